@@ -1,0 +1,133 @@
+/* phx — MI355X-native batched fixed-step RKF78 orbit propagator with the
+ * ParaHPOP full force model. C-ABI of libphx.so (parahpop_amd/libphx.so).
+ *
+ * All buffers are host-owned and structure-of-arrays: state6_soa holds
+ * x[n], y[n], z[n], vx[n], vy[n], vz[n] (km, km/s, ICRF-aligned, Earth-centred).
+ * Every call returns 0 on success and a negative code on failure; the message
+ * is available from phx_last_error(). No C++ exception crosses the ABI.
+ * A context owns one device, one HIP stream and its device memory; it is not
+ * re-entrant (use one context per host thread or serialize calls), except the
+ * serving calls phx_submit/phx_poll, which are thread-safe.
+ */
+#ifndef PHX_H
+#define PHX_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes */
+#define PHX_OK 0
+#define PHX_EINVAL -1      /* invalid configuration, target or argument */
+#define PHX_EDEVICE -2     /* HIP runtime or device failure */
+#define PHX_ENOTDONE -3    /* a target did not reach its end of simulation */
+#define PHX_ENOMEM -4      /* device memory budget exceeded */
+#define PHX_EAGAIN -5      /* serving: result not ready yet */
+
+typedef struct phx_ctx phx_ctx;
+typedef struct phx_batch phx_batch;
+
+typedef struct phx_stats {
+    double object_steps;       /* completed RKF78 steps summed over objects */
+    double kernel_seconds;     /* device time of the step kernels (HIP events) */
+    double epoch_seconds;      /* device time of the epoch-record kernel */
+    double wall_seconds;       /* host wall time of the call */
+    double epoch_records;      /* distinct stage epochs evaluated */
+    double launches;           /* step-kernel launches */
+    int waves;                 /* waves per 64-object tile used */
+    int steps_per_launch;
+} phx_stats;
+
+/* Creates a context from a ParaHPOP CONFIG.json: parses and validates the
+ * "model" and "execution" sections (interface: src/common/driver.cpp:72-86,132-135,
+ * interface/config/Model.h:5-28), loads the environment data it names
+ * (gravity field, space weather, EOP, constants) and uploads all tables.
+ * Replaces the reference's config::Model construction + environment().load(). */
+int phx_create(const char* config_json_path, phx_ctx** out);
+
+/* Propagates n objects for duration_s with the configured fixed step,
+ * shortening each object's final step (propagators/FixedStep.h:7-47), and
+ * writes final epochs (MJD2000 TDB) and states. Inputs as in TARGETS.json:
+ * areas in m^2; the km^2 conversion and cr_eff = cr*srp/drag happen inside
+ * (driver.cpp:99-123). Replaces paraHPOP::run(cfg, samples, duration, step,
+ * maxSteps, threads, gpu=true) (engine/paraHPOP/run.h:5-13). Synchronous. */
+int phx_propagate(phx_ctx* ctx, size_t n, const int64_t* id, const double* epoch_mjd2000_tdb,
+                  const double* state6_soa, const double* mass_kg, const double* drag_area_m2,
+                  const double* srp_area_m2, const double* cd, const double* cr, double duration_s,
+                  double* out_epoch, double* out_state6_soa, phx_stats* stats);
+
+/* Device-resident batches (the same work as phx_propagate split into
+ * upload / run / download so a caller can time or interleave the device
+ * part; replaces the reference's Model::upload + graph replay loop +
+ * download in propagators/device/Propagator.h:33-66). */
+int phx_batch_create(phx_ctx* ctx, size_t n, const int64_t* id, const double* epoch_mjd2000_tdb,
+                     const double* state6_soa, const double* mass_kg, const double* drag_area_m2,
+                     const double* srp_area_m2, const double* cd, const double* cr, double duration_s,
+                     phx_batch** out);
+/* Runs up to max_steps further integration steps (<= 0: to completion). */
+int phx_batch_run(phx_batch* b, int max_steps, phx_stats* stats);
+/* 1 when every object reached its end of simulation, else 0. */
+int phx_batch_done(const phx_batch* b);
+int phx_batch_download(phx_batch* b, double* out_epoch, double* out_state6_soa);
+/* Copies the batch's initial state back in place so it can be run again. */
+int phx_batch_reset(phx_batch* b);
+void phx_batch_destroy(phx_batch* b);
+
+/* Tuning options (no reference counterpart; the reference reads
+ * PARAHPOP_PERF_* environment overrides, AccumulateBodies.cu:20-48):
+ *   "waves"            1, 2, 4 or 8 waves per 64-object tile
+ *   "steps_per_launch" integration steps per kernel launch
+ *   "share0", "share_corr", "share_drag"  relative SH order-block shares
+ *   "sort_altitude"    1: group objects below the drag cutoff into common tiles */
+int phx_set_option(phx_ctx* ctx, const char* key, double value);
+double phx_get_option(const phx_ctx* ctx, const char* key);
+
+/* Serving (continuous batching). A request is admitted when its device
+ * footprint fits; its objects join the running device-resident batch at the
+ * next step boundary and leave it when they finish. */
+typedef struct phx_request {
+    size_t n;
+    const int64_t* id;
+    const double* epoch_mjd2000_tdb;
+    const double* state6_soa;
+    const double* mass_kg;
+    const double* drag_area_m2;
+    const double* srp_area_m2;
+    const double* cd;
+    const double* cr;
+    double duration_s;
+} phx_request;
+
+typedef struct phx_result {
+    size_t n;
+    double* out_epoch;          /* caller-owned, n entries */
+    double* out_state6_soa;     /* caller-owned, 6n entries */
+    double submit_time_s;       /* monotonic clock at submit */
+    double finish_time_s;       /* monotonic clock when the last object finished */
+} phx_result;
+
+int phx_submit(phx_ctx* ctx, const phx_request* req, uint64_t* ticket);
+int phx_poll(phx_ctx* ctx, uint64_t ticket, phx_result* result);
+
+const char* phx_last_error(const phx_ctx* ctx);
+void phx_destroy(phx_ctx* ctx);
+
+/* Diagnostics for tests (no reference counterpart). */
+/* Force model at n states (pos/vel SoA [3][n]) and one epoch (ET seconds):
+ * which = -1 all, 0 central, 1 SH, 2 third bodies, 3 SRP, 4 drag, 5 corrections.
+ * area_km2_over_mass, cr_eff, cd per object; acc out [3][n] km/s^2. */
+int phx_force_probe(phx_ctx* ctx, size_t n, const double* pos, const double* vel, const double* mass,
+                    const double* area_km2, const double* cr_eff, const double* cd, double et, int which,
+                    double* acc);
+/* Epoch record (REC_SIZE doubles) for one ET epoch. */
+int phx_epoch_record(phx_ctx* ctx, double et, double* record, size_t record_len);
+/* NRLMSISE-00 gtd7d on the device for test-driver style inputs (rows of 16:
+ * doy, sec, alt, lat, lon, lst, f107a, f107, ap, ap[7]); out rows of 11. */
+int phx_nrlmsise_probe(phx_ctx* ctx, size_t n, const double* in, int use_ap_array, double* out);
+const char* phx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,53 @@
+/* ORACLE — test infrastructure only (parity checker and timed CPU baseline).
+ * C-ABI of the CPU restatement of the reference host propagator
+ * (propagators/host/Propagator.h:13-34 + parm/integrate/rk/host/Step.h:109-124).
+ * Same argument meaning as phx_propagate in include/phx.h. */
+#ifndef PHX_ORACLE_H
+#define PHX_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_ctx orc_ctx;
+
+typedef struct orc_stats {
+    double object_steps;      /* completed RKF78 steps summed over objects */
+    double rhs_evaluations;   /* force-model evaluations (13 per step) */
+    double seconds;           /* wall time of the stepping loop */
+    int threads;
+} orc_stats;
+
+/* data_dir holds the published tables (tide frequencies, FES2004, NRLMSISE-00). */
+int orc_create(const char* config_json_path, const char* data_dir, orc_ctx** out);
+int orc_propagate(orc_ctx* ctx, size_t n, const int64_t* id, const double* epoch_mjd2000_tdb,
+                  const double* state6_soa, const double* mass_kg, const double* drag_area_m2,
+                  const double* srp_area_m2, const double* cd, const double* cr, double duration_s,
+                  int threads, double* out_epoch, double* out_state6_soa, orc_stats* stats);
+/* One right-hand-side evaluation (acceleration, km/s^2) for unit tests of the
+ * force model: r, v in km, km/s; epoch ET seconds; area km^2; cr effective. */
+int orc_acceleration(orc_ctx* ctx, const double* r, const double* v, double et, double mass,
+                     double area_km2, double cr_eff, double cd, double* acc_out);
+/* Selected force terms at one state (km/s^2): which = 0 central, 1 SH, 2 third
+ * bodies, 3 SRP (with shadow), 4 drag, 5 Earth corrections. */
+int orc_force_term(orc_ctx* ctx, int which, const double* r, const double* v, double et, double mass,
+                   double area_km2, double cr_eff, double cd, double* acc_out);
+/* Raw NRLMSISE-00 gtd7d call (cgs units, switch 0 off as in the published
+ * test driver; use_ap_array selects switch 9 = -1 with the 7-entry ap array). */
+int orc_nrlmsise(const char* data_dir, int doy, double sec, double alt, double glat, double glon, double lst,
+                 double f107a, double f107, double ap, const double* apa, int use_ap_array, double* d9, double* t2);
+/* Geocentric body position (km) from the analytic ephemeris and the analytic
+ * Earth orientation (row-major ICRF->ITRF matrix, omega rad/s) at ET seconds,
+ * with zero polar motion and UT1 = UTC = TT - 69.184 s. */
+int orc_body_position(int naif, double et, double* pos3);
+int orc_earth_orientation(double et, double* R9, double* omega3);
+double orc_step_seconds(const orc_ctx* ctx);
+double orc_duration_seconds(const orc_ctx* ctx);
+const char* orc_last_error(void);
+void orc_destroy(orc_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,21 @@
+"""Build libphx.so (HIP, gfx950) in-tree: ``python parahpop_amd/build.py``."""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def build(jobs=8, verbose=False):
+    csrc = os.path.join(HERE, "csrc")
+    cmd = ["make", "-C", csrc, "-j%d" % jobs]
+    out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if verbose or out.returncode != 0:
+        sys.stdout.write(out.stdout)
+    if out.returncode != 0:
+        raise RuntimeError("libphx build failed")
+    return os.path.join(HERE, "libphx.so")
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

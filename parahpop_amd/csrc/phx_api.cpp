@@ -1,0 +1,852 @@
+// C-ABI of libphx: context (configuration, environment tables on the device),
+// device-resident batches and the propagation driver loop.
+//
+// Reference behaviour kept at this boundary (src/common/ unless noted):
+//   target validation and unit conversion      driver.cpp:99-123
+//   epoch conversion, first/last step sizes    engine/paraHPOP/propagators/FixedStep.h:7-47
+//   end-of-simulation check                    driver.cpp:146-155
+//   device budget guard (60 % of free memory)  engine/paraHPOP/model/environment/Environment.h:977-1050
+// Compiled with -ffp-contract=off: the epoch arithmetic here must round
+// exactly as the CPU reference does.
+#include <dlfcn.h>
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/phx.h"
+#include "model.h"
+#include "phx_common.h"
+#include "phx_kernels.h"
+#include "phx_serve.h"
+
+using namespace phx;
+
+namespace {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+        n = count;
+    }
+    void upload(const T* host, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) check(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+    }
+};
+
+const double kRkC[12] = {2. / 27, 1. / 9, 1. / 6, 5. / 12, 1. / 2, 5. / 6, 1. / 6, 2. / 3, 1. / 3, 1.0, 0, 1.0};
+
+std::string libraryDir() {
+    Dl_info info;
+    if (dladdr((void*)&phx_create, &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        const size_t k = p.find_last_of('/');
+        return k == std::string::npos ? "." : p.substr(0, k);
+    }
+    return ".";
+}
+
+std::string dirName(const std::string& p) {
+    const size_t k = p.find_last_of('/');
+    if (k == std::string::npos) return ".";
+    return k == 0 ? "/" : p.substr(0, k);
+}
+
+}  // namespace
+
+struct phx_ctx {
+    RunConfig run;
+    Environment env;
+    ForceModel fm{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string error;
+    std::string dataDir;
+    // device tables
+    DBuf<double> shTerms, shDiag, nrl, freq, waves, terms, eop, weather;
+    DBuf<float> shTerms32;
+    DBuf<int> shOrderStart;
+    int freqCount[3] = {0, 0, 0};
+    int waveCount = 0;
+    // options
+    int nWaves = 4;
+    int stepsPerLaunch = 32;
+    double share0 = 0.85, shareCorr = 0.85, shareDrag = 0.35;
+    int sortAltitude = 1;
+    // serving
+    std::unique_ptr<Server> server;
+};
+
+struct phx_batch {
+    phx_ctx* ctx = nullptr;
+    size_t n = 0;
+    int nTiles = 0;
+    int nPad = 0;
+    std::vector<int> slotOf;     // original index -> padded slot
+    int totalSteps = 0;          // steps of the longest group
+    int stepsDone = 0;
+    DBuf<double> y, cur, end, nextDt, ams, bcd, records;
+    DBuf<uint8_t> status;
+    DBuf<long long> tileRecord, counters;
+    std::vector<double> y0, cur0, dt0;
+    std::vector<uint8_t> st0;
+    size_t epochRecords = 0;
+    double epochSeconds = 0.0;
+};
+
+namespace {
+
+void loadTables(phx_ctx& c) {
+    const std::string dir = c.dataDir;
+    if (c.env.model.tidesActive()) {
+        std::ifstream in(dir + "/iers2010_solid_tide_frequency.txt");
+        if (!in) throw std::runtime_error("missing table " + dir + "/iers2010_solid_tide_frequency.txt");
+        std::vector<double> rows[3];
+        std::string line;
+        while (std::getline(in, line)) {
+            if (line.empty() || line[0] == '#') continue;
+            std::istringstream s(line);
+            int order, mult[5];
+            double ip, op;
+            s >> order >> mult[0] >> mult[1] >> mult[2] >> mult[3] >> mult[4] >> ip >> op;
+            if (!s || order < 0 || order > 2) throw std::runtime_error("bad tide frequency table row");
+            for (int k = 0; k < 5; ++k) rows[order].push_back(mult[k]);
+            rows[order].push_back(ip);
+            rows[order].push_back(op);
+        }
+        std::vector<double> all;
+        for (int o = 0; o < 3; ++o) {
+            c.freqCount[o] = int(rows[o].size() / 7);
+            all.insert(all.end(), rows[o].begin(), rows[o].end());
+        }
+        c.freq.upload(all.data(), all.size(), c.stream);
+        std::ifstream f(dir + "/fes2004_ocean_tide.txt");
+        if (!f) throw std::runtime_error("missing table " + dir + "/fes2004_ocean_tide.txt");
+        std::vector<double> w, t;
+        while (std::getline(f, line)) {
+            if (line.empty() || line[0] == '#') continue;
+            std::istringstream s(line);
+            char tag;
+            s >> tag;
+            if (tag == 'W') {
+                double v[8];
+                for (double& x : v) s >> x;
+                w.insert(w.end(), v, v + 8);
+            } else if (tag == 'T') {
+                double v[6];
+                for (double& x : v) s >> x;
+                t.insert(t.end(), v, v + 6);
+            }
+            if (!s) throw std::runtime_error("bad FES2004 table row");
+        }
+        c.waveCount = int(w.size() / 8);
+        c.waves.upload(w.data(), w.size(), c.stream);
+        c.terms.upload(t.data(), t.size(), c.stream);
+    }
+    if (c.env.hasWeather) {
+        std::ifstream in(dir + "/nrlmsise00_coefficients.txt");
+        if (!in) throw std::runtime_error("missing table " + dir + "/nrlmsise00_coefficients.txt");
+        // blocks in file order: pt pd ps pdl ptm pdm ptl pma sam pavgm (3300 values)
+        std::vector<double> v;
+        std::string line;
+        while (std::getline(in, line)) {
+            if (line.empty() || line[0] == '#' || std::isalpha((unsigned char)line[0])) continue;
+            std::istringstream s(line);
+            double x;
+            while (s >> x) v.push_back(x);
+        }
+        if (v.size() != 3300) throw std::runtime_error("NRLMSISE-00 coefficient table must hold 3300 values");
+        c.nrl.upload(v.data(), v.size(), c.stream);
+        const WeatherTable& wt = c.env.weather;
+        c.weather.upload(wt.data.data(), wt.data.size(), c.stream);
+    }
+    if (c.env.hasEop) {
+        const EopTable& e = c.env.eop;
+        std::vector<double> v;
+        v.insert(v.end(), e.xp.begin(), e.xp.end());
+        v.insert(v.end(), e.yp.begin(), e.yp.end());
+        v.insert(v.end(), e.dut.begin(), e.dut.end());
+        v.insert(v.end(), e.dat.begin(), e.dat.end());
+        c.eop.upload(v.data(), v.size(), c.stream);
+    }
+}
+
+// Packed SH term stream (see ShTerm in phx_common.h).
+void buildShTables(phx_ctx& c) {
+    const ShField& f = c.env.sh;
+    const int L = f.degree, M = f.order;
+    std::vector<double> terms;
+    std::vector<float> terms32;
+    std::vector<int> start(static_cast<size_t>(L) + 2, 0);
+    std::vector<double> diag(static_cast<size_t>(L) + 1, 0.0);
+    int k = 0;
+    for (int m = 0; m <= L; ++m) {
+        diag[size_t(m)] = f.factors[2 * f.index(m, m)];
+        start[size_t(m)] = k;
+        if (m > M) continue;
+        for (int n = m; n <= L; ++n, ++k) {
+            const double C = f.C[f.index(n, m)], S = m > 0 ? f.S[f.index(n, m)] : 0.0;
+            double e;
+            if (f.normalized) e = n == m ? 0.0 : std::sqrt((2.0 * n + 1.0) * (double(n) * n - double(m) * m) / (2.0 * n - 1.0));
+            else e = double(n + m);
+            double a = 0.0, b = 0.0;
+            if (n < L) {
+                a = f.factors[2 * f.index(n + 1, m)];
+                b = f.factors[2 * f.index(n + 1, m) + 1];
+            }
+            const double t[SHT_STRIDE] = {(n + 1) * C, (n + 1) * S, C, S, e * C, e * S, a, b};
+            terms.insert(terms.end(), t, t + SHT_STRIDE);
+            for (double x : t) terms32.push_back(float(x));
+        }
+    }
+    start[size_t(L) + 1] = k;
+    c.shTerms.upload(terms.data(), terms.size(), c.stream);
+    c.shTerms32.upload(terms32.data(), terms32.size(), c.stream);
+    c.shOrderStart.upload(start.data(), start.size(), c.stream);
+    c.shDiag.upload(diag.data(), diag.size(), c.stream);
+}
+
+void buildForceModel(phx_ctx& c) {
+    const ModelConfig& m = c.env.model;
+    ForceModel& fm = c.fm;
+    std::memset(&fm, 0, sizeof fm);
+    if (m.bodies.size() > size_t(kMaxBodies)) throw std::runtime_error("too many bodies");
+    fm.nBodies = int(m.bodies.size());
+    int occ = 0;
+    for (size_t i = 0; i < m.bodies.size(); ++i) {
+        const BodySpec& b = m.bodies[i];
+        fm.bodyNaif[i] = b.naif;
+        fm.bodyGm[i] = b.gm;
+        fm.bodyPointGravity[i] = b.pointGravity ? 1 : 0;
+        if (b.naif == NAIF_EARTH) {
+            fm.earthPointGravity = b.pointGravity ? 1 : 0;
+            fm.gmEarth = b.gm;
+            if (b.atmosphere && b.nrlmsise) {
+                fm.drag = 1;
+                fm.hCutoff = b.nrl.hCutoff;
+                fm.densityScale = b.nrl.densityScale;
+                fm.earthRadius = b.radius;
+                fm.flattening = b.flattening;
+            }
+        }
+        if (b.radiation) {
+            fm.srp = 1;
+            fm.srpBody = int(i);
+            fm.srpFactor = m.au * m.au * 4.56e-3;
+            fm.sunRadius = b.radius;
+        }
+        if (b.occulting && b.naif != NAIF_SUN) {
+            if (occ >= 4) throw std::runtime_error("at most four occulting bodies are supported");
+            fm.occBody[occ] = int(i);
+            fm.occRadius[occ] = b.radius;
+            ++occ;
+        }
+    }
+    fm.nOccluders = occ;
+    if (c.env.hasSh) {
+        fm.hasSh = 1;
+        fm.shDegree = c.env.sh.degree;
+        fm.shOrder = c.env.sh.order;
+        fm.shRadius = c.env.sh.radius;
+        fm.shGm = c.env.sh.gm;
+        fm.shNormalized = c.env.sh.normalized ? 1 : 0;
+    }
+    fm.tides = m.tidesActive() ? 1 : 0;
+    fm.relativity = m.relativity ? 1 : 0;
+    fm.muEarthSI = 398600.4415e9;
+    fm.radiusSI = 6378.1363e3;
+    fm.clightSI = 299792457.999999984;
+}
+
+EpochArgs epochArgs(const phx_ctx& c) {
+    EpochArgs a{};
+    a.eop = c.eop.p;
+    a.eopDays = c.env.hasEop ? c.env.eop.days : 0;
+    a.eopFirst = c.env.hasEop ? c.env.eop.firstMjdUtc : 0.0;
+    a.solid = c.env.model.solidTides ? 1 : 0;
+    a.ocean = c.env.model.oceanTides ? 1 : 0;
+    a.zeroTide = c.env.model.zeroTide ? 1 : 0;
+    a.freq = c.freq.p;
+    for (int k = 0; k < 3; ++k) a.freqCount[k] = c.freqCount[k];
+    a.waves = c.waves.p;
+    a.waveCount = c.waveCount;
+    a.terms = c.terms.p;
+    a.weather = c.weather.p;
+    a.wDays = c.env.hasWeather ? c.env.weather.days : 0;
+    a.wStart = c.env.hasWeather ? c.env.weather.startMjdUtc : 0.0;
+    return a;
+}
+
+// Order blocks per wave, balanced on term counts with relative shares.
+void orderBlocks(const phx_ctx& c, int W, int out[kMaxWaves + 1]) {
+    const int L = c.fm.shDegree, M = c.fm.shOrder;
+    std::vector<double> share(static_cast<size_t>(W), 1.0);
+    const int dragW = W - 1, corrW = W >= 3 ? W - 2 : W - 1;
+    if (W > 1) {
+        share[0] *= c.share0;
+        if (c.fm.tides || c.fm.relativity || c.fm.srp) share[size_t(corrW)] *= c.shareCorr;
+        if (c.fm.drag) share[size_t(dragW)] *= c.shareDrag;
+    }
+    double total = 0.0;
+    for (int m = 0; m <= M; ++m) total += double(L - m + 1) + 0.6 * m / std::max(1, W);
+    double sumShare = 0.0;
+    for (double s : share) sumShare += s;
+    out[0] = 0;
+    int m = 0;
+    double acc = 0.0, target = 0.0;
+    for (int w = 0; w < W; ++w) {
+        target += total * share[size_t(w)] / sumShare;
+        if (w == W - 1) {
+            m = M + 1;
+        } else {
+            while (m <= M && acc + 0.5 * (double(L - m + 1)) <= target) {
+                acc += double(L - m + 1) + 0.6 * m / std::max(1, W);
+                ++m;
+            }
+        }
+        out[w + 1] = m;
+    }
+    for (int w = W + 1; w <= kMaxWaves; ++w) out[w] = out[W];
+}
+
+void validateTargets(size_t n, const int64_t* id, const double* epoch, const double* state6, const double* mass,
+                     const double* drag, const double* srp, const double* cd, const double* cr) {
+    if (n == 0) throw std::invalid_argument("targets must be a nonempty array.");
+    if (n > size_t(0x7fffffff)) throw std::invalid_argument("Too many targets for the engine index type.");
+    std::set<int64_t> ids;
+    for (size_t i = 0; i < n; ++i) {
+        if (!ids.insert(id[i]).second) throw std::invalid_argument("Target IDs must be unique.");
+        for (int d = 0; d < 6; ++d)
+            if (!std::isfinite(state6[d * n + i])) throw std::invalid_argument("State contains a nonfinite value.");
+        if (!std::isfinite(epoch[i]) || !std::isfinite(mass[i]) || !std::isfinite(drag[i]) || !std::isfinite(srp[i]) ||
+            !std::isfinite(cd[i]) || !std::isfinite(cr[i]) || mass[i] <= 0 || drag[i] <= 0 || srp[i] < 0 || cd[i] < 0 ||
+            cr[i] < 0)
+            throw std::invalid_argument("Invalid target epoch or physical parameter.");
+    }
+}
+
+int fail(phx_ctx* c, const std::exception& e) {
+    if (c) c->error = e.what();
+    if (dynamic_cast<const HipError*>(&e)) return PHX_EDEVICE;
+    if (dynamic_cast<const std::bad_alloc*>(&e)) return PHX_ENOMEM;
+    return PHX_EINVAL;
+}
+
+thread_local std::string g_createError;
+
+}  // namespace
+
+namespace phx {
+
+// Builds the device batch: validation, unit conversion, epoch groups, tiles,
+// per-group step sequence and epoch records.
+std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, const double* epoch, const double* state6,
+                                      const double* mass, const double* drag, const double* srp, const double* cd,
+                                      const double* cr, double duration) {
+    validateTargets(n, id, epoch, state6, mass, drag, srp, cd, cr);
+    if (!(duration > 0) || !std::isfinite(duration)) throw std::invalid_argument("duration_seconds must be finite and positive.");
+    const double step = c.run.stepSeconds;
+    auto b = std::make_unique<phx_batch>();
+    b->ctx = &c;
+    b->n = n;
+    // FixedStep.h:7-29 — ET conversion, first step, launch count
+    std::vector<double> start(n), endEt(n);
+    long long count = 0;
+    for (size_t i = 0; i < n; ++i) {
+        start[i] = epoch[i] * 86400.0 + -43200.0;
+        endEt[i] = (epoch[i] + duration / 86400.0) * 86400.0 + -43200.0;
+        const double span = endEt[i] - start[i];
+        if (!(span > 0) || !std::isfinite(span) || std::ceil(span / step) >= 2147483647.0)
+            throw std::invalid_argument("Duration/step cannot be represented by the integration time/index types.");
+        count = std::max(count, (long long)std::ceil(span / step) + 1);
+    }
+    const long long launches = std::min(count, c.run.maxNumSteps);
+    // epoch groups: identical (start, end) share one step sequence
+    std::map<std::pair<double, double>, int> groupOf;
+    std::vector<int> grp(n);
+    for (size_t i = 0; i < n; ++i) {
+        auto key = std::make_pair(start[i], endEt[i]);
+        auto it = groupOf.find(key);
+        if (it == groupOf.end()) it = groupOf.emplace(key, int(groupOf.size())).first;
+        grp[i] = it->second;
+    }
+    const int G = int(groupOf.size());
+    std::vector<std::pair<double, double>> gkey(static_cast<size_t>(G));
+    for (auto& kv : groupOf) gkey[size_t(kv.second)] = kv.first;
+    // step sequences and record epochs per group
+    std::vector<double> epochs;
+    std::vector<long long> groupRecord(static_cast<size_t>(G));
+    int totalSteps = 0;
+    for (int g = 0; g < G; ++g) {
+        groupRecord[size_t(g)] = (long long)(epochs.size() / 1);
+        double cur = gkey[size_t(g)].first;
+        const double end = gkey[size_t(g)].second;
+        double dt = std::min(step, end - cur);
+        int k = 0;
+        while (k < launches) {
+            for (int s = 0; s < kStages; ++s) {
+                double ts = cur;
+                if (s >= 1) ts += dt * kRkC[s - 1];
+                epochs.push_back(ts);
+            }
+            cur += dt;
+            const double rem = end - cur;
+            const bool fin = rem <= 0;
+            dt = fin ? 0.0 : std::min(step, rem);
+            ++k;
+            if (fin) break;
+        }
+        totalSteps = std::max(totalSteps, k);
+    }
+    for (int g = 0; g < G; ++g) groupRecord[size_t(g)] /= kStages;  // in units of steps
+    b->totalSteps = totalSteps;
+    // order: by group, then by altitude so drag lanes share tiles
+    std::vector<int> order(n);
+    for (size_t i = 0; i < n; ++i) order[i] = int(i);
+    std::vector<double> alt(n);
+    for (size_t i = 0; i < n; ++i)
+        alt[i] = std::sqrt(state6[i] * state6[i] + state6[n + i] * state6[n + i] + state6[2 * n + i] * state6[2 * n + i]);
+    if (c.sortAltitude)
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b2) {
+            if (grp[size_t(a)] != grp[size_t(b2)]) return grp[size_t(a)] < grp[size_t(b2)];
+            return alt[size_t(a)] < alt[size_t(b2)];
+        });
+    else
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return grp[size_t(a)] < grp[size_t(b2)]; });
+    // tiles never mix groups
+    std::vector<int> slotObj;  // slot -> object (-1 padding)
+    std::vector<int> tileGroup;
+    for (size_t k = 0; k < n;) {
+        const int g = grp[size_t(order[k])];
+        size_t e = k;
+        while (e < n && grp[size_t(order[e])] == g) ++e;
+        for (size_t t = k; t < e; t += kTile) {
+            tileGroup.push_back(g);
+            for (int l = 0; l < kTile; ++l) slotObj.push_back(t + size_t(l) < e ? order[t + size_t(l)] : -1);
+        }
+        k = e;
+    }
+    const int nTiles = int(tileGroup.size());
+    const int nPad = nTiles * kTile;
+    b->nTiles = nTiles;
+    b->nPad = nPad;
+    b->slotOf.assign(n, -1);
+    const size_t np = static_cast<size_t>(nPad);
+    std::vector<double> y(6 * np), cur(np), end(np), nd(np), ams(np), bcd(np);
+    std::vector<uint8_t> st(np);
+    std::vector<long long> tileRec(static_cast<size_t>(nTiles));
+    for (int t = 0; t < nTiles; ++t) tileRec[size_t(t)] = groupRecord[size_t(tileGroup[size_t(t)])];
+    for (int s = 0; s < nPad; ++s) {
+        int i = slotObj[size_t(s)];
+        const bool pad = i < 0;
+        if (pad) i = slotObj[size_t(s - s % kTile)];  // copy of the tile's first object
+        else b->slotOf[size_t(i)] = s;
+        for (int d = 0; d < 6; ++d) y[size_t(d) * nPad + s] = state6[size_t(d) * n + i];
+        cur[size_t(s)] = start[size_t(i)];
+        end[size_t(s)] = endEt[size_t(i)];
+        nd[size_t(s)] = std::min(step, endEt[size_t(i)] - start[size_t(i)]);
+        // driver.cpp:117-120: area m^2 -> km^2, cr_eff = cr * srp / drag
+        const double area = drag[i] * 1e-6;
+        const double crEff = cr[i] * srp[i] / drag[i];
+        ams[size_t(s)] = crEff * area / mass[i];
+        bcd[size_t(s)] = cd[i] * area / mass[i];
+        st[size_t(s)] = pad ? 4 : 0;
+    }
+    // device budget guard (Environment.h:977-1050): records + state must fit in 60 % of free memory
+    size_t freeB = 0, totalB = 0;
+    check(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
+    const size_t need = epochs.size() * (REC_SIZE + 1) * sizeof(double) + size_t(nPad) * (12 * sizeof(double) + 1);
+    if (double(need) > 0.6 * double(freeB))
+        throw std::length_error("Device memory budget exceeded (" + std::to_string(need >> 20) + " MiB needed); split into multiple simulations.");
+    hipStream_t s = c.stream;
+    b->y.upload(y.data(), y.size(), s);
+    b->cur.upload(cur.data(), cur.size(), s);
+    b->end.upload(end.data(), end.size(), s);
+    b->nextDt.upload(nd.data(), nd.size(), s);
+    b->ams.upload(ams.data(), ams.size(), s);
+    b->bcd.upload(bcd.data(), bcd.size(), s);
+    b->status.upload(st.data(), st.size(), s);
+    b->tileRecord.upload(tileRec.data(), tileRec.size(), s);
+    long long zero[4] = {0, 0, 0, 0};
+    b->counters.upload(zero, 4, s);
+    b->y0 = y;
+    b->cur0 = cur;
+    b->dt0 = nd;
+    b->st0 = st;
+    // epoch records
+    DBuf<double> dEpochs;
+    dEpochs.upload(epochs.data(), epochs.size(), s);
+    b->records.alloc(epochs.size() * REC_SIZE);
+    EpochArgs ea = epochArgs(c);
+    ea.count = int(epochs.size());
+    ea.epochs = dEpochs.p;
+    ea.out = b->records.p;
+    check(hipEventRecord(c.ev0, s), "hipEventRecord");
+    if (launchEpochRecords(ea, c.fm, s) != 0) throw HipError("epoch-record kernel launch failed");
+    check(hipEventRecord(c.ev1, s), "hipEventRecord");
+    check(hipStreamSynchronize(s), "epoch records");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, c.ev0, c.ev1), "hipEventElapsedTime");
+    b->epochSeconds = ms * 1e-3;
+    b->epochRecords = epochs.size();
+    return b;
+}
+
+int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
+    const auto w0 = std::chrono::steady_clock::now();
+    StepArgs a{};
+    a.nTiles = b.nTiles;
+    a.nPad = b.nPad;
+    a.y = b.y.p;
+    a.cur = b.cur.p;
+    a.end = b.end.p;
+    a.nextDt = b.nextDt.p;
+    a.status = b.status.p;
+    a.ams = b.ams.p;
+    a.bcd = b.bcd.p;
+    a.records = b.records.p;
+    a.tileRecord = b.tileRecord.p;
+    a.step = c.run.stepSeconds;
+    a.shTerms = c.shTerms.p;
+    a.shTerms32 = c.shTerms32.p;
+    a.shOrderStart = c.shOrderStart.p;
+    a.shDiag = c.shDiag.p;
+    a.fp64Degree = c.run.precision.mixed ? c.run.precision.fp64Degree : -1;
+    a.nrlCoeff = c.nrl.p;
+    a.counters = b.counters.p;
+    const int W = c.fm.hasSh ? c.nWaves : 1;
+    if (c.fm.hasSh) orderBlocks(c, W, a.blockStart);
+    else for (int w = 0; w <= kMaxWaves; ++w) a.blockStart[w] = 0;
+    int remaining = b.totalSteps - b.stepsDone;
+    if (maxSteps > 0) remaining = std::min(remaining, maxSteps);
+    long long before = 0;
+    check(hipMemcpyAsync(&before, b.counters.p, sizeof before, hipMemcpyDeviceToHost, c.stream), "counter read");
+    check(hipEventRecord(c.ev0, c.stream), "hipEventRecord");
+    int launches = 0;
+    while (remaining > 0) {
+        const int k = std::min(remaining, std::max(1, c.stepsPerLaunch));
+        a.stepBase = b.stepsDone;
+        a.steps = k;
+        const int rc = launchStep(a, c.fm, W, c.stream);
+        if (rc != 0) throw HipError("step kernel launch failed");
+        b.stepsDone += k;
+        remaining -= k;
+        ++launches;
+    }
+    check(hipEventRecord(c.ev1, c.stream), "hipEventRecord");
+    long long after = 0;
+    check(hipMemcpyAsync(&after, b.counters.p, sizeof after, hipMemcpyDeviceToHost, c.stream), "counter read");
+    check(hipStreamSynchronize(c.stream), "step kernels");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, c.ev0, c.ev1), "hipEventElapsedTime");
+    if (stats) {
+        stats->object_steps = double(after - before);
+        stats->kernel_seconds = ms * 1e-3;
+        stats->epoch_seconds = b.epochSeconds;
+        stats->epoch_records = double(b.epochRecords);
+        stats->launches = launches;
+        stats->waves = W;
+        stats->steps_per_launch = c.stepsPerLaunch;
+        stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
+    }
+    return 0;
+}
+
+void downloadBatch(phx_ctx& c, phx_batch& b, double* outEpoch, double* outState) {
+    const size_t np = static_cast<size_t>(b.nPad);
+    std::vector<double> y(6 * np), cur(np);
+    std::vector<uint8_t> st(np);
+    check(hipMemcpyAsync(y.data(), b.y.p, y.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream), "D2H");
+    check(hipMemcpyAsync(cur.data(), b.cur.p, cur.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream), "D2H");
+    check(hipMemcpyAsync(st.data(), b.status.p, st.size(), hipMemcpyDeviceToHost, c.stream), "D2H");
+    check(hipStreamSynchronize(c.stream), "download");
+    for (size_t i = 0; i < b.n; ++i) {
+        const int s = b.slotOf[i];
+        if (!(st[size_t(s)] & 2)) throw std::range_error("A target stopped without reaching the requested end of simulation.");
+        if (outEpoch) outEpoch[i] = cur[size_t(s)] * (1.0 / 86400.0) + 0.5;   // TimeConversion.h:36-40
+        if (outState)
+            for (int d = 0; d < 6; ++d) outState[size_t(d) * b.n + i] = y[size_t(d) * b.nPad + s];
+    }
+}
+
+}  // namespace phx
+
+extern "C" {
+
+const char* phx_version(void) { return "phx 0.1 (gfx950)"; }
+
+int phx_create(const char* config_json_path, phx_ctx** out) {
+    if (!out || !config_json_path) return PHX_EINVAL;
+    *out = nullptr;
+    auto c = std::make_unique<phx_ctx>();
+    try {
+        int ndev = 0;
+        check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+        if (ndev <= 0) throw HipError("No usable HIP device");
+        const std::string path = config_json_path;
+        json::Value root = json::readFile(path);
+        parseConfig(root, dirName(path), c->run, c->env.model, false);
+        loadEnvironment(c->env);
+        const char* env = std::getenv("PHX_DATA_DIR");
+        c->dataDir = env ? env : libraryDir() + "/data";
+        check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreate(&c->ev0), "hipEventCreate");
+        check(hipEventCreate(&c->ev1), "hipEventCreate");
+        buildForceModel(*c);
+        loadTables(*c);
+        if (c->env.hasSh) buildShTables(*c);
+        check(hipStreamSynchronize(c->stream), "table upload");
+        *out = c.release();
+        return PHX_OK;
+    } catch (const std::exception& e) {
+        g_createError = e.what();
+        return dynamic_cast<const HipError*>(&e) ? PHX_EDEVICE : PHX_EINVAL;
+    }
+}
+
+const char* phx_last_error(const phx_ctx* ctx) { return ctx ? ctx->error.c_str() : g_createError.c_str(); }
+
+void phx_destroy(phx_ctx* ctx) {
+    if (!ctx) return;
+    ctx->server.reset();
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->shTerms.release();
+    ctx->shTerms32.release();
+    ctx->shOrderStart.release();
+    ctx->shDiag.release();
+    ctx->nrl.release();
+    ctx->freq.release();
+    ctx->waves.release();
+    ctx->terms.release();
+    ctx->eop.release();
+    ctx->weather.release();
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int phx_set_option(phx_ctx* c, const char* key, double v) {
+    if (!c || !key) return PHX_EINVAL;
+    const std::string k = key;
+    if (k == "waves") {
+        const int w = int(v);
+        if (w != 1 && w != 2 && w != 4 && w != 8) { c->error = "waves must be 1, 2, 4 or 8"; return PHX_EINVAL; }
+        c->nWaves = w;
+    } else if (k == "steps_per_launch") {
+        if (!(v >= 1)) { c->error = "steps_per_launch must be >= 1"; return PHX_EINVAL; }
+        c->stepsPerLaunch = int(v);
+    } else if (k == "share0") c->share0 = v;
+    else if (k == "share_corr") c->shareCorr = v;
+    else if (k == "share_drag") c->shareDrag = v;
+    else if (k == "sort_altitude") c->sortAltitude = v != 0.0;
+    else { c->error = "unknown option " + k; return PHX_EINVAL; }
+    return PHX_OK;
+}
+
+double phx_get_option(const phx_ctx* c, const char* key) {
+    if (!c || !key) return std::nan("");
+    const std::string k = key;
+    if (k == "waves") return c->nWaves;
+    if (k == "steps_per_launch") return c->stepsPerLaunch;
+    if (k == "share0") return c->share0;
+    if (k == "share_corr") return c->shareCorr;
+    if (k == "share_drag") return c->shareDrag;
+    if (k == "sort_altitude") return c->sortAltitude;
+    if (k == "step_seconds") return c->run.stepSeconds;
+    if (k == "duration_seconds") return c->run.durationSeconds;
+    if (k == "max_num_steps") return double(c->run.maxNumSteps);
+    if (k == "sh_degree") return c->fm.hasSh ? c->fm.shDegree : 0;
+    if (k == "mixed") return c->run.precision.mixed ? 1 : 0;
+    return std::nan("");
+}
+
+int phx_batch_create(phx_ctx* c, size_t n, const int64_t* id, const double* epoch, const double* state6, const double* mass,
+                     const double* drag, const double* srp, const double* cd, const double* cr, double duration,
+                     phx_batch** out) {
+    if (!c || !out) return PHX_EINVAL;
+    try {
+        *out = buildBatch(*c, n, id, epoch, state6, mass, drag, srp, cd, cr, duration).release();
+        return PHX_OK;
+    } catch (const std::length_error& e) {
+        c->error = e.what();
+        return PHX_ENOMEM;
+    } catch (const std::exception& e) {
+        return fail(c, e);
+    }
+}
+
+int phx_batch_run(phx_batch* b, int maxSteps, phx_stats* stats) {
+    if (!b) return PHX_EINVAL;
+    try {
+        return runBatch(*b->ctx, *b, maxSteps, stats);
+    } catch (const std::exception& e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int phx_batch_done(const phx_batch* b) { return b && b->stepsDone >= b->totalSteps ? 1 : 0; }
+
+int phx_batch_download(phx_batch* b, double* outEpoch, double* outState) {
+    if (!b) return PHX_EINVAL;
+    try {
+        downloadBatch(*b->ctx, *b, outEpoch, outState);
+        return PHX_OK;
+    } catch (const std::range_error& e) {
+        b->ctx->error = e.what();
+        return PHX_ENOTDONE;
+    } catch (const std::exception& e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int phx_batch_reset(phx_batch* b) {
+    if (!b) return PHX_EINVAL;
+    try {
+        phx_ctx& c = *b->ctx;
+        check(hipMemcpyAsync(b->y.p, b->y0.data(), b->y0.size() * sizeof(double), hipMemcpyHostToDevice, c.stream), "H2D");
+        check(hipMemcpyAsync(b->cur.p, b->cur0.data(), b->cur0.size() * sizeof(double), hipMemcpyHostToDevice, c.stream), "H2D");
+        check(hipMemcpyAsync(b->nextDt.p, b->dt0.data(), b->dt0.size() * sizeof(double), hipMemcpyHostToDevice, c.stream), "H2D");
+        check(hipMemcpyAsync(b->status.p, b->st0.data(), b->st0.size(), hipMemcpyHostToDevice, c.stream), "H2D");
+        check(hipStreamSynchronize(c.stream), "reset");
+        b->stepsDone = 0;
+        return PHX_OK;
+    } catch (const std::exception& e) {
+        return fail(b->ctx, e);
+    }
+}
+
+void phx_batch_destroy(phx_batch* b) {
+    if (!b) return;
+    if (b->ctx && b->ctx->stream) (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+}
+
+int phx_propagate(phx_ctx* c, size_t n, const int64_t* id, const double* epoch, const double* state6, const double* mass,
+                  const double* drag, const double* srp, const double* cd, const double* cr, double duration,
+                  double* outEpoch, double* outState, phx_stats* stats) {
+    if (!c) return PHX_EINVAL;
+    phx_batch* b = nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = phx_batch_create(c, n, id, epoch, state6, mass, drag, srp, cd, cr, duration, &b);
+    if (rc != PHX_OK) return rc;
+    rc = phx_batch_run(b, 0, stats);
+    if (rc == PHX_OK) rc = phx_batch_download(b, outEpoch, outState);
+    phx_batch_destroy(b);
+    if (stats) stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+int phx_epoch_record(phx_ctx* c, double et, double* record, size_t len) {
+    if (!c || !record || len < size_t(REC_SIZE)) return PHX_EINVAL;
+    try {
+        DBuf<double> e, out;
+        e.upload(&et, 1, c->stream);
+        out.alloc(REC_SIZE);
+        EpochArgs a = epochArgs(*c);
+        a.count = 1;
+        a.epochs = e.p;
+        a.out = out.p;
+        if (launchEpochRecords(a, c->fm, c->stream) != 0) throw HipError("epoch-record kernel launch failed");
+        check(hipMemcpyAsync(record, out.p, REC_SIZE * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipStreamSynchronize(c->stream), "epoch record");
+        return PHX_OK;
+    } catch (const std::exception& e) {
+        return fail(c, e);
+    }
+}
+
+int phx_force_probe(phx_ctx* c, size_t n, const double* pos, const double* vel, const double* mass, const double* area,
+                    const double* crEff, const double* cd, double et, int which, double* acc) {
+    if (!c || n == 0) return PHX_EINVAL;
+    try {
+        std::vector<double> ams(n), bcd(n), rec(REC_SIZE);
+        for (size_t i = 0; i < n; ++i) {
+            ams[i] = crEff[i] * area[i] / mass[i];
+            bcd[i] = cd[i] * area[i] / mass[i];
+        }
+        int rc = phx_epoch_record(c, et, rec.data(), rec.size());
+        if (rc) return rc;
+        DBuf<double> dp, dv, da, db, dr, dacc;
+        dp.upload(pos, 3 * n, c->stream);
+        dv.upload(vel, 3 * n, c->stream);
+        da.upload(ams.data(), n, c->stream);
+        db.upload(bcd.data(), n, c->stream);
+        dr.upload(rec.data(), rec.size(), c->stream);
+        dacc.alloc(3 * n);
+        if (launchForceProbe(int(n), dp.p, dv.p, da.p, db.p, dr.p, which, c->fm, c->shTerms.p, c->shOrderStart.p, c->shDiag.p,
+                             c->nrl.p, dacc.p, c->stream) != 0)
+            throw HipError("force probe launch failed");
+        check(hipMemcpyAsync(acc, dacc.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipStreamSynchronize(c->stream), "force probe");
+        return PHX_OK;
+    } catch (const std::exception& e) {
+        return fail(c, e);
+    }
+}
+
+int phx_nrlmsise_probe(phx_ctx* c, size_t n, const double* in, int useAp, double* out) {
+    if (!c || n == 0) return PHX_EINVAL;
+    try {
+        if (!c->nrl.p) {
+            // load the coefficient table even when the configuration has no drag
+            std::ifstream f(c->dataDir + "/nrlmsise00_coefficients.txt");
+            if (!f) throw std::runtime_error("missing NRLMSISE-00 coefficient table");
+            std::vector<double> v;
+            std::string line;
+            while (std::getline(f, line)) {
+                if (line.empty() || line[0] == '#' || std::isalpha((unsigned char)line[0])) continue;
+                std::istringstream s(line);
+                double x;
+                while (s >> x) v.push_back(x);
+            }
+            if (v.size() != 3300) throw std::runtime_error("NRLMSISE-00 coefficient table must hold 3300 values");
+            c->nrl.upload(v.data(), v.size(), c->stream);
+        }
+        DBuf<double> di, dout;
+        di.upload(in, 16 * n, c->stream);
+        dout.alloc(11 * n);
+        if (launchNrlmsiseProbe(int(n), di.p, useAp, c->nrl.p, dout.p, c->stream) != 0) throw HipError("probe launch failed");
+        check(hipMemcpyAsync(out, dout.p, 11 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream), "D2H");
+        check(hipStreamSynchronize(c->stream), "nrlmsise probe");
+        return PHX_OK;
+    } catch (const std::exception& e) {
+        return fail(c, e);
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+// Host-visible launch wrappers of the HIP kernels (no torch types; plain
+// device pointers and POD argument structs).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "phx_common.h"
+
+namespace phx {
+
+struct EpochArgs {
+    int count;
+    const double* epochs;   // [count] ET seconds
+    double* out;            // [count][REC_SIZE]
+    // EOP table, SoA: xp[days], yp[days], ut1-utc[days], tai-utc[days]
+    const double* eop;
+    int eopDays;
+    double eopFirst;
+    // tides
+    int solid, ocean, zeroTide;
+    const double* freq;     // rows of 7: 5 Delaunay multipliers, in-phase, out-of-phase
+    int freqCount[3];
+    const double* waves;    // rows of 8: 6 Doodson multipliers, first term, end term
+    int waveCount;
+    const double* terms;    // rows of 6: n, m, Cp, Sp, Cm, Sm
+    // NRLMSISE-00 weather (13 x days SoA)
+    const double* weather;
+    int wDays;
+    double wStart;
+};
+
+int launchEpochRecords(const EpochArgs& a, const ForceModel& fm, hipStream_t stream);
+
+// RKF78 step kernel: `waves` waves per 64-object tile (1, 2, 4 or 8).
+int launchStep(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t stream);
+
+// Diagnostics used by the unit tests: one force evaluation per object at the
+// given states and epoch record (acc out [3][n]); `which` selects a force family
+// (-1 all, 0 central, 1 SH, 2 third bodies, 3 SRP, 4 drag, 5 Earth corrections).
+int launchForceProbe(int n, const double* pos, const double* vel, const double* ams, const double* bcd,
+                     const double* record, int which, const ForceModel& fm, const double* shTerms,
+                     const int* shOrderStart, const double* shDiag, const double* nrlCoeff, double* acc,
+                     hipStream_t stream);
+
+// NRLMSISE-00 gtd7d on the device for the published test-driver inputs
+// (in: rows of 16 = doy, sec, alt, lat, lon, lst, f107a, f107, ap, ap[7];
+// out: rows of 11 = d[9], t[2]), switches as in the published driver.
+int launchNrlmsiseProbe(int n, const double* in, int useApArray, const double* nrlCoeff, double* out, hipStream_t stream);
+
+}  // namespace phx

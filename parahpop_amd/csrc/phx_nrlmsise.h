@@ -1,0 +1,788 @@
+// NRLMSISE-00 total-density model on the device (one object per lane).
+// Follows the reference model file
+// src/common/engine/paraHPOP/model/environment/atmosphere/detail/Nrlmsise00Model.inl
+// (tselec :80-100, glatf :108-114, ccor/ccor2 :120-164, scalh :172-178,
+// dnet :186-215, splini/splint/spline :223-338, zeta/densm :346-468,
+// densu :476-576, g0/sumex/sg0 :586-634, prepareGlobeBasis :639-686,
+// globe7 :688-850, glob7s :858-950, gtd7 :958-1065, gtd7d :1073-1078,
+// gts7 :1203-1498). The model's file-scope variables live in a per-lane State.
+// Coefficient blocks come from one device array (offsets below), read with
+// uniform addresses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace phx {
+namespace nrlmsise {
+
+enum : int {
+    OFF_PT = 0, OFF_PD = 150, OFF_PS = 1500, OFF_PDL = 1650, OFF_PTM = 1700, OFF_PDM = 1710,
+    OFF_PTL = 1790, OFF_PMA = 2190, OFF_SAM = 3190, OFF_PAVGM = 3290, COEFF_COUNT = 3300
+};
+
+struct NrlTables {
+    const double* base;
+    __device__ const double* pt() const { return base + OFF_PT; }
+};
+
+
+
+struct Input {
+    int doy;
+    double sec, alt, g_lat, g_long, lst, f107A, f107, ap;
+    double apa[7];
+};
+
+struct Flags {
+    int switches[24];
+    int sw[24];
+    int swc[24];
+};
+
+struct Output { double d[9]; double t[2]; };
+
+// Per-evaluation state (the model's file-scope variables in the C package).
+struct State {
+    double gsurf = 0, re = 0;
+    double dd = 0;
+    double dm04 = 0, dm16 = 0, dm28 = 0, dm32 = 0, dm40 = 0, dm01 = 0, dm14 = 0;
+    double meso_tn1[5] = {}, meso_tn2[4] = {}, meso_tn3[5] = {};
+    double meso_tgn1[2] = {}, meso_tgn2[2] = {}, meso_tgn3[2] = {};
+    double dfa = 0;
+    double plg[4][9] = {};
+    double ctloc = 0, stloc = 0, c2tloc = 0, s2tloc = 0, s3tloc = 0, c3tloc = 0;
+    double apdf = 0, apt[4] = {};
+    bool g0Valid = false;
+    double g0P24 = 0, g0P25 = 0, g0Cache[6] = {};
+};
+
+// Nrlmsise00Model.inl:80-100
+__device__ inline void tselec(Flags& f) {
+    for (int i = 0; i < 24; ++i) {
+        if (i != 9) {
+            f.sw[i] = f.switches[i] == 1 ? 1 : 0;
+            f.swc[i] = f.switches[i] > 0 ? 1 : 0;
+        } else {
+            f.sw[i] = f.switches[i];
+            f.swc[i] = f.switches[i];
+        }
+    }
+}
+
+// :108-114
+__device__ inline void glatf(double lat, double& gv, double& reff) {
+    const double dgtr = 1.74533E-2;
+    const double c2 = cos(2.0 * dgtr * lat);
+    gv = 980.616 * (1.0 - 0.0026373 * c2);
+    reff = 2.0 * gv / (3.085462E-6 + 2.27E-9 * c2) * 1.0E-5;
+}
+
+// :120-137
+__device__ inline double ccor(double alt, double r, double h1, double zh) {
+    double e = (alt - zh) / h1;
+    if (e > 70) return exp(0.0);
+    if (e < -70) return exp(r);
+    const double ex = exp(e);
+    e = r / (1.0 + ex);
+    return exp(e);
+}
+
+// :145-164
+__device__ inline double ccor2(double alt, double r, double h1, double zh, double h2) {
+    const double e1 = (alt - zh) / h1, e2 = (alt - zh) / h2;
+    if (e1 > 70 || e2 > 70) return exp(0.0);
+    if (e1 < -70 && e2 < -70) return exp(r);
+    const double ex1 = exp(e1), ex2 = exp(e2);
+    return exp(r / (1.0 + 0.5 * (ex1 + ex2)));
+}
+
+// :172-178
+__device__ inline double scalh(const State& st, double alt, double xm, double temp) {
+    const double rgas = 831.4;
+    double g = st.gsurf / pow((1.0 + alt / st.re), 2.0);
+    g = rgas * temp / (g * xm);
+    return g;
+}
+
+// :186-215
+__device__ inline double dnet(double dd, double dm, double zhm, double xmm, double xm) {
+    double a = zhm / (xmm - xm);
+    if (!((dm > 0) && (dd > 0))) {
+        if (dd == 0 && dm == 0) dd = 1;
+        if (dm == 0) return dd;
+        if (dd == 0) return dm;
+    }
+    const double ylog = a * log(dm / dd);
+    if (ylog < -10) return dd;
+    if (ylog > 10) return dm;
+    a = dd * pow((1.0 + exp(ylog)), (1.0 / a));
+    return a;
+}
+
+// :223-252
+__device__ inline void splini(const double* xa, const double* ya, const double* y2a, int n, double x, double* y) {
+    double yi = 0;
+    int klo = 0, khi = 1;
+    while ((x > xa[klo]) && (khi < n)) {
+        double xx = x;
+        if (khi < (n - 1)) xx = x < xa[khi] ? x : xa[khi];
+        const double h = xa[khi] - xa[klo];
+        const double a = (xa[khi] - xx) / h;
+        const double b = (xx - xa[klo]) / h;
+        const double a2 = a * a, b2 = b * b;
+        yi += ((1.0 - a2) * ya[klo] / 2.0 + b2 * ya[khi] / 2.0 +
+               ((-(1.0 + a2 * a2) / 4.0 + a2 / 2.0) * y2a[klo] + (b2 * b2 / 4.0 - b2 / 2.0) * y2a[khi]) * h * h / 6.0) * h;
+        klo++;
+        khi++;
+    }
+    *y = yi;
+}
+
+// :260-290
+__device__ inline void splint(const double* xa, const double* ya, const double* y2a, int n, double x, double* y) {
+    int klo = 0, khi = n - 1;
+    while ((khi - klo) > 1) {
+        const int k = (khi + klo) / 2;
+        if (xa[k] > x) khi = k;
+        else klo = k;
+    }
+    const double h = xa[khi] - xa[klo];
+    const double a = (xa[khi] - x) / h;
+    const double b = (x - xa[klo]) / h;
+    *y = a * ya[klo] + b * ya[khi] + ((a * a * a - a) * y2a[klo] + (b * b * b - b) * y2a[khi]) * h * h / 6.0;
+}
+
+// :298-338
+__device__ inline void spline(const double* x, const double* y, int n, double yp1, double ypn, double* y2) {
+    double u[5];
+    if (yp1 > 0.99E30) {
+        y2[0] = 0;
+        u[0] = 0;
+    } else {
+        y2[0] = -0.5;
+        u[0] = (3.0 / (x[1] - x[0])) * ((y[1] - y[0]) / (x[1] - x[0]) - yp1);
+    }
+    for (int i = 1; i < (n - 1); ++i) {
+        const double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+        const double p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        u[i] = (6.0 * ((y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1])) / (x[i + 1] - x[i - 1]) -
+                sig * u[i - 1]) / p;
+    }
+    double qn, un;
+    if (ypn > 0.99E30) {
+        qn = 0;
+        un = 0;
+    } else {
+        qn = 0.5;
+        un = (3.0 / (x[n - 1] - x[n - 2])) * (ypn - (y[n - 1] - y[n - 2]) / (x[n - 1] - x[n - 2]));
+    }
+    y2[n - 1] = (un - qn * u[n - 2]) / (qn * y2[n - 2] + 1.0);
+    for (int k = n - 2; k >= 0; --k) y2[k] = y2[k] * y2[k + 1] + u[k];
+}
+
+// :346-348
+__device__ inline double zeta(const State& st, double zz, double zl) { return ((zz - zl) * (st.re + zl) / (st.re + zz)); }
+
+// :350-468
+__device__ inline double densm(State& st, double alt, double d0, double xm, double* tz, int mn3, const double* zn3, const double* tn3,
+                    const double* tgn3, int mn2, const double* zn2, const double* tn2, const double* tgn2) {
+    double xs[5], ys[5], y2out[5];
+    const double rgas = 831.4;
+    double densm_tmp = d0;
+    if (alt > zn2[0]) return xm == 0.0 ? *tz : d0;
+    double z = alt > zn2[mn2 - 1] ? alt : zn2[mn2 - 1];
+    int mn = mn2;
+    double z1 = zn2[0], z2 = zn2[mn - 1], t1 = tn2[0], t2 = tn2[mn - 1];
+    double zg = zeta(st, z, z1), zgdif = zeta(st, z2, z1);
+    for (int k = 0; k < mn; ++k) {
+        xs[k] = zeta(st, zn2[k], z1) / zgdif;
+        ys[k] = 1.0 / tn2[k];
+    }
+    double yd1 = -tgn2[0] / (t1 * t1) * zgdif;
+    double yd2 = -tgn2[1] / (t2 * t2) * zgdif * (pow(((st.re + z2) / (st.re + z1)), 2.0));
+    spline(xs, ys, mn, yd1, yd2, y2out);
+    double x = zg / zgdif, y;
+    splint(xs, ys, y2out, mn, x, &y);
+    *tz = 1.0 / y;
+    if (xm != 0.0) {
+        const double glb = st.gsurf / (pow((1.0 + z1 / st.re), 2.0));
+        const double gamm = xm * glb * zgdif / rgas;
+        double yi;
+        splini(xs, ys, y2out, mn, x, &yi);
+        double expl = gamm * yi;
+        if (expl > 50.0) expl = 50.0;
+        densm_tmp = densm_tmp * (t1 / *tz) * exp(-expl);
+    }
+    if (alt > zn3[0]) return xm == 0.0 ? *tz : densm_tmp;
+    z = alt;
+    mn = mn3;
+    z1 = zn3[0];
+    z2 = zn3[mn - 1];
+    t1 = tn3[0];
+    t2 = tn3[mn - 1];
+    zg = zeta(st, z, z1);
+    zgdif = zeta(st, z2, z1);
+    for (int k = 0; k < mn; ++k) {
+        xs[k] = zeta(st, zn3[k], z1) / zgdif;
+        ys[k] = 1.0 / tn3[k];
+    }
+    yd1 = -tgn3[0] / (t1 * t1) * zgdif;
+    yd2 = -tgn3[1] / (t2 * t2) * zgdif * (pow(((st.re + z2) / (st.re + z1)), 2.0));
+    spline(xs, ys, mn, yd1, yd2, y2out);
+    x = zg / zgdif;
+    splint(xs, ys, y2out, mn, x, &y);
+    *tz = 1.0 / y;
+    if (xm != 0.0) {
+        const double glb = st.gsurf / (pow((1.0 + z1 / st.re), 2.0));
+        const double gamm = xm * glb * zgdif / rgas;
+        double yi;
+        splini(xs, ys, y2out, mn, x, &yi);
+        double expl = gamm * yi;
+        if (expl > 50.0) expl = 50.0;
+        densm_tmp = densm_tmp * (t1 / *tz) * exp(-expl);
+    }
+    return xm == 0.0 ? *tz : densm_tmp;
+}
+
+// :476-576
+__device__ inline double densu(State& st, double alt, double dlb, double tinf, double tlb, double xm, double alpha, double* tz,
+                    double zlb, double s2, int mn1, const double* zn1, double* tn1, double* tgn1) {
+    double x = 0, y;
+    const double rgas = 831.4;
+    double densu_temp = 1.0;
+    double z1 = 0, t1 = 0, zgdif = 0;
+    int mn = 0;
+    double xs[5], ys[5], y2out[5];
+    const double za = zn1[0];
+    double z = alt > za ? alt : za;
+    const double zg2 = zeta(st, z, zlb);
+    const double tt = tinf - (tinf - tlb) * exp(-s2 * zg2);
+    const double ta = tt;
+    *tz = tt;
+    densu_temp = *tz;
+    if (alt < za) {
+        const double dta = (tinf - ta) * s2 * pow(((st.re + zlb) / (st.re + za)), 2.0);
+        tgn1[0] = dta;
+        tn1[0] = ta;
+        z = alt > zn1[mn1 - 1] ? alt : zn1[mn1 - 1];
+        mn = mn1;
+        z1 = zn1[0];
+        const double z2 = zn1[mn - 1];
+        t1 = tn1[0];
+        const double t2 = tn1[mn - 1];
+        const double zg = zeta(st, z, z1);
+        zgdif = zeta(st, z2, z1);
+        for (int k = 0; k < mn; ++k) {
+            xs[k] = zeta(st, zn1[k], z1) / zgdif;
+            ys[k] = 1.0 / tn1[k];
+        }
+        const double yd1 = -tgn1[0] / (t1 * t1) * zgdif;
+        const double yd2 = -tgn1[1] / (t2 * t2) * zgdif * pow(((st.re + z2) / (st.re + z1)), 2.0);
+        spline(xs, ys, mn, yd1, yd2, y2out);
+        x = zg / zgdif;
+        splint(xs, ys, y2out, mn, x, &y);
+        *tz = 1.0 / y;
+        densu_temp = *tz;
+    }
+    if (xm == 0) return densu_temp;
+    double glb = st.gsurf / pow((1.0 + zlb / st.re), 2.0);
+    const double gamma = xm * glb / (s2 * rgas * tinf);
+    double expl = exp(-s2 * gamma * zg2);
+    if (expl > 50.0) expl = 50.0;
+    if (tt <= 0) expl = 50.0;
+    const double densa = dlb * pow((tlb / tt), ((1.0 + alpha + gamma))) * expl;
+    densu_temp = densa;
+    if (alt >= za) return densu_temp;
+    glb = st.gsurf / pow((1.0 + z1 / st.re), 2.0);
+    const double gamm = xm * glb * zgdif / rgas;
+    double yi;
+    splini(xs, ys, y2out, mn, x, &yi);
+    expl = gamm * yi;
+    if (expl > 50.0) expl = 50.0;
+    if (*tz <= 0) expl = 50.0;
+    densu_temp = densu_temp * pow((t1 / *tz), (1.0 + alpha)) * exp(-expl);
+    return densu_temp;
+}
+
+// :586-593 (p[24] lower bound applied to a local, as in the reference)
+__device__ inline double g0(double a, const double* p) {
+    const double p24 = (p[24] < 1.0E-4) ? 1.0E-4 : sqrt(p[24] * p[24]);
+    return (a - 4.0 + (p[25] - 1.0) * (a - 4.0 + (exp(-p24 * (a - 4.0)) - 1.0) / p24));
+}
+
+// :596-598
+__device__ inline double sumex(double ex) { return (1.0 + (1.0 - pow(ex, 19.0)) / (1.0 - ex) * pow(ex, 0.5)); }
+
+// :601-634
+__device__ inline double sg0(State& st, double ex, const double* p, const double* ap) {
+    double g[6];
+    if (st.g0Valid && p[24] == st.g0P24 && p[25] == st.g0P25) {
+        for (int k = 0; k < 6; ++k) g[k] = st.g0Cache[k];
+    } else {
+        for (int k = 0; k < 6; ++k) g[k] = g0(ap[k + 1], p);
+        if (!st.g0Valid) {
+            st.g0P24 = p[24];
+            st.g0P25 = p[25];
+            for (int k = 0; k < 6; ++k) st.g0Cache[k] = g[k];
+            st.g0Valid = true;
+        }
+    }
+    return (g[0] + (g[1] * ex + g[2] * ex * ex + g[3] * pow(ex, 3.0) +
+                    (g[4] * pow(ex, 4.0) + g[5] * pow(ex, 12.0)) * (1.0 - pow(ex, 8.0)) / (1.0 - ex))) /
+           sumex(ex);
+}
+
+// :639-686
+__device__ inline void prepareGlobeBasis(State& st, const Input& in, const Flags& fl) {
+    const double dgtr = 1.74533E-2, hr = 0.2618;
+    const double tloc = in.lst;
+    const double c = sin(in.g_lat * dgtr), s = cos(in.g_lat * dgtr);
+    const double c2 = c * c, c4 = c2 * c2, s2 = s * s;
+    double(&plg)[4][9] = st.plg;
+    plg[0][1] = c;
+    plg[0][2] = 0.5 * (3.0 * c2 - 1.0);
+    plg[0][3] = 0.5 * (5.0 * c * c2 - 3.0 * c);
+    plg[0][4] = (35.0 * c4 - 30.0 * c2 + 3.0) / 8.0;
+    plg[0][5] = (63.0 * c2 * c2 * c - 70.0 * c2 * c + 15.0 * c) / 8.0;
+    plg[0][6] = (11.0 * c * plg[0][5] - 5.0 * plg[0][4]) / 6.0;
+    plg[1][1] = s;
+    plg[1][2] = 3.0 * c * s;
+    plg[1][3] = 1.5 * (5.0 * c2 - 1.0) * s;
+    plg[1][4] = 2.5 * (7.0 * c2 * c - 3.0 * c) * s;
+    plg[1][5] = 1.875 * (21.0 * c4 - 14.0 * c2 + 1.0) * s;
+    plg[1][6] = (11.0 * c * plg[1][5] - 6.0 * plg[1][4]) / 5.0;
+    plg[2][2] = 3.0 * s2;
+    plg[2][3] = 15.0 * s2 * c;
+    plg[2][4] = 7.5 * (7.0 * c2 - 1.0) * s2;
+    plg[2][5] = 3.0 * c * plg[2][4] - 2.0 * plg[2][3];
+    plg[2][6] = (11.0 * c * plg[2][5] - 7.0 * plg[2][4]) / 4.0;
+    plg[2][7] = (13.0 * c * plg[2][6] - 8.0 * plg[2][5]) / 5.0;
+    plg[3][3] = 15.0 * s2 * s;
+    plg[3][4] = 105.0 * s2 * s * c;
+    plg[3][5] = (9.0 * c * plg[3][4] - 7. * plg[3][3]) / 2.0;
+    plg[3][6] = (11.0 * c * plg[3][5] - 8. * plg[3][4]) / 3.0;
+    if (!(((fl.sw[7] == 0) && (fl.sw[8] == 0)) && (fl.sw[14] == 0))) {
+        st.stloc = sin(hr * tloc);
+        st.ctloc = cos(hr * tloc);
+        st.s2tloc = sin(2.0 * hr * tloc);
+        st.c2tloc = cos(2.0 * hr * tloc);
+        st.s3tloc = sin(3.0 * hr * tloc);
+        st.c3tloc = cos(3.0 * hr * tloc);
+    }
+}
+
+// :688-850
+__device__ inline double globe7(State& st, const double* p, const Input& in, const Flags& fl) {
+    double t[15];
+    const double sr = 7.2722E-5, dgtr = 1.74533E-2, dr = 1.72142E-2, hr = 0.2618;
+    const double tloc = in.lst;
+    const double(&plg)[4][9] = st.plg;
+    for (int j = 0; j < 14; ++j) t[j] = 0;
+    const double cd32 = cos(dr * (in.doy - p[31]));
+    const double cd18 = cos(2.0 * dr * (in.doy - p[17]));
+    const double cd14 = cos(dr * (in.doy - p[13]));
+    const double cd39 = cos(2.0 * dr * (in.doy - p[38]));
+    const double df = in.f107 - in.f107A;
+    st.dfa = in.f107A - 150.0;
+    const double dfa = st.dfa;
+    t[0] = p[19] * df * (1.0 + p[59] * dfa) + p[20] * df * df + p[21] * dfa + p[29] * pow(dfa, 2.0);
+    const double f1 = 1.0 + (p[47] * dfa + p[19] * df + p[20] * df * df) * fl.swc[1];
+    const double f2 = 1.0 + (p[49] * dfa + p[19] * df + p[20] * df * df) * fl.swc[1];
+    t[1] = (p[1] * plg[0][2] + p[2] * plg[0][4] + p[22] * plg[0][6]) + (p[14] * plg[0][2]) * dfa * fl.swc[1] + p[26] * plg[0][1];
+    t[2] = p[18] * cd32;
+    t[3] = (p[15] + p[16] * plg[0][2]) * cd18;
+    t[4] = f1 * (p[9] * plg[0][1] + p[10] * plg[0][3]) * cd14;
+    t[5] = p[37] * plg[0][1] * cd39;
+    if (fl.sw[7]) {
+        const double t71 = (p[11] * plg[1][2]) * cd14 * fl.swc[5];
+        const double t72 = (p[12] * plg[1][2]) * cd14 * fl.swc[5];
+        t[6] = f2 * ((p[3] * plg[1][1] + p[4] * plg[1][3] + p[27] * plg[1][5] + t71) * st.ctloc +
+                     (p[6] * plg[1][1] + p[7] * plg[1][3] + p[28] * plg[1][5] + t72) * st.stloc);
+    }
+    if (fl.sw[8]) {
+        const double t81 = (p[23] * plg[2][3] + p[35] * plg[2][5]) * cd14 * fl.swc[5];
+        const double t82 = (p[33] * plg[2][3] + p[36] * plg[2][5]) * cd14 * fl.swc[5];
+        t[7] = f2 * ((p[5] * plg[2][2] + p[41] * plg[2][4] + t81) * st.c2tloc + (p[8] * plg[2][2] + p[42] * plg[2][4] + t82) * st.s2tloc);
+    }
+    if (fl.sw[14]) {
+        t[13] = f2 * ((p[39] * plg[3][3] + (p[93] * plg[3][4] + p[46] * plg[3][6]) * cd14 * fl.swc[5]) * st.s3tloc +
+                      (p[40] * plg[3][3] + (p[94] * plg[3][4] + p[48] * plg[3][6]) * cd14 * fl.swc[5]) * st.c3tloc);
+    }
+    if (fl.sw[9] == -1) {
+        if (p[51] != 0) {
+            double exp1 = exp(-10800.0 * sqrt(p[51] * p[51]) / (1.0 + p[138] * (45.0 - sqrt(in.g_lat * in.g_lat))));
+            if (exp1 > 0.99999) exp1 = 0.99999;
+            st.apt[0] = sg0(st, exp1, p, in.apa);
+            if (fl.sw[9]) {
+                t[8] = st.apt[0] * (p[50] + p[96] * plg[0][2] + p[54] * plg[0][4] +
+                                    (p[125] * plg[0][1] + p[126] * plg[0][3] + p[127] * plg[0][5]) * cd14 * fl.swc[5] +
+                                    (p[128] * plg[1][1] + p[129] * plg[1][3] + p[130] * plg[1][5]) * fl.swc[7] *
+                                        cos(hr * (tloc - p[131])));
+            }
+        }
+    } else {
+        const double apd = in.ap - 4.0;
+        double p44 = p[43];
+        const double p45 = p[44];
+        if (p44 < 0) p44 = 1.0E-5;
+        st.apdf = apd + (p45 - 1.0) * (apd + (exp(-p44 * apd) - 1.0) / p44);
+        if (fl.sw[9]) {
+            t[8] = st.apdf * (p[32] + p[45] * plg[0][2] + p[34] * plg[0][4] +
+                              (p[100] * plg[0][1] + p[101] * plg[0][3] + p[102] * plg[0][5]) * cd14 * fl.swc[5] +
+                              (p[121] * plg[1][1] + p[122] * plg[1][3] + p[123] * plg[1][5]) * fl.swc[7] *
+                                  cos(hr * (tloc - p[124])));
+        }
+    }
+    if ((fl.sw[10]) && (in.g_long > -1000.0)) {
+        if (fl.sw[11]) {
+            t[10] = (1.0 + p[80] * dfa * fl.swc[1]) *
+                    ((p[64] * plg[1][2] + p[65] * plg[1][4] + p[66] * plg[1][6] + p[103] * plg[1][1] + p[104] * plg[1][3] +
+                      p[105] * plg[1][5] + fl.swc[5] * (p[109] * plg[1][1] + p[110] * plg[1][3] + p[111] * plg[1][5]) * cd14) *
+                         cos(dgtr * in.g_long) +
+                     (p[90] * plg[1][2] + p[91] * plg[1][4] + p[92] * plg[1][6] + p[106] * plg[1][1] + p[107] * plg[1][3] +
+                      p[108] * plg[1][5] + fl.swc[5] * (p[112] * plg[1][1] + p[113] * plg[1][3] + p[114] * plg[1][5]) * cd14) *
+                         sin(dgtr * in.g_long));
+        }
+        if (fl.sw[12]) {
+            t[11] = (1.0 + p[95] * plg[0][1]) * (1.0 + p[81] * dfa * fl.swc[1]) * (1.0 + p[119] * plg[0][1] * fl.swc[5] * cd14) *
+                    ((p[68] * plg[0][1] + p[69] * plg[0][3] + p[70] * plg[0][5]) * cos(sr * (in.sec - p[71])));
+            t[11] += fl.swc[11] * (p[76] * plg[2][3] + p[77] * plg[2][5] + p[78] * plg[2][7]) *
+                     cos(sr * (in.sec - p[79]) + 2.0 * dgtr * in.g_long) * (1.0 + p[137] * dfa * fl.swc[1]);
+        }
+        if (fl.sw[13]) {
+            if (fl.sw[9] == -1) {
+                if (p[51]) {
+                    t[12] = st.apt[0] * fl.swc[11] * (1. + p[132] * plg[0][1]) *
+                                ((p[52] * plg[1][2] + p[98] * plg[1][4] + p[67] * plg[1][6]) * cos(dgtr * (in.g_long - p[97]))) +
+                            st.apt[0] * fl.swc[11] * fl.swc[5] * (p[133] * plg[1][1] + p[134] * plg[1][3] + p[135] * plg[1][5]) *
+                                cd14 * cos(dgtr * (in.g_long - p[136])) +
+                            st.apt[0] * fl.swc[12] * (p[55] * plg[0][1] + p[56] * plg[0][3] + p[57] * plg[0][5]) *
+                                cos(sr * (in.sec - p[58]));
+                }
+            } else {
+                t[12] = st.apdf * fl.swc[11] * (1.0 + p[120] * plg[0][1]) *
+                            ((p[60] * plg[1][2] + p[61] * plg[1][4] + p[62] * plg[1][6]) * cos(dgtr * (in.g_long - p[63]))) +
+                        st.apdf * fl.swc[11] * fl.swc[5] * (p[115] * plg[1][1] + p[116] * plg[1][3] + p[117] * plg[1][5]) * cd14 *
+                            cos(dgtr * (in.g_long - p[118])) +
+                        st.apdf * fl.swc[12] * (p[83] * plg[0][1] + p[84] * plg[0][3] + p[85] * plg[0][5]) *
+                            cos(sr * (in.sec - p[75]));
+            }
+        }
+    }
+    double tinf = p[30];
+    for (int i = 0; i < 14; ++i) tinf = tinf + fabs(double(fl.sw[i + 1])) * t[i];
+    return tinf;
+}
+
+// :858-950
+__device__ inline double glob7s(State& st, const double* p, const Input& in, const Flags& fl) {
+    const double pset = 2.0;
+    double t[14];
+    const double dr = 1.72142E-2, dgtr = 1.74533E-2;
+    const double(&plg)[4][9] = st.plg;
+    const double parameterSet = (p[99] == 0.0) ? pset : p[99];
+    if (parameterSet != pset) return -1;
+    for (int j = 0; j < 14; ++j) t[j] = 0.0;
+    const double cd32 = cos(dr * (in.doy - p[31]));
+    const double cd18 = cos(2.0 * dr * (in.doy - p[17]));
+    const double cd14 = cos(dr * (in.doy - p[13]));
+    const double cd39 = cos(2.0 * dr * (in.doy - p[38]));
+    t[0] = p[21] * st.dfa;
+    t[1] = p[1] * plg[0][2] + p[2] * plg[0][4] + p[22] * plg[0][6] + p[26] * plg[0][1] + p[14] * plg[0][3] + p[59] * plg[0][5];
+    t[2] = (p[18] + p[47] * plg[0][2] + p[29] * plg[0][4]) * cd32;
+    t[3] = (p[15] + p[16] * plg[0][2] + p[30] * plg[0][4]) * cd18;
+    t[4] = (p[9] * plg[0][1] + p[10] * plg[0][3] + p[20] * plg[0][5]) * cd14;
+    t[5] = (p[37] * plg[0][1]) * cd39;
+    if (fl.sw[7]) {
+        const double t71 = p[11] * plg[1][2] * cd14 * fl.swc[5];
+        const double t72 = p[12] * plg[1][2] * cd14 * fl.swc[5];
+        t[6] = ((p[3] * plg[1][1] + p[4] * plg[1][3] + t71) * st.ctloc + (p[6] * plg[1][1] + p[7] * plg[1][3] + t72) * st.stloc);
+    }
+    if (fl.sw[8]) {
+        const double t81 = (p[23] * plg[2][3] + p[35] * plg[2][5]) * cd14 * fl.swc[5];
+        const double t82 = (p[33] * plg[2][3] + p[36] * plg[2][5]) * cd14 * fl.swc[5];
+        t[7] = ((p[5] * plg[2][2] + p[41] * plg[2][4] + t81) * st.c2tloc + (p[8] * plg[2][2] + p[42] * plg[2][4] + t82) * st.s2tloc);
+    }
+    if (fl.sw[14]) t[13] = p[39] * plg[3][3] * st.s3tloc + p[40] * plg[3][3] * st.c3tloc;
+    if (fl.sw[9]) {
+        if (fl.sw[9] == 1) t[8] = st.apdf * (p[32] + p[45] * plg[0][2] * fl.swc[2]);
+        if (fl.sw[9] == -1) t[8] = (p[50] * st.apt[0] + p[96] * plg[0][2] * st.apt[0] * fl.swc[2]);
+    }
+    if (!((fl.sw[10] == 0) || (fl.sw[11] == 0) || (in.g_long <= -1000.0))) {
+        t[10] = (1.0 + plg[0][1] * (p[80] * fl.swc[5] * cos(dr * (in.doy - p[81])) + p[85] * fl.swc[6] * cos(2.0 * dr * (in.doy - p[86]))) +
+                 p[83] * fl.swc[3] * cos(dr * (in.doy - p[84])) + p[87] * fl.swc[4] * cos(2.0 * dr * (in.doy - p[88]))) *
+                ((p[64] * plg[1][2] + p[65] * plg[1][4] + p[66] * plg[1][6] + p[74] * plg[1][1] + p[75] * plg[1][3] + p[76] * plg[1][5]) *
+                     cos(dgtr * in.g_long) +
+                 (p[90] * plg[1][2] + p[91] * plg[1][4] + p[92] * plg[1][6] + p[77] * plg[1][1] + p[78] * plg[1][3] + p[79] * plg[1][5]) *
+                     sin(dgtr * in.g_long));
+    }
+    double tt = 0;
+    for (int i = 0; i < 14; ++i) tt += fabs(double(fl.sw[i + 1])) * t[i];
+    return tt;
+}
+
+// :1203-1498
+__device__ inline void gts7(State& st, const NrlTables& T, Input& in, const Flags& fl, Output& out) {
+    double zn1[5] = {120.0, 110.0, 100.0, 90.0, 72.5};
+    const int mn1 = 5;
+    const double dgtr = 1.74533E-2, dr = 1.72142E-2;
+    const double alpha[9] = {-0.38, 0.0, 0.0, 0.0, 0.17, 0.0, -0.38, 0.0, 0.0};
+    const double altl[8] = {200.0, 300.0, 160.0, 250.0, 240.0, 450.0, 320.0, 450.0};
+    const double za = (T.base+OFF_PDL+25*(1))[15];
+    zn1[0] = za;
+    for (int j = 0; j < 9; ++j) out.d[j] = 0;
+    st.g0Valid = false;
+    prepareGlobeBasis(st, in, fl);
+    double tinf;
+    if (in.alt > zn1[0]) tinf = (T.base+OFF_PTM)[0] * (T.base+OFF_PT)[0] * (1.0 + fl.sw[16] * globe7(st, (T.base+OFF_PT), in, fl));
+    else tinf = (T.base+OFF_PTM)[0] * (T.base+OFF_PT)[0];
+    out.t[0] = tinf;
+    double g0v;
+    if (in.alt > zn1[4]) g0v = (T.base+OFF_PTM)[3] * (T.base+OFF_PS)[0] * (1.0 + fl.sw[19] * globe7(st, (T.base+OFF_PS), in, fl));
+    else g0v = (T.base+OFF_PTM)[3] * (T.base+OFF_PS)[0];
+    const double tlb = (T.base+OFF_PTM)[1] * (1.0 + fl.sw[17] * globe7(st, (T.base+OFF_PD+150*(3)), in, fl)) * (T.base+OFF_PD+150*(3))[0];
+    const double s = g0v / (tinf - tlb);
+    if (in.alt < 300.0) {
+        st.meso_tn1[1] = (T.base+OFF_PTM)[6] * (T.base+OFF_PTL+100*(0))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(0)), in, fl));
+        st.meso_tn1[2] = (T.base+OFF_PTM)[2] * (T.base+OFF_PTL+100*(1))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(1)), in, fl));
+        st.meso_tn1[3] = (T.base+OFF_PTM)[7] * (T.base+OFF_PTL+100*(2))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(2)), in, fl));
+        st.meso_tn1[4] = (T.base+OFF_PTM)[4] * (T.base+OFF_PTL+100*(3))[0] / (1.0 - fl.sw[18] * fl.sw[20] * glob7s(st, (T.base+OFF_PTL+100*(3)), in, fl));
+        st.meso_tgn1[1] = (T.base+OFF_PTM)[8] * (T.base+OFF_PMA+100*(8))[0] * (1.0 + fl.sw[18] * fl.sw[20] * glob7s(st, (T.base+OFF_PMA+100*(8)), in, fl)) * st.meso_tn1[4] *
+                          st.meso_tn1[4] / (pow(((T.base+OFF_PTM)[4] * (T.base+OFF_PTL+100*(3))[0]), 2.0));
+    } else {
+        st.meso_tn1[1] = (T.base+OFF_PTM)[6] * (T.base+OFF_PTL+100*(0))[0];
+        st.meso_tn1[2] = (T.base+OFF_PTM)[2] * (T.base+OFF_PTL+100*(1))[0];
+        st.meso_tn1[3] = (T.base+OFF_PTM)[7] * (T.base+OFF_PTL+100*(2))[0];
+        st.meso_tn1[4] = (T.base+OFF_PTM)[4] * (T.base+OFF_PTL+100*(3))[0];
+        st.meso_tgn1[1] = (T.base+OFF_PTM)[8] * (T.base+OFF_PMA+100*(8))[0] * st.meso_tn1[4] * st.meso_tn1[4] / (pow(((T.base+OFF_PTM)[4] * (T.base+OFF_PTL+100*(3))[0]), 2.0));
+    }
+    const double g28 = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(2)), in, fl);
+    const double zhf = (T.base+OFF_PDL+25*(1))[24] * (1.0 + fl.sw[5] * (T.base+OFF_PDL+25*(0))[24] * sin(dgtr * in.g_lat) * cos(dr * (in.doy - (T.base+OFF_PT)[13])));
+    out.t[0] = tinf;
+    const double xmm = (T.base+OFF_PDM+10*(2))[4];
+    const double z = in.alt;
+    double tz;
+    double* tn1 = st.meso_tn1;
+    double* tgn1 = st.meso_tgn1;
+
+    // N2
+    const double db28 = (T.base+OFF_PDM+10*(2))[0] * exp(g28) * (T.base+OFF_PD+150*(2))[0];
+    out.d[2] = densu(st, z, db28, tinf, tlb, 28.0, alpha[2], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    const double zh28 = (T.base+OFF_PDM+10*(2))[2] * zhf;
+    const double zhm28 = (T.base+OFF_PDM+10*(2))[3] * (T.base+OFF_PDL+25*(1))[5];
+    const double xmd = 28.0 - xmm;
+    double b28 = 0.0;
+    if (!(z > altl[7] && z >= zn1[0]))
+        b28 = densu(st, zh28, db28, tinf, tlb, xmd, (alpha[2] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if ((fl.sw[15]) && (z <= altl[2])) {
+        st.dm28 = densu(st, z, b28, tinf, tlb, xmm, alpha[2], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        out.d[2] = dnet(out.d[2], st.dm28, zhm28, xmm, 28.0);
+    }
+
+    // He
+    const double g4 = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(0)), in, fl);
+    const double db04 = (T.base+OFF_PDM+10*(0))[0] * exp(g4) * (T.base+OFF_PD+150*(0))[0];
+    out.d[0] = densu(st, z, db04, tinf, tlb, 4., alpha[0], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if ((fl.sw[15]) && (z < altl[0])) {
+        const double zh04 = (T.base+OFF_PDM+10*(0))[2];
+        const double b04 = densu(st, zh04, db04, tinf, tlb, 4. - xmm, alpha[0] - 1., &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        st.dm04 = densu(st, z, b04, tinf, tlb, xmm, 0., &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        out.d[0] = dnet(out.d[0], st.dm04, zhm28, xmm, 4.);
+        const double rl = log(b28 * (T.base+OFF_PDM+10*(0))[1] / b04);
+        const double zc04 = (T.base+OFF_PDM+10*(0))[4] * (T.base+OFF_PDL+25*(1))[0];
+        const double hc04 = (T.base+OFF_PDM+10*(0))[5] * (T.base+OFF_PDL+25*(1))[1];
+        out.d[0] = out.d[0] * ccor(z, rl, hc04, zc04);
+    }
+
+    // O
+    const double g16 = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(1)), in, fl);
+    const double db16 = (T.base+OFF_PDM+10*(1))[0] * exp(g16) * (T.base+OFF_PD+150*(1))[0];
+    out.d[1] = densu(st, z, db16, tinf, tlb, 16., alpha[1], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if ((fl.sw[15]) && (z <= altl[1])) {
+        const double zh16 = (T.base+OFF_PDM+10*(1))[2];
+        const double b16 = densu(st, zh16, db16, tinf, tlb, 16.0 - xmm, (alpha[1] - 1.0), &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        st.dm16 = densu(st, z, b16, tinf, tlb, xmm, 0., &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        out.d[1] = dnet(out.d[1], st.dm16, zhm28, xmm, 16.);
+        const double rl = (T.base+OFF_PDM+10*(1))[1] * (T.base+OFF_PDL+25*(1))[16] * (1.0 + fl.sw[1] * (T.base+OFF_PDL+25*(0))[23] * (in.f107A - 150.0));
+        const double hc16 = (T.base+OFF_PDM+10*(1))[5] * (T.base+OFF_PDL+25*(1))[3];
+        const double zc16 = (T.base+OFF_PDM+10*(1))[4] * (T.base+OFF_PDL+25*(1))[2];
+        const double hc216 = (T.base+OFF_PDM+10*(1))[5] * (T.base+OFF_PDL+25*(1))[4];
+        out.d[1] = out.d[1] * ccor2(z, rl, hc16, zc16, hc216);
+        const double hcc16 = (T.base+OFF_PDM+10*(1))[7] * (T.base+OFF_PDL+25*(1))[13];
+        const double zcc16 = (T.base+OFF_PDM+10*(1))[6] * (T.base+OFF_PDL+25*(1))[12];
+        const double rc16 = (T.base+OFF_PDM+10*(1))[3] * (T.base+OFF_PDL+25*(1))[14];
+        out.d[1] = out.d[1] * ccor(z, rc16, hcc16, zcc16);
+    }
+
+    // O2
+    const double g32 = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(4)), in, fl);
+    const double db32 = (T.base+OFF_PDM+10*(3))[0] * exp(g32) * (T.base+OFF_PD+150*(4))[0];
+    out.d[3] = densu(st, z, db32, tinf, tlb, 32., alpha[3], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if (fl.sw[15]) {
+        if (z <= altl[3]) {
+            const double zh32 = (T.base+OFF_PDM+10*(3))[2];
+            const double b32 = densu(st, zh32, db32, tinf, tlb, 32. - xmm, alpha[3] - 1., &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+            st.dm32 = densu(st, z, b32, tinf, tlb, xmm, 0., &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+            out.d[3] = dnet(out.d[3], st.dm32, zhm28, xmm, 32.);
+            const double rl = log(b28 * (T.base+OFF_PDM+10*(3))[1] / b32);
+            const double hc32 = (T.base+OFF_PDM+10*(3))[5] * (T.base+OFF_PDL+25*(1))[7];
+            const double zc32 = (T.base+OFF_PDM+10*(3))[4] * (T.base+OFF_PDL+25*(1))[6];
+            out.d[3] = out.d[3] * ccor(z, rl, hc32, zc32);
+        }
+        const double hcc32 = (T.base+OFF_PDM+10*(3))[7] * (T.base+OFF_PDL+25*(1))[22];
+        const double hcc232 = (T.base+OFF_PDM+10*(3))[7] * (T.base+OFF_PDL+25*(0))[22];
+        const double zcc32 = (T.base+OFF_PDM+10*(3))[6] * (T.base+OFF_PDL+25*(1))[21];
+        const double rc32 = (T.base+OFF_PDM+10*(3))[3] * (T.base+OFF_PDL+25*(1))[23] * (1. + fl.sw[1] * (T.base+OFF_PDL+25*(0))[23] * (in.f107A - 150.));
+        out.d[3] = out.d[3] * ccor2(z, rc32, hcc32, zcc32, hcc232);
+    }
+
+    // Ar
+    const double g40 = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(5)), in, fl);
+    const double db40 = (T.base+OFF_PDM+10*(4))[0] * exp(g40) * (T.base+OFF_PD+150*(5))[0];
+    out.d[4] = densu(st, z, db40, tinf, tlb, 40., alpha[4], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if ((fl.sw[15]) && (z <= altl[4])) {
+        const double zh40 = (T.base+OFF_PDM+10*(4))[2];
+        const double b40 = densu(st, zh40, db40, tinf, tlb, 40. - xmm, alpha[4] - 1., &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        st.dm40 = densu(st, z, b40, tinf, tlb, xmm, 0., &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        out.d[4] = dnet(out.d[4], st.dm40, zhm28, xmm, 40.);
+        const double rl = log(b28 * (T.base+OFF_PDM+10*(4))[1] / b40);
+        const double hc40 = (T.base+OFF_PDM+10*(4))[5] * (T.base+OFF_PDL+25*(1))[9];
+        const double zc40 = (T.base+OFF_PDM+10*(4))[4] * (T.base+OFF_PDL+25*(1))[8];
+        out.d[4] = out.d[4] * ccor(z, rl, hc40, zc40);
+    }
+
+    // H
+    const double g1 = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(6)), in, fl);
+    const double db01 = (T.base+OFF_PDM+10*(5))[0] * exp(g1) * (T.base+OFF_PD+150*(6))[0];
+    out.d[6] = densu(st, z, db01, tinf, tlb, 1., alpha[6], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if ((fl.sw[15]) && (z <= altl[6])) {
+        const double zh01 = (T.base+OFF_PDM+10*(5))[2];
+        const double b01 = densu(st, zh01, db01, tinf, tlb, 1. - xmm, alpha[6] - 1., &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        st.dm01 = densu(st, z, b01, tinf, tlb, xmm, 0., &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        out.d[6] = dnet(out.d[6], st.dm01, zhm28, xmm, 1.);
+        const double rl = log(b28 * (T.base+OFF_PDM+10*(5))[1] * sqrt((T.base+OFF_PDL+25*(1))[17] * (T.base+OFF_PDL+25*(1))[17]) / b01);
+        const double hc01 = (T.base+OFF_PDM+10*(5))[5] * (T.base+OFF_PDL+25*(1))[11];
+        const double zc01 = (T.base+OFF_PDM+10*(5))[4] * (T.base+OFF_PDL+25*(1))[10];
+        out.d[6] = out.d[6] * ccor(z, rl, hc01, zc01);
+        const double hcc01 = (T.base+OFF_PDM+10*(5))[7] * (T.base+OFF_PDL+25*(1))[19];
+        const double zcc01 = (T.base+OFF_PDM+10*(5))[6] * (T.base+OFF_PDL+25*(1))[18];
+        const double rc01 = (T.base+OFF_PDM+10*(5))[3] * (T.base+OFF_PDL+25*(1))[20];
+        out.d[6] = out.d[6] * ccor(z, rc01, hcc01, zcc01);
+    }
+
+    // N
+    const double g14 = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(7)), in, fl);
+    const double db14 = (T.base+OFF_PDM+10*(6))[0] * exp(g14) * (T.base+OFF_PD+150*(7))[0];
+    out.d[7] = densu(st, z, db14, tinf, tlb, 14., alpha[7], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if ((fl.sw[15]) && (z <= altl[7])) {
+        const double zh14 = (T.base+OFF_PDM+10*(6))[2];
+        const double b14 = densu(st, zh14, db14, tinf, tlb, 14. - xmm, alpha[7] - 1., &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        st.dm14 = densu(st, z, b14, tinf, tlb, xmm, 0., &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        out.d[7] = dnet(out.d[7], st.dm14, zhm28, xmm, 14.);
+        const double rl = log(b28 * (T.base+OFF_PDM+10*(6))[1] * sqrt((T.base+OFF_PDL+25*(0))[2] * (T.base+OFF_PDL+25*(0))[2]) / b14);
+        const double hc14 = (T.base+OFF_PDM+10*(6))[5] * (T.base+OFF_PDL+25*(0))[1];
+        const double zc14 = (T.base+OFF_PDM+10*(6))[4] * (T.base+OFF_PDL+25*(0))[0];
+        out.d[7] = out.d[7] * ccor(z, rl, hc14, zc14);
+        const double hcc14 = (T.base+OFF_PDM+10*(6))[7] * (T.base+OFF_PDL+25*(0))[4];
+        const double zcc14 = (T.base+OFF_PDM+10*(6))[6] * (T.base+OFF_PDL+25*(0))[3];
+        const double rc14 = (T.base+OFF_PDM+10*(6))[3] * (T.base+OFF_PDL+25*(0))[5];
+        out.d[7] = out.d[7] * ccor(z, rc14, hcc14, zcc14);
+    }
+
+    // anomalous O
+    const double g16h = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(8)), in, fl);
+    const double db16h = (T.base+OFF_PDM+10*(7))[0] * exp(g16h) * (T.base+OFF_PD+150*(8))[0];
+    const double tho = (T.base+OFF_PDM+10*(7))[9] * (T.base+OFF_PDL+25*(0))[6];
+    const double ddo = densu(st, z, db16h, tho, tho, 16., alpha[8], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    const double zsht = (T.base+OFF_PDM+10*(7))[5];
+    const double zmho = (T.base+OFF_PDM+10*(7))[4];
+    const double zsho = scalh(st, zmho, 16.0, tho);
+    out.d[8] = ddo * exp(-zsht / zsho * (exp(-(z - zmho) / zsht) - 1.));
+
+    out.d[5] = 1.66E-24 * (4.0 * out.d[0] + 16.0 * out.d[1] + 28.0 * out.d[2] + 32.0 * out.d[3] + 40.0 * out.d[4] + out.d[6] + 14.0 * out.d[7]);
+    const double zt = sqrt(in.alt * in.alt);
+    (void)densu(st, zt, 1.0, tinf, tlb, 0.0, 0.0, &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if (fl.sw[0]) {
+        for (int i = 0; i < 9; ++i) out.d[i] = out.d[i] * 1.0E6;
+        out.d[5] = out.d[5] / 1000;
+    }
+}
+
+// :958-1065
+__device__ inline void gtd7(State& st, const NrlTables& T, Input& in, Flags& fl, Output& out) {
+    const int mn3 = 5;
+    const double zn3[5] = {32.5, 20.0, 15.0, 10.0, 0.0};
+    const int mn2 = 4;
+    const double zn2[4] = {72.5, 55.0, 45.0, 32.5};
+    const double zmix = 62.5;
+    Output so{};
+    tselec(fl);
+    double xlat = in.g_lat;
+    if (fl.sw[2] == 0) xlat = 45.0;
+    glatf(xlat, st.gsurf, st.re);
+    const double xmm = (T.base+OFF_PDM+10*(2))[4];
+    double altt = in.alt > zn2[0] ? in.alt : zn2[0];
+    const double tmp = in.alt;
+    in.alt = altt;
+    gts7(st, T, in, fl, so);
+    altt = in.alt;
+    in.alt = tmp;
+    const double dm28m = fl.sw[0] ? st.dm28 * 1.0E6 : st.dm28;
+    out.t[0] = so.t[0];
+    out.t[1] = so.t[1];
+    if (in.alt >= zn2[0]) {
+        for (int i = 0; i < 9; ++i) out.d[i] = so.d[i];
+        return;
+    }
+    st.meso_tgn2[0] = st.meso_tgn1[1];
+    st.meso_tn2[0] = st.meso_tn1[4];
+    st.meso_tn2[1] = (T.base+OFF_PMA+100*(0))[0] * (T.base+OFF_PAVGM)[0] / (1.0 - fl.sw[20] * glob7s(st, (T.base+OFF_PMA+100*(0)), in, fl));
+    st.meso_tn2[2] = (T.base+OFF_PMA+100*(1))[0] * (T.base+OFF_PAVGM)[1] / (1.0 - fl.sw[20] * glob7s(st, (T.base+OFF_PMA+100*(1)), in, fl));
+    st.meso_tn2[3] = (T.base+OFF_PMA+100*(2))[0] * (T.base+OFF_PAVGM)[2] / (1.0 - fl.sw[20] * fl.sw[22] * glob7s(st, (T.base+OFF_PMA+100*(2)), in, fl));
+    st.meso_tgn2[1] = (T.base+OFF_PAVGM)[8] * (T.base+OFF_PMA+100*(9))[0] * (1.0 + fl.sw[20] * fl.sw[22] * glob7s(st, (T.base+OFF_PMA+100*(9)), in, fl)) * st.meso_tn2[3] *
+                      st.meso_tn2[3] / (pow(((T.base+OFF_PMA+100*(2))[0] * (T.base+OFF_PAVGM)[2]), 2.0));
+    st.meso_tn3[0] = st.meso_tn2[3];
+    if (in.alt <= zn3[0]) {
+        st.meso_tgn3[0] = st.meso_tgn2[1];
+        st.meso_tn3[1] = (T.base+OFF_PMA+100*(3))[0] * (T.base+OFF_PAVGM)[3] / (1.0 - fl.sw[22] * glob7s(st, (T.base+OFF_PMA+100*(3)), in, fl));
+        st.meso_tn3[2] = (T.base+OFF_PMA+100*(4))[0] * (T.base+OFF_PAVGM)[4] / (1.0 - fl.sw[22] * glob7s(st, (T.base+OFF_PMA+100*(4)), in, fl));
+        st.meso_tn3[3] = (T.base+OFF_PMA+100*(5))[0] * (T.base+OFF_PAVGM)[5] / (1.0 - fl.sw[22] * glob7s(st, (T.base+OFF_PMA+100*(5)), in, fl));
+        st.meso_tn3[4] = (T.base+OFF_PMA+100*(6))[0] * (T.base+OFF_PAVGM)[6] / (1.0 - fl.sw[22] * glob7s(st, (T.base+OFF_PMA+100*(6)), in, fl));
+        st.meso_tgn3[1] = (T.base+OFF_PMA+100*(7))[0] * (T.base+OFF_PAVGM)[7] * (1.0 + fl.sw[22] * glob7s(st, (T.base+OFF_PMA+100*(7)), in, fl)) * st.meso_tn3[4] *
+                          st.meso_tn3[4] / (pow(((T.base+OFF_PMA+100*(6))[0] * (T.base+OFF_PAVGM)[6]), 2.0));
+    }
+    double dmc = 0;
+    if (in.alt > zmix) dmc = 1.0 - (zn2[0] - in.alt) / (zn2[0] - zmix);
+    const double dz28 = so.d[2];
+    double tz;
+    double dmr = so.d[2] / dm28m - 1.0;
+    out.d[2] = densm(st, in.alt, dm28m, xmm, &tz, mn3, zn3, st.meso_tn3, st.meso_tgn3, mn2, zn2, st.meso_tn2, st.meso_tgn2);
+    out.d[2] = out.d[2] * (1.0 + dmr * dmc);
+    dmr = so.d[0] / (dz28 * (T.base+OFF_PDM+10*(0))[1]) - 1.0;
+    out.d[0] = out.d[2] * (T.base+OFF_PDM+10*(0))[1] * (1.0 + dmr * dmc);
+    out.d[1] = 0;
+    out.d[8] = 0;
+    dmr = so.d[3] / (dz28 * (T.base+OFF_PDM+10*(3))[1]) - 1.0;
+    out.d[3] = out.d[2] * (T.base+OFF_PDM+10*(3))[1] * (1.0 + dmr * dmc);
+    dmr = so.d[4] / (dz28 * (T.base+OFF_PDM+10*(4))[1]) - 1.0;
+    out.d[4] = out.d[2] * (T.base+OFF_PDM+10*(4))[1] * (1.0 + dmr * dmc);
+    out.d[6] = 0;
+    out.d[7] = 0;
+    out.d[5] = 1.66E-24 * (4.0 * out.d[0] + 16.0 * out.d[1] + 28.0 * out.d[2] + 32.0 * out.d[3] + 40.0 * out.d[4] + out.d[6] + 14.0 * out.d[7]);
+    if (fl.sw[0]) out.d[5] = out.d[5] / 1000;
+    st.dd = densm(st, in.alt, 1.0, 0, &tz, mn3, zn3, st.meso_tn3, st.meso_tgn3, mn2, zn2, st.meso_tn2, st.meso_tgn2);
+    out.t[1] = tz;
+}
+
+// :1073-1078
+__device__ inline void gtd7d(State& st, const NrlTables& T, Input& in, Flags& fl, Output& out) {
+    gtd7(st, T, in, fl, out);
+    out.d[5] = 1.66E-24 * (4.0 * out.d[0] + 16.0 * out.d[1] + 28.0 * out.d[2] + 32.0 * out.d[3] + 40.0 * out.d[4] + out.d[6] +
+                           14.0 * out.d[7] + 16.0 * out.d[8]);
+    if (fl.sw[0]) out.d[5] = out.d[5] / 1000;
+}
+
+
+}  // namespace nrlmsise
+}  // namespace phx

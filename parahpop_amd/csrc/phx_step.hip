@@ -1,0 +1,685 @@
+// Batched fixed-step RKF78 propagation kernel for gfx950.
+//
+// Work decomposition: a workgroup owns one tile of 64 objects (one object per
+// lane) and runs W waves (W = 1, 2, 4, 8). Every wave evaluates one block of
+// spherical-harmonic orders for all 64 objects; because all lanes of a wave
+// walk the same (n, m) sequence, the coefficient stream is read with scalar
+// (uniform) loads straight into SGPR operands of FP64 FMAs. The per-wave
+// partial sums meet in LDS and wave 0 reduces them in fixed wave order
+// (deterministic). Wave 0 also owns the RKF78 state: the 13 stage
+// derivatives live in LDS, the state in registers; the other forces are
+// spread over the last waves (drag, then Earth corrections and SRP) so the
+// waves reach the per-stage barrier together. Several integration steps run
+// per launch; epoch-only inputs come from the per-epoch records.
+//
+// Reference semantics (src/common/engine/ unless noted):
+//   stage recursion / final sum   third_party/parm/parm/integrate/rk/host/Step.h:33-124
+//   RKF78 tableau                 third_party/parm/parm/integrate/rk/coefs/tableau.h:19-51
+//   step bookkeeping              paraHPOP/propagators/FixedStep.h:30-37
+//   force terms                   paraHPOP/model/accelerations/PrivateAccelerations.h:29-209
+//   SH evaluation                 paraHPOP/model/accelerations/sphericalharmonics/Evaluation.h:111-159,282-296
+//   point gravity / SRP / drag    PointGravity.h:16-81, SRP.h:62-69, Drag.h:67-73
+//   shadow                        paraHPOP/model/environment/eclipse/Shadow.h:84-142
+//   tide gradient, relativity     paraHPOP/model/accelerations/hpop_compat/HpopCorrections.h:339-376
+//   NRLMSISE-00 inputs            paraHPOP/model/environment/atmosphere/Nrlmsise00.cu:243-327
+//
+// The SH inner loop differs from the reference's per-term form in three
+// exact algebraic rearrangements (rounding-level differences only):
+//   * the longitude phasor cos(m l), sin(m l) is applied once per order to
+//     sums over degree instead of once per term;
+//   * alpha^n is folded into the Legendre recursion (scaled functions);
+//   * for m >= 1 the latitude derivative uses
+//       cos(phi) dP_nm/dphi = -n sin(phi) P_nm + e_nm P_{n-1,m},
+//     e_nm = sqrt((2n+1)(n^2-m^2)/(2n-1)), so the dP recursion and the
+//     per-term division by cos(phi) disappear (one division per wave).
+// Order 0 keeps the dP recursion; the literal pole (cos(phi) == 0) takes the
+// reference form with the analytic m = 1 limit.
+#include <hip/hip_runtime.h>
+
+#include "phx_common.h"
+#include "phx_kernels.h"
+#include "phx_nrlmsise.h"
+
+namespace phx {
+namespace {
+
+__constant__ double kRkA[12][12] = {
+    {2. / 27, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+    {1. / 36, 1. / 12, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+    {1. / 24, 0, 1. / 8, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+    {5. / 12, 0, -25. / 16, 25. / 16, 0, 0, 0, 0, 0, 0, 0, 0},
+    {1. / 20, 0, 0, 1. / 4, 1. / 5, 0, 0, 0, 0, 0, 0, 0},
+    {-25. / 108, 0, 0, 125. / 108, -65. / 27, 125. / 54, 0, 0, 0, 0, 0, 0},
+    {31. / 300, 0, 0, 0, 61. / 225, -2. / 9, 13. / 900, 0, 0, 0, 0, 0},
+    {2.0, 0, 0, -53. / 6, 704. / 45, -107. / 9, 67. / 90, 3.0, 0, 0, 0, 0},
+    {-91. / 108, 0, 0, 23. / 108, -976. / 135, 311. / 54, -19. / 60, 17. / 6, -1. / 12, 0, 0, 0},
+    {2383. / 4100, 0, 0, -341. / 164, 4496. / 1025, -301. / 82, 2133. / 4100, 45. / 82, 45. / 164, 18. / 41, 0, 0},
+    {3. / 205, 0, 0, 0, 0, -6. / 41, -3. / 205, -3. / 41, 3. / 41, 6. / 41, 0, 0},
+    {-1777. / 4100, 0, 0, -341. / 164, 4496. / 1025, -289. / 82, 2193. / 4100, 51. / 82, 33. / 164, 12. / 41, 0, 1.0}};
+__constant__ double kRkB[13] = {0, 0, 0, 0, 0, 34. / 105, 9. / 35, 9. / 35, 9. / 280, 9. / 280, 0, 41. / 840, 41. / 840};
+
+enum : uint8_t { ST_TERMINATED = 1, ST_EOS = 2, ST_PAD = 4 };
+
+// Per-lane geometry of the body-fixed position used by the SH evaluation.
+struct ShGeom {
+    double sinPhi, cosPhi, sinL, cosL, alpha, r;
+    double bx, by, bz;   // body-fixed position
+};
+
+__device__ inline ShGeom shGeometry(const double* __restrict__ rec, double px, double py, double pz, double radius) {
+    ShGeom g;
+    const double* R = rec + REC_R;
+    g.bx = R[0] * px + R[1] * py + R[2] * pz;
+    g.by = R[3] * px + R[4] * py + R[5] * pz;
+    g.bz = R[6] * px + R[7] * py + R[8] * pz;
+    g.r = sqrt(g.bx * g.bx + g.by * g.by + g.bz * g.bz);
+    const double rho = sqrt(g.bx * g.bx + g.by * g.by);
+    g.sinPhi = g.bz / g.r;
+    g.cosPhi = rho / g.r;
+    g.sinL = rho > 1e-14 ? g.by / rho : 0.0;
+    g.cosL = rho > 1e-14 ? g.bx / rho : 1.0;
+    g.alpha = radius / g.r;
+    return g;
+}
+
+// Order block [m0, m1) of the SH sum for one lane; returns the unscaled
+// spherical components (radial, latitude, longitude) of sum_nm alpha^n (...).
+__device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const double* __restrict__ terms,
+                             const int* __restrict__ orderStart, const double* __restrict__ diag, bool normalized,
+                             double& outR, double& outPhi, double& outLam) {
+    const double u = g.alpha * g.sinPhi;   // alpha sin(phi)
+    const double v = g.alpha * g.cosPhi;   // alpha cos(phi)
+    const double w = g.alpha * g.alpha;
+    double ar = 0.0, aphiDirect = 0.0, aphiNum = 0.0, alamNum = 0.0;
+    // Seed P~_{m0,m0} = prod c_k (alpha cos phi)^m0 and cos/sin(m0 lambda).
+    double Pmm = 1.0, cosML = 1.0, sinML = 0.0;
+    for (int k = 1; k <= m0; ++k) {
+        Pmm = diag[k] * v * Pmm;
+        const double s = g.cosL * sinML + g.sinL * cosML;
+        const double c = g.cosL * cosML - g.sinL * sinML;
+        sinML = s;
+        cosML = c;
+    }
+    const bool pole = g.cosPhi == 0.0;
+    for (int m = m0; m < m1; ++m) {
+        const double* t = terms + SHT_STRIDE * orderStart[m];
+        const int count = L - m + 1;
+        if (m == 0 || (pole && m == 1)) {
+            // dP recursion (reference form); seeds P~_00 = 1, dP~_00 = 0 or the m = 1 diagonal.
+            double P0 = Pmm, Pm1 = 0.0;
+            double dP0 = m == 0 ? 0.0 : -diag[1] * g.alpha * g.sinPhi, dPm1 = 0.0;
+            double X1 = 0.0, X2 = 0.0, D1 = 0.0, D2 = 0.0, Lc = 0.0, Ls = 0.0;
+            for (int i = 0; i < count; ++i, t += SHT_STRIDE) {
+                const int n = m + i;
+                X1 = fma(P0, t[SHT_C1], X1);
+                X2 = fma(P0, t[SHT_S1], X2);
+                D1 = fma(dP0, t[SHT_C], D1);
+                D2 = fma(dP0, t[SHT_S], D2);
+                if (m == 1) {
+                    // analytic limit of P_n1 / cos(phi) at the pole (Coefficients.h:123-132)
+                    const double sign = g.sinPhi > 0.0 ? 1.0 : ((n & 1) ? 1.0 : -1.0);
+                    double lim = 0.5 * double(n) * (n + 1.0);
+                    if (normalized) lim *= sqrt(2.0 * (2.0 * n + 1.0) / (double(n) * (n + 1.0)));
+                    double an = 1.0;
+                    for (int k = 0; k < n; ++k) an *= g.alpha;
+                    Lc = fma(sign * lim * an, t[SHT_C], Lc);
+                    Ls = fma(sign * lim * an, t[SHT_S], Ls);
+                }
+                const double a = t[SHT_A], b = t[SHT_B];
+                const double Pn = a * u * P0 - b * w * Pm1;
+                const double dPn = a * (u * dP0 + v * P0) - b * w * dPm1;
+                Pm1 = P0;
+                P0 = Pn;
+                dPm1 = dP0;
+                dP0 = dPn;
+            }
+            ar -= cosML * X1 + sinML * X2;
+            aphiDirect += cosML * D1 + sinML * D2;
+            if (m == 1) alamNum -= double(m) * (sinML * Lc - cosML * Ls);  // already divided by cos(phi)
+        } else if (!pole) {
+            double P0 = Pmm, Pm1 = 0.0;
+            double A1 = 0.0, A2 = 0.0, A5 = 0.0, A6 = 0.0, B1 = 0.0, B2 = 0.0;
+#pragma unroll 4
+            for (int i = 0; i < count; ++i, t += SHT_STRIDE) {
+                A1 = fma(P0, t[SHT_C1], A1);
+                A2 = fma(P0, t[SHT_S1], A2);
+                A5 = fma(P0, t[SHT_C], A5);
+                A6 = fma(P0, t[SHT_S], A6);
+                B1 = fma(Pm1, t[SHT_EC], B1);
+                B2 = fma(Pm1, t[SHT_ES], B2);
+                const double Pn = fma(t[SHT_A] * u, P0, -(t[SHT_B] * w) * Pm1);
+                Pm1 = P0;
+                P0 = Pn;
+            }
+            const double X1 = cosML * A1 + sinML * A2;
+            const double X5 = cosML * A5 + sinML * A6;
+            const double XE = cosML * B1 + sinML * B2;
+            const double X6 = sinML * A5 - cosML * A6;
+            ar -= X1;
+            aphiNum += g.alpha * XE - g.sinPhi * (X1 - X5);
+            alamNum -= double(m) * X6;
+        }
+        // next diagonal and phasor
+        if (m + 1 <= L) Pmm = diag[m + 1] * v * Pmm;
+        const double s = g.cosL * sinML + g.sinL * cosML;
+        const double c = g.cosL * cosML - g.sinL * sinML;
+        sinML = s;
+        cosML = c;
+    }
+    outR = ar;
+    if (pole) {
+        outPhi = aphiDirect;
+        outLam = alamNum;
+    } else {
+        const double inv = 1.0 / g.cosPhi;
+        outPhi = aphiDirect + aphiNum * inv;
+        outLam = alamNum * inv;
+    }
+}
+
+__device__ inline void shToInertial(const double* __restrict__ rec, const ShGeom& g, double ar, double aphi, double alam,
+                                    double gm, double& ax, double& ay, double& az) {
+    const double s = gm / (g.r * g.r);
+    ar *= s;
+    aphi *= s;
+    alam *= s;
+    const double bx = g.cosPhi * g.cosL * ar - g.sinPhi * g.cosL * aphi - g.sinL * alam;
+    const double by = g.cosPhi * g.sinL * ar - g.sinPhi * g.sinL * aphi + g.cosL * alam;
+    const double bz = g.sinPhi * ar + g.cosPhi * aphi;
+    const double* R = rec + REC_R;
+    ax = R[0] * bx + R[3] * by + R[6] * bz;
+    ay = R[1] * bx + R[4] * by + R[7] * bz;
+    az = R[2] * bx + R[5] * by + R[8] * bz;
+}
+
+__device__ inline double clamp1(double x, double lo, double hi) { return fmin(fmax(x, lo), hi); }
+
+// Dual-cone occultation factor (Shadow.h:84-142).
+__device__ double shadowFactor(double sx, double sy, double sz, double bx, double by, double bz, double RSun, double RBody) {
+    const double s2 = sx * sx + sy * sy + sz * sz, b2 = bx * bx + by * by + bz * bz, sb = sx * bx + sy * by + sz * bz;
+    if (b2 <= RBody * RBody) return 0.0;
+    const double rInvS = 1.0 / sqrt(s2), rInvB = 1.0 / sqrt(b2);
+    const double a = asin(clamp1(RSun * rInvS, 0.0, 1.0));
+    const double b = asin(clamp1(RBody * rInvB, 0.0, 1.0));
+    const double c = acos(clamp1(sb * rInvS * rInvB, -1.0, 1.0));
+    if (c >= a + b) return 1.0;
+    if (c <= fabs(b - a)) return b >= a ? 0.0 : 1.0 - (b * b) / (a * a);
+    const double a2 = a * a, bb = b * b, c2 = c * c;
+    const double x = (c2 + a2 - bb) / (2.0 * c);
+    const double y = sqrt(fmax(a2 - x * x, 0.0));
+    const double al = acos(clamp1(x / a, -1.0, 1.0));
+    const double be = acos(clamp1((c - x) / b, -1.0, 1.0));
+    return 1.0 - (a2 * al + bb * be - c * y) / (a2 * 3.14159265358979323846);
+}
+
+__device__ void srpAccel(const ForceModel& fm, const double* __restrict__ rec, double px, double py, double pz, double ams,
+                         double& ax, double& ay, double& az) {
+    const double* sun = rec + REC_BODY + 3 * fm.srpBody;
+    const double dx = px - sun[0], dy = py - sun[1], dz = pz - sun[2];
+    double occ = 1.0;
+    for (int k = 0; k < fm.nOccluders; ++k) {
+        const double* o = rec + REC_BODY + 3 * fm.occBody[k];
+        occ *= shadowFactor(dx, dy, dz, px - o[0], py - o[1], pz - o[2], fm.sunRadius, fm.occRadius[k]);
+    }
+    const double s2 = dx * dx + dy * dy + dz * dz;
+    const double k = occ * (fm.srpFactor * ams) / (s2 * sqrt(s2));
+    ax = k * dx;
+    ay = k * dy;
+    az = k * dz;
+}
+
+// Geodetic latitude/longitude/altitude of an Earth-fixed position (Nrlmsise00.cu:243-266).
+__device__ bool geodetic(double x, double y, double z, double radius, double flat, double& lat, double& lon, double& alt) {
+    const double rho2 = x * x + y * y;
+    if (!(rho2 + z * z > 0.0)) return false;
+    const double e2 = flat * (2.0 - flat);
+    double dz = e2 * z, n = radius;
+    for (int it = 0; it < 20; ++it) {
+        const double zdz = z + dz;
+        const double nh = sqrt(rho2 + zdz * zdz);
+        const double sp = zdz / nh;
+        n = radius / sqrt(1.0 - e2 * sp * sp);
+        const double next = n * e2 * sp;
+        if (fabs(dz - next) < 1.0e-12 * radius) {
+            dz = next;
+            break;
+        }
+        dz = next;
+    }
+    const double zdz = z + dz;
+    const double nh = sqrt(rho2 + zdz * zdz);
+    lon = atan2(y, x);
+    lat = atan2(zdz, sqrt(rho2));
+    alt = nh - n;
+    return isfinite(lat) && isfinite(lon) && isfinite(alt);
+}
+
+__device__ __noinline__ double nrlDensity(const ForceModel& fm, const double* __restrict__ rec, const double* __restrict__ coeff,
+                                          double fx, double fy, double fz) {
+    double lat, lon, alt;
+    if (!geodetic(fx, fy, fz, fm.earthRadius, fm.flattening, lat, lon, alt)) return nan("");
+    if (alt > fm.hCutoff) return 0.0;
+    if (rec[REC_NRL_VALID] == 0.0) return nan("");
+    const double rad2deg = 57.2957795130823208768;
+    nrlmsise::Input in;
+    in.doy = int(rec[REC_NRL_DOY]);
+    in.sec = rec[REC_NRL_SEC];
+    in.alt = alt;
+    in.g_lat = lat * rad2deg;
+    in.g_long = lon * rad2deg;
+    double lst = fmod(rec[REC_NRL_GAST] + lon * rad2deg / 15.0, 24.0);
+    in.lst = lst < 0.0 ? lst + 24.0 : lst;
+    in.f107A = rec[REC_NRL_F107A];
+    in.f107 = rec[REC_NRL_F107];
+    in.ap = rec[REC_NRL_AP];
+    for (int k = 0; k < 7; ++k) in.apa[k] = rec[REC_NRL_AP + k];
+    nrlmsise::Flags fl;
+    for (int k = 0; k < 24; ++k) fl.switches[k] = 1;
+    fl.switches[9] = -1;
+    nrlmsise::Output out;
+    nrlmsise::State st;
+    nrlmsise::NrlTables T{coeff};
+    nrlmsise::gtd7d(st, T, in, fl, out);
+    if (!isfinite(out.d[5]) || out.d[5] < 0.0) return nan("");
+    return (out.d[5] * 1.0e9) * fm.densityScale;
+}
+
+__device__ void dragAccel(const ForceModel& fm, const double* __restrict__ rec, const double* __restrict__ coeff, double px,
+                          double py, double pz, double vx, double vy, double vz, double bcd, double& ax, double& ay,
+                          double& az) {
+    const double* R = rec + REC_R;
+    const double fx = R[0] * px + R[1] * py + R[2] * pz;
+    const double fy = R[3] * px + R[4] * py + R[5] * pz;
+    const double fz = R[6] * px + R[7] * py + R[8] * pz;
+    const double rho = nrlDensity(fm, rec, coeff, fx, fy, fz);
+    const double* om = rec + REC_OMEGA;
+    const double rx = vx - (om[1] * pz - om[2] * py);
+    const double ry = vy - (om[2] * px - om[0] * pz);
+    const double rz = vz - (om[0] * py - om[1] * px);
+    const double f = -0.5 * rho * bcd * sqrt(rx * rx + ry * ry + rz * rz);
+    ax = f * rx;
+    ay = f * ry;
+    az = f * rz;
+}
+
+// Tide gradient (degree <= 6, combined solid + ocean coefficients) and
+// Schwarzschild term, SI inside, km/s^2 out (HpopCorrections.h:339-376,
+// EarthCorrections.h:57-69).
+__device__ void correctionsAccel(const ForceModel& fm, const double* __restrict__ rec, double px, double py, double pz,
+                                 double vx, double vy, double vz, double& ax, double& ay, double& az) {
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    const double mu = fm.muEarthSI;
+    if (fm.tides) {
+        const double* E = rec + REC_R;
+        const double x = 1000.0 * px, y = 1000.0 * py, z = 1000.0 * pz;
+        const double r0 = E[0] * x + E[1] * y + E[2] * z;
+        const double r1 = E[3] * x + E[4] * y + E[5] * z;
+        const double r2 = E[6] * x + E[7] * y + E[8] * z;
+        const double d = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+        const double xy2 = r0 * r0 + r1 * r1, xy = sqrt(xy2);
+        const double s = r2 / d, c = xy / d;
+        const double cl = r0 / xy, sl = r1 / xy;
+        double p[7][7], dp[7][7];
+        for (int n = 0; n < 7; ++n)
+            for (int m = 0; m < 7; ++m) p[n][m] = dp[n][m] = 0.0;
+        p[0][0] = 1.0;
+        p[1][1] = sqrt(3.0) * c;
+        dp[1][1] = -sqrt(3.0) * s;
+        for (int n = 2; n <= 6; ++n) {
+            const double k = sqrt((2.0 * n + 1) / (2.0 * n));
+            p[n][n] = k * c * p[n - 1][n - 1];
+            dp[n][n] = k * (c * dp[n - 1][n - 1] - s * p[n - 1][n - 1]);
+        }
+        for (int n = 1; n <= 6; ++n) {
+            const double k = sqrt(2.0 * n + 1);
+            p[n][n - 1] = k * s * p[n - 1][n - 1];
+            dp[n][n - 1] = k * (c * p[n - 1][n - 1] + s * dp[n - 1][n - 1]);
+        }
+        for (int m = 0; m <= 6; ++m)
+            for (int n = m + 2; n <= 6; ++n) {
+                const double k = sqrt((2.0 * n + 1) / ((n - m) * double(n + m)));
+                const double a = sqrt(2.0 * n - 1);
+                const double b = sqrt(((n + m - 1) * double(n - m - 1)) / (2.0 * n - 3));
+                p[n][m] = k * (a * s * p[n - 1][m] - b * p[n - 2][m]);
+                dp[n][m] = k * (a * s * dp[n - 1][m] + a * c * p[n - 1][m] - b * dp[n - 2][m]);
+            }
+        double cosM[7], sinM[7];
+        cosM[0] = 1.0;
+        sinM[0] = 0.0;
+        for (int m = 1; m <= 6; ++m) {
+            cosM[m] = cosM[m - 1] * cl - sinM[m - 1] * sl;
+            sinM[m] = sinM[m - 1] * cl + cosM[m - 1] * sl;
+        }
+        const double* TC = rec + REC_TIDE_C;
+        const double* TS = rec + REC_TIDE_S;
+        const double ratio = fm.radiusSI / d;
+        double power = 1.0, dUdr = 0.0, dUdlat = 0.0, dUdlon = 0.0;
+        for (int n = 0; n <= 6; ++n) {
+            const double b1 = (-mu / (d * d)) * power * (n + 1);
+            const double b2 = (mu / d) * power;
+            double q1 = 0.0, q2 = 0.0, q3 = 0.0;
+            for (int m = 0; m <= n; ++m) {
+                const double Cnm = TC[n * (n + 1) / 2 + m], Snm = TS[n * (n + 1) / 2 + m];
+                const double value = Cnm * cosM[m] + Snm * sinM[m];
+                q1 += p[n][m] * value;
+                q2 += dp[n][m] * value;
+                q3 += m * p[n][m] * (Snm * cosM[m] - Cnm * sinM[m]);
+            }
+            dUdr += q1 * b1;
+            dUdlat += q2 * b2;
+            dUdlon += q3 * b2;
+            power *= ratio;
+        }
+        const double q = (1.0 / d * dUdr - r2 / (d * d * xy) * dUdlat);
+        const double bf0 = q * r0 - (1.0 / xy2 * dUdlon) * r1;
+        const double bf1 = q * r1 + (1.0 / xy2 * dUdlon) * r0;
+        const double bf2 = 1.0 / d * dUdr * r2 + xy / (d * d) * dUdlat;
+        sx = E[0] * bf0 + E[3] * bf1 + E[6] * bf2;
+        sy = E[1] * bf0 + E[4] * bf1 + E[7] * bf2;
+        sz = E[2] * bf0 + E[5] * bf1 + E[8] * bf2;
+    }
+    if (fm.relativity) {
+        const double x = 1000.0 * px, y = 1000.0 * py, z = 1000.0 * pz;
+        const double u = 1000.0 * vx, v = 1000.0 * vy, w = 1000.0 * vz;
+        const double r = sqrt(x * x + y * y + z * z);
+        const double vv = u * u + v * v + w * w;
+        const double rv = x * u + y * v + z * w;
+        const double f = mu / (fm.clightSI * fm.clightSI * r * r * r);
+        const double g = 4.0 * mu / r - vv;
+        sx += f * (g * x + 4.0 * rv * u);
+        sy += f * (g * y + 4.0 * rv * v);
+        sz += f * (g * z + 4.0 * rv * w);
+    }
+    ax = 1.0e-3 * sx;
+    ay = 1.0e-3 * sy;
+    az = 1.0e-3 * sz;
+}
+
+// Central and third-body point gravity (PointGravity.h:16-81); the indirect
+// third-body term is an epoch-only sum taken from the record.
+__device__ void pointGravity(const ForceModel& fm, const double* __restrict__ rec, double px, double py, double pz, double& ax,
+                             double& ay, double& az) {
+    ax = ay = az = 0.0;
+    if (fm.earthPointGravity) {
+        const double s2 = px * px + py * py + pz * pz;
+        const double rc = 1.0 / (s2 * sqrt(s2));
+        ax = (-fm.gmEarth * px) * rc;
+        ay = (-fm.gmEarth * py) * rc;
+        az = (-fm.gmEarth * pz) * rc;
+    }
+    for (int b = 0; b < fm.nBodies; ++b) {
+        if (!fm.bodyPointGravity[b] || fm.bodyNaif[b] == 399) continue;
+        const double* q = rec + REC_BODY + 3 * b;
+        const double dx = px - q[0], dy = py - q[1], dz = pz - q[2];
+        const double s2 = dx * dx + dy * dy + dz * dz;
+        const double rc = 1.0 / (s2 * sqrt(s2));
+        const double g = fm.bodyGm[b];
+        ax += (-g * dx) * rc;
+        ay += (-g * dy) * rc;
+        az += (-g * dz) * rc;
+    }
+    ax += rec[REC_INDIRECT + 0];
+    ay += rec[REC_INDIRECT + 1];
+    az += rec[REC_INDIRECT + 2];
+}
+
+// Which wave carries which extra force family for a W-wave tile.
+__device__ constexpr int dragWave(int W) { return W - 1; }
+__device__ constexpr int corrWave(int W) { return W >= 3 ? W - 2 : W - 1; }
+
+template <int W>
+__global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm) {
+    __shared__ double kst[kStages][6][kTile];
+    __shared__ double xch[6][kTile];
+    __shared__ double part[W][3][kTile];
+    __shared__ double extra[2][3][kTile];
+    __shared__ int anyActive;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tile = blockIdx.x;
+    const int obj = tile * kTile + lane;
+    const int nPad = a.nPad;
+
+    double y[6];
+    double dt = 0.0;
+    uint8_t st = ST_PAD;
+    double ams = 0.0, bcd = 0.0;
+    if (wave == 0) {
+        for (int d = 0; d < 6; ++d) y[d] = a.y[d * nPad + obj];
+        st = a.status[obj];
+        dt = a.nextDt[obj];
+    } else {
+        for (int d = 0; d < 6; ++d) y[d] = 0.0;
+    }
+    ams = a.ams[obj];
+    bcd = a.bcd[obj];
+    long long completed = 0;
+    const long long recBase = a.tileRecord[tile];
+    const int m0 = a.blockStart[wave], m1 = a.blockStart[wave + 1];
+
+    for (int step = 0; step < a.steps; ++step) {
+        if (wave == 0) {
+            const bool active = (st & (ST_TERMINATED | ST_PAD)) == 0;
+            const unsigned long long ballot = __ballot(active);
+            if (lane == 0) anyActive = ballot != 0ull;
+        }
+        __syncthreads();
+        if (!anyActive) break;
+        const double* recStep = a.records + (recBase + (long long)(a.stepBase + step) * kStages) * REC_SIZE;
+        for (int s = 0; s < kStages; ++s) {
+            const double* rec = recStep + s * REC_SIZE;
+            if (wave == 0) {
+                double ys[6];
+                for (int d = 0; d < 6; ++d) ys[d] = y[d];
+                if (s >= 1) {
+                    for (int j = 0; j < s; ++j) {
+                        const double aij = kRkA[s - 1][j];
+                        if (aij != 0.0) {
+                            const double coef = dt * aij;
+                            for (int d = 0; d < 6; ++d) ys[d] += coef * kst[j][d][lane];
+                        }
+                    }
+                }
+                for (int d = 0; d < 6; ++d) xch[d][lane] = ys[d];
+            }
+            __syncthreads();
+            const double px = xch[0][lane], py = xch[1][lane], pz = xch[2][lane];
+            // spherical harmonics, this wave's order block
+            ShGeom g;
+            double sr = 0.0, sp = 0.0, sl = 0.0;
+            if (fm.hasSh) {
+                g = shGeometry(rec, px, py, pz, fm.shRadius);
+                if (m0 < m1)
+                    shOrderBlock(g, m0, m1, fm.shDegree, a.shTerms, a.shOrderStart, a.shDiag, fm.shNormalized, sr, sp, sl);
+                part[wave][0][lane] = sr;
+                part[wave][1][lane] = sp;
+                part[wave][2][lane] = sl;
+            }
+            // extra force families
+            if (wave == dragWave(W)) {
+                double ax = 0.0, ay = 0.0, az = 0.0;
+                if (fm.drag) dragAccel(fm, rec, a.nrlCoeff, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], bcd, ax, ay, az);
+                extra[0][0][lane] = ax;
+                extra[0][1][lane] = ay;
+                extra[0][2][lane] = az;
+            }
+            if (wave == corrWave(W)) {
+                double ax = 0.0, ay = 0.0, az = 0.0;
+                if (fm.tides || fm.relativity)
+                    correctionsAccel(fm, rec, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], ax, ay, az);
+                if (fm.srp) {
+                    double bx, by, bz;
+                    srpAccel(fm, rec, px, py, pz, ams, bx, by, bz);
+                    ax += bx;
+                    ay += by;
+                    az += bz;
+                }
+                extra[1][0][lane] = ax;
+                extra[1][1][lane] = ay;
+                extra[1][2][lane] = az;
+            }
+            __syncthreads();
+            if (wave == 0) {
+                double ax, ay, az;
+                pointGravity(fm, rec, px, py, pz, ax, ay, az);
+                if (fm.hasSh) {
+                    double r = 0.0, p = 0.0, l = 0.0;
+                    for (int w = 0; w < W; ++w) {
+                        r += part[w][0][lane];
+                        p += part[w][1][lane];
+                        l += part[w][2][lane];
+                    }
+                    double hx, hy, hz;
+                    shToInertial(rec, g, r, p, l, fm.shGm, hx, hy, hz);
+                    ax += hx;
+                    ay += hy;
+                    az += hz;
+                }
+                ax += extra[1][0][lane] + extra[0][0][lane];
+                ay += extra[1][1][lane] + extra[0][1][lane];
+                az += extra[1][2][lane] + extra[0][2][lane];
+                kst[s][0][lane] = xch[3][lane];
+                kst[s][1][lane] = xch[4][lane];
+                kst[s][2][lane] = xch[5][lane];
+                kst[s][3][lane] = ax;
+                kst[s][4][lane] = ay;
+                kst[s][5][lane] = az;
+            }
+        }
+        if (wave == 0) {
+            const bool active = (st & (ST_TERMINATED | ST_PAD)) == 0;
+            if (active) {
+                for (int s = 0; s < kStages; ++s) {
+                    const double bs = kRkB[s];
+                    if (bs != 0.0) {
+                        const double cb = dt * bs;
+                        for (int d = 0; d < 6; ++d) y[d] += cb * kst[s][d][lane];
+                    }
+                }
+                // FixedStep.h:30-37
+                const double cur = a.cur[obj] + dt;
+                a.cur[obj] = cur;
+                const double remaining = a.end[obj] - cur;
+                const bool finished = remaining <= 0.0;
+                st = finished ? (ST_TERMINATED | ST_EOS) : 0;
+                dt = finished ? 0.0 : fmin(a.step, remaining);
+                ++completed;
+            }
+        }
+    }
+    if (wave == 0) {
+        for (int d = 0; d < 6; ++d) a.y[d * nPad + obj] = y[d];
+        a.status[obj] = st;
+        a.nextDt[obj] = dt;
+        // object-steps completed in this launch, one atomic per wave
+        unsigned long long c = (unsigned long long)completed;
+        for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+        if (lane == 0 && a.counters) atomicAdd((unsigned long long*)a.counters, c);
+    }
+}
+
+__global__ void __launch_bounds__(64) forceProbeKernel(int n, const double* pos, const double* vel, const double* ams,
+                                                       const double* bcd, const double* rec, int which, ForceModel fm,
+                                                       const double* shTerms, const int* shOrderStart, const double* shDiag,
+                                                       const double* nrlCoeff, double* acc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double px = pos[i], py = pos[n + i], pz = pos[2 * n + i];
+    const double vx = vel[i], vy = vel[n + i], vz = vel[2 * n + i];
+    double ax = 0.0, ay = 0.0, az = 0.0, bx, by, bz;
+    if (which < 0 || which == 0 || which == 2) {
+        ForceModel f = fm;
+        if (which == 0) f.nBodies = 0;        // central term only
+        if (which == 2) f.earthPointGravity = 0;  // third bodies only
+        pointGravity(f, rec, px, py, pz, bx, by, bz);
+        if (which == 0) {
+            bx -= rec[REC_INDIRECT];
+            by -= rec[REC_INDIRECT + 1];
+            bz -= rec[REC_INDIRECT + 2];
+        }
+        ax += bx, ay += by, az += bz;
+    }
+    if ((which < 0 || which == 1) && fm.hasSh) {
+        const ShGeom g = shGeometry(rec, px, py, pz, fm.shRadius);
+        double r, p, l;
+        shOrderBlock(g, 0, fm.shOrder + 1, fm.shDegree, shTerms, shOrderStart, shDiag, fm.shNormalized, r, p, l);
+        shToInertial(rec, g, r, p, l, fm.shGm, bx, by, bz);
+        ax += bx, ay += by, az += bz;
+    }
+    if ((which < 0 || which == 3) && fm.srp) {
+        srpAccel(fm, rec, px, py, pz, ams[i], bx, by, bz);
+        ax += bx, ay += by, az += bz;
+    }
+    if ((which < 0 || which == 4) && fm.drag) {
+        dragAccel(fm, rec, nrlCoeff, px, py, pz, vx, vy, vz, bcd[i], bx, by, bz);
+        ax += bx, ay += by, az += bz;
+    }
+    if ((which < 0 || which == 5) && (fm.tides || fm.relativity)) {
+        correctionsAccel(fm, rec, px, py, pz, vx, vy, vz, bx, by, bz);
+        ax += bx, ay += by, az += bz;
+    }
+    acc[i] = ax;
+    acc[n + i] = ay;
+    acc[2 * n + i] = az;
+}
+
+__global__ void __launch_bounds__(64) nrlProbeKernel(int n, const double* in, int useApArray, const double* coeff, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double* r = in + 16 * i;
+    nrlmsise::Input x;
+    x.doy = int(r[0]);
+    x.sec = r[1];
+    x.alt = r[2];
+    x.g_lat = r[3];
+    x.g_long = r[4];
+    x.lst = r[5];
+    x.f107A = r[6];
+    x.f107 = r[7];
+    x.ap = r[8];
+    for (int k = 0; k < 7; ++k) x.apa[k] = r[9 + k];
+    nrlmsise::Flags fl;
+    for (int k = 0; k < 24; ++k) fl.switches[k] = 1;
+    fl.switches[0] = 0;
+    if (useApArray) fl.switches[9] = -1;
+    nrlmsise::Output o;
+    nrlmsise::State st;
+    nrlmsise::NrlTables T{coeff};
+    nrlmsise::gtd7d(st, T, x, fl, o);
+    for (int k = 0; k < 9; ++k) out[11 * i + k] = o.d[k];
+    out[11 * i + 9] = o.t[0];
+    out[11 * i + 10] = o.t[1];
+}
+
+}  // namespace
+
+int launchStep(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t stream) {
+    if (a.nTiles <= 0) return 0;
+    const dim3 grid(a.nTiles);
+    switch (waves) {
+        case 1: hipLaunchKernelGGL(rkf78Kernel<1>, grid, dim3(64), 0, stream, a, fm); break;
+        case 2: hipLaunchKernelGGL(rkf78Kernel<2>, grid, dim3(128), 0, stream, a, fm); break;
+        case 4: hipLaunchKernelGGL(rkf78Kernel<4>, grid, dim3(256), 0, stream, a, fm); break;
+        case 8: hipLaunchKernelGGL(rkf78Kernel<8>, grid, dim3(512), 0, stream, a, fm); break;
+        default: return -2;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launchForceProbe(int n, const double* pos, const double* vel, const double* ams, const double* bcd, const double* record,
+                     int which, const ForceModel& fm, const double* shTerms, const int* shOrderStart, const double* shDiag,
+                     const double* nrlCoeff, double* acc, hipStream_t stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(forceProbeKernel, dim3((n + 63) / 64), dim3(64), 0, stream, n, pos, vel, ams, bcd, record, which, fm,
+                       shTerms, shOrderStart, shDiag, nrlCoeff, acc);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launchNrlmsiseProbe(int n, const double* in, int useApArray, const double* nrlCoeff, double* out, hipStream_t stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(nrlProbeKernel, dim3((n + 63) / 64), dim3(64), 0, stream, n, in, useApArray, nrlCoeff, out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace phx

@@ -1,0 +1,299 @@
+"""Seeded synthetic inputs for the five propagation workloads.
+
+Writes the reference's own file formats so the same directory drives either
+implementation: CONFIG.json / TARGETS.json (docs/configuration.md), a PDS-SHA
+gravity file (interface/config/model/sphericalharmonics/Coefficients.h:188-362),
+a CelesTrak SW-All file (model/environment/atmosphere/Nrlmsise00Atmosphere.h:146-188),
+an HPOP EOP-All file (model/environment/earthcorrections/EarthCorrections.h:141-177)
+and constants.json (data/constants.json).
+
+Workloads (object populations and force models):
+  1  1,024 LEO objects, central gravity only, 60 s step, 600 s
+  2  16,384 LEO/MEO objects, 70x70 field + NRLMSISE-00 + SRP/shadow + Sun,
+     Moon, planets + solid/ocean tides + relativity, 60 s step, 1 day
+  3  as 2 with a 120x120 Kaula-rule field, 30 s step, mixed precision
+  4  131,072 objects, uneven epochs and durations, 70% LEO/20% MEO/10% GEO
+  5  serving stream (request generator in serve.py)
+"""
+import json
+import math
+import os
+
+import numpy as np
+
+GM_EARTH = 398600.4415
+R_EARTH = 6378.137
+R_FIELD = 6378.1363
+EPOCH0 = 4035.0
+
+CONSTANTS = [
+    {"CONSTANTS": "synthetic set (HPOP 4.2.2 / GGM03C / DE440 compatible values)",
+     "AU": 149597870.7, "CLIGHT": 299792.458, "EMRAT": 81.30056822149722, "PS": 4.559821181358739e-09},
+    {"NAME": "Sun", "NAIFID": 10, "GM": 132712440041.27942, "SOI": 10000000000.0, "R": 696000.0,
+     "A_MEAN": 0.0, "J2": 0.0},
+    {"NAME": "Moon", "NAIFID": 301, "MAIN_BODY": "Earth", "GM": 4902.800192171393, "SOIOld": 66194.3772804927,
+     "R": 1738.0, "A_MEAN": 384466.5276989999, "J2": 0.00020343},
+    {"NAME": "Earth", "NAIFID": 399, "MAIN_BODY": "Sun", "GM": 398600.4415, "SOIOld": 924646.788599425,
+     "R": 6378.137, "A_MEAN": 149597870.7, "J2": 0.0010826355254902923},
+]
+
+PLANET_GM = {
+    "MercuryBarycenter": 22031.868551, "VenusBarycenter": 324858.592, "MarsBarycenter": 42828.375816,
+    "JupiterBarycenter": 126712764.1, "SaturnBarycenter": 37940584.8418, "UranusBarycenter": 5794556.4,
+    "NeptuneBarycenter": 6836527.10058, "PlutoBarycenter": 975.5,
+}
+
+
+def kepler_to_cartesian(a, e, inc, raan, argp, M, gm=GM_EARTH):
+    """Classical elements (km, rad) -> ICRF position/velocity (km, km/s), vectorised."""
+    E = M.copy()
+    for _ in range(12):
+        E = E - (E - e * np.sin(E) - M) / (1.0 - e * np.cos(E))
+    cE, sE = np.cos(E), np.sin(E)
+    sq = np.sqrt(1.0 - e * e)
+    x_p = a * (cE - e)
+    y_p = a * sq * sE
+    r = a * (1.0 - e * cE)
+    n = np.sqrt(gm / a ** 3)
+    vx_p = -a * n * sE / (1.0 - e * cE)
+    vy_p = a * n * sq * cE / (1.0 - e * cE)
+    cO, sO, cw, sw, ci, si = np.cos(raan), np.sin(raan), np.cos(argp), np.sin(argp), np.cos(inc), np.sin(inc)
+    P = np.stack([cO * cw - sO * sw * ci, sO * cw + cO * sw * ci, sw * si])
+    Q = np.stack([-cO * sw - sO * cw * ci, -sO * sw + cO * cw * ci, cw * si])
+    pos = P * x_p + Q * y_p
+    vel = P * vx_p + Q * vy_p
+    del r
+    return pos, vel
+
+
+def population(n, seed, mix=(1.0, 0.0, 0.0), leo=(300.0, 2000.0), meo=(19000.0, 24000.0), physical=True):
+    """Objects drawn from seeded distributions; mix = fractions (LEO, MEO, GEO)."""
+    rng = np.random.default_rng(seed)
+    kind = rng.choice(3, size=n, p=np.array(mix) / sum(mix))
+    alt = np.where(kind == 0, rng.uniform(*leo, n), rng.uniform(*meo, n))
+    a = R_EARTH + alt
+    a = np.where(kind == 2, 42164.0 + rng.uniform(-50.0, 50.0, n), a)
+    e = rng.uniform(0.0, 0.02, n)
+    inc = np.radians(rng.uniform(0.0, 180.0, n))
+    raan = np.radians(rng.uniform(0.0, 360.0, n))
+    argp = np.radians(rng.uniform(0.0, 360.0, n))
+    M = np.radians(rng.uniform(0.0, 360.0, n))
+    # keep perigee above 150 km for the LEO shell
+    a = np.maximum(a, (R_EARTH + 150.0) / (1.0 - e))
+    pos, vel = kepler_to_cartesian(a, e, inc, raan, argp, M)
+    out = {"state": np.vstack([pos, vel]), "kind": kind}
+    if physical:
+        mass = np.exp(rng.uniform(math.log(10.0), math.log(5000.0), n))
+        am = np.exp(rng.normal(math.log(0.01), 1.0, n))  # m^2/kg
+        area = am * mass
+        out.update(mass=mass, drag_area=area, srp_area=area.copy(), cd=np.full(n, 2.2), cr=np.full(n, 1.3))
+    else:
+        out.update(mass=np.full(n, 1000.0), drag_area=np.full(n, 10.0), srp_area=np.full(n, 10.0),
+                   cd=np.full(n, 2.2), cr=np.full(n, 1.3))
+    return out
+
+
+def write_targets(path, pop, epochs, ids=None):
+    n = pop["state"].shape[1]
+    ids = np.arange(1, n + 1) if ids is None else ids
+    st = pop["state"]
+    targets = []
+    for i in range(n):
+        targets.append({
+            "id": int(ids[i]),
+            "epoch_mjd2000_tdb": float(epochs[i]),
+            "state_km_km_s": [float(st[d, i]) for d in range(6)],
+            "mass_kg": float(pop["mass"][i]),
+            "drag_area_m2": float(pop["drag_area"][i]),
+            "srp_area_m2": float(pop["srp_area"][i]),
+            "cd": float(pop["cd"][i]),
+            "cr": float(pop["cr"][i]),
+        })
+    with open(path, "w") as f:
+        json.dump({"description": "synthetic population", "targets": targets}, f, separators=(",", ":"))
+
+
+def write_gravity(path, degree, seed, kaula=False):
+    """Fully normalized PDS-SHA field: C20 = -4.8417e-4, other C/S ~ N(0, (s/n^2)^2), S_n0 = 0."""
+    rng = np.random.default_rng(seed)
+    scale = 1e-5
+    lines = ["%.10f, %.10f, 0.0, %d, %d, 1, 0.0, 0.0" % (R_FIELD, GM_EARTH, degree, degree)]
+    for n in range(1, degree + 1):
+        for m in range(n + 1):
+            if n == 1:
+                c = s = 0.0
+            else:
+                sig = scale / n ** 2
+                c = rng.normal(0.0, sig)
+                s = rng.normal(0.0, sig) if m > 0 else 0.0
+                if n == 2 and m == 0:
+                    c = -4.8417e-4
+            lines.append("%d, %d, %.17e, %.17e" % (n, m, c, s))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def mjd_to_calendar(mjd):
+    jd = int(math.floor(mjd)) + 2400001
+    a = jd + 32044
+    b = (4 * a + 3) // 146097
+    c = a - 146097 * b // 4
+    d = (4 * c + 3) // 1461
+    e = c - 1461 * d // 4
+    m = (5 * e + 2) // 153
+    day = e - (153 * m + 2) // 5 + 1
+    month = m + 3 - 12 * (m // 10)
+    year = 100 * b + d - 4800 + m // 10
+    return year, month, day
+
+
+def write_space_weather(path, mjd_first, days, f107=150.0, ap=15):
+    """CelesTrak SW-All layout: y m d bsrn nd kp*8 sum ap*8 avg cp c9 isn adjF q adjC81 adjL81 obsF obsC81 obsL81."""
+    lines = []
+    for k in range(days):
+        y, mo, d = mjd_to_calendar(mjd_first + k)
+        kp = " ".join(["27"] * 8)
+        aps = " ".join([str(ap)] * 8)
+        lines.append("%d %02d %02d 2420 %d %s 216 %s %d 0.8 4 50 %.1f 0 %.1f %.1f %.1f %.1f %.1f" % (
+            y, mo, d, k + 1, kp, aps, ap, f107, f107, f107, f107, f107, f107))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def write_eop(path, mjd_first, days):
+    """HPOP EOP-All rows: y m d mjd xp yp ut1-utc lod dpsi deps dx dy tai-utc (constant, zero pole)."""
+    lines = []
+    for k in range(days):
+        y, mo, d = mjd_to_calendar(mjd_first + k)
+        lines.append("%d %2d %2d %d 0.000000 0.000000 0.0000000 0.0000000 0.000000 0.000000 0.000000 0.000000 37"
+                     % (y, mo, d, mjd_first + k))
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def full_model(degree, gravity_file, start_mjd_utc, end_mjd_utc):
+    bodies = {
+        "Earth": {
+            "occulting": True,
+            "flattening": 0.0033528106647474805,
+            "gravity": {"shape": {"body": "Earth", "file": gravity_file, "delimiter": ",", "headerLines": 0,
+                                  "degree": degree, "order": degree}},
+            "atmosphere": {"nrlmsise00": {"spaceWeatherFile": "SW-All.txt", "coverageStartMjdUtc": start_mjd_utc,
+                                          "coverageEndMjdUtc": end_mjd_utc, "hCutoff": 1000.0, "densityScale": 1.0}},
+        },
+        "Sun": {"gravity": True, "radiation": True},
+        "Moon": {"gravity": True},
+    }
+    for name, gm in PLANET_GM.items():
+        bodies[name] = {"gm": gm, "gravity": True}
+    return {"environment": {
+        "bodies": bodies, "ephemeris": "analytic", "orientations": "analytic", "constants": "constants.json",
+        "preferSSB": True,
+        "earthCorrections": {"model": "iers2010_fes2004", "tideSystem": "zero_tide", "solidEarthTides": True,
+                             "oceanTides": True, "relativity": True, "eopFile": "EOP-All.txt"}}}
+
+
+def central_model():
+    return {"environment": {"bodies": {"Earth": {"gravity": True}}, "ephemeris": "analytic",
+                            "constants": "constants.json", "preferSSB": True}}
+
+
+def write_config(path, model, duration, step, max_steps, precision=None):
+    cfg = {"run": {"duration_seconds": float(duration)},
+           "execution": {"maxThreads": 1, "integration": {"initialStepSize": float(step), "maxNumSteps": int(max_steps)}},
+           "model": model}
+    if precision:
+        cfg["execution"]["precision"] = precision
+    with open(path, "w") as f:
+        json.dump(cfg, f, indent=2)
+
+
+def environment_files(out, degree, seed, mjd_lo, mjd_hi, kaula=False):
+    with open(os.path.join(out, "constants.json"), "w") as f:
+        json.dump(CONSTANTS, f, indent=1)
+    gfile = "gravity_%d.tab" % degree
+    write_gravity(os.path.join(out, gfile), degree, seed, kaula)
+    first = int(math.floor(mjd_lo)) - 10
+    days = int(math.ceil(mjd_hi)) - first + 10
+    write_space_weather(os.path.join(out, "SW-All.txt"), first, days)
+    write_eop(os.path.join(out, "EOP-All.txt"), first, days)
+    return gfile
+
+
+def make_workload(out, which, n=None, duration=None, step=None, degree=None, seed=None, precision=None):
+    """Write CONFIG.json, TARGETS.json and data files for workload 1..4; returns the paths."""
+    os.makedirs(out, exist_ok=True)
+    mjd_utc0 = EPOCH0 + 51544.5
+    if which == 1:
+        n = n or 1024
+        pop = population(n, seed or 1, mix=(1, 0, 0), leo=(400.0, 1200.0), physical=False)
+        epochs = np.full(n, EPOCH0)
+        with open(os.path.join(out, "constants.json"), "w") as f:
+            json.dump(CONSTANTS, f, indent=1)
+        model = central_model()
+        duration = duration or 600.0
+        step = step or 60.0
+    elif which in (2, 3):
+        n = n or 16384
+        pop = population(n, seed or 2, mix=(0.8, 0.2, 0.0))
+        epochs = np.full(n, EPOCH0)
+        degree = degree or (70 if which == 2 else 120)
+        duration = duration or 86400.0
+        step = step or (60.0 if which == 2 else 30.0)
+        gfile = environment_files(out, degree, 70 if which == 2 else 120, mjd_utc0, mjd_utc0 + duration / 86400.0)
+        model = full_model(degree, gfile, mjd_utc0 - 1, mjd_utc0 + duration / 86400.0 + 1)
+        if which == 3 and precision is None:
+            precision = {"mode": "mixed", "fp64Degree": 20}
+    elif which == 4:
+        n = n or 131072
+        pop = population(n, seed or 4, mix=(0.7, 0.2, 0.1))
+        rng = np.random.default_rng((seed or 4) + 1000)
+        epochs = EPOCH0 + rng.uniform(0.0, 30.0, n)
+        degree = degree or 70
+        step = step or 60.0
+        duration = duration or 86400.0
+        gfile = environment_files(out, degree, 70, mjd_utc0, mjd_utc0 + 30.0 + 8.0)
+        model = full_model(degree, gfile, mjd_utc0 - 1, mjd_utc0 + 39.0)
+    else:
+        raise ValueError("workload must be 1..4")
+    max_steps = int(math.ceil(duration / step)) + 2
+    write_config(os.path.join(out, "CONFIG.json"), model, duration, step, max_steps, precision)
+    write_targets(os.path.join(out, "TARGETS.json"), pop, epochs)
+    return os.path.join(out, "CONFIG.json"), os.path.join(out, "TARGETS.json")
+
+
+def load_targets(path):
+    """TARGETS.json -> dict of numpy arrays in the C-ABI layout (state SoA [6][n])."""
+    with open(path) as f:
+        t = json.load(f)["targets"]
+    n = len(t)
+    return {
+        "id": np.array([x["id"] for x in t], dtype=np.int64),
+        "epoch": np.array([x["epoch_mjd2000_tdb"] for x in t], dtype=np.float64),
+        "state": np.ascontiguousarray(np.array([x["state_km_km_s"] for x in t], dtype=np.float64).T),
+        "mass": np.array([x["mass_kg"] for x in t], dtype=np.float64),
+        "drag_area": np.array([x["drag_area_m2"] for x in t], dtype=np.float64),
+        "srp_area": np.array([x["srp_area_m2"] for x in t], dtype=np.float64),
+        "cd": np.array([x["cd"] for x in t], dtype=np.float64),
+        "cr": np.array([x["cr"] for x in t], dtype=np.float64),
+        "n": n,
+    }
+
+
+def targets_from_population(pop, epochs):
+    n = pop["state"].shape[1]
+    return {"id": np.arange(1, n + 1, dtype=np.int64), "epoch": np.asarray(epochs, dtype=np.float64),
+            "state": np.ascontiguousarray(pop["state"], dtype=np.float64), "mass": pop["mass"],
+            "drag_area": pop["drag_area"], "srp_area": pop["srp_area"], "cd": pop["cd"], "cr": pop["cr"], "n": n}
+
+
+if __name__ == "__main__":
+    import argparse
+
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("workload", type=int)
+    ap.add_argument("out")
+    ap.add_argument("--n", type=int)
+    ap.add_argument("--duration", type=float)
+    args = ap.parse_args()
+    print(make_workload(args.out, args.workload, n=args.n, duration=args.duration))

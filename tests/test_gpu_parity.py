@@ -1,0 +1,183 @@
+"""Parity of the HIP library (through the C-ABI) against the CPU oracle.
+
+Tolerances (FP64 everywhere):
+  * central gravity, 1,024 objects, 600 s: final state within 1e-9 relative
+    (the workload's bit-level contract), epochs identical;
+  * each force family at random states: within 1e-9 of that family's
+    magnitude (the kernel's SH sum is an exact algebraic rearrangement of the
+    reference's, so only rounding differs);
+  * full force model: position within 1 mm and velocity within 1 um/s.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+EPOCH0_ET = 4035.0 * 86400.0 - 43200.0
+
+
+def _ctx(cfg, **opt):
+    import parahpop_amd
+    return parahpop_amd.Context(cfg, **opt)
+
+
+def _oracle(cfg):
+    import oracle
+    return oracle.Oracle(cfg)
+
+
+def test_library_loaded(gpu_lib):
+    assert gpu_lib.phx_version().decode().startswith("phx")
+
+
+def test_central_matches_oracle(central_workload):
+    cfg, t = central_workload
+    e_g, s_g, st = _ctx(cfg).propagate(t, 600.0)
+    e_o, s_o, _ = _oracle(cfg).propagate(t, 600.0, threads=8)
+    assert st["object_steps"] == 10 * t["n"]
+    np.testing.assert_array_equal(e_g, e_o)
+    r = np.linalg.norm(s_o[:3], axis=0)
+    v = np.linalg.norm(s_o[3:], axis=0)
+    assert np.max(np.linalg.norm(s_g[:3] - s_o[:3], axis=0) / r) < 1e-9
+    assert np.max(np.linalg.norm(s_g[3:] - s_o[3:], axis=0) / v) < 1e-9
+
+
+@pytest.mark.parametrize("which,name", [(0, "central"), (1, "sh"), (2, "third"), (3, "srp"), (4, "drag"), (5, "corrections")])
+def test_force_family_matches_oracle(full_workload, which, name):
+    cfg, t = full_workload
+    ctx = _ctx(cfg)
+    orc = _oracle(cfg)
+    n = t["n"]
+    area = t["drag_area"] * 1e-6
+    cr_eff = t["cr"] * t["srp_area"] / t["drag_area"]
+    et = EPOCH0_ET + 1234.5
+    acc = ctx.force(t["state"][:3], t["state"][3:], t["mass"], area, cr_eff, t["cd"], et, which)
+    ref = np.stack([orc.force(t["state"][:3, i], t["state"][3:, i], et, t["mass"][i], area[i], cr_eff[i], t["cd"][i], which)
+                    for i in range(n)], axis=1)
+    scale = np.max(np.linalg.norm(ref, axis=0))
+    assert scale > 0 or name in ("drag",)
+    err = np.max(np.linalg.norm(acc - ref, axis=0))
+    assert err <= 1e-9 * max(scale, 1e-30), (name, err, scale)
+
+
+def test_nrlmsise_device_published_vectors(central_workload):
+    here = os.path.dirname(os.path.abspath(__file__))
+    doc = json.load(open(os.path.join(here, "golden", "nrlmsise00_testdriver.json")))
+    ctx = _ctx(central_workload[0])
+    for use_ap in (False, True):
+        cases = [c for c in doc["cases"] if (c["ap_array"] is not None) == use_ap]
+        rows = np.array([list(c["input"]) + (c["ap_array"] or [c["input"][8]] * 7) for c in cases], dtype=float)
+        out = ctx.nrlmsise(rows, use_ap_array=use_ap)
+        for c, o in zip(cases, out):
+            exp = np.array(c["output"])
+            got = np.concatenate([o[:5], [o[5] - 1.66e-24 * 16.0 * o[8]], o[6:9], o[9:11]])
+            np.testing.assert_allclose(got, exp, rtol=6e-7, atol=0, err_msg=str(c["input"]))
+
+
+def _compare(s_g, s_o, pos_mm=1.0, vel_um_s=1.0):
+    dp = np.linalg.norm(s_g[:3] - s_o[:3], axis=0) * 1e6   # km -> mm
+    dv = np.linalg.norm(s_g[3:] - s_o[3:], axis=0) * 1e9   # km/s -> um/s
+    assert np.max(dp) <= pos_mm, np.max(dp)
+    assert np.max(dv) <= vel_um_s, np.max(dv)
+    return np.max(dp), np.max(dv)
+
+
+def test_full_force_one_hour_matches_oracle(full_workload):
+    cfg, t = full_workload
+    e_g, s_g, st = _ctx(cfg).propagate(t, 3600.0)
+    e_o, s_o, _ = _oracle(cfg).propagate(t, 3600.0, threads=8)
+    np.testing.assert_array_equal(e_g, e_o)
+    assert st["object_steps"] == 60 * t["n"]
+    _compare(s_g, s_o)
+
+
+def test_full_force_one_day_matches_oracle(workdir):
+    """64 objects of the full-force population over one day at 60 s: <= 1 mm, <= 1 um/s."""
+    from parahpop_amd import synth
+    cfg, tg = synth.make_workload(str(workdir / "w2day"), 2, n=64, duration=86400.0)
+    t = synth.load_targets(tg)
+    e_g, s_g, _ = _ctx(cfg).propagate(t, 86400.0)
+    e_o, s_o, _ = _oracle(cfg).propagate(t, 86400.0, threads=8)
+    np.testing.assert_array_equal(e_g, e_o)
+    _compare(s_g, s_o)
+
+
+@pytest.mark.parametrize("waves", [1, 2, 4, 8])
+def test_wave_split_invariance(full_workload, waves):
+    """The order-block split across waves changes only the summation order."""
+    cfg, t = full_workload
+    ref = _ctx(cfg, waves=1).propagate(t, 600.0)[1]
+    got = _ctx(cfg, waves=waves, steps_per_launch=3).propagate(t, 600.0)[1]
+    rel = np.max(np.linalg.norm(got[:3] - ref[:3], axis=0) / np.linalg.norm(ref[:3], axis=0))
+    assert rel < 1e-12
+
+
+def test_partial_tile_and_mixed_epochs(workdir):
+    """33 targets with different epochs and masses (a tile with a partial tail,
+    several epoch groups), and a single target; both against the oracle."""
+    from parahpop_amd import synth
+    cfg, tg = synth.make_workload(str(workdir / "w2mix"), 2, n=33, duration=1800.0)
+    t = synth.load_targets(tg)
+    t["epoch"] = t["epoch"] + np.arange(33) % 3 * 0.25
+    t["mass"] = t["mass"] * (1 + np.arange(33) % 5)
+    e_g, s_g, _ = _ctx(cfg).propagate(t, 1800.0)
+    e_o, s_o, _ = _oracle(cfg).propagate(t, 1800.0, threads=4)
+    np.testing.assert_array_equal(e_g, e_o)
+    _compare(s_g, s_o)
+    one = {k: (v[:1] if hasattr(v, "shape") and v.ndim == 1 else v) for k, v in t.items()}
+    one["state"] = t["state"][:, :1].copy()
+    one["n"] = 1
+    e_g, s_g, _ = _ctx(cfg).propagate(one, 1800.0)
+    e_o, s_o, _ = _oracle(cfg).propagate(one, 1800.0, threads=1)
+    np.testing.assert_array_equal(e_g, e_o)
+    _compare(s_g, s_o)
+
+
+@pytest.mark.parametrize("duration", [65.0, 5.0, 600.0])
+def test_shortened_final_step(central_workload, duration):
+    """The final step is shortened so every object ends at its requested epoch
+    (validate.py full_65s_endpoint / full_5s_endpoint)."""
+    cfg, t = central_workload
+    e_g, s_g, st = _ctx(cfg).propagate(t, duration)
+    e_o, s_o, _ = _oracle(cfg).propagate(t, duration, threads=8)
+    np.testing.assert_array_equal(e_g, e_o)
+    assert np.max(np.abs(e_g - (t["epoch"] + duration / 86400.0))) * 86400.0 < 1e-4
+    rel = np.max(np.linalg.norm(s_g[:3] - s_o[:3], axis=0) / np.linalg.norm(s_o[:3], axis=0))
+    assert rel < 1e-9
+
+
+def test_batch_rerun_is_reproducible(full_workload):
+    cfg, t = full_workload
+    ctx = _ctx(cfg)
+    b = ctx.batch(t, 600.0)
+    b.run()
+    e1, s1 = b.download()
+    b.reset()
+    b.run(3)
+    assert not b.done()
+    b.run()
+    e2, s2 = b.download()
+    np.testing.assert_array_equal(s1, s2)
+    np.testing.assert_array_equal(e1, e2)
+
+
+def test_rejections(central_workload):
+    import parahpop_amd
+    cfg, t = central_workload
+    ctx = _ctx(cfg)
+    bad = dict(t)
+    bad["id"] = t["id"].copy()
+    bad["id"][1] = bad["id"][0]
+    with pytest.raises(parahpop_amd.PhxError) as ex:
+        ctx.propagate(bad, 600.0)
+    assert "unique" in str(ex.value)
+    bad = dict(t)
+    bad["mass"] = t["mass"].copy()
+    bad["mass"][3] = 0.0
+    with pytest.raises(parahpop_amd.PhxError):
+        ctx.propagate(bad, 600.0)
+    with pytest.raises(parahpop_amd.PhxError):
+        ctx.propagate(t, -1.0)
