@@ -614,13 +614,14 @@ int phx_create(const char* config_json_path, phx_ctx** out) {
     *out = nullptr;
     auto c = std::make_unique<phx_ctx>();
     try {
-        int ndev = 0;
-        check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
-        if (ndev <= 0) throw HipError("No usable HIP device");
         const std::string path = config_json_path;
         json::Value root = json::readFile(path);
         parseConfig(root, dirName(path), c->run, c->env.model, false);
         loadEnvironment(c->env);
+        int ndev = 0;
+        const hipError_t de = hipGetDeviceCount(&ndev);
+        if (de != hipSuccess || ndev <= 0)
+            throw HipError(std::string("No usable HIP device: ") + hipGetErrorString(de));
         const char* env = std::getenv("PHX_DATA_DIR");
         c->dataDir = env ? env : libraryDir() + "/data";
         check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
