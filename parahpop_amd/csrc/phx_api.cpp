@@ -107,6 +107,9 @@ struct phx_ctx {
     int stepsPerLaunch = 32;
     double share0 = 0.85, shareCorr = 0.85, shareDrag = 0.35;
     int sortAltitude = 1;
+    int exactGroupsMax = 16;     // more distinct (start, end) groups than this -> grid epochs
+    double gridStep = 60.0;      // s, spacing of the epoch grid
+    int forceGrid = 0;
     // serving
     std::unique_ptr<Server> server;
 };
@@ -126,6 +129,9 @@ struct phx_batch {
     std::vector<uint8_t> st0;
     size_t epochRecords = 0;
     double epochSeconds = 0.0;
+    bool grid = false;
+    int gridCount = 0;
+    double gridT0 = 0.0, gridInvH = 0.0;
 };
 
 namespace {
@@ -405,48 +411,68 @@ std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, c
         grp[i] = it->second;
     }
     const int G = int(groupOf.size());
-    std::vector<std::pair<double, double>> gkey(static_cast<size_t>(G));
-    for (auto& kv : groupOf) gkey[size_t(kv.second)] = kv.first;
-    // step sequences and record epochs per group
+    const bool grid = c.forceGrid || G > c.exactGroupsMax;
+    b->grid = grid;
     std::vector<double> epochs;
-    std::vector<long long> groupRecord(static_cast<size_t>(G));
+    std::vector<long long> groupRecord(static_cast<size_t>(G), 0);
     int totalSteps = 0;
-    for (int g = 0; g < G; ++g) {
-        groupRecord[size_t(g)] = (long long)(epochs.size() / 1);
-        double cur = gkey[size_t(g)].first;
-        const double end = gkey[size_t(g)].second;
-        double dt = std::min(step, end - cur);
-        int k = 0;
-        while (k < launches) {
-            for (int s = 0; s < kStages; ++s) {
-                double ts = cur;
-                if (s >= 1) ts += dt * kRkC[s - 1];
-                epochs.push_back(ts);
+    if (!grid) {
+        std::vector<std::pair<double, double>> gkey(static_cast<size_t>(G));
+        for (auto& kv : groupOf) gkey[size_t(kv.second)] = kv.first;
+        // per-group step sequences; record epochs are the stage epochs themselves
+        for (int g = 0; g < G; ++g) {
+            groupRecord[size_t(g)] = (long long)(epochs.size() / kStages);
+            double cur = gkey[size_t(g)].first;
+            const double end = gkey[size_t(g)].second;
+            double dt = std::min(step, end - cur);
+            int k = 0;
+            while (k < launches) {
+                for (int s = 0; s < kStages; ++s) {
+                    double ts = cur;
+                    if (s >= 1) ts += dt * kRkC[s - 1];
+                    epochs.push_back(ts);
+                }
+                cur += dt;
+                const double rem = end - cur;
+                const bool fin = rem <= 0;
+                dt = fin ? 0.0 : std::min(step, rem);
+                ++k;
+                if (fin) break;
             }
-            cur += dt;
-            const double rem = end - cur;
-            const bool fin = rem <= 0;
-            dt = fin ? 0.0 : std::min(step, rem);
-            ++k;
-            if (fin) break;
+            totalSteps = std::max(totalSteps, k);
         }
-        totalSteps = std::max(totalSteps, k);
+    } else {
+        // uniform grid over the whole batch window, two rows of margin each side
+        double lo = start[0], hi = endEt[0];
+        for (size_t i = 0; i < n; ++i) {
+            lo = std::min(lo, start[i]);
+            hi = std::max(hi, endEt[i]);
+        }
+        const double h = c.gridStep;
+        b->gridT0 = lo - 2.0 * h;
+        b->gridInvH = 1.0 / h;
+        b->gridCount = int(std::ceil((hi - b->gridT0) / h)) + 3;
+        for (int k = 0; k < b->gridCount; ++k) epochs.push_back(b->gridT0 + k * h);
+        totalSteps = int(launches);
     }
-    for (int g = 0; g < G; ++g) groupRecord[size_t(g)] /= kStages;  // in units of steps
     b->totalSteps = totalSteps;
-    // order: by group, then by altitude so drag lanes share tiles
+    // order: by group (grid: by start epoch), then by altitude so drag lanes share tiles
     std::vector<int> order(n);
     for (size_t i = 0; i < n; ++i) order[i] = int(i);
     std::vector<double> alt(n);
     for (size_t i = 0; i < n; ++i)
         alt[i] = std::sqrt(state6[i] * state6[i] + state6[n + i] * state6[n + i] + state6[2 * n + i] * state6[2 * n + i]);
-    if (c.sortAltitude)
+    if (grid) {
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return start[size_t(a)] < start[size_t(b2)]; });
+        for (size_t i = 0; i < n; ++i) grp[i] = 0;
+    } else if (c.sortAltitude) {
         std::stable_sort(order.begin(), order.end(), [&](int a, int b2) {
             if (grp[size_t(a)] != grp[size_t(b2)]) return grp[size_t(a)] < grp[size_t(b2)];
             return alt[size_t(a)] < alt[size_t(b2)];
         });
-    else
+    } else {
         std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return grp[size_t(a)] < grp[size_t(b2)]; });
+    }
     // tiles never mix groups
     std::vector<int> slotObj;  // slot -> object (-1 padding)
     std::vector<int> tileGroup;
@@ -548,6 +574,13 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
     a.fp64Degree = c.run.precision.mixed ? c.run.precision.fp64Degree : -1;
     a.nrlCoeff = c.nrl.p;
     a.counters = b.counters.p;
+    a.grid = b.grid ? 1 : 0;
+    a.gridCount = b.gridCount;
+    a.gridT0 = b.gridT0;
+    a.gridInvH = b.gridInvH;
+    a.weather = c.weather.p;
+    a.wDays = c.env.hasWeather ? c.env.weather.days : 0;
+    a.wStart = c.env.hasWeather ? c.env.weather.startMjdUtc : 0.0;
     const int W = c.fm.hasSh ? c.nWaves : 1;
     if (c.fm.hasSh) orderBlocks(c, W, a.blockStart);
     else for (int w = 0; w <= kMaxWaves; ++w) a.blockStart[w] = 0;
@@ -675,6 +708,11 @@ int phx_set_option(phx_ctx* c, const char* key, double v) {
     else if (k == "share_corr") c->shareCorr = v;
     else if (k == "share_drag") c->shareDrag = v;
     else if (k == "sort_altitude") c->sortAltitude = v != 0.0;
+    else if (k == "exact_groups_max") c->exactGroupsMax = int(v);
+    else if (k == "grid_step") {
+        if (!(v > 0 && v <= 600)) { c->error = "grid_step must be in (0, 600] s"; return PHX_EINVAL; }
+        c->gridStep = v;
+    } else if (k == "force_grid") c->forceGrid = v != 0.0;
     else { c->error = "unknown option " + k; return PHX_EINVAL; }
     return PHX_OK;
 }
@@ -688,6 +726,9 @@ double phx_get_option(const phx_ctx* c, const char* key) {
     if (k == "share_corr") return c->shareCorr;
     if (k == "share_drag") return c->shareDrag;
     if (k == "sort_altitude") return c->sortAltitude;
+    if (k == "exact_groups_max") return c->exactGroupsMax;
+    if (k == "grid_step") return c->gridStep;
+    if (k == "force_grid") return c->forceGrid;
     if (k == "step_seconds") return c->run.stepSeconds;
     if (k == "duration_seconds") return c->run.durationSeconds;
     if (k == "max_num_steps") return double(c->run.maxNumSteps);

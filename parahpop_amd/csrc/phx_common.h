@@ -103,6 +103,15 @@ struct StepArgs {
     int fp64Degree;              // mixed precision: degrees above this use FP32 terms (-1: all FP64)
     const double* nrlCoeff;      // NRLMSISE-00 coefficient blocks (device)
     long long* counters;         // [4] diagnostics: object-steps completed
+    // Uneven epochs: records on a uniform time grid t_k = gridT0 + k / gridInvH,
+    // interpolated per object (cubic Lagrange, rows k-1..k+2); NRLMSISE-00
+    // time inputs are then computed per object from the weather table.
+    int grid;
+    int gridCount;
+    double gridT0, gridInvH;
+    const double* weather;       // 13 x wDays SoA (CelesTrak components)
+    int wDays;
+    double wStart;
 };
 
 }  // namespace phx

@@ -38,6 +38,7 @@
 
 #include "phx_common.h"
 #include "phx_kernels.h"
+#include "phx_epoch_device.h"
 #include "phx_nrlmsise.h"
 
 namespace phx {
@@ -60,18 +61,46 @@ __constant__ double kRkB[13] = {0, 0, 0, 0, 0, 34. / 105, 9. / 35, 9. / 35, 9. /
 
 enum : uint8_t { ST_TERMINATED = 1, ST_EOS = 2, ST_PAD = 4 };
 
+// Epoch-record accessors: one record per stage epoch shared by the tile
+// (uniform loads), or four grid rows around the object's own stage epoch
+// combined with cubic Lagrange weights (uneven epochs).
+struct ExactRec {
+    const double* __restrict__ p;
+    __device__ double operator()(int f) const { return p[f]; }
+};
+struct GridRec {
+    const double* __restrict__ p;  // row k-1
+    double w0, w1, w2, w3;
+    __device__ double operator()(int f) const {
+        return w0 * p[f] + w1 * p[REC_SIZE + f] + w2 * p[2 * REC_SIZE + f] + w3 * p[3 * REC_SIZE + f];
+    }
+};
+__device__ inline GridRec gridRec(const double* table, int count, double t0, double invH, double ts) {
+    const double x = (ts - t0) * invH;
+    int k = int(floor(x));
+    k = k < 1 ? 1 : (k > count - 3 ? count - 3 : k);
+    const double f = x - double(k);
+    GridRec r;
+    r.p = table + size_t(k - 1) * REC_SIZE;
+    r.w0 = -f * (f - 1.0) * (f - 2.0) * (1.0 / 6.0);
+    r.w1 = (f + 1.0) * (f - 1.0) * (f - 2.0) * 0.5;
+    r.w2 = -(f + 1.0) * f * (f - 2.0) * 0.5;
+    r.w3 = (f + 1.0) * f * (f - 1.0) * (1.0 / 6.0);
+    return r;
+}
+
 // Per-lane geometry of the body-fixed position used by the SH evaluation.
 struct ShGeom {
     double sinPhi, cosPhi, sinL, cosL, alpha, r;
     double bx, by, bz;   // body-fixed position
 };
 
-__device__ inline ShGeom shGeometry(const double* __restrict__ rec, double px, double py, double pz, double radius) {
+template <class Rec>
+__device__ inline ShGeom shGeometry(const Rec& rec, double px, double py, double pz, double radius) {
     ShGeom g;
-    const double* R = rec + REC_R;
-    g.bx = R[0] * px + R[1] * py + R[2] * pz;
-    g.by = R[3] * px + R[4] * py + R[5] * pz;
-    g.bz = R[6] * px + R[7] * py + R[8] * pz;
+    g.bx = rec(REC_R + 0) * px + rec(REC_R + 1) * py + rec(REC_R + 2) * pz;
+    g.by = rec(REC_R + 3) * px + rec(REC_R + 4) * py + rec(REC_R + 5) * pz;
+    g.bz = rec(REC_R + 6) * px + rec(REC_R + 7) * py + rec(REC_R + 8) * pz;
     g.r = sqrt(g.bx * g.bx + g.by * g.by + g.bz * g.bz);
     const double rho = sqrt(g.bx * g.bx + g.by * g.by);
     g.sinPhi = g.bz / g.r;
@@ -177,7 +206,8 @@ __device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const doubl
     }
 }
 
-__device__ inline void shToInertial(const double* __restrict__ rec, const ShGeom& g, double ar, double aphi, double alam,
+template <class Rec>
+__device__ inline void shToInertial(const Rec& rec, const ShGeom& g, double ar, double aphi, double alam,
                                     double gm, double& ax, double& ay, double& az) {
     const double s = gm / (g.r * g.r);
     ar *= s;
@@ -186,10 +216,9 @@ __device__ inline void shToInertial(const double* __restrict__ rec, const ShGeom
     const double bx = g.cosPhi * g.cosL * ar - g.sinPhi * g.cosL * aphi - g.sinL * alam;
     const double by = g.cosPhi * g.sinL * ar - g.sinPhi * g.sinL * aphi + g.cosL * alam;
     const double bz = g.sinPhi * ar + g.cosPhi * aphi;
-    const double* R = rec + REC_R;
-    ax = R[0] * bx + R[3] * by + R[6] * bz;
-    ay = R[1] * bx + R[4] * by + R[7] * bz;
-    az = R[2] * bx + R[5] * by + R[8] * bz;
+    ax = rec(REC_R + 0) * bx + rec(REC_R + 3) * by + rec(REC_R + 6) * bz;
+    ay = rec(REC_R + 1) * bx + rec(REC_R + 4) * by + rec(REC_R + 7) * bz;
+    az = rec(REC_R + 2) * bx + rec(REC_R + 5) * by + rec(REC_R + 8) * bz;
 }
 
 __device__ inline double clamp1(double x, double lo, double hi) { return fmin(fmax(x, lo), hi); }
@@ -212,14 +241,15 @@ __device__ double shadowFactor(double sx, double sy, double sz, double bx, doubl
     return 1.0 - (a2 * al + bb * be - c * y) / (a2 * 3.14159265358979323846);
 }
 
-__device__ void srpAccel(const ForceModel& fm, const double* __restrict__ rec, double px, double py, double pz, double ams,
-                         double& ax, double& ay, double& az) {
-    const double* sun = rec + REC_BODY + 3 * fm.srpBody;
-    const double dx = px - sun[0], dy = py - sun[1], dz = pz - sun[2];
+template <class Rec>
+__device__ void srpAccel(const ForceModel& fm, const Rec& rec, double px, double py, double pz, double ams, double& ax,
+                         double& ay, double& az) {
+    const int sb = REC_BODY + 3 * fm.srpBody;
+    const double dx = px - rec(sb), dy = py - rec(sb + 1), dz = pz - rec(sb + 2);
     double occ = 1.0;
     for (int k = 0; k < fm.nOccluders; ++k) {
-        const double* o = rec + REC_BODY + 3 * fm.occBody[k];
-        occ *= shadowFactor(dx, dy, dz, px - o[0], py - o[1], pz - o[2], fm.sunRadius, fm.occRadius[k]);
+        const int ob = REC_BODY + 3 * fm.occBody[k];
+        occ *= shadowFactor(dx, dy, dz, px - rec(ob), py - rec(ob + 1), pz - rec(ob + 2), fm.sunRadius, fm.occRadius[k]);
     }
     const double s2 = dx * dx + dy * dy + dz * dz;
     const double k = occ * (fm.srpFactor * ams) / (s2 * sqrt(s2));
@@ -254,8 +284,10 @@ __device__ bool geodetic(double x, double y, double z, double radius, double fla
     return isfinite(lat) && isfinite(lon) && isfinite(alt);
 }
 
-__device__ __noinline__ double nrlDensity(const ForceModel& fm, const double* __restrict__ rec, const double* __restrict__ coeff,
+// nrl: the NRLMSISE-00 epoch inputs (record fields REC_NRL_* - REC_NRL).
+__device__ __noinline__ double nrlDensity(const ForceModel& fm, const double* nrl, const double* __restrict__ coeff,
                                           double fx, double fy, double fz) {
+    const double* rec = nrl - REC_NRL;
     double lat, lon, alt;
     if (!geodetic(fx, fy, fz, fm.earthRadius, fm.flattening, lat, lon, alt)) return nan("");
     if (alt > fm.hCutoff) return 0.0;
@@ -284,15 +316,15 @@ __device__ __noinline__ double nrlDensity(const ForceModel& fm, const double* __
     return (out.d[5] * 1.0e9) * fm.densityScale;
 }
 
-__device__ void dragAccel(const ForceModel& fm, const double* __restrict__ rec, const double* __restrict__ coeff, double px,
+template <class Rec>
+__device__ void dragAccel(const ForceModel& fm, const Rec& rec, const double* nrl, const double* __restrict__ coeff, double px,
                           double py, double pz, double vx, double vy, double vz, double bcd, double& ax, double& ay,
                           double& az) {
-    const double* R = rec + REC_R;
-    const double fx = R[0] * px + R[1] * py + R[2] * pz;
-    const double fy = R[3] * px + R[4] * py + R[5] * pz;
-    const double fz = R[6] * px + R[7] * py + R[8] * pz;
-    const double rho = nrlDensity(fm, rec, coeff, fx, fy, fz);
-    const double* om = rec + REC_OMEGA;
+    const double fx = rec(REC_R + 0) * px + rec(REC_R + 1) * py + rec(REC_R + 2) * pz;
+    const double fy = rec(REC_R + 3) * px + rec(REC_R + 4) * py + rec(REC_R + 5) * pz;
+    const double fz = rec(REC_R + 6) * px + rec(REC_R + 7) * py + rec(REC_R + 8) * pz;
+    const double rho = nrlDensity(fm, nrl, coeff, fx, fy, fz);
+    const double om[3] = {rec(REC_OMEGA), rec(REC_OMEGA + 1), rec(REC_OMEGA + 2)};
     const double rx = vx - (om[1] * pz - om[2] * py);
     const double ry = vy - (om[2] * px - om[0] * pz);
     const double rz = vz - (om[0] * py - om[1] * px);
@@ -305,12 +337,14 @@ __device__ void dragAccel(const ForceModel& fm, const double* __restrict__ rec, 
 // Tide gradient (degree <= 6, combined solid + ocean coefficients) and
 // Schwarzschild term, SI inside, km/s^2 out (HpopCorrections.h:339-376,
 // EarthCorrections.h:57-69).
-__device__ void correctionsAccel(const ForceModel& fm, const double* __restrict__ rec, double px, double py, double pz,
-                                 double vx, double vy, double vz, double& ax, double& ay, double& az) {
+template <class Rec>
+__device__ void correctionsAccel(const ForceModel& fm, const Rec& rec, double px, double py, double pz, double vx, double vy,
+                                 double vz, double& ax, double& ay, double& az) {
     double sx = 0.0, sy = 0.0, sz = 0.0;
     const double mu = fm.muEarthSI;
     if (fm.tides) {
-        const double* E = rec + REC_R;
+        double E[9];
+        for (int k = 0; k < 9; ++k) E[k] = rec(REC_R + k);
         const double x = 1000.0 * px, y = 1000.0 * py, z = 1000.0 * pz;
         const double r0 = E[0] * x + E[1] * y + E[2] * z;
         const double r1 = E[3] * x + E[4] * y + E[5] * z;
@@ -350,8 +384,7 @@ __device__ void correctionsAccel(const ForceModel& fm, const double* __restrict_
             cosM[m] = cosM[m - 1] * cl - sinM[m - 1] * sl;
             sinM[m] = sinM[m - 1] * cl + cosM[m - 1] * sl;
         }
-        const double* TC = rec + REC_TIDE_C;
-        const double* TS = rec + REC_TIDE_S;
+
         const double ratio = fm.radiusSI / d;
         double power = 1.0, dUdr = 0.0, dUdlat = 0.0, dUdlon = 0.0;
         for (int n = 0; n <= 6; ++n) {
@@ -359,7 +392,7 @@ __device__ void correctionsAccel(const ForceModel& fm, const double* __restrict_
             const double b2 = (mu / d) * power;
             double q1 = 0.0, q2 = 0.0, q3 = 0.0;
             for (int m = 0; m <= n; ++m) {
-                const double Cnm = TC[n * (n + 1) / 2 + m], Snm = TS[n * (n + 1) / 2 + m];
+                const double Cnm = rec(REC_TIDE_C + n * (n + 1) / 2 + m), Snm = rec(REC_TIDE_S + n * (n + 1) / 2 + m);
                 const double value = Cnm * cosM[m] + Snm * sinM[m];
                 q1 += p[n][m] * value;
                 q2 += dp[n][m] * value;
@@ -397,8 +430,9 @@ __device__ void correctionsAccel(const ForceModel& fm, const double* __restrict_
 
 // Central and third-body point gravity (PointGravity.h:16-81); the indirect
 // third-body term is an epoch-only sum taken from the record.
-__device__ void pointGravity(const ForceModel& fm, const double* __restrict__ rec, double px, double py, double pz, double& ax,
-                             double& ay, double& az) {
+template <class Rec>
+__device__ void pointGravity(const ForceModel& fm, const Rec& rec, double px, double py, double pz, double& ax, double& ay,
+                             double& az) {
     ax = ay = az = 0.0;
     if (fm.earthPointGravity) {
         const double s2 = px * px + py * py + pz * pz;
@@ -409,8 +443,8 @@ __device__ void pointGravity(const ForceModel& fm, const double* __restrict__ re
     }
     for (int b = 0; b < fm.nBodies; ++b) {
         if (!fm.bodyPointGravity[b] || fm.bodyNaif[b] == 399) continue;
-        const double* q = rec + REC_BODY + 3 * b;
-        const double dx = px - q[0], dy = py - q[1], dz = pz - q[2];
+        const int q = REC_BODY + 3 * b;
+        const double dx = px - rec(q), dy = py - rec(q + 1), dz = pz - rec(q + 2);
         const double s2 = dx * dx + dy * dy + dz * dz;
         const double rc = 1.0 / (s2 * sqrt(s2));
         const double g = fm.bodyGm[b];
@@ -418,19 +452,81 @@ __device__ void pointGravity(const ForceModel& fm, const double* __restrict__ re
         ay += (-g * dy) * rc;
         az += (-g * dz) * rc;
     }
-    ax += rec[REC_INDIRECT + 0];
-    ay += rec[REC_INDIRECT + 1];
-    az += rec[REC_INDIRECT + 2];
+    ax += rec(REC_INDIRECT + 0);
+    ay += rec(REC_INDIRECT + 1);
+    az += rec(REC_INDIRECT + 2);
 }
 
 // Which wave carries which extra force family for a W-wave tile.
 __device__ constexpr int dragWave(int W) { return W - 1; }
 __device__ constexpr int corrWave(int W) { return W >= 3 ? W - 2 : W - 1; }
 
-template <int W>
+__constant__ double kRkC[12] = {2. / 27, 1. / 9, 1. / 6, 5. / 12, 1. / 2, 5. / 6, 1. / 6, 2. / 3, 1. / 3, 1.0, 0, 1.0};
+
+// One stage's force evaluation for the tile: SH order block of this wave,
+// the extra families assigned to this wave, then (wave 0) the sum.
+template <int W, class Rec>
+__device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, const Rec& rec, const double* nrl, int wave,
+                                   int lane, int m0, int m1, double ams, double bcd, double (*xch)[kTile],
+                                   double (*part)[3][kTile], double (*extra)[3][kTile], double out[3]) {
+    const double px = xch[0][lane], py = xch[1][lane], pz = xch[2][lane];
+    ShGeom g;
+    if (fm.hasSh) {
+        double sr = 0.0, sp = 0.0, sl = 0.0;
+        g = shGeometry(rec, px, py, pz, fm.shRadius);
+        if (m0 < m1) shOrderBlock(g, m0, m1, fm.shDegree, a.shTerms, a.shOrderStart, a.shDiag, fm.shNormalized, sr, sp, sl);
+        part[wave][0][lane] = sr;
+        part[wave][1][lane] = sp;
+        part[wave][2][lane] = sl;
+    }
+    if (wave == dragWave(W)) {
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        if (fm.drag) dragAccel(fm, rec, nrl, a.nrlCoeff, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], bcd, ax, ay, az);
+        extra[0][0][lane] = ax;
+        extra[0][1][lane] = ay;
+        extra[0][2][lane] = az;
+    }
+    if (wave == corrWave(W)) {
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        if (fm.tides || fm.relativity) correctionsAccel(fm, rec, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], ax, ay, az);
+        if (fm.srp) {
+            double bx, by, bz;
+            srpAccel(fm, rec, px, py, pz, ams, bx, by, bz);
+            ax += bx;
+            ay += by;
+            az += bz;
+        }
+        extra[1][0][lane] = ax;
+        extra[1][1][lane] = ay;
+        extra[1][2][lane] = az;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        double ax, ay, az;
+        pointGravity(fm, rec, px, py, pz, ax, ay, az);
+        if (fm.hasSh) {
+            double r = 0.0, p = 0.0, l = 0.0;
+            for (int w = 0; w < W; ++w) {
+                r += part[w][0][lane];
+                p += part[w][1][lane];
+                l += part[w][2][lane];
+            }
+            double hx, hy, hz;
+            shToInertial(rec, g, r, p, l, fm.shGm, hx, hy, hz);
+            ax += hx;
+            ay += hy;
+            az += hz;
+        }
+        out[0] = ax + (extra[1][0][lane] + extra[0][0][lane]);
+        out[1] = ay + (extra[1][1][lane] + extra[0][1][lane]);
+        out[2] = az + (extra[1][2][lane] + extra[0][2][lane]);
+    }
+}
+
+template <int W, bool GRID>
 __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm) {
     __shared__ double kst[kStages][6][kTile];
-    __shared__ double xch[6][kTile];
+    __shared__ double xch[7][kTile];          // staged position, velocity, stage epoch
     __shared__ double part[W][3][kTile];
     __shared__ double extra[2][3][kTile];
     __shared__ int anyActive;
@@ -442,21 +538,25 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
     const int nPad = a.nPad;
 
     double y[6];
-    double dt = 0.0;
+    double dt = 0.0, cur = 0.0;
     uint8_t st = ST_PAD;
-    double ams = 0.0, bcd = 0.0;
     if (wave == 0) {
         for (int d = 0; d < 6; ++d) y[d] = a.y[d * nPad + obj];
         st = a.status[obj];
         dt = a.nextDt[obj];
+        cur = a.cur[obj];
     } else {
         for (int d = 0; d < 6; ++d) y[d] = 0.0;
     }
-    ams = a.ams[obj];
-    bcd = a.bcd[obj];
+    const double ams = a.ams[obj];
+    const double bcd = a.bcd[obj];
     long long completed = 0;
-    const long long recBase = a.tileRecord[tile];
+    const long long recBase = GRID ? 0 : a.tileRecord[tile];
     const int m0 = a.blockStart[wave], m1 = a.blockStart[wave + 1];
+    EpochArgs wa{};
+    wa.weather = a.weather;
+    wa.wDays = a.wDays;
+    wa.wStart = a.wStart;
 
     for (int step = 0; step < a.steps; ++step) {
         if (wave == 0) {
@@ -466,9 +566,8 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
         }
         __syncthreads();
         if (!anyActive) break;
-        const double* recStep = a.records + (recBase + (long long)(a.stepBase + step) * kStages) * REC_SIZE;
+        const double* recStep = GRID ? a.records : a.records + (recBase + (long long)(a.stepBase + step) * kStages) * REC_SIZE;
         for (int s = 0; s < kStages; ++s) {
-            const double* rec = recStep + s * REC_SIZE;
             if (wave == 0) {
                 double ys[6];
                 for (int d = 0; d < 6; ++d) ys[d] = y[d];
@@ -482,69 +581,30 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
                     }
                 }
                 for (int d = 0; d < 6; ++d) xch[d][lane] = ys[d];
+                double ts = cur;
+                if (s >= 1) ts += dt * kRkC[s - 1];
+                xch[6][lane] = ts;
             }
             __syncthreads();
-            const double px = xch[0][lane], py = xch[1][lane], pz = xch[2][lane];
-            // spherical harmonics, this wave's order block
-            ShGeom g;
-            double sr = 0.0, sp = 0.0, sl = 0.0;
-            if (fm.hasSh) {
-                g = shGeometry(rec, px, py, pz, fm.shRadius);
-                if (m0 < m1)
-                    shOrderBlock(g, m0, m1, fm.shDegree, a.shTerms, a.shOrderStart, a.shDiag, fm.shNormalized, sr, sp, sl);
-                part[wave][0][lane] = sr;
-                part[wave][1][lane] = sp;
-                part[wave][2][lane] = sl;
+            double acc[3];
+            if (GRID) {
+                const double ts = xch[6][lane];
+                const GridRec rec = gridRec(a.records, a.gridCount, a.gridT0, a.gridInvH, ts);
+                double nrl[13];
+                if (fm.drag && wave == dragWave(W)) epochdev::nrlEpoch(wa, ts, nrl);
+                stageForces<W>(a, fm, rec, nrl, wave, lane, m0, m1, ams, bcd, xch, part, extra, acc);
+            } else {
+                const ExactRec rec{recStep + s * REC_SIZE};
+                stageForces<W>(a, fm, rec, recStep + s * REC_SIZE + REC_NRL, wave, lane, m0, m1, ams, bcd, xch, part, extra,
+                               acc);
             }
-            // extra force families
-            if (wave == dragWave(W)) {
-                double ax = 0.0, ay = 0.0, az = 0.0;
-                if (fm.drag) dragAccel(fm, rec, a.nrlCoeff, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], bcd, ax, ay, az);
-                extra[0][0][lane] = ax;
-                extra[0][1][lane] = ay;
-                extra[0][2][lane] = az;
-            }
-            if (wave == corrWave(W)) {
-                double ax = 0.0, ay = 0.0, az = 0.0;
-                if (fm.tides || fm.relativity)
-                    correctionsAccel(fm, rec, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], ax, ay, az);
-                if (fm.srp) {
-                    double bx, by, bz;
-                    srpAccel(fm, rec, px, py, pz, ams, bx, by, bz);
-                    ax += bx;
-                    ay += by;
-                    az += bz;
-                }
-                extra[1][0][lane] = ax;
-                extra[1][1][lane] = ay;
-                extra[1][2][lane] = az;
-            }
-            __syncthreads();
             if (wave == 0) {
-                double ax, ay, az;
-                pointGravity(fm, rec, px, py, pz, ax, ay, az);
-                if (fm.hasSh) {
-                    double r = 0.0, p = 0.0, l = 0.0;
-                    for (int w = 0; w < W; ++w) {
-                        r += part[w][0][lane];
-                        p += part[w][1][lane];
-                        l += part[w][2][lane];
-                    }
-                    double hx, hy, hz;
-                    shToInertial(rec, g, r, p, l, fm.shGm, hx, hy, hz);
-                    ax += hx;
-                    ay += hy;
-                    az += hz;
-                }
-                ax += extra[1][0][lane] + extra[0][0][lane];
-                ay += extra[1][1][lane] + extra[0][1][lane];
-                az += extra[1][2][lane] + extra[0][2][lane];
                 kst[s][0][lane] = xch[3][lane];
                 kst[s][1][lane] = xch[4][lane];
                 kst[s][2][lane] = xch[5][lane];
-                kst[s][3][lane] = ax;
-                kst[s][4][lane] = ay;
-                kst[s][5][lane] = az;
+                kst[s][3][lane] = acc[0];
+                kst[s][4][lane] = acc[1];
+                kst[s][5][lane] = acc[2];
             }
         }
         if (wave == 0) {
@@ -558,8 +618,7 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
                     }
                 }
                 // FixedStep.h:30-37
-                const double cur = a.cur[obj] + dt;
-                a.cur[obj] = cur;
+                cur = cur + dt;
                 const double remaining = a.end[obj] - cur;
                 const bool finished = remaining <= 0.0;
                 st = finished ? (ST_TERMINATED | ST_EOS) : 0;
@@ -572,6 +631,7 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
         for (int d = 0; d < 6; ++d) a.y[d * nPad + obj] = y[d];
         a.status[obj] = st;
         a.nextDt[obj] = dt;
+        a.cur[obj] = cur;
         // object-steps completed in this launch, one atomic per wave
         unsigned long long c = (unsigned long long)completed;
         for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
@@ -580,13 +640,14 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
 }
 
 __global__ void __launch_bounds__(64) forceProbeKernel(int n, const double* pos, const double* vel, const double* ams,
-                                                       const double* bcd, const double* rec, int which, ForceModel fm,
+                                                       const double* bcd, const double* recp, int which, ForceModel fm,
                                                        const double* shTerms, const int* shOrderStart, const double* shDiag,
                                                        const double* nrlCoeff, double* acc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double px = pos[i], py = pos[n + i], pz = pos[2 * n + i];
     const double vx = vel[i], vy = vel[n + i], vz = vel[2 * n + i];
+    const ExactRec rec{recp};
     double ax = 0.0, ay = 0.0, az = 0.0, bx, by, bz;
     if (which < 0 || which == 0 || which == 2) {
         ForceModel f = fm;
@@ -594,9 +655,9 @@ __global__ void __launch_bounds__(64) forceProbeKernel(int n, const double* pos,
         if (which == 2) f.earthPointGravity = 0;  // third bodies only
         pointGravity(f, rec, px, py, pz, bx, by, bz);
         if (which == 0) {
-            bx -= rec[REC_INDIRECT];
-            by -= rec[REC_INDIRECT + 1];
-            bz -= rec[REC_INDIRECT + 2];
+            bx -= rec(REC_INDIRECT);
+            by -= rec(REC_INDIRECT + 1);
+            bz -= rec(REC_INDIRECT + 2);
         }
         ax += bx, ay += by, az += bz;
     }
@@ -612,7 +673,7 @@ __global__ void __launch_bounds__(64) forceProbeKernel(int n, const double* pos,
         ax += bx, ay += by, az += bz;
     }
     if ((which < 0 || which == 4) && fm.drag) {
-        dragAccel(fm, rec, nrlCoeff, px, py, pz, vx, vy, vz, bcd[i], bx, by, bz);
+        dragAccel(fm, rec, recp + REC_NRL, nrlCoeff, px, py, pz, vx, vy, vz, bcd[i], bx, by, bz);
         ax += bx, ay += by, az += bz;
     }
     if ((which < 0 || which == 5) && (fm.tides || fm.relativity)) {
@@ -657,12 +718,22 @@ __global__ void __launch_bounds__(64) nrlProbeKernel(int n, const double* in, in
 int launchStep(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t stream) {
     if (a.nTiles <= 0) return 0;
     const dim3 grid(a.nTiles);
-    switch (waves) {
-        case 1: hipLaunchKernelGGL(rkf78Kernel<1>, grid, dim3(64), 0, stream, a, fm); break;
-        case 2: hipLaunchKernelGGL(rkf78Kernel<2>, grid, dim3(128), 0, stream, a, fm); break;
-        case 4: hipLaunchKernelGGL(rkf78Kernel<4>, grid, dim3(256), 0, stream, a, fm); break;
-        case 8: hipLaunchKernelGGL(rkf78Kernel<8>, grid, dim3(512), 0, stream, a, fm); break;
-        default: return -2;
+    if (a.grid) {
+        switch (waves) {
+            case 1: hipLaunchKernelGGL((rkf78Kernel<1, true>), grid, dim3(64), 0, stream, a, fm); break;
+            case 2: hipLaunchKernelGGL((rkf78Kernel<2, true>), grid, dim3(128), 0, stream, a, fm); break;
+            case 4: hipLaunchKernelGGL((rkf78Kernel<4, true>), grid, dim3(256), 0, stream, a, fm); break;
+            case 8: hipLaunchKernelGGL((rkf78Kernel<8, true>), grid, dim3(512), 0, stream, a, fm); break;
+            default: return -2;
+        }
+    } else {
+        switch (waves) {
+            case 1: hipLaunchKernelGGL((rkf78Kernel<1, false>), grid, dim3(64), 0, stream, a, fm); break;
+            case 2: hipLaunchKernelGGL((rkf78Kernel<2, false>), grid, dim3(128), 0, stream, a, fm); break;
+            case 4: hipLaunchKernelGGL((rkf78Kernel<4, false>), grid, dim3(256), 0, stream, a, fm); break;
+            case 8: hipLaunchKernelGGL((rkf78Kernel<8, false>), grid, dim3(512), 0, stream, a, fm); break;
+            default: return -2;
+        }
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

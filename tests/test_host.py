@@ -1,0 +1,126 @@
+"""Host-side checks that run without a GPU: the C-ABI library loads and exports
+every entry point include/phx.h declares, configuration validation mirrors the
+reference's rejections, the CLI keeps the reference's process contract, and the
+data loaders read the reference formats."""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "phx.h")).read()
+    return sorted(set(re.findall(r"\b(phx_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol(gpu_lib):
+    names = _declared()
+    assert len(names) >= 18
+    missing = [n for n in names if not hasattr(gpu_lib, n)]
+    assert not missing, missing
+
+
+def test_package_lists_exports():
+    import parahpop_amd
+    assert sorted(parahpop_amd.EXPORTED) == _declared()
+
+
+def _write_cfg(tmp, model, **ex):
+    from parahpop_amd import synth
+    with open(os.path.join(tmp, "constants.json"), "w") as f:
+        json.dump(synth.CONSTANTS, f)
+    cfg = {"run": {"duration_seconds": 600.0},
+           "execution": {"maxThreads": 1, "integration": {"initialStepSize": 60.0, "maxNumSteps": 100}},
+           "model": model}
+    cfg["execution"].update(ex)
+    p = os.path.join(tmp, "CONFIG.json")
+    with open(p, "w") as f:
+        json.dump(cfg, f)
+    return p
+
+
+def _create_error(path):
+    import ctypes
+    import parahpop_amd
+    lib = parahpop_amd.library()
+    h = ctypes.c_void_p()
+    rc = lib.phx_create(path.encode(), ctypes.byref(h))
+    if rc == 0:
+        lib.phx_destroy(h)
+        return rc, ""
+    return rc, lib.phx_last_error(None).decode()
+
+
+@pytest.mark.parametrize("model,expect", [
+    ({"environment": {"bodies": {"Earth": {"gravity": True}}, "constants": "constants.json", "bogus": 1}}, "Unsupported model.environment option"),
+    ({"environment": {"bodies": {"Sun": {"gravity": True}}, "constants": "constants.json"}}, "Earth must be configured"),
+    ({"environment": {"bodies": {"Earth": {"gravity": {"shape": {"file": "missing.tab"}}}}, "constants": "constants.json"}}, "Required data file not found"),
+    ({"environment": {"bodies": {"Earth": {"gravity": True}}, "ephemeris": ["de440s.brie"], "constants": "constants.json"}}, "analytic"),
+    ({"environment": {"bodies": {"Earth": {"gravity": True}, "Moon": {"radiation": True}}, "constants": "constants.json"}}, "Radiation is supported for the Sun only"),
+    ({"environment": {"bodies": {"Earth": {"gravity": True}, "Vulcan": {"gravity": True}}, "constants": "constants.json"}}, "Unsupported body"),
+    ({"environment": {"bodies": {"Earth": {"gravity": True}}, "constants": "constants.json",
+                      "earthCorrections": {"model": "iers2010_fes2004", "solidEarthTides": True}}}, "eopFile"),
+])
+def test_config_rejections(tmp_path, model, expect):
+    rc, msg = _create_error(_write_cfg(str(tmp_path), model))
+    assert rc == -1 and expect in msg, msg
+
+
+def test_execution_rejections(tmp_path):
+    model = {"environment": {"bodies": {"Earth": {"gravity": True}}, "constants": "constants.json"}}
+    p = _write_cfg(str(tmp_path), model)
+    cfg = json.load(open(p))
+    for bad, expect in [({"maxNumSteps": 0}, "maxNumSteps and maxThreads"), ({"initialStepSize": -1.0}, "initialStepSize"),
+                        ({"absTol": 1e-9}, "Unsupported integration option")]:
+        c = json.loads(json.dumps(cfg))
+        c["execution"]["integration"].update(bad)
+        json.dump(c, open(p, "w"))
+        rc, msg = _create_error(p)
+        assert rc == -1 and expect in msg, msg
+    c = json.loads(json.dumps(cfg))
+    c["execution"]["maxThreads"] = -1
+    json.dump(c, open(p, "w"))
+    rc, msg = _create_error(p)
+    assert rc == -1 and "maxNumSteps and maxThreads" in msg
+
+
+def test_cli_contract(tmp_path):
+    exe = os.path.join(ROOT, "parahpop_amd", "phx_gpu")
+    out = subprocess.run([exe, "--help"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    assert out.returncode == 0 and "CONFIG.json TARGETS.json OUTPUT_DIRECTORY" in out.stdout
+    out = subprocess.run([exe, "a"], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    assert out.returncode == 1 and out.stderr.startswith("ParaHPOP: Expected CONFIG.json")
+    d = tmp_path / "out"
+    d.mkdir()
+    (d / "summary.json").write_text("{}")
+    out = subprocess.run([exe, "c.json", "t.json", str(d)], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    assert out.returncode == 1 and "Output exists" in out.stderr
+
+
+def test_loaders_read_reference_formats(workdir):
+    """The oracle context parses the synthetic gravity, SW-All and EOP files the
+    generator writes in the reference formats (shared host loader)."""
+    import oracle
+    from parahpop_amd import synth
+    cfg, _ = synth.make_workload(str(workdir / "loaders"), 2, n=4, duration=600.0)
+    o = oracle.Oracle(cfg)
+    assert o.h
+    tab = open(os.path.join(os.path.dirname(cfg), "gravity_70.tab")).read().splitlines()
+    assert len(tab) == 1 + sum(n + 1 for n in range(1, 71))
+    assert float(tab[3].split(",")[2]) == -4.8417e-4  # (2, 0) row
+
+
+def test_population_distributions():
+    from parahpop_amd import synth
+    pop = synth.population(20000, 2, mix=(0.8, 0.2, 0.0))
+    r = np.linalg.norm(pop["state"][:3], axis=0)
+    leo = pop["kind"] == 0
+    assert abs(leo.mean() - 0.8) < 0.02
+    assert np.all(r[leo] > synth.R_EARTH + 150) and np.all(r[leo] < synth.R_EARTH + 2100)
+    am = pop["drag_area"] / pop["mass"]
+    assert abs(np.median(np.log(am)) - np.log(0.01)) < 0.05
