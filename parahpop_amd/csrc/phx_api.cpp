@@ -240,6 +240,11 @@ void buildShTables(phx_ctx& c) {
         }
     }
     start[size_t(L) + 1] = k;
+    // one zero term of padding: the pipelined loop reads one term ahead
+    for (int p = 0; p < SHT_STRIDE; ++p) {
+        terms.push_back(0.0);
+        terms32.push_back(0.0f);
+    }
     c.shTerms.upload(terms.data(), terms.size(), c.stream);
     c.shTerms32.upload(terms32.data(), terms32.size(), c.stream);
     c.shOrderStart.upload(start.data(), start.size(), c.stream);
@@ -311,6 +316,7 @@ EpochArgs epochArgs(const phx_ctx& c) {
     a.waves = c.waves.p;
     a.waveCount = c.waveCount;
     a.terms = c.terms.p;
+    a.nrlCoeff = c.nrl.p;
     a.weather = c.weather.p;
     a.wDays = c.env.hasWeather ? c.env.weather.days : 0;
     a.wStart = c.env.hasWeather ? c.env.weather.startMjdUtc : 0.0;
@@ -321,11 +327,10 @@ EpochArgs epochArgs(const phx_ctx& c) {
 void orderBlocks(const phx_ctx& c, int W, int out[kMaxWaves + 1]) {
     const int L = c.fm.shDegree, M = c.fm.shOrder;
     std::vector<double> share(static_cast<size_t>(W), 1.0);
-    const int dragW = W - 1, corrW = W >= 3 ? W - 2 : W - 1;
+    const int corrW = W >= 3 ? W - 2 : W - 1;
     if (W > 1) {
         share[0] *= c.share0;
         if (c.fm.tides || c.fm.relativity || c.fm.srp) share[size_t(corrW)] *= c.shareCorr;
-        if (c.fm.drag) share[size_t(dragW)] *= c.shareDrag;
     }
     double total = 0.0;
     for (int m = 0; m <= M; ++m) total += double(L - m + 1) + 0.6 * m / std::max(1, W);
@@ -421,7 +426,7 @@ std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, c
         for (auto& kv : groupOf) gkey[size_t(kv.second)] = kv.first;
         // per-group step sequences; record epochs are the stage epochs themselves
         for (int g = 0; g < G; ++g) {
-            groupRecord[size_t(g)] = (long long)(epochs.size() / kStages);
+            groupRecord[size_t(g)] = (long long)epochs.size();  // record index of (step 0, stage 0)
             double cur = gkey[size_t(g)].first;
             const double end = gkey[size_t(g)].second;
             double dt = std::min(step, end - cur);
@@ -661,6 +666,7 @@ int phx_create(const char* config_json_path, phx_ctx** out) {
         check(hipEventCreate(&c->ev0), "hipEventCreate");
         check(hipEventCreate(&c->ev1), "hipEventCreate");
         buildForceModel(*c);
+        if (uploadTideFactors() != 0) throw HipError("tide factor upload failed");
         loadTables(*c);
         if (c->env.hasSh) buildShTables(*c);
         check(hipStreamSynchronize(c->stream), "table upload");

@@ -28,8 +28,9 @@ enum EpochRecord : int {
     REC_NRL_F107A = REC_NRL + 4,
     REC_NRL_AP = REC_NRL + 5,     // 7 entries (daily Ap, then the 3-hour history)
     REC_NRL_VALID = REC_NRL + 12,
-    REC_USED = REC_NRL + 13,
-    REC_SIZE = 128
+    REC_NRL_GLOBE = REC_NRL + 13, // epoch-only globe7 terms of the 11 tables (nrlmsise::GE_COUNT)
+    REC_USED = REC_NRL_GLOBE + 199,
+    REC_SIZE = 320
 };
 static_assert(REC_USED <= REC_SIZE, "epoch record overflow");
 
@@ -91,7 +92,7 @@ struct StepArgs {
     const double* ams;           // [nPad] cr*area/mass (km^2/kg)
     const double* bcd;           // [nPad] cd*area/mass
     const double* records;       // epoch records
-    const long long* tileRecord; // [nTiles] record index of this tile's step 0, stage 0
+    const long long* tileRecord; // [nTiles] record index of this tile's (step 0, stage 0)
     int stepBase;                // steps completed before this launch
     int steps;                   // steps to run in this launch
     double step;                 // nominal step (s)

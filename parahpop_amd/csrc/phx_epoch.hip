@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(64) epochRecordKernel(EpochArgs a, ForceModel 
                 rec[REC_TIDE_S + n * (n + 1) / 2 + m] = S[n][m];
             }
     }
-    if (fm.drag) nrlEpoch(a, et, rec + REC_NRL);
+    if (fm.drag) nrlEpoch(a, et, rec + REC_NRL, rec + REC_NRL_GLOBE);
 }
 
 }  // namespace

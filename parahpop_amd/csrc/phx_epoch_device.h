@@ -18,6 +18,7 @@
 
 #include "phx_common.h"
 #include "phx_kernels.h"
+#include "phx_nrlmsise.h"
 
 namespace phx {
 namespace epochdev {
@@ -398,8 +399,9 @@ __device__ double gastHours(double mjdUtc) {
     return wrapHours((gr + dpsi / 3600.0 * kDeg * cos(eps)) * 24.0 / kTwoPi);
 }
 
-// Writes the NRLMSISE-00 epoch inputs at dst[REC_NRL_* - REC_NRL].
-__device__ void nrlEpoch(const EpochArgs& a, double et, double* dst) {
+// Writes the NRLMSISE-00 epoch inputs at dst[REC_NRL_* - REC_NRL] and, when
+// globe is given, the epoch-only globe7 terms (nrlmsise::GE_COUNT values).
+__device__ void nrlEpoch(const EpochArgs& a, double et, double* dst, double* globe) {
     double* rec = dst - REC_NRL;
     const double mjd = etToMjdUtcNrl(et);
     // calendar date -> day of year and seconds of day (Nrlmsise00.cu:113-140)
@@ -456,6 +458,10 @@ __device__ void nrlEpoch(const EpochArgs& a, double et, double* dst) {
     rec[REC_NRL_AP + 5] = s5 / 8.0;
     rec[REC_NRL_AP + 6] = s6 / 8.0;
     rec[REC_NRL_VALID] = 1.0;
+    if (globe && a.nrlCoeff) {
+        nrlmsise::NrlTables T{a.nrlCoeff};
+        nrlmsise::globeEpoch(T, doy, rec[REC_NRL_SEC], rec[REC_NRL_F107], rec[REC_NRL_F107A], rec + REC_NRL_AP, globe);
+    }
 }
 
 

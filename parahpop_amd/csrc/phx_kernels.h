@@ -23,13 +23,17 @@ struct EpochArgs {
     const double* waves;    // rows of 8: 6 Doodson multipliers, first term, end term
     int waveCount;
     const double* terms;    // rows of 6: n, m, Cp, Sp, Cm, Sm
-    // NRLMSISE-00 weather (13 x days SoA)
+    // NRLMSISE-00 weather (13 x days SoA) and coefficient blocks
+    const double* nrlCoeff;
     const double* weather;
     int wDays;
     double wStart;
 };
 
 int launchEpochRecords(const EpochArgs& a, const ForceModel& fm, hipStream_t stream);
+
+// Uploads the degree-6 Legendre recursion factors used by the tide gradient.
+int uploadTideFactors();
 
 // RKF78 step kernel: `waves` waves per 64-object tile (1, 2, 4 or 8).
 int launchStep(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t stream);

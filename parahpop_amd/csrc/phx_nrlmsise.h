@@ -52,6 +52,7 @@ struct State {
     double plg[4][9] = {};
     double ctloc = 0, stloc = 0, c2tloc = 0, s2tloc = 0, s3tloc = 0, c3tloc = 0;
     double apdf = 0, apt[4] = {};
+    double cglong = 0, sglong = 0;   // cos/sin(dgtr * g_long), fast path
     bool g0Valid = false;
     double g0P24 = 0, g0P25 = 0, g0Cache[6] = {};
 };
@@ -523,7 +524,8 @@ __device__ inline double glob7s(State& st, const double* p, const Input& in, con
 }
 
 // :1203-1498
-__device__ inline void gts7(State& st, const NrlTables& T, Input& in, const Flags& fl, Output& out) {
+template <class Globe>
+__device__ inline void gts7(State& st, const NrlTables& T, Input& in, const Flags& fl, Output& out, const Globe& G) {
     double zn1[5] = {120.0, 110.0, 100.0, 90.0, 72.5};
     const int mn1 = 5;
     const double dgtr = 1.74533E-2, dr = 1.72142E-2;
@@ -533,15 +535,15 @@ __device__ inline void gts7(State& st, const NrlTables& T, Input& in, const Flag
     zn1[0] = za;
     for (int j = 0; j < 9; ++j) out.d[j] = 0;
     st.g0Valid = false;
-    prepareGlobeBasis(st, in, fl);
+    G.basis(st, in, fl);
     double tinf;
-    if (in.alt > zn1[0]) tinf = (T.base+OFF_PTM)[0] * (T.base+OFF_PT)[0] * (1.0 + fl.sw[16] * globe7(st, (T.base+OFF_PT), in, fl));
+    if (in.alt > zn1[0]) tinf = (T.base+OFF_PTM)[0] * (T.base+OFF_PT)[0] * (1.0 + fl.sw[16] * G(st, 0, (T.base+OFF_PT), in, fl));
     else tinf = (T.base+OFF_PTM)[0] * (T.base+OFF_PT)[0];
     out.t[0] = tinf;
     double g0v;
-    if (in.alt > zn1[4]) g0v = (T.base+OFF_PTM)[3] * (T.base+OFF_PS)[0] * (1.0 + fl.sw[19] * globe7(st, (T.base+OFF_PS), in, fl));
+    if (in.alt > zn1[4]) g0v = (T.base+OFF_PTM)[3] * (T.base+OFF_PS)[0] * (1.0 + fl.sw[19] * G(st, 1, (T.base+OFF_PS), in, fl));
     else g0v = (T.base+OFF_PTM)[3] * (T.base+OFF_PS)[0];
-    const double tlb = (T.base+OFF_PTM)[1] * (1.0 + fl.sw[17] * globe7(st, (T.base+OFF_PD+150*(3)), in, fl)) * (T.base+OFF_PD+150*(3))[0];
+    const double tlb = (T.base+OFF_PTM)[1] * (1.0 + fl.sw[17] * G(st, 5, (T.base+OFF_PD+150*(3)), in, fl)) * (T.base+OFF_PD+150*(3))[0];
     const double s = g0v / (tinf - tlb);
     if (in.alt < 300.0) {
         st.meso_tn1[1] = (T.base+OFF_PTM)[6] * (T.base+OFF_PTL+100*(0))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(0)), in, fl));
@@ -557,7 +559,7 @@ __device__ inline void gts7(State& st, const NrlTables& T, Input& in, const Flag
         st.meso_tn1[4] = (T.base+OFF_PTM)[4] * (T.base+OFF_PTL+100*(3))[0];
         st.meso_tgn1[1] = (T.base+OFF_PTM)[8] * (T.base+OFF_PMA+100*(8))[0] * st.meso_tn1[4] * st.meso_tn1[4] / (pow(((T.base+OFF_PTM)[4] * (T.base+OFF_PTL+100*(3))[0]), 2.0));
     }
-    const double g28 = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(2)), in, fl);
+    const double g28 = fl.sw[21] * G(st, 4, (T.base+OFF_PD+150*(2)), in, fl);
     const double zhf = (T.base+OFF_PDL+25*(1))[24] * (1.0 + fl.sw[5] * (T.base+OFF_PDL+25*(0))[24] * sin(dgtr * in.g_lat) * cos(dr * (in.doy - (T.base+OFF_PT)[13])));
     out.t[0] = tinf;
     const double xmm = (T.base+OFF_PDM+10*(2))[4];
@@ -581,7 +583,7 @@ __device__ inline void gts7(State& st, const NrlTables& T, Input& in, const Flag
     }
 
     // He
-    const double g4 = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(0)), in, fl);
+    const double g4 = fl.sw[21] * G(st, 2, (T.base+OFF_PD+150*(0)), in, fl);
     const double db04 = (T.base+OFF_PDM+10*(0))[0] * exp(g4) * (T.base+OFF_PD+150*(0))[0];
     out.d[0] = densu(st, z, db04, tinf, tlb, 4., alpha[0], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
     if ((fl.sw[15]) && (z < altl[0])) {
@@ -596,7 +598,7 @@ __device__ inline void gts7(State& st, const NrlTables& T, Input& in, const Flag
     }
 
     // O
-    const double g16 = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(1)), in, fl);
+    const double g16 = fl.sw[21] * G(st, 3, (T.base+OFF_PD+150*(1)), in, fl);
     const double db16 = (T.base+OFF_PDM+10*(1))[0] * exp(g16) * (T.base+OFF_PD+150*(1))[0];
     out.d[1] = densu(st, z, db16, tinf, tlb, 16., alpha[1], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
     if ((fl.sw[15]) && (z <= altl[1])) {
@@ -616,7 +618,7 @@ __device__ inline void gts7(State& st, const NrlTables& T, Input& in, const Flag
     }
 
     // O2
-    const double g32 = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(4)), in, fl);
+    const double g32 = fl.sw[21] * G(st, 6, (T.base+OFF_PD+150*(4)), in, fl);
     const double db32 = (T.base+OFF_PDM+10*(3))[0] * exp(g32) * (T.base+OFF_PD+150*(4))[0];
     out.d[3] = densu(st, z, db32, tinf, tlb, 32., alpha[3], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
     if (fl.sw[15]) {
@@ -638,7 +640,7 @@ __device__ inline void gts7(State& st, const NrlTables& T, Input& in, const Flag
     }
 
     // Ar
-    const double g40 = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(5)), in, fl);
+    const double g40 = fl.sw[21] * G(st, 7, (T.base+OFF_PD+150*(5)), in, fl);
     const double db40 = (T.base+OFF_PDM+10*(4))[0] * exp(g40) * (T.base+OFF_PD+150*(5))[0];
     out.d[4] = densu(st, z, db40, tinf, tlb, 40., alpha[4], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
     if ((fl.sw[15]) && (z <= altl[4])) {
@@ -653,7 +655,7 @@ __device__ inline void gts7(State& st, const NrlTables& T, Input& in, const Flag
     }
 
     // H
-    const double g1 = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(6)), in, fl);
+    const double g1 = fl.sw[21] * G(st, 8, (T.base+OFF_PD+150*(6)), in, fl);
     const double db01 = (T.base+OFF_PDM+10*(5))[0] * exp(g1) * (T.base+OFF_PD+150*(6))[0];
     out.d[6] = densu(st, z, db01, tinf, tlb, 1., alpha[6], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
     if ((fl.sw[15]) && (z <= altl[6])) {
@@ -672,7 +674,7 @@ __device__ inline void gts7(State& st, const NrlTables& T, Input& in, const Flag
     }
 
     // N
-    const double g14 = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(7)), in, fl);
+    const double g14 = fl.sw[21] * G(st, 9, (T.base+OFF_PD+150*(7)), in, fl);
     const double db14 = (T.base+OFF_PDM+10*(6))[0] * exp(g14) * (T.base+OFF_PD+150*(7))[0];
     out.d[7] = densu(st, z, db14, tinf, tlb, 14., alpha[7], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
     if ((fl.sw[15]) && (z <= altl[7])) {
@@ -691,7 +693,7 @@ __device__ inline void gts7(State& st, const NrlTables& T, Input& in, const Flag
     }
 
     // anomalous O
-    const double g16h = fl.sw[21] * globe7(st, (T.base+OFF_PD+150*(8)), in, fl);
+    const double g16h = fl.sw[21] * G(st, 10, (T.base+OFF_PD+150*(8)), in, fl);
     const double db16h = (T.base+OFF_PDM+10*(7))[0] * exp(g16h) * (T.base+OFF_PD+150*(8))[0];
     const double tho = (T.base+OFF_PDM+10*(7))[9] * (T.base+OFF_PDL+25*(0))[6];
     const double ddo = densu(st, z, db16h, tho, tho, 16., alpha[8], &out.t[1], (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
@@ -710,7 +712,275 @@ __device__ inline void gts7(State& st, const NrlTables& T, Input& in, const Flag
 }
 
 // :958-1065
-__device__ inline void gtd7(State& st, const NrlTables& T, Input& in, Flags& fl, Output& out) {
+
+// ---------------------------------------------------------------------------
+// gts7 split for the GPU: the common prefix (exospheric temperature, lower
+// boundary temperature and gradient, lower thermosphere nodes) and one
+// function per species, so the waves of a tile can evaluate the species
+// concurrently. Same statements as gts7 above (Nrlmsise00Model.inl:1203-1498);
+// each species recomputes the N2 mixed density at Zlb it consumes. Densu's
+// writes to the lower thermosphere nodes depend only on (tinf, tlb, s), so the
+// species are order-independent. Valid for alt >= zn2[0] = 72.5 km.
+struct Pre {
+    double tinf, tlb, s, zhf, xmm;
+    double zn1[5];
+};
+
+template <class Globe>
+__device__ inline void gts7Prefix(State& st, const NrlTables& T, Input& in, const Flags& fl, const Globe& G, Pre& pre) {
+    double zn1[5] = {120.0, 110.0, 100.0, 90.0, 72.5};
+    const double dgtr = 1.74533E-2, dr = 1.72142E-2;
+    const double za = (T.base+OFF_PDL+25*(1))[15];
+    zn1[0] = za;
+    st.g0Valid = false;
+    G.basis(st, in, fl);
+    double tinf;
+    if (in.alt > zn1[0]) tinf = (T.base+OFF_PTM)[0] * (T.base+OFF_PT)[0] * (1.0 + fl.sw[16] * G(st, 0, (T.base+OFF_PT), in, fl));
+    else tinf = (T.base+OFF_PTM)[0] * (T.base+OFF_PT)[0];
+    double g0v;
+    if (in.alt > zn1[4]) g0v = (T.base+OFF_PTM)[3] * (T.base+OFF_PS)[0] * (1.0 + fl.sw[19] * G(st, 1, (T.base+OFF_PS), in, fl));
+    else g0v = (T.base+OFF_PTM)[3] * (T.base+OFF_PS)[0];
+    const double tlb = (T.base+OFF_PTM)[1] * (1.0 + fl.sw[17] * G(st, 5, (T.base+OFF_PD+150*(3)), in, fl)) * (T.base+OFF_PD+150*(3))[0];
+    const double s = g0v / (tinf - tlb);
+    if (in.alt < 300.0) {
+        st.meso_tn1[1] = (T.base+OFF_PTM)[6] * (T.base+OFF_PTL+100*(0))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(0)), in, fl));
+        st.meso_tn1[2] = (T.base+OFF_PTM)[2] * (T.base+OFF_PTL+100*(1))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(1)), in, fl));
+        st.meso_tn1[3] = (T.base+OFF_PTM)[7] * (T.base+OFF_PTL+100*(2))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(2)), in, fl));
+        st.meso_tn1[4] = (T.base+OFF_PTM)[4] * (T.base+OFF_PTL+100*(3))[0] / (1.0 - fl.sw[18] * fl.sw[20] * glob7s(st, (T.base+OFF_PTL+100*(3)), in, fl));
+        st.meso_tgn1[1] = (T.base+OFF_PTM)[8] * (T.base+OFF_PMA+100*(8))[0] * (1.0 + fl.sw[18] * fl.sw[20] * glob7s(st, (T.base+OFF_PMA+100*(8)), in, fl)) * st.meso_tn1[4] *
+                          st.meso_tn1[4] / (pow(((T.base+OFF_PTM)[4] * (T.base+OFF_PTL+100*(3))[0]), 2.0));
+    } else {
+        st.meso_tn1[1] = (T.base+OFF_PTM)[6] * (T.base+OFF_PTL+100*(0))[0];
+        st.meso_tn1[2] = (T.base+OFF_PTM)[2] * (T.base+OFF_PTL+100*(1))[0];
+        st.meso_tn1[3] = (T.base+OFF_PTM)[7] * (T.base+OFF_PTL+100*(2))[0];
+        st.meso_tn1[4] = (T.base+OFF_PTM)[4] * (T.base+OFF_PTL+100*(3))[0];
+        st.meso_tgn1[1] = (T.base+OFF_PTM)[8] * (T.base+OFF_PMA+100*(8))[0] * st.meso_tn1[4] * st.meso_tn1[4] / (pow(((T.base+OFF_PTM)[4] * (T.base+OFF_PTL+100*(3))[0]), 2.0));
+    }
+    pre.tinf = tinf;
+    pre.tlb = tlb;
+    pre.s = s;
+    pre.zhf = (T.base+OFF_PDL+25*(1))[24] * (1.0 + fl.sw[5] * (T.base+OFF_PDL+25*(0))[24] * sin(dgtr * in.g_lat) * cos(dr * (in.doy - (T.base+OFF_PT)[13])));
+    pre.xmm = (T.base+OFF_PDM+10*(2))[4];
+    for (int k = 0; k < 5; ++k) pre.zn1[k] = zn1[k];
+}
+
+// Species k: 0 N2, 1 He, 2 O, 3 O2, 4 Ar, 5 H, 6 N, 7 anomalous O. Returns the
+// number density in the model's cgs units (before the metric factor).
+template <class Globe>
+__device__ inline double gts7Species(int k, State& st, const NrlTables& T, Input& in, const Flags& fl, const Globe& G, const Pre& pre) {
+    const double* tlbp = nullptr; (void)tlbp;
+    const int mn1 = 5;
+    const double alpha[9] = {-0.38, 0.0, 0.0, 0.0, 0.17, 0.0, -0.38, 0.0, 0.0};
+    const double altl[8] = {200.0, 300.0, 160.0, 250.0, 240.0, 450.0, 320.0, 450.0};
+    const double tinf = pre.tinf, tlb = pre.tlb, s = pre.s, zhf = pre.zhf, xmm = pre.xmm;
+    double zn1[5] = {pre.zn1[0], pre.zn1[1], pre.zn1[2], pre.zn1[3], pre.zn1[4]};
+    const double z = in.alt;
+    double tz;
+    double* tn1 = st.meso_tn1;
+    double* tgn1 = st.meso_tgn1;
+    const double zhm28 = (T.base+OFF_PDM+10*(2))[3] * (T.base+OFF_PDL+25*(1))[5];
+    (void)alpha; (void)altl; (void)zhf; (void)zhm28;
+    if (k == 0) {  // N2 (d[2])
+    const double g28 = fl.sw[21] * G(st, 4, (T.base+OFF_PD+150*(2)), in, fl);
+    const double db28 = (T.base+OFF_PDM+10*(2))[0] * exp(g28) * (T.base+OFF_PD+150*(2))[0];
+    double d = densu(st, z, db28, tinf, tlb, 28.0, alpha[2], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    const double zh28 = (T.base+OFF_PDM+10*(2))[2] * zhf;
+    const double xmd = 28.0 - xmm;
+    double b28 = 0.0;
+    if (!(z > altl[7] && z >= zn1[0]))
+        b28 = densu(st, zh28, db28, tinf, tlb, xmd, (alpha[2] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if ((fl.sw[15]) && (z <= altl[2])) {
+        st.dm28 = densu(st, z, b28, tinf, tlb, xmm, alpha[2], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        d = dnet(d, st.dm28, zhm28, xmm, 28.0);
+    }
+
+    return d;
+    }
+    if (k == 1) {  // He (d[0])
+    // N2 mixed density at Zlb, consumed by the mixing-ratio corrections
+    double b28 = 0.0;
+    if (!(z > altl[7] && z >= zn1[0])) {
+        const double g28 = fl.sw[21] * G(st, 4, (T.base+OFF_PD+150*(2)), in, fl);
+        const double db28 = (T.base+OFF_PDM+10*(2))[0] * exp(g28) * (T.base+OFF_PD+150*(2))[0];
+        const double zh28 = (T.base+OFF_PDM+10*(2))[2] * zhf;
+        b28 = densu(st, zh28, db28, tinf, tlb, 28.0 - xmm, (alpha[2] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    }
+    const double g4 = fl.sw[21] * G(st, 2, (T.base+OFF_PD+150*(0)), in, fl);
+    const double db04 = (T.base+OFF_PDM+10*(0))[0] * exp(g4) * (T.base+OFF_PD+150*(0))[0];
+    double d = densu(st, z, db04, tinf, tlb, 4., alpha[0], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if ((fl.sw[15]) && (z < altl[0])) {
+        const double zh04 = (T.base+OFF_PDM+10*(0))[2];
+        const double b04 = densu(st, zh04, db04, tinf, tlb, 4. - xmm, alpha[0] - 1., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        st.dm04 = densu(st, z, b04, tinf, tlb, xmm, 0., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        d = dnet(d, st.dm04, zhm28, xmm, 4.);
+        const double rl = log(b28 * (T.base+OFF_PDM+10*(0))[1] / b04);
+        const double zc04 = (T.base+OFF_PDM+10*(0))[4] * (T.base+OFF_PDL+25*(1))[0];
+        const double hc04 = (T.base+OFF_PDM+10*(0))[5] * (T.base+OFF_PDL+25*(1))[1];
+        d = d * ccor(z, rl, hc04, zc04);
+    }
+
+    return d;
+    }
+    if (k == 2) {  // O (d[1])
+    const double g16 = fl.sw[21] * G(st, 3, (T.base+OFF_PD+150*(1)), in, fl);
+    const double db16 = (T.base+OFF_PDM+10*(1))[0] * exp(g16) * (T.base+OFF_PD+150*(1))[0];
+    double d = densu(st, z, db16, tinf, tlb, 16., alpha[1], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if ((fl.sw[15]) && (z <= altl[1])) {
+        const double zh16 = (T.base+OFF_PDM+10*(1))[2];
+        const double b16 = densu(st, zh16, db16, tinf, tlb, 16.0 - xmm, (alpha[1] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        st.dm16 = densu(st, z, b16, tinf, tlb, xmm, 0., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        d = dnet(d, st.dm16, zhm28, xmm, 16.);
+        const double rl = (T.base+OFF_PDM+10*(1))[1] * (T.base+OFF_PDL+25*(1))[16] * (1.0 + fl.sw[1] * (T.base+OFF_PDL+25*(0))[23] * (in.f107A - 150.0));
+        const double hc16 = (T.base+OFF_PDM+10*(1))[5] * (T.base+OFF_PDL+25*(1))[3];
+        const double zc16 = (T.base+OFF_PDM+10*(1))[4] * (T.base+OFF_PDL+25*(1))[2];
+        const double hc216 = (T.base+OFF_PDM+10*(1))[5] * (T.base+OFF_PDL+25*(1))[4];
+        d = d * ccor2(z, rl, hc16, zc16, hc216);
+        const double hcc16 = (T.base+OFF_PDM+10*(1))[7] * (T.base+OFF_PDL+25*(1))[13];
+        const double zcc16 = (T.base+OFF_PDM+10*(1))[6] * (T.base+OFF_PDL+25*(1))[12];
+        const double rc16 = (T.base+OFF_PDM+10*(1))[3] * (T.base+OFF_PDL+25*(1))[14];
+        d = d * ccor(z, rc16, hcc16, zcc16);
+    }
+
+    return d;
+    }
+    if (k == 3) {  // O2 (d[3])
+    // N2 mixed density at Zlb, consumed by the mixing-ratio corrections
+    double b28 = 0.0;
+    if (!(z > altl[7] && z >= zn1[0])) {
+        const double g28 = fl.sw[21] * G(st, 4, (T.base+OFF_PD+150*(2)), in, fl);
+        const double db28 = (T.base+OFF_PDM+10*(2))[0] * exp(g28) * (T.base+OFF_PD+150*(2))[0];
+        const double zh28 = (T.base+OFF_PDM+10*(2))[2] * zhf;
+        b28 = densu(st, zh28, db28, tinf, tlb, 28.0 - xmm, (alpha[2] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    }
+    const double g32 = fl.sw[21] * G(st, 6, (T.base+OFF_PD+150*(4)), in, fl);
+    const double db32 = (T.base+OFF_PDM+10*(3))[0] * exp(g32) * (T.base+OFF_PD+150*(4))[0];
+    double d = densu(st, z, db32, tinf, tlb, 32., alpha[3], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if (fl.sw[15]) {
+        if (z <= altl[3]) {
+            const double zh32 = (T.base+OFF_PDM+10*(3))[2];
+            const double b32 = densu(st, zh32, db32, tinf, tlb, 32. - xmm, alpha[3] - 1., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+            st.dm32 = densu(st, z, b32, tinf, tlb, xmm, 0., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+            d = dnet(d, st.dm32, zhm28, xmm, 32.);
+            const double rl = log(b28 * (T.base+OFF_PDM+10*(3))[1] / b32);
+            const double hc32 = (T.base+OFF_PDM+10*(3))[5] * (T.base+OFF_PDL+25*(1))[7];
+            const double zc32 = (T.base+OFF_PDM+10*(3))[4] * (T.base+OFF_PDL+25*(1))[6];
+            d = d * ccor(z, rl, hc32, zc32);
+        }
+        const double hcc32 = (T.base+OFF_PDM+10*(3))[7] * (T.base+OFF_PDL+25*(1))[22];
+        const double hcc232 = (T.base+OFF_PDM+10*(3))[7] * (T.base+OFF_PDL+25*(0))[22];
+        const double zcc32 = (T.base+OFF_PDM+10*(3))[6] * (T.base+OFF_PDL+25*(1))[21];
+        const double rc32 = (T.base+OFF_PDM+10*(3))[3] * (T.base+OFF_PDL+25*(1))[23] * (1. + fl.sw[1] * (T.base+OFF_PDL+25*(0))[23] * (in.f107A - 150.));
+        d = d * ccor2(z, rc32, hcc32, zcc32, hcc232);
+    }
+
+    return d;
+    }
+    if (k == 4) {  // Ar (d[4])
+    // N2 mixed density at Zlb, consumed by the mixing-ratio corrections
+    double b28 = 0.0;
+    if (!(z > altl[7] && z >= zn1[0])) {
+        const double g28 = fl.sw[21] * G(st, 4, (T.base+OFF_PD+150*(2)), in, fl);
+        const double db28 = (T.base+OFF_PDM+10*(2))[0] * exp(g28) * (T.base+OFF_PD+150*(2))[0];
+        const double zh28 = (T.base+OFF_PDM+10*(2))[2] * zhf;
+        b28 = densu(st, zh28, db28, tinf, tlb, 28.0 - xmm, (alpha[2] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    }
+    const double g40 = fl.sw[21] * G(st, 7, (T.base+OFF_PD+150*(5)), in, fl);
+    const double db40 = (T.base+OFF_PDM+10*(4))[0] * exp(g40) * (T.base+OFF_PD+150*(5))[0];
+    double d = densu(st, z, db40, tinf, tlb, 40., alpha[4], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if ((fl.sw[15]) && (z <= altl[4])) {
+        const double zh40 = (T.base+OFF_PDM+10*(4))[2];
+        const double b40 = densu(st, zh40, db40, tinf, tlb, 40. - xmm, alpha[4] - 1., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        st.dm40 = densu(st, z, b40, tinf, tlb, xmm, 0., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        d = dnet(d, st.dm40, zhm28, xmm, 40.);
+        const double rl = log(b28 * (T.base+OFF_PDM+10*(4))[1] / b40);
+        const double hc40 = (T.base+OFF_PDM+10*(4))[5] * (T.base+OFF_PDL+25*(1))[9];
+        const double zc40 = (T.base+OFF_PDM+10*(4))[4] * (T.base+OFF_PDL+25*(1))[8];
+        d = d * ccor(z, rl, hc40, zc40);
+    }
+
+    return d;
+    }
+    if (k == 5) {  // H (d[6])
+    // N2 mixed density at Zlb, consumed by the mixing-ratio corrections
+    double b28 = 0.0;
+    if (!(z > altl[7] && z >= zn1[0])) {
+        const double g28 = fl.sw[21] * G(st, 4, (T.base+OFF_PD+150*(2)), in, fl);
+        const double db28 = (T.base+OFF_PDM+10*(2))[0] * exp(g28) * (T.base+OFF_PD+150*(2))[0];
+        const double zh28 = (T.base+OFF_PDM+10*(2))[2] * zhf;
+        b28 = densu(st, zh28, db28, tinf, tlb, 28.0 - xmm, (alpha[2] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    }
+    const double g1 = fl.sw[21] * G(st, 8, (T.base+OFF_PD+150*(6)), in, fl);
+    const double db01 = (T.base+OFF_PDM+10*(5))[0] * exp(g1) * (T.base+OFF_PD+150*(6))[0];
+    double d = densu(st, z, db01, tinf, tlb, 1., alpha[6], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if ((fl.sw[15]) && (z <= altl[6])) {
+        const double zh01 = (T.base+OFF_PDM+10*(5))[2];
+        const double b01 = densu(st, zh01, db01, tinf, tlb, 1. - xmm, alpha[6] - 1., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        st.dm01 = densu(st, z, b01, tinf, tlb, xmm, 0., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        d = dnet(d, st.dm01, zhm28, xmm, 1.);
+        const double rl = log(b28 * (T.base+OFF_PDM+10*(5))[1] * sqrt((T.base+OFF_PDL+25*(1))[17] * (T.base+OFF_PDL+25*(1))[17]) / b01);
+        const double hc01 = (T.base+OFF_PDM+10*(5))[5] * (T.base+OFF_PDL+25*(1))[11];
+        const double zc01 = (T.base+OFF_PDM+10*(5))[4] * (T.base+OFF_PDL+25*(1))[10];
+        d = d * ccor(z, rl, hc01, zc01);
+        const double hcc01 = (T.base+OFF_PDM+10*(5))[7] * (T.base+OFF_PDL+25*(1))[19];
+        const double zcc01 = (T.base+OFF_PDM+10*(5))[6] * (T.base+OFF_PDL+25*(1))[18];
+        const double rc01 = (T.base+OFF_PDM+10*(5))[3] * (T.base+OFF_PDL+25*(1))[20];
+        d = d * ccor(z, rc01, hcc01, zcc01);
+    }
+
+    return d;
+    }
+    if (k == 6) {  // N (d[7])
+    // N2 mixed density at Zlb, consumed by the mixing-ratio corrections
+    double b28 = 0.0;
+    if (!(z > altl[7] && z >= zn1[0])) {
+        const double g28 = fl.sw[21] * G(st, 4, (T.base+OFF_PD+150*(2)), in, fl);
+        const double db28 = (T.base+OFF_PDM+10*(2))[0] * exp(g28) * (T.base+OFF_PD+150*(2))[0];
+        const double zh28 = (T.base+OFF_PDM+10*(2))[2] * zhf;
+        b28 = densu(st, zh28, db28, tinf, tlb, 28.0 - xmm, (alpha[2] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    }
+    const double g14 = fl.sw[21] * G(st, 9, (T.base+OFF_PD+150*(7)), in, fl);
+    const double db14 = (T.base+OFF_PDM+10*(6))[0] * exp(g14) * (T.base+OFF_PD+150*(7))[0];
+    double d = densu(st, z, db14, tinf, tlb, 14., alpha[7], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if ((fl.sw[15]) && (z <= altl[7])) {
+        const double zh14 = (T.base+OFF_PDM+10*(6))[2];
+        const double b14 = densu(st, zh14, db14, tinf, tlb, 14. - xmm, alpha[7] - 1., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        st.dm14 = densu(st, z, b14, tinf, tlb, xmm, 0., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+        d = dnet(d, st.dm14, zhm28, xmm, 14.);
+        const double rl = log(b28 * (T.base+OFF_PDM+10*(6))[1] * sqrt((T.base+OFF_PDL+25*(0))[2] * (T.base+OFF_PDL+25*(0))[2]) / b14);
+        const double hc14 = (T.base+OFF_PDM+10*(6))[5] * (T.base+OFF_PDL+25*(0))[1];
+        const double zc14 = (T.base+OFF_PDM+10*(6))[4] * (T.base+OFF_PDL+25*(0))[0];
+        d = d * ccor(z, rl, hc14, zc14);
+        const double hcc14 = (T.base+OFF_PDM+10*(6))[7] * (T.base+OFF_PDL+25*(0))[4];
+        const double zcc14 = (T.base+OFF_PDM+10*(6))[6] * (T.base+OFF_PDL+25*(0))[3];
+        const double rc14 = (T.base+OFF_PDM+10*(6))[3] * (T.base+OFF_PDL+25*(0))[5];
+        d = d * ccor(z, rc14, hcc14, zcc14);
+    }
+
+    return d;
+    }
+    if (k == 7) {  // anomalous O (d[8])
+    const double g16h = fl.sw[21] * G(st, 10, (T.base+OFF_PD+150*(8)), in, fl);
+    const double db16h = (T.base+OFF_PDM+10*(7))[0] * exp(g16h) * (T.base+OFF_PD+150*(8))[0];
+    const double tho = (T.base+OFF_PDM+10*(7))[9] * (T.base+OFF_PDL+25*(0))[6];
+    const double ddo = densu(st, z, db16h, tho, tho, 16., alpha[8], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    const double zsht = (T.base+OFF_PDM+10*(7))[5];
+    const double zmho = (T.base+OFF_PDM+10*(7))[4];
+    const double zsho = scalh(st, zmho, 16.0, tho);
+    const double d = ddo * exp(-zsht / zsho * (exp(-(z - zmho) / zsht) - 1.));
+
+    return d;
+    }
+    return 0.0;
+}
+
+// Mass-density weights of the species in gtd7d (amu): d[5] = 1.66e-24 sum w_k d_k.
+__device__ inline double speciesMass(int k) {
+    const double w[8] = {28.0, 4.0, 16.0, 32.0, 40.0, 1.0, 14.0, 16.0};
+    return w[k];
+}
+
+template <class Globe>
+__device__ inline void gtd7(State& st, const NrlTables& T, Input& in, Flags& fl, Output& out, const Globe& G) {
     const int mn3 = 5;
     const double zn3[5] = {32.5, 20.0, 15.0, 10.0, 0.0};
     const int mn2 = 4;
@@ -725,7 +995,7 @@ __device__ inline void gtd7(State& st, const NrlTables& T, Input& in, Flags& fl,
     double altt = in.alt > zn2[0] ? in.alt : zn2[0];
     const double tmp = in.alt;
     in.alt = altt;
-    gts7(st, T, in, fl, so);
+    gts7(st, T, in, fl, so, G);
     altt = in.alt;
     in.alt = tmp;
     const double dm28m = fl.sw[0] ? st.dm28 * 1.0E6 : st.dm28;
@@ -776,13 +1046,197 @@ __device__ inline void gtd7(State& st, const NrlTables& T, Input& in, Flags& fl,
 }
 
 // :1073-1078
-__device__ inline void gtd7d(State& st, const NrlTables& T, Input& in, Flags& fl, Output& out) {
-    gtd7(st, T, in, fl, out);
+template <class Globe>
+__device__ inline void gtd7d(State& st, const NrlTables& T, Input& in, Flags& fl, Output& out, const Globe& G) {
+    gtd7(st, T, in, fl, out, G);
     out.d[5] = 1.66E-24 * (4.0 * out.d[0] + 16.0 * out.d[1] + 28.0 * out.d[2] + 32.0 * out.d[3] + 40.0 * out.d[4] + out.d[6] +
                            14.0 * out.d[7] + 16.0 * out.d[8]);
     if (fl.sw[0]) out.d[5] = out.d[5] / 1000;
 }
 
+
+// ---------------------------------------------------------------------------
+// Policies for the globe7 evaluations inside gts7.
+// GenericGlobe: the model as published (any switches) — used by the test probe.
+struct GenericGlobe {
+    __device__ void basis(State& st, const Input& in, const Flags& fl) const { prepareGlobeBasis(st, in, fl); }
+    __device__ double operator()(State& st, int, const double* p, const Input& in, const Flags& fl) const {
+        return globe7(st, p, in, fl);
+    }
+};
+
+// Epoch-only parts of globe7 for one coefficient table with the propagator's
+// switch setting (all on, switch 9 = -1): day-of-year cosines, flux factors,
+// UT cosines, the 3-hour Ap function where it does not depend on latitude
+// (p[138] = 0), and the g0 values where it does. 18 values per table.
+enum : int {
+    GE_CD14 = 0, GE_CD18, GE_CD32, GE_CD39, GE_F1, GE_F2, GE_T0, GE_UT71, GE_UT79C, GE_UT79S, GE_UT58, GE_APT0,
+    GE_G0 = 12,  // 6 values
+    GE_STRIDE = 18,
+    GE_TABLES = 11,
+    GE_DFA = GE_TABLES * GE_STRIDE,
+    GE_COUNT = GE_DFA + 1
+};
+
+// Table j: 0 = pt, 1 = ps, 2 + k = pd[k].
+__device__ inline const double* globeTable(const NrlTables& T, int j) {
+    return j == 0 ? T.base + OFF_PT : (j == 1 ? T.base + OFF_PS : T.base + OFF_PD + 150 * (j - 2));
+}
+
+// Fills out[GE_COUNT] from the epoch inputs (doy, sec, f107, f107A, ap[7]).
+__device__ inline void globeEpoch(const NrlTables& T, int doy, double sec, double f107, double f107A, const double* ap,
+                                  double* out) {
+    const double sr = 7.2722E-5, dr = 1.72142E-2;
+    const double df = f107 - f107A;
+    const double dfa = f107A - 150.0;
+    out[GE_DFA] = dfa;
+    for (int j = 0; j < GE_TABLES; ++j) {
+        const double* p = globeTable(T, j);
+        double* e = out + GE_STRIDE * j;
+        e[GE_CD32] = cos(dr * (doy - p[31]));
+        e[GE_CD18] = cos(2.0 * dr * (doy - p[17]));
+        e[GE_CD14] = cos(dr * (doy - p[13]));
+        e[GE_CD39] = cos(2.0 * dr * (doy - p[38]));
+        e[GE_T0] = p[19] * df * (1.0 + p[59] * dfa) + p[20] * df * df + p[21] * dfa + p[29] * pow(dfa, 2.0);
+        e[GE_F1] = 1.0 + (p[47] * dfa + p[19] * df + p[20] * df * df);
+        e[GE_F2] = 1.0 + (p[49] * dfa + p[19] * df + p[20] * df * df);
+        e[GE_UT71] = cos(sr * (sec - p[71]));
+        e[GE_UT79C] = cos(sr * (sec - p[79]));
+        e[GE_UT79S] = sin(sr * (sec - p[79]));
+        e[GE_UT58] = cos(sr * (sec - p[58]));
+        for (int k = 0; k < 6; ++k) e[GE_G0 + k] = g0(ap[k + 1], p);
+        e[GE_APT0] = 0.0;
+        if (p[51] != 0 && p[138] == 0.0) {
+            double exp1 = exp(-10800.0 * sqrt(p[51] * p[51]) / 1.0);
+            if (exp1 > 0.99999) exp1 = 0.99999;
+            const double* g = e + GE_G0;
+            e[GE_APT0] = (g[0] + (g[1] * exp1 + g[2] * exp1 * exp1 + g[3] * pow(exp1, 3.0) +
+                                  (g[4] * pow(exp1, 4.0) + g[5] * pow(exp1, 12.0)) * (1.0 - pow(exp1, 8.0)) / (1.0 - exp1))) /
+                         sumex(exp1);
+        }
+    }
+}
+
+// Per-object globe7 with the epoch terms E (this table's GE_STRIDE block) and
+// the basis of State (plg, local-time and longitude trigonometry). The
+// expansion is term by term the published one (Nrlmsise00Model.inl:688-850)
+// with cos(x - p) = cos x cos p + sin x sin p for the longitude / local-time
+// phases and powers of the Ap decay factor taken by products.
+__device__ inline double globe7Fast(const State& st, const double* p, const double* E, double dfa, double glat, double& aptOut) {
+    const double hr = 0.2618, dgtr = 1.74533E-2;
+    const double(&plg)[4][9] = st.plg;
+    const double cd14 = E[GE_CD14], cd18 = E[GE_CD18], f2 = E[GE_F2];
+    double tinf = p[30];
+    tinf += E[GE_T0];
+    tinf += (p[1] * plg[0][2] + p[2] * plg[0][4] + p[22] * plg[0][6]) + (p[14] * plg[0][2]) * dfa + p[26] * plg[0][1];
+    tinf += p[18] * E[GE_CD32];
+    tinf += (p[15] + p[16] * plg[0][2]) * cd18;
+    tinf += E[GE_F1] * (p[9] * plg[0][1] + p[10] * plg[0][3]) * cd14;
+    tinf += p[37] * plg[0][1] * E[GE_CD39];
+    {
+        const double t71 = (p[11] * plg[1][2]) * cd14, t72 = (p[12] * plg[1][2]) * cd14;
+        tinf += f2 * ((p[3] * plg[1][1] + p[4] * plg[1][3] + p[27] * plg[1][5] + t71) * st.ctloc +
+                      (p[6] * plg[1][1] + p[7] * plg[1][3] + p[28] * plg[1][5] + t72) * st.stloc);
+    }
+    {
+        const double t81 = (p[23] * plg[2][3] + p[35] * plg[2][5]) * cd14, t82 = (p[33] * plg[2][3] + p[36] * plg[2][5]) * cd14;
+        tinf += f2 * ((p[5] * plg[2][2] + p[41] * plg[2][4] + t81) * st.c2tloc + (p[8] * plg[2][2] + p[42] * plg[2][4] + t82) * st.s2tloc);
+    }
+    double apt0 = 0.0;
+    if (p[51] != 0) {
+        if (p[138] == 0.0) {
+            apt0 = E[GE_APT0];
+        } else {
+            double ex = exp(-10800.0 * sqrt(p[51] * p[51]) / (1.0 + p[138] * (45.0 - sqrt(glat * glat))));
+            if (ex > 0.99999) ex = 0.99999;
+            const double* g = E + GE_G0;
+            const double e2 = ex * ex, e3 = e2 * ex, e4 = e2 * e2, e8 = e4 * e4, e12 = e8 * e4, e16 = e8 * e8, e19 = e16 * e3;
+            const double sx = 1.0 + (1.0 - e19) / (1.0 - ex) * sqrt(ex);
+            apt0 = (g[0] + (g[1] * ex + g[2] * e2 + g[3] * e3 + (g[4] * e4 + g[5] * e12) * (1.0 - e8) / (1.0 - ex))) / sx;
+        }
+        const double c131 = st.ctloc * cos(hr * p[131]) + st.stloc * sin(hr * p[131]);
+        tinf += apt0 * (p[50] + p[96] * plg[0][2] + p[54] * plg[0][4] +
+                        (p[125] * plg[0][1] + p[126] * plg[0][3] + p[127] * plg[0][5]) * cd14 +
+                        (p[128] * plg[1][1] + p[129] * plg[1][3] + p[130] * plg[1][5]) * c131);
+    }
+    const double cg = st.cglong, sg = st.sglong;
+    tinf += (1.0 + p[80] * dfa) *
+            ((p[64] * plg[1][2] + p[65] * plg[1][4] + p[66] * plg[1][6] + p[103] * plg[1][1] + p[104] * plg[1][3] + p[105] * plg[1][5] +
+              (p[109] * plg[1][1] + p[110] * plg[1][3] + p[111] * plg[1][5]) * cd14) * cg +
+             (p[90] * plg[1][2] + p[91] * plg[1][4] + p[92] * plg[1][6] + p[106] * plg[1][1] + p[107] * plg[1][3] + p[108] * plg[1][5] +
+              (p[112] * plg[1][1] + p[113] * plg[1][3] + p[114] * plg[1][5]) * cd14) * sg);
+    {
+        const double c2g = cg * cg - sg * sg, s2g = 2.0 * sg * cg;
+        double t11 = (1.0 + p[95] * plg[0][1]) * (1.0 + p[81] * dfa) * (1.0 + p[119] * plg[0][1] * cd14) *
+                     ((p[68] * plg[0][1] + p[69] * plg[0][3] + p[70] * plg[0][5]) * E[GE_UT71]);
+        t11 += (p[76] * plg[2][3] + p[77] * plg[2][5] + p[78] * plg[2][7]) * (E[GE_UT79C] * c2g - E[GE_UT79S] * s2g) *
+               (1.0 + p[137] * dfa);
+        tinf += t11;
+    }
+    if (p[51] != 0) {
+        const double c97 = cg * cos(dgtr * p[97]) + sg * sin(dgtr * p[97]);
+        const double c136 = cg * cos(dgtr * p[136]) + sg * sin(dgtr * p[136]);
+        tinf += apt0 * (1. + p[132] * plg[0][1]) * ((p[52] * plg[1][2] + p[98] * plg[1][4] + p[67] * plg[1][6]) * c97) +
+                apt0 * (p[133] * plg[1][1] + p[134] * plg[1][3] + p[135] * plg[1][5]) * cd14 * c136 +
+                apt0 * (p[55] * plg[0][1] + p[56] * plg[0][3] + p[57] * plg[0][5]) * E[GE_UT58];
+    }
+    tinf += f2 * ((p[39] * plg[3][3] + (p[93] * plg[3][4] + p[46] * plg[3][6]) * cd14) * st.s3tloc +
+                  (p[40] * plg[3][3] + (p[94] * plg[3][4] + p[48] * plg[3][6]) * cd14) * st.c3tloc);
+    aptOut = apt0;
+    return tinf;
+}
+
+// Fast policy: epoch terms from the per-epoch record.
+struct FastGlobe {
+    const double* E;  // GE_COUNT values
+    __device__ void basis(State& st, const Input& in, const Flags&) const {
+        const double dgtr = 1.74533E-2, hr = 0.2618;
+        const double c = sin(in.g_lat * dgtr), s = cos(in.g_lat * dgtr);
+        const double c2 = c * c, c4 = c2 * c2, s2 = s * s;
+        double(&plg)[4][9] = st.plg;
+        plg[0][1] = c;
+        plg[0][2] = 0.5 * (3.0 * c2 - 1.0);
+        plg[0][3] = 0.5 * (5.0 * c * c2 - 3.0 * c);
+        plg[0][4] = (35.0 * c4 - 30.0 * c2 + 3.0) / 8.0;
+        plg[0][5] = (63.0 * c2 * c2 * c - 70.0 * c2 * c + 15.0 * c) / 8.0;
+        plg[0][6] = (11.0 * c * plg[0][5] - 5.0 * plg[0][4]) / 6.0;
+        plg[1][1] = s;
+        plg[1][2] = 3.0 * c * s;
+        plg[1][3] = 1.5 * (5.0 * c2 - 1.0) * s;
+        plg[1][4] = 2.5 * (7.0 * c2 * c - 3.0 * c) * s;
+        plg[1][5] = 1.875 * (21.0 * c4 - 14.0 * c2 + 1.0) * s;
+        plg[1][6] = (11.0 * c * plg[1][5] - 6.0 * plg[1][4]) / 5.0;
+        plg[2][2] = 3.0 * s2;
+        plg[2][3] = 15.0 * s2 * c;
+        plg[2][4] = 7.5 * (7.0 * c2 - 1.0) * s2;
+        plg[2][5] = 3.0 * c * plg[2][4] - 2.0 * plg[2][3];
+        plg[2][6] = (11.0 * c * plg[2][5] - 7.0 * plg[2][4]) / 4.0;
+        plg[2][7] = (13.0 * c * plg[2][6] - 8.0 * plg[2][5]) / 5.0;
+        plg[3][3] = 15.0 * s2 * s;
+        plg[3][4] = 105.0 * s2 * s * c;
+        plg[3][5] = (9.0 * c * plg[3][4] - 7. * plg[3][3]) / 2.0;
+        plg[3][6] = (11.0 * c * plg[3][5] - 8. * plg[3][4]) / 3.0;
+        // local time harmonics from one sin/cos pair (double and triple angles)
+        const double sl = sin(hr * in.lst), cl = cos(hr * in.lst);
+        st.stloc = sl;
+        st.ctloc = cl;
+        st.s2tloc = 2.0 * sl * cl;
+        st.c2tloc = cl * cl - sl * sl;
+        st.s3tloc = sl * (3.0 - 4.0 * sl * sl);
+        st.c3tloc = cl * (4.0 * cl * cl - 3.0);
+        st.cglong = cos(dgtr * in.g_long);
+        st.sglong = sin(dgtr * in.g_long);
+        st.dfa = E[GE_DFA];
+    }
+    __device__ double operator()(State& st, int j, const double* p, const Input& in, const Flags&) const {
+        // globe7 also leaves apt[0] for glob7s (lower thermosphere)
+        const double* e = E + GE_STRIDE * j;
+        double apt0 = 0.0;
+        const double v = globe7Fast(st, p, e, E[GE_DFA], in.g_lat, apt0);
+        if (p[51] != 0) st.apt[0] = apt0;
+        return v;
+    }
+};
 
 }  // namespace nrlmsise
 }  // namespace phx

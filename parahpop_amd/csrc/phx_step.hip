@@ -111,11 +111,23 @@ __device__ inline ShGeom shGeometry(const Rec& rec, double px, double py, double
     return g;
 }
 
+// Read-only tables in the constant address space: uniform indices become
+// scalar (SMEM) loads into SGPR operands of the FMAs.
+typedef const __attribute__((address_space(4))) double* ConstD;
+typedef const __attribute__((address_space(4))) int* ConstI;
+
 // Order block [m0, m1) of the SH sum for one lane; returns the unscaled
 // spherical components (radial, latitude, longitude) of sum_nm alpha^n (...).
-__device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const double* __restrict__ terms,
-                             const int* __restrict__ orderStart, const double* __restrict__ diag, bool normalized,
-                             double& outR, double& outPhi, double& outLam) {
+// All control flow over (n, m) is uniform across the wave; only the literal
+// pole (x = y = 0 in the body frame) takes a divergent fix-up after the loop.
+__device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const double* termsG, const int* orderStartG,
+                             const double* diagG, bool normalized, double& outR, double& outPhi, double& outLam) {
+    ConstD terms = (ConstD)termsG;
+    ConstI orderStart = (ConstI)orderStartG;
+    ConstD diag = (ConstD)diagG;
+    m0 = __builtin_amdgcn_readfirstlane(m0);
+    m1 = __builtin_amdgcn_readfirstlane(m1);
+    L = __builtin_amdgcn_readfirstlane(L);
     const double u = g.alpha * g.sinPhi;   // alpha sin(phi)
     const double v = g.alpha * g.cosPhi;   // alpha cos(phi)
     const double w = g.alpha * g.alpha;
@@ -129,45 +141,31 @@ __device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const doubl
         sinML = s;
         cosML = c;
     }
-    const bool pole = g.cosPhi == 0.0;
     for (int m = m0; m < m1; ++m) {
-        const double* t = terms + SHT_STRIDE * orderStart[m];
+        ConstD t = terms + SHT_STRIDE * orderStart[m];
         const int count = L - m + 1;
-        if (m == 0 || (pole && m == 1)) {
-            // dP recursion (reference form); seeds P~_00 = 1, dP~_00 = 0 or the m = 1 diagonal.
-            double P0 = Pmm, Pm1 = 0.0;
-            double dP0 = m == 0 ? 0.0 : -diag[1] * g.alpha * g.sinPhi, dPm1 = 0.0;
-            double X1 = 0.0, X2 = 0.0, D1 = 0.0, D2 = 0.0, Lc = 0.0, Ls = 0.0;
+        if (m == 0) {
+            // order 0: dP recursion (reference form), seeds P~_00 = 1, dP~_00 = 0
+            double P0 = Pmm, Pm1 = 0.0, dP0 = 0.0, dPm1 = 0.0;
+            double X1 = 0.0, D1 = 0.0;
             for (int i = 0; i < count; ++i, t += SHT_STRIDE) {
-                const int n = m + i;
                 X1 = fma(P0, t[SHT_C1], X1);
-                X2 = fma(P0, t[SHT_S1], X2);
                 D1 = fma(dP0, t[SHT_C], D1);
-                D2 = fma(dP0, t[SHT_S], D2);
-                if (m == 1) {
-                    // analytic limit of P_n1 / cos(phi) at the pole (Coefficients.h:123-132)
-                    const double sign = g.sinPhi > 0.0 ? 1.0 : ((n & 1) ? 1.0 : -1.0);
-                    double lim = 0.5 * double(n) * (n + 1.0);
-                    if (normalized) lim *= sqrt(2.0 * (2.0 * n + 1.0) / (double(n) * (n + 1.0)));
-                    double an = 1.0;
-                    for (int k = 0; k < n; ++k) an *= g.alpha;
-                    Lc = fma(sign * lim * an, t[SHT_C], Lc);
-                    Ls = fma(sign * lim * an, t[SHT_S], Ls);
-                }
-                const double a = t[SHT_A], b = t[SHT_B];
-                const double Pn = a * u * P0 - b * w * Pm1;
-                const double dPn = a * (u * dP0 + v * P0) - b * w * dPm1;
+                const double au = t[SHT_A] * u, bw = t[SHT_B] * w;
+                const double Pn = fma(au, P0, -bw * Pm1);
+                const double dPn = fma(t[SHT_A], fma(v, P0, u * dP0), -bw * dPm1);
                 Pm1 = P0;
                 P0 = Pn;
                 dPm1 = dP0;
                 dP0 = dPn;
             }
-            ar -= cosML * X1 + sinML * X2;
-            aphiDirect += cosML * D1 + sinML * D2;
-            if (m == 1) alamNum -= double(m) * (sinML * Lc - cosML * Ls);  // already divided by cos(phi)
-        } else if (!pole) {
+            ar -= X1;
+            aphiDirect += D1;
+        } else {
             double P0 = Pmm, Pm1 = 0.0;
             double A1 = 0.0, A2 = 0.0, A5 = 0.0, A6 = 0.0, B1 = 0.0, B2 = 0.0;
+            // Four terms per trip: their 4 x 64 B of uniform coefficients are
+            // requested together so one scalar-load latency covers 40 FMAs.
 #pragma unroll 4
             for (int i = 0; i < count; ++i, t += SHT_STRIDE) {
                 A1 = fma(P0, t[SHT_C1], A1);
@@ -196,13 +194,41 @@ __device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const doubl
         cosML = c;
     }
     outR = ar;
-    if (pole) {
-        outPhi = aphiDirect;
-        outLam = alamNum;
-    } else {
+    if (g.cosPhi != 0.0) {
         const double inv = 1.0 / g.cosPhi;
         outPhi = aphiDirect + aphiNum * inv;
         outLam = alamNum * inv;
+        return;
+    }
+    // Literal pole: every P~_nm with m >= 1 vanishes; order 1 keeps a finite
+    // latitude derivative (dP recursion from dP~_11 = -c_1 alpha sin(phi)) and
+    // the analytic limit of P_n1 / cos(phi) (Coefficients.h:123-132).
+    outPhi = aphiDirect;
+    outLam = 0.0;
+    if (m0 <= 1 && 1 < m1 && L >= 1) {
+        ConstD t = terms + SHT_STRIDE * orderStart[1];
+        double P0 = 0.0, Pm1 = 0.0, dP0 = -diag[1] * g.alpha * g.sinPhi, dPm1 = 0.0;
+        double D1 = 0.0, D2 = 0.0, Lc = 0.0, Ls = 0.0, an = g.alpha;
+        for (int n = 1; n <= L; ++n, t += SHT_STRIDE) {
+            D1 = fma(dP0, t[SHT_C], D1);
+            D2 = fma(dP0, t[SHT_S], D2);
+            const double sign = g.sinPhi > 0.0 ? 1.0 : ((n & 1) ? 1.0 : -1.0);
+            double lim = 0.5 * double(n) * (n + 1.0);
+            if (normalized) lim *= sqrt(2.0 * (2.0 * n + 1.0) / (double(n) * (n + 1.0)));
+            Lc = fma(sign * lim * an, t[SHT_C], Lc);
+            Ls = fma(sign * lim * an, t[SHT_S], Ls);
+            const double a = t[SHT_A], b = t[SHT_B];
+            const double Pn = a * u * P0 - b * w * Pm1;
+            const double dPn = a * (u * dP0 + v * P0) - b * w * dPm1;
+            Pm1 = P0;
+            P0 = Pn;
+            dPm1 = dP0;
+            dP0 = dPn;
+            an *= g.alpha;
+        }
+        // order-1 phasor: cos(l), sin(l)
+        outPhi += g.cosL * D1 + g.sinL * D2;
+        outLam = -(g.sinL * Lc - g.cosL * Ls);
     }
 }
 
@@ -247,7 +273,9 @@ __device__ void srpAccel(const ForceModel& fm, const Rec& rec, double px, double
     const int sb = REC_BODY + 3 * fm.srpBody;
     const double dx = px - rec(sb), dy = py - rec(sb + 1), dz = pz - rec(sb + 2);
     double occ = 1.0;
-    for (int k = 0; k < fm.nOccluders; ++k) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k >= fm.nOccluders) break;
         const int ob = REC_BODY + 3 * fm.occBody[k];
         occ *= shadowFactor(dx, dy, dz, px - rec(ob), py - rec(ob + 1), pz - rec(ob + 2), fm.sunRadius, fm.occRadius[k]);
     }
@@ -284,59 +312,137 @@ __device__ bool geodetic(double x, double y, double z, double radius, double fla
     return isfinite(lat) && isfinite(lon) && isfinite(alt);
 }
 
-// nrl: the NRLMSISE-00 epoch inputs (record fields REC_NRL_* - REC_NRL).
-__device__ __noinline__ double nrlDensity(const ForceModel& fm, const double* nrl, const double* __restrict__ coeff,
-                                          double fx, double fy, double fz) {
-    const double* rec = nrl - REC_NRL;
-    double lat, lon, alt;
-    if (!geodetic(fx, fy, fz, fm.earthRadius, fm.flattening, lat, lon, alt)) return nan("");
-    if (alt > fm.hCutoff) return 0.0;
-    if (rec[REC_NRL_VALID] == 0.0) return nan("");
-    const double rad2deg = 57.2957795130823208768;
+// NRLMSISE-00 inputs of one lane: Earth-fixed position -> geodetic latitude,
+// longitude, altitude and the model input record (Nrlmsise00.cu:270-327).
+struct NrlLane {
     nrlmsise::Input in;
-    in.doy = int(rec[REC_NRL_DOY]);
-    in.sec = rec[REC_NRL_SEC];
-    in.alt = alt;
-    in.g_lat = lat * rad2deg;
-    in.g_long = lon * rad2deg;
-    double lst = fmod(rec[REC_NRL_GAST] + lon * rad2deg / 15.0, 24.0);
-    in.lst = lst < 0.0 ? lst + 24.0 : lst;
-    in.f107A = rec[REC_NRL_F107A];
-    in.f107 = rec[REC_NRL_F107];
-    in.ap = rec[REC_NRL_AP];
-    for (int k = 0; k < 7; ++k) in.apa[k] = rec[REC_NRL_AP + k];
+    int state;   // 0 above hCutoff (rho = 0), 1 model (alt >= 72.5 km), 2 full model below 72.5 km, 3 invalid
+};
+
+template <class Rec>
+__device__ inline NrlLane nrlLane(const ForceModel& fm, const Rec& rec, const double* nrl, double px, double py, double pz) {
+    NrlLane L;
+    const double fx = rec(REC_R + 0) * px + rec(REC_R + 1) * py + rec(REC_R + 2) * pz;
+    const double fy = rec(REC_R + 3) * px + rec(REC_R + 4) * py + rec(REC_R + 5) * pz;
+    const double fz = rec(REC_R + 6) * px + rec(REC_R + 7) * py + rec(REC_R + 8) * pz;
+    double lat, lon, alt;
+    const double* r = nrl - REC_NRL;
+    L.state = 0;
+    if (!geodetic(fx, fy, fz, fm.earthRadius, fm.flattening, lat, lon, alt)) {
+        L.state = 3;
+        alt = 0.0;
+        lat = lon = 0.0;
+    } else if (alt > fm.hCutoff) {
+        L.state = 0;
+    } else if (r[REC_NRL_VALID] == 0.0) {
+        L.state = 3;
+    } else {
+        L.state = alt >= 72.5 ? 1 : 2;
+    }
+    const double rad2deg = 57.2957795130823208768;
+    L.in.doy = int(r[REC_NRL_DOY]);
+    L.in.sec = r[REC_NRL_SEC];
+    L.in.alt = alt;
+    L.in.g_lat = lat * rad2deg;
+    L.in.g_long = lon * rad2deg;
+    double lst = fmod(r[REC_NRL_GAST] + lon * rad2deg / 15.0, 24.0);
+    L.in.lst = lst < 0.0 ? lst + 24.0 : lst;
+    L.in.f107A = r[REC_NRL_F107A];
+    L.in.f107 = r[REC_NRL_F107];
+    L.in.ap = r[REC_NRL_AP];
+    for (int k = 0; k < 7; ++k) L.in.apa[k] = r[REC_NRL_AP + k];
+    return L;
+}
+
+__device__ inline nrlmsise::Flags nrlFlags() {
     nrlmsise::Flags fl;
+    for (int k = 0; k < 24; ++k) fl.switches[k] = 1;
+    fl.switches[9] = -1;
+    nrlmsise::tselec(fl);
+    return fl;
+}
+
+// Species k0, k0 + stride, ... of the thermospheric model for this lane
+// (cgs number densities before the metric factor) into spec[k][lane].
+template <bool FAST>
+__device__ void nrlSpecies(const NrlLane& L, const double* nrl, const double* __restrict__ coeff, int k0, int stride,
+                           double (*spec)[kTile], int lane) {
+    nrlmsise::Input in = L.in;
+    const nrlmsise::Flags fl = nrlFlags();
+    nrlmsise::State st;
+    const nrlmsise::NrlTables T{coeff};
+    nrlmsise::Pre pre;
+    // gtd7's gravity/radius at this latitude (switch 2 on), Nrlmsise00Model.inl:976-980
+    nrlmsise::glatf(in.g_lat, st.gsurf, st.re);
+    if (FAST) {
+        const nrlmsise::FastGlobe G{nrl + (REC_NRL_GLOBE - REC_NRL)};
+        nrlmsise::gts7Prefix(st, T, in, fl, G, pre);
+        for (int k = k0; k < 8; k += stride) spec[k][lane] = nrlmsise::gts7Species(k, st, T, in, fl, G, pre);
+    } else {
+        const nrlmsise::GenericGlobe G{};
+        nrlmsise::gts7Prefix(st, T, in, fl, G, pre);
+        for (int k = k0; k < 8; k += stride) spec[k][lane] = nrlmsise::gts7Species(k, st, T, in, fl, G, pre);
+    }
+}
+
+// Full model for one lane (serial gtd7d), kg/km^3.
+template <bool FAST>
+__device__ __noinline__ double nrlDensity(const ForceModel& fm, const double* nrl, const double* __restrict__ coeff,
+                                          nrlmsise::Input in) {
+    nrlmsise::Flags fl = nrlFlags();
     for (int k = 0; k < 24; ++k) fl.switches[k] = 1;
     fl.switches[9] = -1;
     nrlmsise::Output out;
     nrlmsise::State st;
     nrlmsise::NrlTables T{coeff};
-    nrlmsise::gtd7d(st, T, in, fl, out);
+    if (FAST) nrlmsise::gtd7d(st, T, in, fl, out, nrlmsise::FastGlobe{nrl + (REC_NRL_GLOBE - REC_NRL)});
+    else nrlmsise::gtd7d(st, T, in, fl, out, nrlmsise::GenericGlobe{});
     if (!isfinite(out.d[5]) || out.d[5] < 0.0) return nan("");
     return (out.d[5] * 1.0e9) * fm.densityScale;
 }
 
-template <class Rec>
+// Total density (kg/km^3) from the species: gtd7d's sum and units.
+__device__ inline double nrlRhoFromSpecies(const ForceModel& fm, double (*spec)[kTile], int lane) {
+    // d[i] = 1e6 n_i; d5 = 1.66e-24 (4 He + 16 O + 28 N2 + 32 O2 + 40 Ar + H + 14 N + 16 anomalous O) / 1000
+    const double He = 1.0E6 * spec[1][lane], O = 1.0E6 * spec[2][lane], N2 = 1.0E6 * spec[0][lane], O2 = 1.0E6 * spec[3][lane];
+    const double Ar = 1.0E6 * spec[4][lane], H = 1.0E6 * spec[5][lane], N = 1.0E6 * spec[6][lane], AO = 1.0E6 * spec[7][lane];
+    double d5 = 1.66E-24 * (4.0 * He + 16.0 * O + 28.0 * N2 + 32.0 * O2 + 40.0 * Ar + H + 14.0 * N + 16.0 * AO);
+    d5 = d5 / 1000;
+    if (!isfinite(d5) || d5 < 0.0) return nan("");
+    return (d5 * 1.0e9) * fm.densityScale;
+}
+
+// Drag of one lane with the serial full model (diagnostic probe).
+template <bool FAST, class Rec>
 __device__ void dragAccel(const ForceModel& fm, const Rec& rec, const double* nrl, const double* __restrict__ coeff, double px,
                           double py, double pz, double vx, double vy, double vz, double bcd, double& ax, double& ay,
                           double& az) {
-    const double fx = rec(REC_R + 0) * px + rec(REC_R + 1) * py + rec(REC_R + 2) * pz;
-    const double fy = rec(REC_R + 3) * px + rec(REC_R + 4) * py + rec(REC_R + 5) * pz;
-    const double fz = rec(REC_R + 6) * px + rec(REC_R + 7) * py + rec(REC_R + 8) * pz;
-    const double rho = nrlDensity(fm, nrl, coeff, fx, fy, fz);
-    const double om[3] = {rec(REC_OMEGA), rec(REC_OMEGA + 1), rec(REC_OMEGA + 2)};
-    const double rx = vx - (om[1] * pz - om[2] * py);
-    const double ry = vy - (om[2] * px - om[0] * pz);
-    const double rz = vz - (om[0] * py - om[1] * px);
+    const NrlLane L = nrlLane(fm, rec, nrl, px, py, pz);
+    double rho = 0.0;
+    if (L.state == 1 || L.state == 2) rho = nrlDensity<FAST>(fm, nrl, coeff, L.in);
+    else if (L.state == 3) rho = nan("");
+    const double om0 = rec(REC_OMEGA), om1 = rec(REC_OMEGA + 1), om2 = rec(REC_OMEGA + 2);
+    const double rx = vx - (om1 * pz - om2 * py);
+    const double ry = vy - (om2 * px - om0 * pz);
+    const double rz = vz - (om0 * py - om1 * px);
     const double f = -0.5 * rho * bcd * sqrt(rx * rx + ry * ry + rz * rz);
     ax = f * rx;
     ay = f * ry;
     az = f * rz;
 }
 
+// Fully normalized Legendre recursion factors for degree <= 6, the same
+// formulas as the reference's legendre() (HpopCorrections.h:121-145):
+// kTideK = sqrt((2n+1)/((n-m)(n+m))), kTideA = sqrt(2n-1),
+// kTideB = sqrt((n+m-1)(n-m-1)/(2n-3)) at [n][m] for n >= m+2;
+// kTideD[n] = sqrt((2n+1)/(2n)) (sectoral), kTideS[m] = sqrt(2m+3) (first sub-diagonal).
+__constant__ double kTideK[7][7], kTideA[7][7], kTideB[7][7], kTideD[7], kTideS[7];
+
 // Tide gradient (degree <= 6, combined solid + ocean coefficients) and
 // Schwarzschild term, SI inside, km/s^2 out (HpopCorrections.h:339-376,
-// EarthCorrections.h:57-69).
+// EarthCorrections.h:57-69). The Legendre values are generated column by
+// column (order m outer, degree n inner) and consumed at once, so nothing
+// but the two previous degrees of one column is kept in registers.
 template <class Rec>
 __device__ void correctionsAccel(const ForceModel& fm, const Rec& rec, double px, double py, double pz, double vx, double vy,
                                  double vz, double& ax, double& ay, double& az) {
@@ -353,55 +459,57 @@ __device__ void correctionsAccel(const ForceModel& fm, const Rec& rec, double px
         const double xy2 = r0 * r0 + r1 * r1, xy = sqrt(xy2);
         const double s = r2 / d, c = xy / d;
         const double cl = r0 / xy, sl = r1 / xy;
-        double p[7][7], dp[7][7];
-        for (int n = 0; n < 7; ++n)
-            for (int m = 0; m < 7; ++m) p[n][m] = dp[n][m] = 0.0;
-        p[0][0] = 1.0;
-        p[1][1] = sqrt(3.0) * c;
-        dp[1][1] = -sqrt(3.0) * s;
-        for (int n = 2; n <= 6; ++n) {
-            const double k = sqrt((2.0 * n + 1) / (2.0 * n));
-            p[n][n] = k * c * p[n - 1][n - 1];
-            dp[n][n] = k * (c * dp[n - 1][n - 1] - s * p[n - 1][n - 1]);
-        }
-        for (int n = 1; n <= 6; ++n) {
-            const double k = sqrt(2.0 * n + 1);
-            p[n][n - 1] = k * s * p[n - 1][n - 1];
-            dp[n][n - 1] = k * (c * p[n - 1][n - 1] + s * dp[n - 1][n - 1]);
-        }
-        for (int m = 0; m <= 6; ++m)
-            for (int n = m + 2; n <= 6; ++n) {
-                const double k = sqrt((2.0 * n + 1) / ((n - m) * double(n + m)));
-                const double a = sqrt(2.0 * n - 1);
-                const double b = sqrt(((n + m - 1) * double(n - m - 1)) / (2.0 * n - 3));
-                p[n][m] = k * (a * s * p[n - 1][m] - b * p[n - 2][m]);
-                dp[n][m] = k * (a * s * dp[n - 1][m] + a * c * p[n - 1][m] - b * dp[n - 2][m]);
-            }
-        double cosM[7], sinM[7];
-        cosM[0] = 1.0;
-        sinM[0] = 0.0;
-        for (int m = 1; m <= 6; ++m) {
-            cosM[m] = cosM[m - 1] * cl - sinM[m - 1] * sl;
-            sinM[m] = sinM[m - 1] * cl + cosM[m - 1] * sl;
-        }
-
         const double ratio = fm.radiusSI / d;
-        double power = 1.0, dUdr = 0.0, dUdlat = 0.0, dUdlon = 0.0;
-        for (int n = 0; n <= 6; ++n) {
-            const double b1 = (-mu / (d * d)) * power * (n + 1);
-            const double b2 = (mu / d) * power;
-            double q1 = 0.0, q2 = 0.0, q3 = 0.0;
-            for (int m = 0; m <= n; ++m) {
-                const double Cnm = rec(REC_TIDE_C + n * (n + 1) / 2 + m), Snm = rec(REC_TIDE_S + n * (n + 1) / 2 + m);
-                const double value = Cnm * cosM[m] + Snm * sinM[m];
-                q1 += p[n][m] * value;
-                q2 += dp[n][m] * value;
-                q3 += m * p[n][m] * (Snm * cosM[m] - Cnm * sinM[m]);
+        // b1_n = (-mu/d^2) (R/d)^n (n+1), b2_n = (mu/d) (R/d)^n
+        double pw[7];
+        pw[0] = 1.0;
+        for (int n = 1; n <= 6; ++n) pw[n] = pw[n - 1] * ratio;
+        double dUdr = 0.0, dUdlat = 0.0, dUdlon = 0.0;
+        double pmm = 1.0, dpmm = 0.0, cm = 1.0, sm = 0.0;
+        for (int m = 0; m <= 6; ++m) {
+            if (m == 1) {
+                pmm = sqrt(3.0) * c;
+                dpmm = -sqrt(3.0) * s;
+            } else if (m >= 2) {
+                const double k = kTideD[m];
+                const double np = k * c * pmm;
+                dpmm = k * (c * dpmm - s * pmm);
+                pmm = np;
             }
-            dUdr += q1 * b1;
-            dUdlat += q2 * b2;
-            dUdlon += q3 * b2;
-            power *= ratio;
+            if (m >= 1) {
+                const double nc = cm * cl - sm * sl;
+                sm = sm * cl + cm * sl;
+                cm = nc;
+            }
+            double p2 = 0.0, dp2 = 0.0, p1 = pmm, dp1 = dpmm;
+            for (int n = m; n <= 6; ++n) {
+                double pn, dpn;
+                if (n == m) {
+                    pn = pmm;
+                    dpn = dpmm;
+                } else if (n == m + 1) {
+                    const double k = kTideS[m];
+                    pn = k * s * pmm;
+                    dpn = k * (c * pmm + s * dpmm);
+                } else {
+                    const double k = kTideK[n][m], a = kTideA[n][m], b = kTideB[n][m];
+                    pn = k * (a * s * p1 - b * p2);
+                    dpn = k * (a * s * dp1 + a * c * p1 - b * dp2);
+                }
+                if (n > m) {
+                    p2 = p1;
+                    dp2 = dp1;
+                    p1 = pn;
+                    dp1 = dpn;
+                }
+                const double Cnm = rec(REC_TIDE_C + n * (n + 1) / 2 + m), Snm = rec(REC_TIDE_S + n * (n + 1) / 2 + m);
+                const double value = Cnm * cm + Snm * sm;
+                const double b1 = (-mu / (d * d)) * pw[n] * (n + 1);
+                const double b2 = (mu / d) * pw[n];
+                dUdr += pn * value * b1;
+                dUdlat += dpn * value * b2;
+                dUdlon += m * pn * (Snm * cm - Cnm * sm) * b2;
+            }
         }
         const double q = (1.0 / d * dUdr - r2 / (d * d * xy) * dUdlat);
         const double bf0 = q * r0 - (1.0 / xy2 * dUdlon) * r1;
@@ -441,7 +549,9 @@ __device__ void pointGravity(const ForceModel& fm, const Rec& rec, double px, do
         ay = (-fm.gmEarth * py) * rc;
         az = (-fm.gmEarth * pz) * rc;
     }
-    for (int b = 0; b < fm.nBodies; ++b) {
+#pragma unroll
+    for (int b = 0; b < kMaxBodies; ++b) {
+        if (b >= fm.nBodies) break;
         if (!fm.bodyPointGravity[b] || fm.bodyNaif[b] == 399) continue;
         const int q = REC_BODY + 3 * b;
         const double dx = px - rec(q), dy = py - rec(q + 1), dz = pz - rec(q + 2);
@@ -458,17 +568,17 @@ __device__ void pointGravity(const ForceModel& fm, const Rec& rec, double px, do
 }
 
 // Which wave carries which extra force family for a W-wave tile.
-__device__ constexpr int dragWave(int W) { return W - 1; }
 __device__ constexpr int corrWave(int W) { return W >= 3 ? W - 2 : W - 1; }
 
 __constant__ double kRkC[12] = {2. / 27, 1. / 9, 1. / 6, 5. / 12, 1. / 2, 5. / 6, 1. / 6, 2. / 3, 1. / 3, 1.0, 0, 1.0};
 
 // One stage's force evaluation for the tile: SH order block of this wave,
 // the extra families assigned to this wave, then (wave 0) the sum.
-template <int W, class Rec>
+template <int W, bool FAST, class Rec>
 __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, const Rec& rec, const double* nrl, int wave,
                                    int lane, int m0, int m1, double ams, double bcd, double (*xch)[kTile],
-                                   double (*part)[3][kTile], double (*extra)[3][kTile], double out[3]) {
+                                   double (*part)[3][kTile], double (*extra)[3][kTile], double (*spec)[kTile],
+                                   double out[3]) {
     const double px = xch[0][lane], py = xch[1][lane], pz = xch[2][lane];
     ShGeom g;
     if (fm.hasSh) {
@@ -479,12 +589,13 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
         part[wave][1][lane] = sp;
         part[wave][2][lane] = sl;
     }
-    if (wave == dragWave(W)) {
-        double ax = 0.0, ay = 0.0, az = 0.0;
-        if (fm.drag) dragAccel(fm, rec, nrl, a.nrlCoeff, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], bcd, ax, ay, az);
-        extra[0][0][lane] = ax;
-        extra[0][1][lane] = ay;
-        extra[0][2][lane] = az;
+    // NRLMSISE-00: every wave evaluates its share of the species for the
+    // lanes below the cutoff; wave 0 assembles the density after the barrier.
+    NrlLane nl;
+    nl.state = 0;
+    if (fm.drag) {
+        nl = nrlLane(fm, rec, nrl, px, py, pz);
+        if (__ballot(nl.state == 1) != 0ull) nrlSpecies<FAST>(nl, nrl, a.nrlCoeff, wave, W, spec, lane);
     }
     if (wave == corrWave(W)) {
         double ax = 0.0, ay = 0.0, az = 0.0;
@@ -517,9 +628,26 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
             ay += hy;
             az += hz;
         }
-        out[0] = ax + (extra[1][0][lane] + extra[0][0][lane]);
-        out[1] = ay + (extra[1][1][lane] + extra[0][1][lane]);
-        out[2] = az + (extra[1][2][lane] + extra[0][2][lane]);
+        double dx = 0.0, dy = 0.0, dz = 0.0;
+        if (fm.drag && nl.state != 0) {
+            double rho;
+            if (nl.state == 1) rho = nrlRhoFromSpecies(fm, spec, lane);
+            else if (nl.state == 2) rho = nrlDensity<FAST>(fm, nrl, a.nrlCoeff, nl.in);
+            else rho = nan("");
+            // Drag.h:67-73 with v_rel = v - omega x r (Earth is the centre)
+            const double vx = xch[3][lane], vy = xch[4][lane], vz = xch[5][lane];
+            const double om0 = rec(REC_OMEGA), om1 = rec(REC_OMEGA + 1), om2 = rec(REC_OMEGA + 2);
+            const double rx = vx - (om1 * pz - om2 * py);
+            const double ry = vy - (om2 * px - om0 * pz);
+            const double rz = vz - (om0 * py - om1 * px);
+            const double f = -0.5 * rho * bcd * sqrt(rx * rx + ry * ry + rz * rz);
+            dx = f * rx;
+            dy = f * ry;
+            dz = f * rz;
+        }
+        out[0] = ax + (extra[1][0][lane] + dx);
+        out[1] = ay + (extra[1][1][lane] + dy);
+        out[2] = az + (extra[1][2][lane] + dz);
     }
 }
 
@@ -529,6 +657,7 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
     __shared__ double xch[7][kTile];          // staged position, velocity, stage epoch
     __shared__ double part[W][3][kTile];
     __shared__ double extra[2][3][kTile];
+    __shared__ double spec[8][kTile];          // NRLMSISE-00 species densities
     __shared__ int anyActive;
 
     const int lane = threadIdx.x & 63;
@@ -591,12 +720,12 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
                 const double ts = xch[6][lane];
                 const GridRec rec = gridRec(a.records, a.gridCount, a.gridT0, a.gridInvH, ts);
                 double nrl[13];
-                if (fm.drag && wave == dragWave(W)) epochdev::nrlEpoch(wa, ts, nrl);
-                stageForces<W>(a, fm, rec, nrl, wave, lane, m0, m1, ams, bcd, xch, part, extra, acc);
+                if (fm.drag) epochdev::nrlEpoch(wa, ts, nrl, nullptr);
+                stageForces<W, false>(a, fm, rec, nrl, wave, lane, m0, m1, ams, bcd, xch, part, extra, spec, acc);
             } else {
                 const ExactRec rec{recStep + s * REC_SIZE};
-                stageForces<W>(a, fm, rec, recStep + s * REC_SIZE + REC_NRL, wave, lane, m0, m1, ams, bcd, xch, part, extra,
-                               acc);
+                stageForces<W, true>(a, fm, rec, recStep + s * REC_SIZE + REC_NRL, wave, lane, m0, m1, ams, bcd, xch, part, extra,
+                                     spec, acc);
             }
             if (wave == 0) {
                 kst[s][0][lane] = xch[3][lane];
@@ -673,7 +802,7 @@ __global__ void __launch_bounds__(64) forceProbeKernel(int n, const double* pos,
         ax += bx, ay += by, az += bz;
     }
     if ((which < 0 || which == 4) && fm.drag) {
-        dragAccel(fm, rec, recp + REC_NRL, nrlCoeff, px, py, pz, vx, vy, vz, bcd[i], bx, by, bz);
+        dragAccel<true>(fm, rec, recp + REC_NRL, nrlCoeff, px, py, pz, vx, vy, vz, bcd[i], bx, by, bz);
         ax += bx, ay += by, az += bz;
     }
     if ((which < 0 || which == 5) && (fm.tides || fm.relativity)) {
@@ -707,13 +836,31 @@ __global__ void __launch_bounds__(64) nrlProbeKernel(int n, const double* in, in
     nrlmsise::Output o;
     nrlmsise::State st;
     nrlmsise::NrlTables T{coeff};
-    nrlmsise::gtd7d(st, T, x, fl, o);
+    nrlmsise::gtd7d(st, T, x, fl, o, nrlmsise::GenericGlobe{});
     for (int k = 0; k < 9; ++k) out[11 * i + k] = o.d[k];
     out[11 * i + 9] = o.t[0];
     out[11 * i + 10] = o.t[1];
 }
 
 }  // namespace
+
+int uploadTideFactors() {
+    double K[7][7] = {}, A[7][7] = {}, B[7][7] = {}, D[7] = {}, S[7] = {};
+    for (int m = 0; m <= 6; ++m)
+        for (int n = m + 2; n <= 6; ++n) {
+            K[n][m] = sqrt((2.0 * n + 1) / ((n - m) * double(n + m)));
+            A[n][m] = sqrt(2.0 * n - 1);
+            B[n][m] = sqrt(((n + m - 1) * double(n - m - 1)) / (2.0 * n - 3));
+        }
+    for (int n = 2; n <= 6; ++n) D[n] = sqrt((2.0 * n + 1) / (2.0 * n));
+    for (int m = 0; m <= 6; ++m) S[m] = sqrt(2.0 * m + 3);
+    if (hipMemcpyToSymbol(HIP_SYMBOL(kTideK), K, sizeof K) != hipSuccess) return -1;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(kTideA), A, sizeof A) != hipSuccess) return -1;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(kTideB), B, sizeof B) != hipSuccess) return -1;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(kTideD), D, sizeof D) != hipSuccess) return -1;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(kTideS), S, sizeof S) != hipSuccess) return -1;
+    return 0;
+}
 
 int launchStep(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t stream) {
     if (a.nTiles <= 0) return 0;
