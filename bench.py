@@ -1,16 +1,30 @@
-"""Benchmark: object-steps/s of the full-force 70x70 RKF78 propagation.
+"""Benchmark: object-steps/s of the full-force 70x70 RKF78 propagation, FP64
+vs mixed precision, uneven batches, and the serving loop's latency.
 
-Workload: 16,384 LEO/MEO objects (seed 2 population), 70x70 synthetic field,
-NRLMSISE-00 drag, SRP with Earth shadow, Sun/Moon/8 planetary barycenters,
-solid + ocean tides, relativity; 60 s fixed step over one day. One "step"
-is one RKF78 integration step of the whole batch (13 force evaluations per
-object). Inputs are device-resident when the timed region starts.
+Headline (`value`): 16,384 LEO/MEO objects (seed-2 population), 70x70
+synthetic field, NRLMSISE-00 drag, SRP with Earth shadow, Sun/Moon/8 planetary
+barycenters, solid + ocean tides, relativity; 60 s fixed step over one day in
+FP64. A "step" is one RKF78 integration step of the whole batch (13 force
+evaluations per object); the timed region starts with the batch resident in
+HBM.
 
-Prints one JSON line (rank 0, one GPU). --gpus N > 1 exits: this build runs on
-one GPU and the path shards by object slots.
+Further legs (one JSON object each in the same line):
+  mixed   16,384 objects, 120x120 Kaula-rule field, 30 s step: FP64 vs mixed
+          precision throughput and the mixed result's position error against
+          the FP64 run (all objects) and the CPU oracle (64-object subset)
+  uneven  131,072 objects, epochs over 30 days, durations log-uniform
+          10 min - 7 days (shortened final steps), autotuned tile shape,
+          multi-step launches until every object has ended
+  serve   Poisson arrivals of requests (1 - 50,000 objects log-uniform, 1 h -
+          3 d log-uniform, uneven-batch distributions) on a mixed-precision
+          70x70 model: p50/p99 latency, object-steps/s, error vs the FP64 path
+
+--gpus N > 1 exits: this build runs on one GPU (the propagation shards by
+object slots with no exchange between GPUs).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import tempfile
@@ -37,108 +51,283 @@ def flops_per_object_step(L):
     return 13.0 * (f_sh + F_OTHER) + F_RK
 
 
-def cpu_baseline(cfg, t, sample_objects, sample_seconds, threads):
-    import oracle
-    idx = np.linspace(0, t["n"] - 1, sample_objects).astype(int)
-    sub = {k: (v[idx] if k not in ("state", "n") else v) for k, v in t.items()}
-    sub["state"] = np.ascontiguousarray(t["state"][:, idx])
-    sub["n"] = len(idx)
-    o = oracle.Oracle(cfg)
-    e, s, st = o.propagate(sub, sample_seconds, threads=threads)
-    return sub, s, st
+def subset(t, idx):
+    idx = np.asarray(idx)
+    out = {k: (v[idx] if hasattr(v, "shape") and v.ndim == 1 and k != "state" else v) for k, v in t.items()}
+    out["state"] = np.ascontiguousarray(t["state"][:, idx])
+    out["n"] = len(idx)
+    return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=240)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--objects", type=int, default=16384)
-    ap.add_argument("--degree", type=int, default=70)
-    ap.add_argument("--waves", type=int, default=0, help="waves per 64-object tile (0: library default)")
-    ap.add_argument("--steps-per-launch", type=int, default=0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-objects", type=int, default=1024)
-    ap.add_argument("--cpu-seconds", type=float, default=3600.0)
-    ap.add_argument("--workdir", default=None)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: the propagation shards by object slots with no exchange; "
-              "multi-GPU runs are not part of this build (requested --gpus %d)" % args.gpus, file=sys.stderr)
-        sys.exit(2)
+def pos_err_mm(a, b):
+    d = np.linalg.norm(a[:3] - b[:3], axis=0) * 1e6
+    return float(np.sqrt(np.mean(d ** 2))), float(np.max(d))
 
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_steps(b, total, chunk):
+    """Runs `total` steps in calls of at most `chunk`; returns summed stats."""
+    acc = {"object_steps": 0.0, "kernel_seconds": 0.0, "launches": 0.0}
+    done = 0
+    while done < total:
+        k = min(chunk, total - done)
+        st = b.run(k)
+        for key in acc:
+            acc[key] += st[key]
+        acc["waves"], acc["steps_per_launch"] = st["waves"], st["steps_per_launch"]
+        done += k
+    return acc
+
+
+def leg_main(args, work):
     import parahpop_amd
     from parahpop_amd import synth
-
-    work = args.workdir or tempfile.mkdtemp(prefix="phx_bench_")
     duration = 86400.0
-    cfg, tg = synth.make_workload(work, 2, n=args.objects, duration=duration, degree=args.degree)
+    cfg, tg = synth.make_workload(os.path.join(work, "w2"), 2, n=args.objects, duration=duration, degree=args.degree)
     t = synth.load_targets(tg)
     ctx = parahpop_amd.Context(cfg)
     if args.waves:
         ctx.set_option("waves", args.waves)
-    if args.steps_per_launch:
-        ctx.set_option("steps_per_launch", args.steps_per_launch)
+    spl = args.steps_per_launch or int(ctx.option("steps_per_launch"))
+    ctx.set_option("steps_per_launch", spl)
     b = ctx.batch(t, duration)
     day_steps = int(round(duration / 60.0))
-
-    pos = [0]
-
-    def run(k):
-        stats = []
-        while k > 0:
-            if pos[0] >= day_steps:
-                b.reset()
-                pos[0] = 0
-            c = min(k, day_steps - pos[0])
-            stats.append(b.run(c))
-            pos[0] += c
-            k -= c
-        return stats
-
     if args.warmup:
-        run(args.warmup)
+        run_steps(b, min(args.warmup, day_steps), spl)
     b.reset()
-    pos[0] = 0
+    steps = min(args.steps, day_steps)
     t0 = time.perf_counter()
-    stats = run(args.steps)
+    st = run_steps(b, steps, steps)
     wall = time.perf_counter() - t0
-    obj_steps = sum(s["object_steps"] for s in stats)
-    kern = sum(s["kernel_seconds"] for s in stats)
-    value = obj_steps / wall
+    obj_steps = st["object_steps"]
+    kern = st["kernel_seconds"]
     fps = flops_per_object_step(args.degree)
-    launches = sum(s["launches"] for s in stats)
     achieved = fps * obj_steps / kern / 1e12
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": obj_steps / wall,
         "unit": "object-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
+        "ms_per_step": 1e3 * wall / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded population, synthetic 70x70 field, constant space weather/EOP, analytic ephemeris)",
         "config": {"workload": "full-force 70x70, 16384 LEO/MEO objects, 60 s step, 1 day",
-                   "objects": args.objects, "degree": args.degree, "waves": stats[0]["waves"],
-                   "steps_per_launch": stats[0]["steps_per_launch"]},
+                   "objects": args.objects, "degree": args.degree, "waves": st["waves"],
+                   "steps_per_launch": st["steps_per_launch"]},
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel_ms_avg": 1e3 * kern / max(launches, 1), "flops_per_object_step": fps},
+                     "kernel_ms_avg": 1e3 * kern / max(st["launches"], 1), "launches": st["launches"],
+                     "flops_per_object_step": fps},
     }
+    traffic = os.path.join(ROOT, "profiles", "rkf78_full_force_70x70_traffic.json")
+    if os.path.exists(traffic):
+        tr = json.load(open(traffic))
+        line["roofline"]["traffic"] = tr["hbm_bytes_per_step"] * st["steps_per_launch"]
+        line["roofline"]["traffic_source"] = os.path.relpath(traffic, ROOT)
     if not args.no_cpu:
+        import oracle
         threads = min(16, os.cpu_count() or 1)
-        sub, s_cpu, st = cpu_baseline(cfg, t, args.cpu_objects, args.cpu_seconds, threads)
+        idx = np.linspace(0, t["n"] - 1, args.cpu_objects).astype(int)
+        sub = subset(t, idx)
+        e_o, s_o, so = oracle.Oracle(cfg).propagate(sub, args.cpu_seconds, threads=threads)
         e_g, s_g, _ = ctx.propagate(sub, args.cpu_seconds)
-        err_mm = float(np.max(np.linalg.norm(s_g[:3] - s_cpu[:3], axis=0)) * 1e6)
-        line["cpu_baseline"] = {"value": st["object_steps"] / st["seconds"], "unit": "object-steps/s", "cores": threads,
+        line["cpu_baseline"] = {"value": so["object_steps"] / so["seconds"], "unit": "object-steps/s", "cores": threads,
                                 "kind": "port",
                                 "sample": "%d evenly spaced objects of the same population, %.0f s (%d object-steps)"
-                                          % (sub["n"], args.cpu_seconds, int(st["object_steps"]))}
-        line["error_mm_vs_cpu"] = err_mm
+                                          % (sub["n"], args.cpu_seconds, int(so["object_steps"]))}
+        rms, mx = pos_err_mm(s_g, s_o)
+        dv = float(np.max(np.linalg.norm(s_g[3:] - s_o[3:], axis=0)) * 1e9)
+        line["error_vs_cpu"] = {"pos_mm_rms": rms, "pos_mm_max": mx, "vel_um_s_max": dv,
+                                "epochs_identical": bool(np.array_equal(e_g, e_o))}
+    return line
+
+
+def leg_mixed(args, work):
+    import parahpop_amd
+    from parahpop_amd import synth
+    duration = args.mixed_seconds
+    d = os.path.join(work, "w3")
+    cfg, tg = synth.make_workload(d, 3, n=args.objects, duration=duration)
+    t = synth.load_targets(tg)
+    doc = json.load(open(cfg))
+    doc["execution"]["precision"] = {"mode": "fp64"}
+    cfg64 = os.path.join(d, "CONFIG_fp64.json")
+    json.dump(doc, open(cfg64, "w"))
+    out = {"workload": "full-force 120x120 Kaula field, 16384 objects, 30 s step, %.0f s" % duration}
+    states = {}
+    for mode, path in (("fp64", cfg64), ("mixed", cfg)):
+        ctx = parahpop_amd.Context(path)
+        b = ctx.batch(t, duration)
+        b.run(2)
+        b.reset()
+        t0 = time.perf_counter()
+        st = b.run()
+        wall = time.perf_counter() - t0
+        states[mode] = b.download()[1]
+        out[mode] = {"value": st["object_steps"] / wall, "kernel_seconds": st["kernel_seconds"], "waves": st["waves"]}
+        log("mixed leg: %s %.3g object-steps/s" % (mode, out[mode]["value"]))
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    rms, mx = pos_err_mm(states["mixed"], states["fp64"])
+    out["mixed_vs_fp64_pos_mm"] = {"rms": rms, "max": mx}
+    if not args.no_cpu:
+        import oracle
+        idx = np.linspace(0, t["n"] - 1, 64).astype(int)
+        e_o, s_o, _ = oracle.Oracle(cfg64).propagate(subset(t, idx), duration, threads=min(16, os.cpu_count() or 1))
+        rms, mx = pos_err_mm(states["mixed"][:, idx], s_o)
+        out["mixed_vs_oracle_pos_mm_64obj"] = {"rms": rms, "max": mx}
+        rms, mx = pos_err_mm(states["fp64"][:, idx], s_o)
+        out["fp64_vs_oracle_pos_mm_64obj"] = {"rms": rms, "max": mx}
+    return out
+
+
+def leg_uneven(args, work):
+    import parahpop_amd
+    from parahpop_amd import synth
+    n = args.uneven_objects
+    cfg, tg = synth.make_workload(os.path.join(work, "w4"), 4, n=n, duration=604800.0)
+    t = synth.load_targets(tg)
+    rng = np.random.default_rng(44)
+    dur = np.exp(rng.uniform(math.log(600.0), math.log(604800.0), n))
+    ctx = parahpop_amd.Context(cfg, steps_per_launch=args.uneven_steps_per_launch)
+    b = ctx.batch(t, dur)
+    tune = b.autotune(4)
+    t0 = time.perf_counter()
+    st = b.run()
+    wall = time.perf_counter() - t0
+    assert b.done()
+    e, _ = b.download()
+    end_err = float(np.max(np.abs((e - t["epoch"]) * 86400.0 - dur)))
+    return {"workload": "uneven: %d objects, epochs U[4035,4065], durations log-U[600 s, 7 d], 70x70 full force" % n,
+            "value": st["object_steps"] / wall, "object_steps": st["object_steps"],
+            "expected_object_steps": float(np.ceil(dur / 60.0).sum()), "wall_seconds": wall,
+            "kernel_seconds": st["kernel_seconds"], "launches": st["launches"], "epoch_records": st["epoch_records"],
+            "tuned": {"waves": int(ctx.option("waves")), "share0": ctx.option("share0"),
+                      "trial_ms_per_step": 1e3 * tune["kernel_seconds"] / 4},
+            "max_end_epoch_error_s": end_err}
+
+
+def leg_serve(args, work):
+    import parahpop_amd
+    from parahpop_amd import synth
+    d = os.path.join(work, "w5")
+    pool_n = args.serve_pool
+    cfg, tg = synth.make_workload(d, 4, n=pool_n, duration=3 * 86400.0)
+    t = synth.load_targets(tg)
+    doc = json.load(open(cfg))
+    doc["execution"]["precision"] = {"mode": "mixed", "fp64Degree": 20}
+    cfg_mixed = os.path.join(d, "CONFIG_mixed.json")
+    json.dump(doc, open(cfg_mixed, "w"))
+    rng = np.random.default_rng(5)
+    nreq = args.serve_requests
+    sizes = np.exp(rng.uniform(0.0, math.log(args.serve_max_objects), nreq)).astype(int).clip(1, pool_n)
+    durs = np.exp(rng.uniform(math.log(3600.0), math.log(3 * 86400.0), nreq))
+    gaps = rng.exponential(1.0 / args.serve_rate, nreq)
+    starts = rng.integers(0, pool_n - sizes + 1)
+    reqs = [(subset(t, np.arange(s, s + k)), float(du)) for s, k, du in zip(starts, sizes, durs)]
+    ctx = parahpop_amd.Context(cfg_mixed, serve_steps_per_tick=args.serve_steps_per_tick)
+    # warm the server (pool allocation, graph capture) with a tiny request
+    ctx.wait(ctx.submit(subset(t, [0]), 600.0))
+    steps0 = ctx.option("serve_object_steps")
+    tickets, results = [], {}
+    t0 = time.perf_counter()
+    next_t = t0
+    for i, (r, du) in enumerate(reqs):
+        next_t += gaps[i]
+        while time.perf_counter() < next_t:
+            for k in list(tickets):
+                res = ctx.poll(k)
+                if res is not None:
+                    results[k] = res
+                    tickets.remove(k)
+            time.sleep(0.001)
+        tickets.append(ctx.submit(r, du))
+        if i == 0:
+            first = tickets[-1]
+        log("serve: submitted %d/%d (%d objects, %.0f s)" % (i + 1, nreq, r["n"], du))
+    order = list(tickets)
+    all_tickets = list(range(first, first + nreq))
+    for k in order:
+        results[k] = ctx.wait(k, timeout=900.0)
+    wall = time.perf_counter() - t0
+    lat = np.array([results[k][3] - results[k][2] for k in all_tickets])
+    obj_steps = ctx.option("serve_object_steps") - steps0
+    # error vs the FP64 drop-in path for the smallest requests
+    small = np.argsort(sizes)[: min(4, nreq)]
+    ctx64 = parahpop_amd.Context(cfg)
+    errs = []
+    for j in small:
+        r, du = reqs[j]
+        s64 = ctx64.propagate(r, du)[1]
+        errs.append(pos_err_mm(results[all_tickets[j]][1], s64)[1])
+    return {"workload": "serving: %d Poisson requests at %.2f/s, 1-%d objects log-U, 1 h-3 d log-U, mixed 70x70"
+                        % (nreq, args.serve_rate, args.serve_max_objects),
+            "object_steps_per_s": obj_steps / wall, "object_steps": obj_steps, "wall_seconds": wall,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+            "latency_max_s": float(lat.max()), "graph_captures": ctx.option("serve_graph_captures"),
+            "ticks": ctx.option("serve_ticks"),
+            "mixed_vs_fp64_pos_mm_max_smallest4": float(max(errs)) if errs else None}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1440, help="timed steps of the headline leg (1440 = one day)")
+    ap.add_argument("--warmup", type=int, default=32)
+    ap.add_argument("--objects", type=int, default=16384)
+    ap.add_argument("--degree", type=int, default=70)
+    ap.add_argument("--waves", type=int, default=0, help="waves per 64-object tile (0: library default)")
+    ap.add_argument("--steps-per-launch", type=int, default=0)
+    ap.add_argument("--legs", default="main,mixed,uneven,serve")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-objects", type=int, default=1024)
+    ap.add_argument("--cpu-seconds", type=float, default=86400.0)
+    ap.add_argument("--mixed-seconds", type=float, default=86400.0)
+    ap.add_argument("--uneven-objects", type=int, default=131072)
+    ap.add_argument("--uneven-steps-per-launch", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=1.0)
+    ap.add_argument("--serve-max-objects", type=int, default=50000)
+    ap.add_argument("--serve-pool", type=int, default=65536)
+    ap.add_argument("--serve-steps-per-tick", type=int, default=16)
+    ap.add_argument("--workdir", default=None)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (scales the HBM byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the propagation shards by object slots with no exchange; "
+              "multi-GPU runs are not part of this build (requested --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "copies": 5}))
+        return
+
+    work = args.workdir or tempfile.mkdtemp(prefix="phx_bench_")
+    legs = [x for x in args.legs.split(",") if x]
+    line = leg_main(args, work) if "main" in legs else {"metric": METRIC}
+    log("main: %.4g object-steps/s" % line.get("value", float("nan")))
+    if "mixed" in legs:
+        line["mixed"] = leg_mixed(args, work)
+    if "uneven" in legs:
+        line["uneven"] = leg_uneven(args, work)
+        log("uneven: %.4g object-steps/s" % line["uneven"]["value"])
+    if "serve" in legs:
+        line["serve"] = leg_serve(args, work)
+        log("serve: p50 %.2f s p99 %.2f s" % (line["serve"]["latency_p50_s"], line["serve"]["latency_p99_s"]))
     print(json.dumps(line))
 
 

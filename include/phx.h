@@ -65,6 +65,21 @@ int phx_batch_create(phx_ctx* ctx, size_t n, const int64_t* id, const double* ep
                      const double* state6_soa, const double* mass_kg, const double* drag_area_m2,
                      const double* srp_area_m2, const double* cd, const double* cr, double duration_s,
                      phx_batch** out);
+/* As phx_batch_create with one duration per object (duration_s[n]): the
+ * uneven batch of epochs, durations and shortened final steps runs as one
+ * device-resident batch (epoch records on a uniform grid, tiles ordered
+ * longest work first). Extension: the reference takes one duration per run
+ * (engine/paraHPOP/run.h:5-13). */
+int phx_batch_create_uneven(phx_ctx* ctx, size_t n, const int64_t* id, const double* epoch_mjd2000_tdb,
+                            const double* state6_soa, const double* mass_kg, const double* drag_area_m2,
+                            const double* srp_area_m2, const double* cd, const double* cr, const double* duration_s,
+                            phx_batch** out);
+/* Autotuner (no reference counterpart; the reference reads fixed
+ * PARAHPOP_PERF_* overrides, AccumulateBodies.cu:20-48): times trial_steps
+ * steps of this batch for each candidate tile shape (waves per 64-object tile,
+ * share of the SH order blocks on wave 0), keeps the fastest in the context's
+ * options, resets the batch and reports the winner's trial statistics. */
+int phx_batch_autotune(phx_batch* b, int trial_steps, phx_stats* best);
 /* Runs up to max_steps further integration steps (<= 0: to completion). */
 int phx_batch_run(phx_batch* b, int max_steps, phx_stats* stats);
 /* 1 when every object reached its end of simulation, else 0. */

@@ -12,12 +12,12 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libphx.so")
-REC_SIZE = 128
+REC_SIZE = 320
 
 PHX_OK, PHX_EINVAL, PHX_EDEVICE, PHX_ENOTDONE, PHX_ENOMEM, PHX_EAGAIN = 0, -1, -2, -3, -4, -5
 
 EXPORTED = [
-    "phx_create", "phx_propagate", "phx_batch_create", "phx_batch_run", "phx_batch_done", "phx_batch_download",
+    "phx_create", "phx_propagate", "phx_batch_create", "phx_batch_create_uneven", "phx_batch_autotune", "phx_batch_run", "phx_batch_done", "phx_batch_download",
     "phx_batch_reset", "phx_batch_destroy", "phx_set_option", "phx_get_option", "phx_submit", "phx_poll",
     "phx_last_error", "phx_destroy", "phx_force_probe", "phx_epoch_record", "phx_nrlmsise_probe", "phx_version",
 ]
@@ -66,6 +66,8 @@ def library():
     lib.phx_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
     lib.phx_propagate.argtypes = [vp, sz] + [vp] * 8 + [d, vp, vp, vp]
     lib.phx_batch_create.argtypes = [vp, sz] + [vp] * 8 + [d, ctypes.POINTER(vp)]
+    lib.phx_batch_create_uneven.argtypes = [vp, sz] + [vp] * 9 + [ctypes.POINTER(vp)]
+    lib.phx_batch_autotune.argtypes = [vp, i, vp]
     lib.phx_batch_run.argtypes = [vp, i, vp]
     lib.phx_batch_done.argtypes = [vp]
     lib.phx_batch_download.argtypes = [vp, vp, vp]
@@ -83,9 +85,8 @@ def library():
     lib.phx_epoch_record.argtypes = [vp, d, vp, sz]
     lib.phx_nrlmsise_probe.argtypes = [vp, sz, vp, i, vp]
     lib.phx_version.restype = ctypes.c_char_p
-    if hasattr(lib, "phx_submit"):
-        lib.phx_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]
-        lib.phx_poll.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(Result)]
+    lib.phx_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]
+    lib.phx_poll.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(Result)]
     _lib = lib
     return lib
 
@@ -157,7 +158,43 @@ class Context:
         return out_e, out_s, st.as_dict()
 
     def batch(self, targets, duration):
+        """Device-resident batch; duration is a scalar or one value per object."""
         return Batch(self, targets, duration)
+
+    def submit(self, targets, duration):
+        """Queue a propagation request on the context's server; returns a ticket."""
+        n, arrs = self._inputs(targets)
+        req = Request(n, *[_ptr(a) for a in arrs], float(duration))
+        ticket = ctypes.c_uint64()
+        self._check(self.lib.phx_submit(self.h, ctypes.byref(req), ctypes.byref(ticket)))
+        self._pending = getattr(self, "_pending", {})
+        self._pending[ticket.value] = n
+        return ticket.value
+
+    def poll(self, ticket):
+        """None while the request is queued or running; else (epochs, states[6][n], submit_s, finish_s)."""
+        n = self._pending[ticket]
+        e = np.zeros(n)
+        s = np.zeros((6, n))
+        res = Result(n, _ptr(e), _ptr(s), 0.0, 0.0)
+        rc = self.lib.phx_poll(self.h, ctypes.c_uint64(ticket), ctypes.byref(res))
+        if rc == PHX_EAGAIN:
+            return None
+        del self._pending[ticket]
+        self._check(rc)
+        return e, s, res.submit_time_s, res.finish_time_s
+
+    def wait(self, ticket, timeout=600.0, interval=0.002):
+        """Poll until the request is done (raises TimeoutError)."""
+        import time
+        t_end = time.monotonic() + timeout
+        while True:
+            r = self.poll(ticket)
+            if r is not None:
+                return r
+            if time.monotonic() > t_end:
+                raise TimeoutError("request %d not done after %.0f s" % (ticket, timeout))
+            time.sleep(interval)
 
     def force(self, pos, vel, mass, area_km2, cr_eff, cd, et, which=-1):
         pos, vel = _f64(pos).reshape(3, -1), _f64(vel).reshape(3, -1)
@@ -190,7 +227,12 @@ class Batch:
         self.n = n
         self._keep = arrs
         h = ctypes.c_void_p()
-        rc = ctx.lib.phx_batch_create(ctx.h, n, *[_ptr(a) for a in arrs], float(duration), ctypes.byref(h))
+        if np.ndim(duration) == 0:
+            rc = ctx.lib.phx_batch_create(ctx.h, n, *[_ptr(a) for a in arrs], float(duration), ctypes.byref(h))
+        else:
+            self._dur = _f64(duration)
+            assert self._dur.shape == (n,)
+            rc = ctx.lib.phx_batch_create_uneven(ctx.h, n, *[_ptr(a) for a in arrs], _ptr(self._dur), ctypes.byref(h))
         ctx._check(rc)
         self.h = h
 
@@ -201,6 +243,12 @@ class Batch:
 
     def done(self):
         return bool(self.ctx.lib.phx_batch_done(self.h))
+
+    def autotune(self, trial_steps=4):
+        """Pick the fastest tile shape for this batch (sets the context's options)."""
+        st = Stats()
+        self.ctx._check(self.ctx.lib.phx_batch_autotune(self.h, int(trial_steps), ctypes.byref(st)))
+        return st.as_dict()
 
     def reset(self):
         self.ctx._check(self.ctx.lib.phx_batch_reset(self.h))

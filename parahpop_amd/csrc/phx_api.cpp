@@ -25,50 +25,12 @@
 #include <string>
 #include <vector>
 
-#include "../../include/phx.h"
-#include "model.h"
-#include "phx_common.h"
-#include "phx_kernels.h"
+#include "phx_internal.h"
 #include "phx_serve.h"
 
 using namespace phx;
 
 namespace {
-
-struct HipError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-void check(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <class T>
-struct DBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DBuf() = default;
-    DBuf(const DBuf&) = delete;
-    DBuf& operator=(const DBuf&) = delete;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(size_t count) {
-        release();
-        if (count == 0) return;
-        check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
-        n = count;
-    }
-    void upload(const T* host, size_t count, hipStream_t s) {
-        alloc(count);
-        if (count) check(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
-    }
-};
-
-const double kRkC[12] = {2. / 27, 1. / 9, 1. / 6, 5. / 12, 1. / 2, 5. / 6, 1. / 6, 2. / 3, 1. / 3, 1.0, 0, 1.0};
 
 std::string libraryDir() {
     Dl_info info;
@@ -88,51 +50,9 @@ std::string dirName(const std::string& p) {
 
 }  // namespace
 
-struct phx_ctx {
-    RunConfig run;
-    Environment env;
-    ForceModel fm{};
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::string error;
-    std::string dataDir;
-    // device tables
-    DBuf<double> shTerms, shDiag, nrl, freq, waves, terms, eop, weather;
-    DBuf<float> shTerms32;
-    DBuf<int> shOrderStart;
-    int freqCount[3] = {0, 0, 0};
-    int waveCount = 0;
-    // options
-    int nWaves = 4;
-    int stepsPerLaunch = 32;
-    double share0 = 0.85, shareCorr = 0.85, shareDrag = 0.35;
-    int sortAltitude = 1;
-    int exactGroupsMax = 16;     // more distinct (start, end) groups than this -> grid epochs
-    double gridStep = 60.0;      // s, spacing of the epoch grid
-    int forceGrid = 0;
-    // serving
-    std::unique_ptr<Server> server;
-};
-
-struct phx_batch {
-    phx_ctx* ctx = nullptr;
-    size_t n = 0;
-    int nTiles = 0;
-    int nPad = 0;
-    std::vector<int> slotOf;     // original index -> padded slot
-    int totalSteps = 0;          // steps of the longest group
-    int stepsDone = 0;
-    DBuf<double> y, cur, end, nextDt, ams, bcd, records;
-    DBuf<uint8_t> status;
-    DBuf<long long> tileRecord, counters;
-    std::vector<double> y0, cur0, dt0;
-    std::vector<uint8_t> st0;
-    size_t epochRecords = 0;
-    double epochSeconds = 0.0;
-    bool grid = false;
-    int gridCount = 0;
-    double gridT0 = 0.0, gridInvH = 0.0;
-};
+namespace phx {
+const double kRkC[12] = {2. / 27, 1. / 9, 1. / 6, 5. / 12, 1. / 2, 5. / 6, 1. / 6, 2. / 3, 1. / 3, 1.0, 0, 1.0};
+}  // namespace phx
 
 namespace {
 
@@ -303,6 +223,10 @@ void buildForceModel(phx_ctx& c) {
     fm.clightSI = 299792457.999999984;
 }
 
+}  // namespace
+
+namespace phx {
+
 EpochArgs epochArgs(const phx_ctx& c) {
     EpochArgs a{};
     a.eop = c.eop.p;
@@ -323,7 +247,6 @@ EpochArgs epochArgs(const phx_ctx& c) {
     return a;
 }
 
-// Order blocks per wave, balanced on term counts with relative shares.
 void orderBlocks(const phx_ctx& c, int W, int out[kMaxWaves + 1]) {
     const int L = c.fm.shDegree, M = c.fm.shOrder;
     std::vector<double> share(static_cast<size_t>(W), 1.0);
@@ -370,6 +293,10 @@ void validateTargets(size_t n, const int64_t* id, const double* epoch, const dou
     }
 }
 
+}  // namespace phx
+
+namespace {
+
 int fail(phx_ctx* c, const std::exception& e) {
     if (c) c->error = e.what();
     if (dynamic_cast<const HipError*>(&e)) return PHX_EDEVICE;
@@ -387,9 +314,10 @@ namespace phx {
 // per-group step sequence and epoch records.
 std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, const double* epoch, const double* state6,
                                       const double* mass, const double* drag, const double* srp, const double* cd,
-                                      const double* cr, double duration) {
+                                      const double* cr, double duration, const double* durations) {
     validateTargets(n, id, epoch, state6, mass, drag, srp, cd, cr);
-    if (!(duration > 0) || !std::isfinite(duration)) throw std::invalid_argument("duration_seconds must be finite and positive.");
+    if (!durations && (!(duration > 0) || !std::isfinite(duration)))
+        throw std::invalid_argument("duration_seconds must be finite and positive.");
     const double step = c.run.stepSeconds;
     auto b = std::make_unique<phx_batch>();
     b->ctx = &c;
@@ -398,8 +326,10 @@ std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, c
     std::vector<double> start(n), endEt(n);
     long long count = 0;
     for (size_t i = 0; i < n; ++i) {
+        const double d = durations ? durations[i] : duration;
+        if (!(d > 0) || !std::isfinite(d)) throw std::invalid_argument("duration_seconds must be finite and positive.");
         start[i] = epoch[i] * 86400.0 + -43200.0;
-        endEt[i] = (epoch[i] + duration / 86400.0) * 86400.0 + -43200.0;
+        endEt[i] = (epoch[i] + d / 86400.0) * 86400.0 + -43200.0;
         const double span = endEt[i] - start[i];
         if (!(span > 0) || !std::isfinite(span) || std::ceil(span / step) >= 2147483647.0)
             throw std::invalid_argument("Duration/step cannot be represented by the integration time/index types.");
@@ -421,6 +351,8 @@ std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, c
     std::vector<double> epochs;
     std::vector<long long> groupRecord(static_cast<size_t>(G), 0);
     int totalSteps = 0;
+    double gridT0 = 0.0;
+    int gridRows = 0;
     if (!grid) {
         std::vector<std::pair<double, double>> gkey(static_cast<size_t>(G));
         for (auto& kv : groupOf) gkey[size_t(kv.second)] = kv.first;
@@ -454,10 +386,10 @@ std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, c
             hi = std::max(hi, endEt[i]);
         }
         const double h = c.gridStep;
-        b->gridT0 = lo - 2.0 * h;
+        gridT0 = lo - 2.0 * h;
         b->gridInvH = 1.0 / h;
-        b->gridCount = int(std::ceil((hi - b->gridT0) / h)) + 3;
-        for (int k = 0; k < b->gridCount; ++k) epochs.push_back(b->gridT0 + k * h);
+        gridRows = int(std::ceil((hi - gridT0) / h)) + 3;
+        for (int k = 0; k < gridRows; ++k) epochs.push_back(gridT0 + k * h);
         totalSteps = int(launches);
     }
     b->totalSteps = totalSteps;
@@ -468,7 +400,14 @@ std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, c
     for (size_t i = 0; i < n; ++i)
         alt[i] = std::sqrt(state6[i] * state6[i] + state6[n + i] * state6[n + i] + state6[2 * n + i] * state6[2 * n + i]);
     if (grid) {
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b2) { return start[size_t(a)] < start[size_t(b2)]; });
+        // longest work first (tiles finish together and the tail is short), then by
+        // start epoch so a tile's lanes read neighbouring grid rows
+        std::vector<double> span(n);
+        for (size_t i = 0; i < n; ++i) span[i] = std::ceil((endEt[i] - start[i]) / step);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b2) {
+            if (span[size_t(a)] != span[size_t(b2)]) return span[size_t(a)] > span[size_t(b2)];
+            return start[size_t(a)] < start[size_t(b2)];
+        });
         for (size_t i = 0; i < n; ++i) grp[i] = 0;
     } else if (c.sortAltitude) {
         std::stable_sort(order.begin(), order.end(), [&](int a, int b2) {
@@ -532,6 +471,15 @@ std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, c
     b->bcd.upload(bcd.data(), bcd.size(), s);
     b->status.upload(st.data(), st.size(), s);
     b->tileRecord.upload(tileRec.data(), tileRec.size(), s);
+    if (grid) {
+        // one grid segment (record row 0) shared by every slot
+        std::vector<long long> sr(np, 0);
+        std::vector<double> st0v(np, gridT0);
+        std::vector<int> rows(np, gridRows);
+        b->slotRecord.upload(sr.data(), np, s);
+        b->slotT0.upload(st0v.data(), np, s);
+        b->slotRows.upload(rows.data(), np, s);
+    }
     long long zero[4] = {0, 0, 0, 0};
     b->counters.upload(zero, 4, s);
     b->y0 = y;
@@ -557,9 +505,43 @@ std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, c
     return b;
 }
 
+StepArgs contextStepArgs(const phx_ctx& c, int& W) {
+    StepArgs a{};
+    a.step = c.run.stepSeconds;
+    a.shTerms = c.shTerms.p;
+    a.shTerms32 = c.shTerms32.p;
+    a.shOrderStart = c.shOrderStart.p;
+    a.shDiag = c.shDiag.p;
+    a.fp64Degree = c.run.precision.mixed ? c.run.precision.fp64Degree : -1;
+    a.nrlCoeff = c.nrl.p;
+    a.weather = c.weather.p;
+    a.wDays = c.env.hasWeather ? c.env.weather.days : 0;
+    a.wStart = c.env.hasWeather ? c.env.weather.startMjdUtc : 0.0;
+    a.gridInvH = 1.0 / c.gridStep;
+    W = c.fm.hasSh ? c.nWaves : 1;
+    if (c.fm.hasSh) orderBlocks(c, W, a.blockStart);
+    else
+        for (int w = 0; w <= kMaxWaves; ++w) a.blockStart[w] = 0;
+    return a;
+}
+
+int stepsToEnd(double startEt, double endEt, double step) {
+    double cur = startEt;
+    double dt = std::min(step, endEt - cur);
+    int k = 0;
+    for (;;) {
+        cur += dt;
+        ++k;
+        const double rem = endEt - cur;
+        if (rem <= 0) return k;
+        dt = std::min(step, rem);
+    }
+}
+
 int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
     const auto w0 = std::chrono::steady_clock::now();
-    StepArgs a{};
+    int W = 1;
+    StepArgs a = contextStepArgs(c, W);
     a.nTiles = b.nTiles;
     a.nPad = b.nPad;
     a.y = b.y.p;
@@ -571,24 +553,12 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
     a.bcd = b.bcd.p;
     a.records = b.records.p;
     a.tileRecord = b.tileRecord.p;
-    a.step = c.run.stepSeconds;
-    a.shTerms = c.shTerms.p;
-    a.shTerms32 = c.shTerms32.p;
-    a.shOrderStart = c.shOrderStart.p;
-    a.shDiag = c.shDiag.p;
-    a.fp64Degree = c.run.precision.mixed ? c.run.precision.fp64Degree : -1;
-    a.nrlCoeff = c.nrl.p;
     a.counters = b.counters.p;
     a.grid = b.grid ? 1 : 0;
-    a.gridCount = b.gridCount;
-    a.gridT0 = b.gridT0;
     a.gridInvH = b.gridInvH;
-    a.weather = c.weather.p;
-    a.wDays = c.env.hasWeather ? c.env.weather.days : 0;
-    a.wStart = c.env.hasWeather ? c.env.weather.startMjdUtc : 0.0;
-    const int W = c.fm.hasSh ? c.nWaves : 1;
-    if (c.fm.hasSh) orderBlocks(c, W, a.blockStart);
-    else for (int w = 0; w <= kMaxWaves; ++w) a.blockStart[w] = 0;
+    a.slotRecord = b.slotRecord.p;
+    a.slotT0 = b.slotT0.p;
+    a.slotRows = b.slotRows.p;
     int remaining = b.totalSteps - b.stepsDone;
     if (maxSteps > 0) remaining = std::min(remaining, maxSteps);
     long long before = 0;
@@ -719,7 +689,15 @@ int phx_set_option(phx_ctx* c, const char* key, double v) {
         if (!(v > 0 && v <= 600)) { c->error = "grid_step must be in (0, 600] s"; return PHX_EINVAL; }
         c->gridStep = v;
     } else if (k == "force_grid") c->forceGrid = v != 0.0;
-    else { c->error = "unknown option " + k; return PHX_EINVAL; }
+    else if (k == "serve_steps_per_tick") {
+        if (!(v >= 1)) { c->error = "serve_steps_per_tick must be >= 1"; return PHX_EINVAL; }
+        c->serveStepsPerTick = int(v);
+    } else if (k == "serve_slots" || k == "serve_record_mib") {
+        if (c->server) { c->error = k + " must be set before the first phx_submit"; return PHX_EINVAL; }
+        if (!(v >= 1)) { c->error = k + " must be >= 1"; return PHX_EINVAL; }
+        if (k == "serve_slots") c->serveSlots = (long long)v;
+        else c->serveRecordMiB = v;
+    } else { c->error = "unknown option " + k; return PHX_EINVAL; }
     return PHX_OK;
 }
 
@@ -740,6 +718,10 @@ double phx_get_option(const phx_ctx* c, const char* key) {
     if (k == "max_num_steps") return double(c->run.maxNumSteps);
     if (k == "sh_degree") return c->fm.hasSh ? c->fm.shDegree : 0;
     if (k == "mixed") return c->run.precision.mixed ? 1 : 0;
+    if (k == "serve_steps_per_tick") return c->serveStepsPerTick;
+    if (k == "serve_slots" && !c->server) return double(c->serveSlots);
+    if (k == "serve_record_mib") return c->serveRecordMiB;
+    if (k.rfind("serve_", 0) == 0) return c->server ? c->server->stat(k) : 0.0;
     return std::nan("");
 }
 
@@ -748,7 +730,22 @@ int phx_batch_create(phx_ctx* c, size_t n, const int64_t* id, const double* epoc
                      phx_batch** out) {
     if (!c || !out) return PHX_EINVAL;
     try {
-        *out = buildBatch(*c, n, id, epoch, state6, mass, drag, srp, cd, cr, duration).release();
+        *out = buildBatch(*c, n, id, epoch, state6, mass, drag, srp, cd, cr, duration, nullptr).release();
+        return PHX_OK;
+    } catch (const std::length_error& e) {
+        c->error = e.what();
+        return PHX_ENOMEM;
+    } catch (const std::exception& e) {
+        return fail(c, e);
+    }
+}
+
+int phx_batch_create_uneven(phx_ctx* c, size_t n, const int64_t* id, const double* epoch, const double* state6,
+                            const double* mass, const double* drag, const double* srp, const double* cd, const double* cr,
+                            const double* duration_s, phx_batch** out) {
+    if (!c || !out || !duration_s) return PHX_EINVAL;
+    try {
+        *out = buildBatch(*c, n, id, epoch, state6, mass, drag, srp, cd, cr, 0.0, duration_s).release();
         return PHX_OK;
     } catch (const std::length_error& e) {
         c->error = e.what();
@@ -764,6 +761,58 @@ int phx_batch_run(phx_batch* b, int maxSteps, phx_stats* stats) {
         return runBatch(*b->ctx, *b, maxSteps, stats);
     } catch (const std::exception& e) {
         return fail(b->ctx, e);
+    }
+}
+
+int phx_batch_autotune(phx_batch* b, int trialSteps, phx_stats* best) {
+    if (!b || trialSteps < 1) return PHX_EINVAL;
+    phx_ctx& c = *b->ctx;
+    try {
+        struct Cand {
+            int waves;
+            double share0;
+        };
+        std::vector<Cand> cands;
+        if (c.fm.hasSh) {
+            for (int w : {1, 2, 4, 8})
+                for (double s0 : {0.7, 0.85, 1.0}) {
+                    if (w == 1 && s0 != 1.0) continue;
+                    cands.push_back({w, s0});
+                }
+        } else {
+            cands.push_back({1, 1.0});
+        }
+        const int keepWaves = c.nWaves;
+        const double keepShare = c.share0;
+        double bestT = 0.0;
+        int bestI = -1;
+        phx_stats st{}, bestSt{};
+        for (size_t i = 0; i < cands.size(); ++i) {
+            c.nWaves = cands[i].waves;
+            c.share0 = cands[i].share0;
+            int rc = phx_batch_reset(b);
+            if (rc) throw std::runtime_error(c.error);
+            runBatch(c, *b, trialSteps, &st);
+            if (st.object_steps <= 0) continue;
+            const double t = st.kernel_seconds / st.object_steps;
+            if (bestI < 0 || t < bestT) {
+                bestT = t;
+                bestI = int(i);
+                bestSt = st;
+            }
+        }
+        if (bestI < 0) {
+            c.nWaves = keepWaves;
+            c.share0 = keepShare;
+        } else {
+            c.nWaves = cands[size_t(bestI)].waves;
+            c.share0 = cands[size_t(bestI)].share0;
+        }
+        const int rc = phx_batch_reset(b);
+        if (best) *best = bestSt;
+        return rc;
+    } catch (const std::exception& e) {
+        return fail(&c, e);
     }
 }
 
@@ -816,6 +865,38 @@ int phx_propagate(phx_ctx* c, size_t n, const int64_t* id, const double* epoch, 
     if (rc == PHX_OK) rc = phx_batch_download(b, outEpoch, outState);
     phx_batch_destroy(b);
     if (stats) stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+int phx_submit(phx_ctx* c, const phx_request* req, uint64_t* ticket) {
+    if (!c || !req || !ticket) return PHX_EINVAL;
+    try {
+        Server* s = nullptr;
+        {
+            static std::mutex createMu;
+            std::lock_guard<std::mutex> lk(createMu);
+            if (!c->server) c->server = std::make_unique<Server>(*c);
+            s = c->server.get();
+        }
+        *ticket = s->submit(*req);
+        return PHX_OK;
+    } catch (const std::length_error& e) {
+        c->error = e.what();
+        return PHX_ENOMEM;
+    } catch (const std::exception& e) {
+        return fail(c, e);
+    }
+}
+
+int phx_poll(phx_ctx* c, uint64_t ticket, phx_result* result) {
+    if (!c) return PHX_EINVAL;
+    if (!c->server) {
+        c->error = "no request was submitted";
+        return PHX_EINVAL;
+    }
+    std::string err;
+    const int rc = c->server->poll(ticket, result, err);
+    if (rc != PHX_OK && rc != PHX_EAGAIN) c->error = err;
     return rc;
 }
 

@@ -92,7 +92,8 @@ struct StepArgs {
     const double* ams;           // [nPad] cr*area/mass (km^2/kg)
     const double* bcd;           // [nPad] cd*area/mass
     const double* records;       // epoch records
-    const long long* tileRecord; // [nTiles] record index of this tile's (step 0, stage 0)
+    const long long* tileRecord; // exact epochs: [nTiles] record index of this tile's (step 0, stage 0)
+    const int* tileList;         // [nTiles] pool tile of each workgroup (nullptr: workgroup i runs tile i)
     int stepBase;                // steps completed before this launch
     int steps;                   // steps to run in this launch
     double step;                 // nominal step (s)
@@ -104,12 +105,15 @@ struct StepArgs {
     int fp64Degree;              // mixed precision: degrees above this use FP32 terms (-1: all FP64)
     const double* nrlCoeff;      // NRLMSISE-00 coefficient blocks (device)
     long long* counters;         // [4] diagnostics: object-steps completed
-    // Uneven epochs: records on a uniform time grid t_k = gridT0 + k / gridInvH,
+    // Uneven epochs: each slot reads its own segment of grid records, rows
+    // t_k = slotT0 + k / gridInvH at record index slotRecord + k (k < slotRows),
     // interpolated per object (cubic Lagrange, rows k-1..k+2); NRLMSISE-00
     // time inputs are then computed per object from the weather table.
     int grid;
-    int gridCount;
-    double gridT0, gridInvH;
+    double gridInvH;
+    const long long* slotRecord;
+    const double* slotT0;
+    const int* slotRows;
     const double* weather;       // 13 x wDays SoA (CelesTrak components)
     int wDays;
     double wStart;

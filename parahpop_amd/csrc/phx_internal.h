@@ -1,0 +1,125 @@
+// Host-side internals shared by the C-ABI (phx_api.cpp) and the request
+// server (phx_serve.cpp): device buffers, the context and batch objects and the
+// helpers that fill the step-kernel arguments.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/phx.h"
+#include "model.h"
+#include "phx_common.h"
+#include "phx_kernels.h"
+
+namespace phx {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Owning device array (hipMalloc/hipFree).
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+        n = count;
+    }
+    void upload(const T* host, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) check(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+    }
+};
+
+class Server;
+
+// RKF78 node fractions c_2..c_13 (parm/integrate/rk/coefs/tableau.h:19-31).
+extern const double kRkC[12];
+
+}  // namespace phx
+
+struct phx_ctx {
+    phx::RunConfig run;
+    phx::Environment env;
+    phx::ForceModel fm{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string error;
+    std::string dataDir;
+    // device tables
+    phx::DBuf<double> shTerms, shDiag, nrl, freq, waves, terms, eop, weather;
+    phx::DBuf<float> shTerms32;
+    phx::DBuf<int> shOrderStart;
+    int freqCount[3] = {0, 0, 0};
+    int waveCount = 0;
+    // options
+    int nWaves = 4;
+    int stepsPerLaunch = 32;
+    double share0 = 0.85, shareCorr = 0.85, shareDrag = 0.35;
+    int sortAltitude = 1;
+    int exactGroupsMax = 16;     // more distinct (start, end) groups than this -> grid epochs
+    double gridStep = 60.0;      // s, spacing of the epoch grid
+    int forceGrid = 0;
+    // serving options
+    int serveStepsPerTick = 16;
+    long long serveSlots = 262144;      // object slots of the device pool
+    double serveRecordMiB = 8192.0;     // epoch-record arena
+    std::unique_ptr<phx::Server> server;
+};
+
+struct phx_batch {
+    phx_ctx* ctx = nullptr;
+    size_t n = 0;
+    int nTiles = 0;
+    int nPad = 0;
+    std::vector<int> slotOf;     // original index -> padded slot
+    int totalSteps = 0;          // steps of the longest group
+    int stepsDone = 0;
+    phx::DBuf<double> y, cur, end, nextDt, ams, bcd, records, slotT0;
+    phx::DBuf<uint8_t> status;
+    phx::DBuf<long long> tileRecord, counters, slotRecord;
+    phx::DBuf<int> slotRows;
+    std::vector<double> y0, cur0, dt0;
+    std::vector<uint8_t> st0;
+    size_t epochRecords = 0;
+    double epochSeconds = 0.0;
+    bool grid = false;
+    double gridInvH = 0.0;
+};
+
+namespace phx {
+
+EpochArgs epochArgs(const phx_ctx& c);
+// Order blocks per wave, balanced on term counts with relative shares.
+void orderBlocks(const phx_ctx& c, int W, int out[kMaxWaves + 1]);
+// Target validation of driver.cpp:99-112 (throws std::invalid_argument).
+void validateTargets(size_t n, const int64_t* id, const double* epoch, const double* state6, const double* mass,
+                     const double* drag, const double* srp, const double* cd, const double* cr);
+// Context-wide step-kernel arguments (tables, weather, wave split); the caller
+// fills the per-batch buffers.
+StepArgs contextStepArgs(const phx_ctx& c, int& waves);
+// Steps an object needs to reach its end (propagators/FixedStep.h:7-37).
+int stepsToEnd(double startEt, double endEt, double step);
+
+}  // namespace phx

@@ -183,6 +183,85 @@ __device__ inline void spline(const double* x, const double* y, int n, double yp
 }
 
 // :346-348
+// Fixed-size forms of the three spline routines above for the N = 5 node
+// profiles of densu: every array index is a compile-time constant after
+// unrolling, so the nodes stay in registers (the pointer forms index private
+// memory). Same arithmetic in the same order; the interval search of splint is
+// the binary search's result for increasing nodes (the zeta-scaled nodes of
+// densu increase strictly): the last node at or below x, clamped to [0, N-2].
+template <int N>
+__device__ inline void spliniN(const double (&xa)[N], const double (&ya)[N], const double (&y2a)[N], double x, double* y) {
+    double yi = 0;
+    bool go = true;
+#pragma unroll
+    for (int klo = 0; klo < N - 1; ++klo) {
+        const int khi = klo + 1;
+        go = go && (x > xa[klo]);
+        if (go) {
+            double xx = x;
+            if (khi < (N - 1)) xx = x < xa[khi] ? x : xa[khi];
+            const double h = xa[khi] - xa[klo];
+            const double a = (xa[khi] - xx) / h;
+            const double b = (xx - xa[klo]) / h;
+            const double a2 = a * a, b2 = b * b;
+            yi += ((1.0 - a2) * ya[klo] / 2.0 + b2 * ya[khi] / 2.0 +
+                   ((-(1.0 + a2 * a2) / 4.0 + a2 / 2.0) * y2a[klo] + (b2 * b2 / 4.0 - b2 / 2.0) * y2a[khi]) * h * h / 6.0) * h;
+        }
+    }
+    *y = yi;
+}
+
+template <int N>
+__device__ inline void splintN(const double (&xa)[N], const double (&ya)[N], const double (&y2a)[N], double x, double* y) {
+    // interval values carried by conditional moves (no index variable)
+    double xlo = xa[0], xhi = xa[1], ylo = ya[0], yhi = ya[1], y2lo = y2a[0], y2hi = y2a[1];
+#pragma unroll
+    for (int k = 1; k < N - 1; ++k) {
+        const bool up = xa[k] <= x;
+        xlo = up ? xa[k] : xlo;
+        xhi = up ? xa[k + 1] : xhi;
+        ylo = up ? ya[k] : ylo;
+        yhi = up ? ya[k + 1] : yhi;
+        y2lo = up ? y2a[k] : y2lo;
+        y2hi = up ? y2a[k + 1] : y2hi;
+    }
+    const double h = xhi - xlo;
+    const double a = (xhi - x) / h;
+    const double b = (x - xlo) / h;
+    *y = a * ylo + b * yhi + ((a * a * a - a) * y2lo + (b * b * b - b) * y2hi) * h * h / 6.0;
+}
+
+template <int N>
+__device__ inline void splineN(const double (&x)[N], const double (&y)[N], double yp1, double ypn, double (&y2)[N]) {
+    double u[N];
+    if (yp1 > 0.99E30) {
+        y2[0] = 0;
+        u[0] = 0;
+    } else {
+        y2[0] = -0.5;
+        u[0] = (3.0 / (x[1] - x[0])) * ((y[1] - y[0]) / (x[1] - x[0]) - yp1);
+    }
+#pragma unroll
+    for (int i = 1; i < (N - 1); ++i) {
+        const double sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+        const double p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        u[i] = (6.0 * ((y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1])) / (x[i + 1] - x[i - 1]) -
+                sig * u[i - 1]) / p;
+    }
+    double qn, un;
+    if (ypn > 0.99E30) {
+        qn = 0;
+        un = 0;
+    } else {
+        qn = 0.5;
+        un = (3.0 / (x[N - 1] - x[N - 2])) * (ypn - (y[N - 1] - y[N - 2]) / (x[N - 1] - x[N - 2]));
+    }
+    y2[N - 1] = (un - qn * u[N - 2]) / (qn * y2[N - 2] + 1.0);
+#pragma unroll
+    for (int k = N - 2; k >= 0; --k) y2[k] = y2[k] * y2[k + 1] + u[k];
+}
+
 __device__ inline double zeta(const State& st, double zz, double zl) { return ((zz - zl) * (st.re + zl) / (st.re + zz)); }
 
 // :350-468
@@ -247,14 +326,16 @@ __device__ inline double densm(State& st, double alt, double d0, double xm, doub
 }
 
 // :476-576
+// mn1 is 5 at every call site (the five lower-thermosphere nodes of gts7).
 __device__ inline double densu(State& st, double alt, double dlb, double tinf, double tlb, double xm, double alpha, double* tz,
                     double zlb, double s2, int mn1, const double* zn1, double* tn1, double* tgn1) {
+    constexpr int MN = 5;
+    (void)mn1;
     double x = 0, y;
     const double rgas = 831.4;
     double densu_temp = 1.0;
     double z1 = 0, t1 = 0, zgdif = 0;
-    int mn = 0;
-    double xs[5], ys[5], y2out[5];
+    double xs[MN] = {}, ys[MN] = {}, y2out[MN] = {};
     const double za = zn1[0];
     double z = alt > za ? alt : za;
     const double zg2 = zeta(st, z, zlb);
@@ -266,23 +347,23 @@ __device__ inline double densu(State& st, double alt, double dlb, double tinf, d
         const double dta = (tinf - ta) * s2 * pow(((st.re + zlb) / (st.re + za)), 2.0);
         tgn1[0] = dta;
         tn1[0] = ta;
-        z = alt > zn1[mn1 - 1] ? alt : zn1[mn1 - 1];
-        mn = mn1;
+        z = alt > zn1[MN - 1] ? alt : zn1[MN - 1];
         z1 = zn1[0];
-        const double z2 = zn1[mn - 1];
+        const double z2 = zn1[MN - 1];
         t1 = tn1[0];
-        const double t2 = tn1[mn - 1];
+        const double t2 = tn1[MN - 1];
         const double zg = zeta(st, z, z1);
         zgdif = zeta(st, z2, z1);
-        for (int k = 0; k < mn; ++k) {
+#pragma unroll
+        for (int k = 0; k < MN; ++k) {
             xs[k] = zeta(st, zn1[k], z1) / zgdif;
             ys[k] = 1.0 / tn1[k];
         }
         const double yd1 = -tgn1[0] / (t1 * t1) * zgdif;
         const double yd2 = -tgn1[1] / (t2 * t2) * zgdif * pow(((st.re + z2) / (st.re + z1)), 2.0);
-        spline(xs, ys, mn, yd1, yd2, y2out);
+        splineN(xs, ys, yd1, yd2, y2out);
         x = zg / zgdif;
-        splint(xs, ys, y2out, mn, x, &y);
+        splintN(xs, ys, y2out, x, &y);
         *tz = 1.0 / y;
         densu_temp = *tz;
     }
@@ -298,7 +379,7 @@ __device__ inline double densu(State& st, double alt, double dlb, double tinf, d
     glb = st.gsurf / pow((1.0 + z1 / st.re), 2.0);
     const double gamm = xm * glb * zgdif / rgas;
     double yi;
-    splini(xs, ys, y2out, mn, x, &yi);
+    spliniN(xs, ys, y2out, x, &yi);
     expl = gamm * yi;
     if (expl > 50.0) expl = 50.0;
     if (*tz <= 0) expl = 50.0;
@@ -764,213 +845,108 @@ __device__ inline void gts7Prefix(State& st, const NrlTables& T, Input& in, cons
     for (int k = 0; k < 5; ++k) pre.zn1[k] = zn1[k];
 }
 
-// Species k: 0 N2, 1 He, 2 O, 3 O2, 4 Ar, 5 H, 6 N, 7 anomalous O. Returns the
-// number density in the model's cgs units (before the metric factor).
+// Per-species constants of gts7 (Nrlmsise00Model.inl:1203-1498) for species
+// 0 N2, 1 He, 2 O, 3 O2, 4 Ar, 5 H, 6 N: pd block, globe7 table slot (FastGlobe
+// epoch block), pdm row, molecular mass, thermal-diffusion alpha, mixing
+// altitude altl, and flat pdl indices (pdl[0][i] = i, pdl[1][i] = 25 + i) of the
+// chemistry/dissociation correction terms (-1: the species has none).
+struct SpeciesPar {
+    int pd, globe, row;
+    double xm, alpha, altl;
+    int fIdx, hIdx, zIdx, rcIdx, hccIdx, zccIdx;
+};
+static __constant__ SpeciesPar kSpeciesPar[7] = {
+    {2, 4, 2, 28.0, 0.0, 160.0, -1, -1, -1, -1, -1, -1},   // N2
+    {0, 2, 0, 4.0, -0.38, 200.0, -1, 26, 25, -1, -1, -1},  // He
+    {1, 3, 1, 16.0, 0.0, 300.0, -1, 28, 27, 39, 38, 37},   // O  (ccor2: h1 pdl[1][3], zh pdl[1][2], h2 pdl[1][4])
+    {4, 6, 3, 32.0, 0.0, 250.0, -1, 32, 31, -1, -1, -1},   // O2
+    {5, 7, 4, 40.0, 0.17, 240.0, -1, 34, 33, -1, -1, -1},  // Ar
+    {6, 8, 5, 1.0, -0.38, 320.0, 42, 36, 35, 45, 44, 43},  // H
+    {7, 9, 6, 14.0, 0.0, 450.0, 2, 1, 0, 5, 4, 3},         // N
+};
+
+// Species k (0 N2, 1 He, 2 O, 3 O2, 4 Ar, 5 H, 6 N, 7 anomalous O): number
+// density in the model's cgs units (before the metric factor). One code path
+// for species 0-6 driven by kSpeciesPar (k is uniform across the wave, so the
+// parameters are scalar loads); each species performs the same operations in
+// the same order as its block of gts7. Densu's side effects (lower-thermosphere
+// nodes, tz) depend only on (tinf, tlb, s), so the N2 density at Zlb (b28) may
+// be evaluated before the species' own density.
 template <class Globe>
-__device__ inline double gts7Species(int k, State& st, const NrlTables& T, Input& in, const Flags& fl, const Globe& G, const Pre& pre) {
-    const double* tlbp = nullptr; (void)tlbp;
+__device__ __attribute__((always_inline)) inline double gts7Species(int k, State& st, const NrlTables& T, Input& in,
+                                                                    const Flags& fl, const Globe& G, const Pre& pre) {
     const int mn1 = 5;
-    const double alpha[9] = {-0.38, 0.0, 0.0, 0.0, 0.17, 0.0, -0.38, 0.0, 0.0};
-    const double altl[8] = {200.0, 300.0, 160.0, 250.0, 240.0, 450.0, 320.0, 450.0};
     const double tinf = pre.tinf, tlb = pre.tlb, s = pre.s, zhf = pre.zhf, xmm = pre.xmm;
     double zn1[5] = {pre.zn1[0], pre.zn1[1], pre.zn1[2], pre.zn1[3], pre.zn1[4]};
     const double z = in.alt;
     double tz;
     double* tn1 = st.meso_tn1;
     double* tgn1 = st.meso_tgn1;
-    const double zhm28 = (T.base+OFF_PDM+10*(2))[3] * (T.base+OFF_PDL+25*(1))[5];
-    (void)alpha; (void)altl; (void)zhf; (void)zhm28;
-    if (k == 0) {  // N2 (d[2])
-    const double g28 = fl.sw[21] * G(st, 4, (T.base+OFF_PD+150*(2)), in, fl);
-    const double db28 = (T.base+OFF_PDM+10*(2))[0] * exp(g28) * (T.base+OFF_PD+150*(2))[0];
-    double d = densu(st, z, db28, tinf, tlb, 28.0, alpha[2], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-    const double zh28 = (T.base+OFF_PDM+10*(2))[2] * zhf;
-    const double xmd = 28.0 - xmm;
-    double b28 = 0.0;
-    if (!(z > altl[7] && z >= zn1[0]))
-        b28 = densu(st, zh28, db28, tinf, tlb, xmd, (alpha[2] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-    if ((fl.sw[15]) && (z <= altl[2])) {
-        st.dm28 = densu(st, z, b28, tinf, tlb, xmm, alpha[2], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-        d = dnet(d, st.dm28, zhm28, xmm, 28.0);
+    const double* pdm = T.base + OFF_PDM;   // pdm[row][i] = pdm[10 row + i]
+    const double* pdl = T.base + OFF_PDL;   // pdl[blk][i] = pdl[25 blk + i]
+    const double zlbT = (T.base + OFF_PTM)[5];
+    if (k == 7) {  // anomalous O (d[8])
+        const double g16h = fl.sw[21] * G(st, 10, (T.base + OFF_PD + 150 * (8)), in, fl);
+        const double db16h = pdm[70] * exp(g16h) * (T.base + OFF_PD + 150 * (8))[0];
+        const double tho = pdm[79] * pdl[6];
+        const double ddo = densu(st, z, db16h, tho, tho, 16., 0.0, &tz, zlbT, s, mn1, zn1, tn1, tgn1);
+        const double zsht = pdm[75];
+        const double zmho = pdm[74];
+        const double zsho = scalh(st, zmho, 16.0, tho);
+        return ddo * exp(-zsht / zsho * (exp(-(z - zmho) / zsht) - 1.));
     }
-
-    return d;
+    const SpeciesPar P = kSpeciesPar[k];
+    const double* pdk = T.base + OFF_PD + 150 * P.pd;
+    const double* pdmk = pdm + 10 * P.row;
+    const double alpha2 = 0.0;   // alpha[2] (N2)
+    // N2 mixed density at Zlb: N2's own mixing and the corrections of
+    // He, O2, Ar, H and N consume it; O never does
+    const bool wantB28 = k != 2 && !(z > 450.0 && z >= zn1[0]);
+    double db28 = 0.0, b28 = 0.0;
+    if (wantB28 || k == 0) {
+        const double g28 = fl.sw[21] * G(st, 4, (T.base + OFF_PD + 150 * (2)), in, fl);
+        db28 = pdm[20] * exp(g28) * (T.base + OFF_PD + 150 * (2))[0];
     }
-    if (k == 1) {  // He (d[0])
-    // N2 mixed density at Zlb, consumed by the mixing-ratio corrections
-    double b28 = 0.0;
-    if (!(z > altl[7] && z >= zn1[0])) {
-        const double g28 = fl.sw[21] * G(st, 4, (T.base+OFF_PD+150*(2)), in, fl);
-        const double db28 = (T.base+OFF_PDM+10*(2))[0] * exp(g28) * (T.base+OFF_PD+150*(2))[0];
-        const double zh28 = (T.base+OFF_PDM+10*(2))[2] * zhf;
-        b28 = densu(st, zh28, db28, tinf, tlb, 28.0 - xmm, (alpha[2] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
+    if (wantB28) {
+        const double zh28 = pdm[22] * zhf;
+        b28 = densu(st, zh28, db28, tinf, tlb, 28.0 - xmm, (alpha2 - 1.0), &tz, zlbT, s, mn1, zn1, tn1, tgn1);
     }
-    const double g4 = fl.sw[21] * G(st, 2, (T.base+OFF_PD+150*(0)), in, fl);
-    const double db04 = (T.base+OFF_PDM+10*(0))[0] * exp(g4) * (T.base+OFF_PD+150*(0))[0];
-    double d = densu(st, z, db04, tinf, tlb, 4., alpha[0], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-    if ((fl.sw[15]) && (z < altl[0])) {
-        const double zh04 = (T.base+OFF_PDM+10*(0))[2];
-        const double b04 = densu(st, zh04, db04, tinf, tlb, 4. - xmm, alpha[0] - 1., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-        st.dm04 = densu(st, z, b04, tinf, tlb, xmm, 0., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-        d = dnet(d, st.dm04, zhm28, xmm, 4.);
-        const double rl = log(b28 * (T.base+OFF_PDM+10*(0))[1] / b04);
-        const double zc04 = (T.base+OFF_PDM+10*(0))[4] * (T.base+OFF_PDL+25*(1))[0];
-        const double hc04 = (T.base+OFF_PDM+10*(0))[5] * (T.base+OFF_PDL+25*(1))[1];
-        d = d * ccor(z, rl, hc04, zc04);
+    double dbk = db28;
+    if (k != 0) {
+        const double gk = fl.sw[21] * G(st, P.globe, pdk, in, fl);
+        dbk = pdmk[0] * exp(gk) * pdk[0];
     }
-
-    return d;
-    }
-    if (k == 2) {  // O (d[1])
-    const double g16 = fl.sw[21] * G(st, 3, (T.base+OFF_PD+150*(1)), in, fl);
-    const double db16 = (T.base+OFF_PDM+10*(1))[0] * exp(g16) * (T.base+OFF_PD+150*(1))[0];
-    double d = densu(st, z, db16, tinf, tlb, 16., alpha[1], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-    if ((fl.sw[15]) && (z <= altl[1])) {
-        const double zh16 = (T.base+OFF_PDM+10*(1))[2];
-        const double b16 = densu(st, zh16, db16, tinf, tlb, 16.0 - xmm, (alpha[1] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-        st.dm16 = densu(st, z, b16, tinf, tlb, xmm, 0., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-        d = dnet(d, st.dm16, zhm28, xmm, 16.);
-        const double rl = (T.base+OFF_PDM+10*(1))[1] * (T.base+OFF_PDL+25*(1))[16] * (1.0 + fl.sw[1] * (T.base+OFF_PDL+25*(0))[23] * (in.f107A - 150.0));
-        const double hc16 = (T.base+OFF_PDM+10*(1))[5] * (T.base+OFF_PDL+25*(1))[3];
-        const double zc16 = (T.base+OFF_PDM+10*(1))[4] * (T.base+OFF_PDL+25*(1))[2];
-        const double hc216 = (T.base+OFF_PDM+10*(1))[5] * (T.base+OFF_PDL+25*(1))[4];
-        d = d * ccor2(z, rl, hc16, zc16, hc216);
-        const double hcc16 = (T.base+OFF_PDM+10*(1))[7] * (T.base+OFF_PDL+25*(1))[13];
-        const double zcc16 = (T.base+OFF_PDM+10*(1))[6] * (T.base+OFF_PDL+25*(1))[12];
-        const double rc16 = (T.base+OFF_PDM+10*(1))[3] * (T.base+OFF_PDL+25*(1))[14];
-        d = d * ccor(z, rc16, hcc16, zcc16);
-    }
-
-    return d;
-    }
-    if (k == 3) {  // O2 (d[3])
-    // N2 mixed density at Zlb, consumed by the mixing-ratio corrections
-    double b28 = 0.0;
-    if (!(z > altl[7] && z >= zn1[0])) {
-        const double g28 = fl.sw[21] * G(st, 4, (T.base+OFF_PD+150*(2)), in, fl);
-        const double db28 = (T.base+OFF_PDM+10*(2))[0] * exp(g28) * (T.base+OFF_PD+150*(2))[0];
-        const double zh28 = (T.base+OFF_PDM+10*(2))[2] * zhf;
-        b28 = densu(st, zh28, db28, tinf, tlb, 28.0 - xmm, (alpha[2] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-    }
-    const double g32 = fl.sw[21] * G(st, 6, (T.base+OFF_PD+150*(4)), in, fl);
-    const double db32 = (T.base+OFF_PDM+10*(3))[0] * exp(g32) * (T.base+OFF_PD+150*(4))[0];
-    double d = densu(st, z, db32, tinf, tlb, 32., alpha[3], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-    if (fl.sw[15]) {
-        if (z <= altl[3]) {
-            const double zh32 = (T.base+OFF_PDM+10*(3))[2];
-            const double b32 = densu(st, zh32, db32, tinf, tlb, 32. - xmm, alpha[3] - 1., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-            st.dm32 = densu(st, z, b32, tinf, tlb, xmm, 0., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-            d = dnet(d, st.dm32, zhm28, xmm, 32.);
-            const double rl = log(b28 * (T.base+OFF_PDM+10*(3))[1] / b32);
-            const double hc32 = (T.base+OFF_PDM+10*(3))[5] * (T.base+OFF_PDL+25*(1))[7];
-            const double zc32 = (T.base+OFF_PDM+10*(3))[4] * (T.base+OFF_PDL+25*(1))[6];
-            d = d * ccor(z, rl, hc32, zc32);
+    double d = densu(st, z, dbk, tinf, tlb, P.xm, P.alpha, &tz, zlbT, s, mn1, zn1, tn1, tgn1);
+    const double zhm28 = pdm[23] * pdl[25 + 5];
+    const bool mix = fl.sw[15] && (k == 1 ? z < P.altl : z <= P.altl);
+    if (mix) {
+        if (k == 0) {
+            const double dm28 = densu(st, z, b28, tinf, tlb, xmm, alpha2, &tz, zlbT, s, mn1, zn1, tn1, tgn1);
+            d = dnet(d, dm28, zhm28, xmm, 28.0);
+        } else {
+            const double zh = pdmk[2];
+            const double bk = densu(st, zh, dbk, tinf, tlb, P.xm - xmm, P.alpha - 1.0, &tz, zlbT, s, mn1, zn1, tn1, tgn1);
+            const double dmk = densu(st, z, bk, tinf, tlb, xmm, 0., &tz, zlbT, s, mn1, zn1, tn1, tgn1);
+            d = dnet(d, dmk, zhm28, xmm, P.xm);
+            if (k == 2) {
+                const double rl = pdmk[1] * pdl[25 + 16] * (1.0 + fl.sw[1] * pdl[23] * (in.f107A - 150.0));
+                d = d * ccor2(z, rl, pdmk[5] * pdl[P.hIdx], pdmk[4] * pdl[P.zIdx], pdmk[5] * pdl[25 + 4]);
+            } else {
+                const double f = P.fIdx >= 0 ? sqrt(pdl[P.fIdx] * pdl[P.fIdx]) : 1.0;
+                const double rl = P.fIdx >= 0 ? log(b28 * pdmk[1] * f / bk) : log(b28 * pdmk[1] / bk);
+                d = d * ccor(z, rl, pdmk[5] * pdl[P.hIdx], pdmk[4] * pdl[P.zIdx]);
+            }
+            if (P.rcIdx >= 0) d = d * ccor(z, pdmk[3] * pdl[P.rcIdx], pdmk[7] * pdl[P.hccIdx], pdmk[6] * pdl[P.zccIdx]);
         }
-        const double hcc32 = (T.base+OFF_PDM+10*(3))[7] * (T.base+OFF_PDL+25*(1))[22];
-        const double hcc232 = (T.base+OFF_PDM+10*(3))[7] * (T.base+OFF_PDL+25*(0))[22];
-        const double zcc32 = (T.base+OFF_PDM+10*(3))[6] * (T.base+OFF_PDL+25*(1))[21];
-        const double rc32 = (T.base+OFF_PDM+10*(3))[3] * (T.base+OFF_PDL+25*(1))[23] * (1. + fl.sw[1] * (T.base+OFF_PDL+25*(0))[23] * (in.f107A - 150.));
+    }
+    if (k == 3 && fl.sw[15]) {
+        const double hcc32 = pdmk[7] * pdl[25 + 22];
+        const double hcc232 = pdmk[7] * pdl[22];
+        const double zcc32 = pdmk[6] * pdl[25 + 21];
+        const double rc32 = pdmk[3] * pdl[25 + 23] * (1. + fl.sw[1] * pdl[23] * (in.f107A - 150.));
         d = d * ccor2(z, rc32, hcc32, zcc32, hcc232);
     }
-
     return d;
-    }
-    if (k == 4) {  // Ar (d[4])
-    // N2 mixed density at Zlb, consumed by the mixing-ratio corrections
-    double b28 = 0.0;
-    if (!(z > altl[7] && z >= zn1[0])) {
-        const double g28 = fl.sw[21] * G(st, 4, (T.base+OFF_PD+150*(2)), in, fl);
-        const double db28 = (T.base+OFF_PDM+10*(2))[0] * exp(g28) * (T.base+OFF_PD+150*(2))[0];
-        const double zh28 = (T.base+OFF_PDM+10*(2))[2] * zhf;
-        b28 = densu(st, zh28, db28, tinf, tlb, 28.0 - xmm, (alpha[2] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-    }
-    const double g40 = fl.sw[21] * G(st, 7, (T.base+OFF_PD+150*(5)), in, fl);
-    const double db40 = (T.base+OFF_PDM+10*(4))[0] * exp(g40) * (T.base+OFF_PD+150*(5))[0];
-    double d = densu(st, z, db40, tinf, tlb, 40., alpha[4], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-    if ((fl.sw[15]) && (z <= altl[4])) {
-        const double zh40 = (T.base+OFF_PDM+10*(4))[2];
-        const double b40 = densu(st, zh40, db40, tinf, tlb, 40. - xmm, alpha[4] - 1., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-        st.dm40 = densu(st, z, b40, tinf, tlb, xmm, 0., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-        d = dnet(d, st.dm40, zhm28, xmm, 40.);
-        const double rl = log(b28 * (T.base+OFF_PDM+10*(4))[1] / b40);
-        const double hc40 = (T.base+OFF_PDM+10*(4))[5] * (T.base+OFF_PDL+25*(1))[9];
-        const double zc40 = (T.base+OFF_PDM+10*(4))[4] * (T.base+OFF_PDL+25*(1))[8];
-        d = d * ccor(z, rl, hc40, zc40);
-    }
-
-    return d;
-    }
-    if (k == 5) {  // H (d[6])
-    // N2 mixed density at Zlb, consumed by the mixing-ratio corrections
-    double b28 = 0.0;
-    if (!(z > altl[7] && z >= zn1[0])) {
-        const double g28 = fl.sw[21] * G(st, 4, (T.base+OFF_PD+150*(2)), in, fl);
-        const double db28 = (T.base+OFF_PDM+10*(2))[0] * exp(g28) * (T.base+OFF_PD+150*(2))[0];
-        const double zh28 = (T.base+OFF_PDM+10*(2))[2] * zhf;
-        b28 = densu(st, zh28, db28, tinf, tlb, 28.0 - xmm, (alpha[2] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-    }
-    const double g1 = fl.sw[21] * G(st, 8, (T.base+OFF_PD+150*(6)), in, fl);
-    const double db01 = (T.base+OFF_PDM+10*(5))[0] * exp(g1) * (T.base+OFF_PD+150*(6))[0];
-    double d = densu(st, z, db01, tinf, tlb, 1., alpha[6], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-    if ((fl.sw[15]) && (z <= altl[6])) {
-        const double zh01 = (T.base+OFF_PDM+10*(5))[2];
-        const double b01 = densu(st, zh01, db01, tinf, tlb, 1. - xmm, alpha[6] - 1., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-        st.dm01 = densu(st, z, b01, tinf, tlb, xmm, 0., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-        d = dnet(d, st.dm01, zhm28, xmm, 1.);
-        const double rl = log(b28 * (T.base+OFF_PDM+10*(5))[1] * sqrt((T.base+OFF_PDL+25*(1))[17] * (T.base+OFF_PDL+25*(1))[17]) / b01);
-        const double hc01 = (T.base+OFF_PDM+10*(5))[5] * (T.base+OFF_PDL+25*(1))[11];
-        const double zc01 = (T.base+OFF_PDM+10*(5))[4] * (T.base+OFF_PDL+25*(1))[10];
-        d = d * ccor(z, rl, hc01, zc01);
-        const double hcc01 = (T.base+OFF_PDM+10*(5))[7] * (T.base+OFF_PDL+25*(1))[19];
-        const double zcc01 = (T.base+OFF_PDM+10*(5))[6] * (T.base+OFF_PDL+25*(1))[18];
-        const double rc01 = (T.base+OFF_PDM+10*(5))[3] * (T.base+OFF_PDL+25*(1))[20];
-        d = d * ccor(z, rc01, hcc01, zcc01);
-    }
-
-    return d;
-    }
-    if (k == 6) {  // N (d[7])
-    // N2 mixed density at Zlb, consumed by the mixing-ratio corrections
-    double b28 = 0.0;
-    if (!(z > altl[7] && z >= zn1[0])) {
-        const double g28 = fl.sw[21] * G(st, 4, (T.base+OFF_PD+150*(2)), in, fl);
-        const double db28 = (T.base+OFF_PDM+10*(2))[0] * exp(g28) * (T.base+OFF_PD+150*(2))[0];
-        const double zh28 = (T.base+OFF_PDM+10*(2))[2] * zhf;
-        b28 = densu(st, zh28, db28, tinf, tlb, 28.0 - xmm, (alpha[2] - 1.0), &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-    }
-    const double g14 = fl.sw[21] * G(st, 9, (T.base+OFF_PD+150*(7)), in, fl);
-    const double db14 = (T.base+OFF_PDM+10*(6))[0] * exp(g14) * (T.base+OFF_PD+150*(7))[0];
-    double d = densu(st, z, db14, tinf, tlb, 14., alpha[7], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-    if ((fl.sw[15]) && (z <= altl[7])) {
-        const double zh14 = (T.base+OFF_PDM+10*(6))[2];
-        const double b14 = densu(st, zh14, db14, tinf, tlb, 14. - xmm, alpha[7] - 1., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-        st.dm14 = densu(st, z, b14, tinf, tlb, xmm, 0., &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-        d = dnet(d, st.dm14, zhm28, xmm, 14.);
-        const double rl = log(b28 * (T.base+OFF_PDM+10*(6))[1] * sqrt((T.base+OFF_PDL+25*(0))[2] * (T.base+OFF_PDL+25*(0))[2]) / b14);
-        const double hc14 = (T.base+OFF_PDM+10*(6))[5] * (T.base+OFF_PDL+25*(0))[1];
-        const double zc14 = (T.base+OFF_PDM+10*(6))[4] * (T.base+OFF_PDL+25*(0))[0];
-        d = d * ccor(z, rl, hc14, zc14);
-        const double hcc14 = (T.base+OFF_PDM+10*(6))[7] * (T.base+OFF_PDL+25*(0))[4];
-        const double zcc14 = (T.base+OFF_PDM+10*(6))[6] * (T.base+OFF_PDL+25*(0))[3];
-        const double rc14 = (T.base+OFF_PDM+10*(6))[3] * (T.base+OFF_PDL+25*(0))[5];
-        d = d * ccor(z, rc14, hcc14, zcc14);
-    }
-
-    return d;
-    }
-    if (k == 7) {  // anomalous O (d[8])
-    const double g16h = fl.sw[21] * G(st, 10, (T.base+OFF_PD+150*(8)), in, fl);
-    const double db16h = (T.base+OFF_PDM+10*(7))[0] * exp(g16h) * (T.base+OFF_PD+150*(8))[0];
-    const double tho = (T.base+OFF_PDM+10*(7))[9] * (T.base+OFF_PDL+25*(0))[6];
-    const double ddo = densu(st, z, db16h, tho, tho, 16., alpha[8], &tz, (T.base+OFF_PTM)[5], s, mn1, zn1, tn1, tgn1);
-    const double zsht = (T.base+OFF_PDM+10*(7))[5];
-    const double zmho = (T.base+OFF_PDM+10*(7))[4];
-    const double zsho = scalh(st, zmho, 16.0, tho);
-    const double d = ddo * exp(-zsht / zsho * (exp(-(z - zmho) / zsht) - 1.));
-
-    return d;
-    }
-    return 0.0;
 }
 
 // Mass-density weights of the species in gtd7d (amu): d[5] = 1.66e-24 sum w_k d_k.
@@ -1122,7 +1098,7 @@ __device__ inline void globeEpoch(const NrlTables& T, int doy, double sec, doubl
 // expansion is term by term the published one (Nrlmsise00Model.inl:688-850)
 // with cos(x - p) = cos x cos p + sin x sin p for the longitude / local-time
 // phases and powers of the Ap decay factor taken by products.
-__device__ inline double globe7Fast(const State& st, const double* p, const double* E, double dfa, double glat, double& aptOut) {
+__device__ __attribute__((always_inline)) inline double globe7Fast(const State& st, const double* p, const double* E, double dfa, double glat, double& aptOut) {
     const double hr = 0.2618, dgtr = 1.74533E-2;
     const double(&plg)[4][9] = st.plg;
     const double cd14 = E[GE_CD14], cd18 = E[GE_CD18], f2 = E[GE_F2];
@@ -1189,7 +1165,7 @@ __device__ inline double globe7Fast(const State& st, const double* p, const doub
 // Fast policy: epoch terms from the per-epoch record.
 struct FastGlobe {
     const double* E;  // GE_COUNT values
-    __device__ void basis(State& st, const Input& in, const Flags&) const {
+    __device__ __attribute__((always_inline)) void basis(State& st, const Input& in, const Flags&) const {
         const double dgtr = 1.74533E-2, hr = 0.2618;
         const double c = sin(in.g_lat * dgtr), s = cos(in.g_lat * dgtr);
         const double c2 = c * c, c4 = c2 * c2, s2 = s * s;
@@ -1228,7 +1204,7 @@ struct FastGlobe {
         st.sglong = sin(dgtr * in.g_long);
         st.dfa = E[GE_DFA];
     }
-    __device__ double operator()(State& st, int j, const double* p, const Input& in, const Flags&) const {
+    __device__ __attribute__((always_inline)) double operator()(State& st, int j, const double* p, const Input& in, const Flags&) const {
         // globe7 also leaves apt[0] for glob7s (lower thermosphere)
         const double* e = E + GE_STRIDE * j;
         double apt0 = 0.0;

@@ -114,23 +114,72 @@ __device__ inline ShGeom shGeometry(const Rec& rec, double px, double py, double
 // Read-only tables in the constant address space: uniform indices become
 // scalar (SMEM) loads into SGPR operands of the FMAs.
 typedef const __attribute__((address_space(4))) double* ConstD;
+typedef const __attribute__((address_space(4))) float* ConstF;
 typedef const __attribute__((address_space(4))) int* ConstI;
+
+// One run of consecutive degrees of order m >= 1 (column recursion
+// P~_{n+1} = a u P~_n - b w P~_{n-1}); T is the arithmetic type of the run, the
+// six partial sums are returned in T.
+template <class T, class CP>
+__device__ __attribute__((always_inline)) inline void shRun(CP t, int count, T u, T w, T& P0, T& Pm1, T& A1, T& A2, T& A5,
+                                                            T& A6, T& B1, T& B2) {
+    // Four terms per trip: their uniform coefficients are requested together
+    // so one scalar-load latency covers the FMAs of four terms.
+#pragma unroll 4
+    for (int i = 0; i < count; ++i, t += SHT_STRIDE) {
+        A1 = fma(P0, t[SHT_C1], A1);
+        A2 = fma(P0, t[SHT_S1], A2);
+        A5 = fma(P0, t[SHT_C], A5);
+        A6 = fma(P0, t[SHT_S], A6);
+        B1 = fma(Pm1, t[SHT_EC], B1);
+        B2 = fma(Pm1, t[SHT_ES], B2);
+        const T Pn = fma(t[SHT_A] * u, P0, -(t[SHT_B] * w) * Pm1);
+        Pm1 = P0;
+        P0 = Pn;
+    }
+}
+
+// Order 0: P~ and its latitude derivative by the reference's dP recursion.
+template <class T, class CP>
+__device__ __attribute__((always_inline)) inline void shRun0(CP t, int count, T u, T v, T w, T& P0, T& Pm1, T& dP0, T& dPm1,
+                                                             T& X1, T& D1) {
+    for (int i = 0; i < count; ++i, t += SHT_STRIDE) {
+        X1 = fma(P0, t[SHT_C1], X1);
+        D1 = fma(dP0, t[SHT_C], D1);
+        const T au = t[SHT_A] * u, bw = t[SHT_B] * w;
+        const T Pn = fma(au, P0, -bw * Pm1);
+        const T dPn = fma(t[SHT_A], fma(v, P0, u * dP0), -bw * dPm1);
+        Pm1 = P0;
+        P0 = Pn;
+        dPm1 = dP0;
+        dP0 = dPn;
+    }
+}
 
 // Order block [m0, m1) of the SH sum for one lane; returns the unscaled
 // spherical components (radial, latitude, longitude) of sum_nm alpha^n (...).
 // All control flow over (n, m) is uniform across the wave; only the literal
 // pole (x = y = 0 in the body frame) takes a divergent fix-up after the loop.
-__device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const double* termsG, const int* orderStartG,
-                             const double* diagG, bool normalized, double& outR, double& outPhi, double& outLam) {
+// Mixed precision (fp64Degree >= 0): degrees n <= fp64Degree run in FP64; the
+// terms of higher degree run in FP32 (recursion and per-order partial sums, FP32
+// coefficient stream), and each order's FP32 partial sums are accumulated into
+// the FP64 sums; the diagonal seeds and the longitude phasor stay FP64.
+__device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const double* termsG, const float* terms32G,
+                             int fp64Degree, const int* orderStartG, const double* diagG, bool normalized, double& outR,
+                             double& outPhi, double& outLam) {
     ConstD terms = (ConstD)termsG;
+    ConstF terms32 = (ConstF)terms32G;
     ConstI orderStart = (ConstI)orderStartG;
     ConstD diag = (ConstD)diagG;
     m0 = __builtin_amdgcn_readfirstlane(m0);
     m1 = __builtin_amdgcn_readfirstlane(m1);
     L = __builtin_amdgcn_readfirstlane(L);
+    // first degree evaluated in FP32 (L + 1: none)
+    const int nSplit = __builtin_amdgcn_readfirstlane(fp64Degree < 0 ? L + 1 : min(fp64Degree + 1, L + 1));
     const double u = g.alpha * g.sinPhi;   // alpha sin(phi)
     const double v = g.alpha * g.cosPhi;   // alpha cos(phi)
     const double w = g.alpha * g.alpha;
+    const float uf = float(u), vf = float(v), wf = float(w);
     double ar = 0.0, aphiDirect = 0.0, aphiNum = 0.0, alamNum = 0.0;
     // Seed P~_{m0,m0} = prod c_k (alpha cos phi)^m0 and cos/sin(m0 lambda).
     double Pmm = 1.0, cosML = 1.0, sinML = 0.0;
@@ -142,41 +191,36 @@ __device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const doubl
         cosML = c;
     }
     for (int m = m0; m < m1; ++m) {
-        ConstD t = terms + SHT_STRIDE * orderStart[m];
+        const int first = orderStart[m];
         const int count = L - m + 1;
+        const int c64 = max(0, min(count, nSplit - m));   // degrees m .. m + c64 - 1 in FP64
+        const int c32 = count - c64;
         if (m == 0) {
-            // order 0: dP recursion (reference form), seeds P~_00 = 1, dP~_00 = 0
             double P0 = Pmm, Pm1 = 0.0, dP0 = 0.0, dPm1 = 0.0;
             double X1 = 0.0, D1 = 0.0;
-            for (int i = 0; i < count; ++i, t += SHT_STRIDE) {
-                X1 = fma(P0, t[SHT_C1], X1);
-                D1 = fma(dP0, t[SHT_C], D1);
-                const double au = t[SHT_A] * u, bw = t[SHT_B] * w;
-                const double Pn = fma(au, P0, -bw * Pm1);
-                const double dPn = fma(t[SHT_A], fma(v, P0, u * dP0), -bw * dPm1);
-                Pm1 = P0;
-                P0 = Pn;
-                dPm1 = dP0;
-                dP0 = dPn;
+            shRun0<double>(terms + SHT_STRIDE * first, c64, u, v, w, P0, Pm1, dP0, dPm1, X1, D1);
+            if (c32 > 0) {
+                float p0 = float(P0), pm1 = float(Pm1), dp0 = float(dP0), dpm1 = float(dPm1), x1 = 0.0f, d1 = 0.0f;
+                shRun0<float>(terms32 + SHT_STRIDE * (first + c64), c32, uf, vf, wf, p0, pm1, dp0, dpm1, x1, d1);
+                X1 += double(x1);
+                D1 += double(d1);
             }
             ar -= X1;
             aphiDirect += D1;
         } else {
             double P0 = Pmm, Pm1 = 0.0;
             double A1 = 0.0, A2 = 0.0, A5 = 0.0, A6 = 0.0, B1 = 0.0, B2 = 0.0;
-            // Four terms per trip: their 4 x 64 B of uniform coefficients are
-            // requested together so one scalar-load latency covers 40 FMAs.
-#pragma unroll 4
-            for (int i = 0; i < count; ++i, t += SHT_STRIDE) {
-                A1 = fma(P0, t[SHT_C1], A1);
-                A2 = fma(P0, t[SHT_S1], A2);
-                A5 = fma(P0, t[SHT_C], A5);
-                A6 = fma(P0, t[SHT_S], A6);
-                B1 = fma(Pm1, t[SHT_EC], B1);
-                B2 = fma(Pm1, t[SHT_ES], B2);
-                const double Pn = fma(t[SHT_A] * u, P0, -(t[SHT_B] * w) * Pm1);
-                Pm1 = P0;
-                P0 = Pn;
+            shRun<double>(terms + SHT_STRIDE * first, c64, u, w, P0, Pm1, A1, A2, A5, A6, B1, B2);
+            if (c32 > 0) {
+                float p0 = float(P0), pm1 = float(Pm1);
+                float a1 = 0.0f, a2 = 0.0f, a5 = 0.0f, a6 = 0.0f, b1 = 0.0f, b2 = 0.0f;
+                shRun<float>(terms32 + SHT_STRIDE * (first + c64), c32, uf, wf, p0, pm1, a1, a2, a5, a6, b1, b2);
+                A1 += double(a1);
+                A2 += double(a2);
+                A5 += double(a5);
+                A6 += double(a6);
+                B1 += double(b1);
+                B2 += double(b2);
             }
             const double X1 = cosML * A1 + sinML * A2;
             const double X5 = cosML * A5 + sinML * A6;
@@ -202,7 +246,7 @@ __device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const doubl
     }
     // Literal pole: every P~_nm with m >= 1 vanishes; order 1 keeps a finite
     // latitude derivative (dP recursion from dP~_11 = -c_1 alpha sin(phi)) and
-    // the analytic limit of P_n1 / cos(phi) (Coefficients.h:123-132).
+    // the analytic limit of P_n1 / cos(phi) (Coefficients.h:123-132). FP64.
     outPhi = aphiDirect;
     outLam = 0.0;
     if (m0 <= 1 && 1 < m1 && L >= 1) {
@@ -365,7 +409,7 @@ __device__ inline nrlmsise::Flags nrlFlags() {
 // Species k0, k0 + stride, ... of the thermospheric model for this lane
 // (cgs number densities before the metric factor) into spec[k][lane].
 template <bool FAST>
-__device__ void nrlSpecies(const NrlLane& L, const double* nrl, const double* __restrict__ coeff, int k0, int stride,
+__device__ __attribute__((always_inline)) inline void nrlSpecies(const NrlLane& L, const double* nrl, const double* __restrict__ coeff, int k0, int stride,
                            double (*spec)[kTile], int lane) {
     nrlmsise::Input in = L.in;
     const nrlmsise::Flags fl = nrlFlags();
@@ -584,7 +628,9 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
     if (fm.hasSh) {
         double sr = 0.0, sp = 0.0, sl = 0.0;
         g = shGeometry(rec, px, py, pz, fm.shRadius);
-        if (m0 < m1) shOrderBlock(g, m0, m1, fm.shDegree, a.shTerms, a.shOrderStart, a.shDiag, fm.shNormalized, sr, sp, sl);
+        if (m0 < m1)
+            shOrderBlock(g, m0, m1, fm.shDegree, a.shTerms, a.shTerms32, a.fp64Degree, a.shOrderStart, a.shDiag, fm.shNormalized, sr,
+                         sp, sl);
         part[wave][0][lane] = sr;
         part[wave][1][lane] = sp;
         part[wave][2][lane] = sl;
@@ -662,7 +708,7 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int tile = blockIdx.x;
+    const int tile = __builtin_amdgcn_readfirstlane(a.tileList ? a.tileList[blockIdx.x] : int(blockIdx.x));
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
 
@@ -681,6 +727,14 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
     const double bcd = a.bcd[obj];
     long long completed = 0;
     const long long recBase = GRID ? 0 : a.tileRecord[tile];
+    const double* gridBase = nullptr;
+    double gridT0 = 0.0;
+    int gridRows = 0;
+    if (GRID) {
+        gridBase = a.records + a.slotRecord[obj] * REC_SIZE;
+        gridT0 = a.slotT0[obj];
+        gridRows = a.slotRows[obj];
+    }
     const int m0 = a.blockStart[wave], m1 = a.blockStart[wave + 1];
     EpochArgs wa{};
     wa.weather = a.weather;
@@ -718,7 +772,7 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
             double acc[3];
             if (GRID) {
                 const double ts = xch[6][lane];
-                const GridRec rec = gridRec(a.records, a.gridCount, a.gridT0, a.gridInvH, ts);
+                const GridRec rec = gridRec(gridBase, gridRows, gridT0, a.gridInvH, ts);
                 double nrl[13];
                 if (fm.drag) epochdev::nrlEpoch(wa, ts, nrl, nullptr);
                 stageForces<W, false>(a, fm, rec, nrl, wave, lane, m0, m1, ams, bcd, xch, part, extra, spec, acc);
@@ -793,7 +847,7 @@ __global__ void __launch_bounds__(64) forceProbeKernel(int n, const double* pos,
     if ((which < 0 || which == 1) && fm.hasSh) {
         const ShGeom g = shGeometry(rec, px, py, pz, fm.shRadius);
         double r, p, l;
-        shOrderBlock(g, 0, fm.shOrder + 1, fm.shDegree, shTerms, shOrderStart, shDiag, fm.shNormalized, r, p, l);
+        shOrderBlock(g, 0, fm.shOrder + 1, fm.shDegree, shTerms, nullptr, -1, shOrderStart, shDiag, fm.shNormalized, r, p, l);
         shToInertial(rec, g, r, p, l, fm.shGm, bx, by, bz);
         ax += bx, ay += by, az += bz;
     }

@@ -121,6 +121,7 @@ def test_population_distributions():
     r = np.linalg.norm(pop["state"][:3], axis=0)
     leo = pop["kind"] == 0
     assert abs(leo.mean() - 0.8) < 0.02
-    assert np.all(r[leo] > synth.R_EARTH + 150) and np.all(r[leo] < synth.R_EARTH + 2100)
+    # perigee >= 150 km; apogee <= a (1 + e) with a <= R_E + 2000 km and e <= 0.02
+    assert np.all(r[leo] > synth.R_EARTH + 150) and np.all(r[leo] <= (synth.R_EARTH + 2000.0) * 1.02)
     am = pop["drag_area"] / pop["mass"]
     assert abs(np.median(np.log(am)) - np.log(0.01)) < 0.05
