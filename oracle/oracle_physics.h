@@ -11,6 +11,19 @@
 // series and the analytic Earth orientation, which stand in for the .brie
 // ephemeris and the .brot2 orientation files that the configurations do not
 // provide) follow the published methods named at each function.
+//
+// What pins each part (tests/test_oracle_pins.py and the NRLMSISE-00 golden
+// vectors): RKF78 tableau identities; closed-form Kepler propagation and the
+// energy / angular-momentum invariants for the central-gravity driver; final
+// step shortening; the SH sum against the analytic J2 formula (zonal field)
+// and against finite differences of the potential; shadow geometry limits;
+// the relativity closed form; the analytic ephemeris against its published
+// accuracy bands; NRLMSISE-00 against the published test-driver outputs.
+// Parity unpinned (restatement only, no reference-held vector or closed form
+// covers it): the EOP time model (timeAt, ET -> UTC iterations), the solid
+// Earth tide coefficients (IERS 2010 frequency-dependent terms), the FES2004
+// ocean tide synthesis, and the NRLMSISE-00 input preparation (ET -> UTC,
+// day of year, GAST hours, weather lookup).
 #pragma once
 
 #include <cmath>

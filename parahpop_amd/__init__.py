@@ -7,6 +7,7 @@ when libphx.so is missing or cannot be loaded.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -109,11 +110,15 @@ class Context:
         if rc != PHX_OK:
             raise PhxError(rc, self.lib.phx_last_error(None).decode())
         self.h = h
+        self._batches = weakref.WeakSet()
         for k, v in options.items():
             self.set_option(k, v)
 
     def close(self):
         if getattr(self, "h", None):
+            # batches hold the context: release them first
+            for b in list(getattr(self, "_batches", ())):
+                b.close()
             self.lib.phx_destroy(self.h)
             self.h = None
 
@@ -235,6 +240,7 @@ class Batch:
             rc = ctx.lib.phx_batch_create_uneven(ctx.h, n, *[_ptr(a) for a in arrs], _ptr(self._dur), ctypes.byref(h))
         ctx._check(rc)
         self.h = h
+        ctx._batches.add(self)
 
     def run(self, max_steps=0):
         st = Stats()
@@ -261,7 +267,8 @@ class Batch:
 
     def close(self):
         if getattr(self, "h", None):
-            self.ctx.lib.phx_batch_destroy(self.h)
+            if getattr(self.ctx, "h", None):
+                self.ctx.lib.phx_batch_destroy(self.h)
             self.h = None
 
     __del__ = close

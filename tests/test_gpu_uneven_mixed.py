@@ -41,17 +41,20 @@ def test_uneven_batch_per_object_durations(workdir):
         st = b.run()
         assert b.done()
         e, s = b.download()
-    steps = np.ceil(dur / 60.0)
-    assert st["object_steps"] == steps.sum()
     np.testing.assert_allclose((e - t["epoch"]) * 86400.0, dur, atol=1e-4)
     orc = oracle.Oracle(cfg)
+    oracle_steps = 0
     for d in classes:
         idx = np.nonzero(dur == d)[0]
-        e_o, s_o, _ = orc.propagate(_subset(t, idx), d, threads=8)
+        e_o, s_o, so = orc.propagate(_subset(t, idx), d, threads=8)
+        oracle_steps += so["object_steps"]
         np.testing.assert_array_equal(e[idx], e_o)
         dp = np.linalg.norm(s[:3, idx] - s_o[:3], axis=0) * 1e6
         dv = np.linalg.norm(s[3:, idx] - s_o[3:], axis=0) * 1e9
         assert dp.max() <= 1.0 and dv.max() <= 1.0, (d, dp.max(), dv.max())
+    # ET rounding can leave a sub-second final step (FixedStep.h:30-37), so the
+    # step count is the oracle's, not ceil(duration / step)
+    assert st["object_steps"] == oracle_steps
 
 
 def test_autotune_keeps_results(full_workload):
