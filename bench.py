@@ -129,9 +129,12 @@ def leg_main(args, work):
     }
     traffic = os.path.join(ROOT, "profiles", "rkf78_full_force_70x70_traffic.json")
     if os.path.exists(traffic):
+        # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs (scaled by a
+        # device copy of known size), valid for the same launch shape
         tr = json.load(open(traffic))
-        line["roofline"]["traffic"] = tr["hbm_bytes_per_step"] * st["steps_per_launch"]
-        line["roofline"]["traffic_source"] = os.path.relpath(traffic, ROOT)
+        if tr["steps_per_launch"] == st["steps_per_launch"] and args.objects == 16384 and args.degree == 70:
+            line["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
+            line["roofline"]["traffic_source"] = os.path.relpath(traffic, ROOT)
     if not args.no_cpu:
         import oracle
         threads = min(16, os.cpu_count() or 1)

@@ -400,13 +400,15 @@ std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, c
     for (size_t i = 0; i < n; ++i)
         alt[i] = std::sqrt(state6[i] * state6[i] + state6[n + i] * state6[n + i] + state6[2 * n + i] * state6[2 * n + i]);
     if (grid) {
-        // longest work first (tiles finish together and the tail is short), then by
-        // start epoch so a tile's lanes read neighbouring grid rows
-        std::vector<double> span(n);
-        for (size_t i = 0; i < n; ++i) span[i] = std::ceil((endEt[i] - start[i]) / step);
+        // longest work first, in classes of step counts 10 % wide (a tile's lanes end
+        // within ~10 % of each other and the tail is short); inside a class by
+        // radius, so the objects below the drag cutoff share tiles
+        std::vector<int> cls(n);
+        for (size_t i = 0; i < n; ++i)
+            cls[i] = int(std::floor(std::log(std::max(1.0, std::ceil((endEt[i] - start[i]) / step))) / std::log(1.1)));
         std::stable_sort(order.begin(), order.end(), [&](int a, int b2) {
-            if (span[size_t(a)] != span[size_t(b2)]) return span[size_t(a)] > span[size_t(b2)];
-            return start[size_t(a)] < start[size_t(b2)];
+            if (cls[size_t(a)] != cls[size_t(b2)]) return cls[size_t(a)] > cls[size_t(b2)];
+            return alt[size_t(a)] < alt[size_t(b2)];
         });
         for (size_t i = 0; i < n; ++i) grp[i] = 0;
     } else if (c.sortAltitude) {

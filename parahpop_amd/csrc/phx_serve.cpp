@@ -228,12 +228,23 @@ bool Server::admitOne(Request& r) {
     }
     const size_t m = size_t(r.padEnd - s0);
     const size_t nn = r.n;
+    // objects by radius, so those below the drag cutoff share tiles
+    r.perm.resize(nn);
+    {
+        std::vector<double> rad(nn);
+        for (size_t i = 0; i < nn; ++i) {
+            const double x = r.y[i], y = r.y[nn + i], z = r.y[2 * nn + i];
+            rad[i] = std::sqrt(x * x + y * y + z * z);
+            r.perm[i] = int(i);
+        }
+        std::stable_sort(r.perm.begin(), r.perm.end(), [&](int a, int b) { return rad[size_t(a)] < rad[size_t(b)]; });
+    }
     std::vector<double> y(6 * m), cur(m), end(m), dt(m), ams(m), bcd(m), t0(m, r.gridT0);
     std::vector<uint8_t> st(m);
     std::vector<long long> recIdx(m, rec0);
     std::vector<int> rows(m, int(r.rows));
     for (size_t j = 0; j < m; ++j) {
-        const size_t i = j < nn ? j : 0;
+        const size_t i = size_t(r.perm[j < nn ? j : 0]);
         for (int d = 0; d < 6; ++d) y[size_t(d) * m + j] = r.y[size_t(d) * nn + i];
         cur[j] = r.start[i];
         end[j] = r.end[i];
@@ -369,11 +380,13 @@ void Server::tick() {
         q->outEpoch.resize(q->n);
         q->outState.resize(6 * q->n);
         bool ok = true;
-        for (size_t i = 0; i < q->n; ++i) {
-            if (!(q->dlSt[i] & ST_EOS)) ok = false;
-            q->outEpoch[i] = q->dlCur[i] * (1.0 / 86400.0) + 0.5;   // TimeConversion.h:36-40
+        const size_t nn = q->n;
+        for (size_t j = 0; j < nn; ++j) {
+            const size_t i = size_t(q->perm[j]);   // slot j holds object i
+            if (!(q->dlSt[j] & ST_EOS)) ok = false;
+            q->outEpoch[i] = q->dlCur[j] * (1.0 / 86400.0) + 0.5;   // TimeConversion.h:36-40
+            for (int d = 0; d < 6; ++d) q->outState[size_t(d) * nn + i] = q->dlY[size_t(d) * nn + j];
         }
-        q->outState.swap(q->dlY);
         if (!ok) {
             q->rc = PHX_ENOTDONE;
             q->error = "A target stopped without reaching the requested end of simulation.";

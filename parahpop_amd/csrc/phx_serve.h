@@ -66,6 +66,7 @@ private:
         uint64_t ticket = 0;
         size_t n = 0;
         std::vector<double> y, start, end, dt0, ams, bcd;   // SoA inputs (ET s, km, km/s)
+        std::vector<int> perm;         // slot order: slot0 + j holds object perm[j]
         int steps = 0;                 // steps until every object has ended (capped by maxNumSteps)
         double gridT0 = 0.0;
         long long rows = 0;            // epoch-record rows of the request's segment
