@@ -442,6 +442,16 @@ std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, c
     std::vector<uint8_t> st(np);
     std::vector<long long> tileRec(static_cast<size_t>(nTiles));
     for (int t = 0; t < nTiles; ++t) tileRec[size_t(t)] = groupRecord[size_t(tileGroup[size_t(t)])];
+    // steps each tile needs at most (ceil(span / step) + 1 bounds the shortened-step sequence)
+    b->tileSteps.assign(static_cast<size_t>(nTiles), 0);
+    for (int sIdx = 0; sIdx < nPad; ++sIdx) {
+        const int i = slotObj[size_t(sIdx)];
+        if (i < 0) continue;
+        const double span = endEt[size_t(i)] - start[size_t(i)];
+        const long long need = std::min<long long>((long long)std::ceil(span / step) + 1, launches);
+        int& ts = b->tileSteps[size_t(sIdx / kTile)];
+        ts = std::max(ts, int(need));
+    }
     for (int s = 0; s < nPad; ++s) {
         int i = slotObj[size_t(s)];
         const bool pad = i < 0;
@@ -519,6 +529,7 @@ StepArgs contextStepArgs(const phx_ctx& c, int& W) {
     a.weather = c.weather.p;
     a.wDays = c.env.hasWeather ? c.env.weather.days : 0;
     a.wStart = c.env.hasWeather ? c.env.weather.startMjdUtc : 0.0;
+    a.nrlSlots = c.nrlSlots.p;
     a.gridInvH = 1.0 / c.gridStep;
     W = c.fm.hasSh ? c.nWaves : 1;
     if (c.fm.hasSh) orderBlocks(c, W, a.blockStart);
@@ -567,11 +578,28 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
     check(hipMemcpyAsync(&before, b.counters.p, sizeof before, hipMemcpyDeviceToHost, c.stream), "counter read");
     check(hipEventRecord(c.ev0, c.stream), "hipEventRecord");
     int launches = 0;
+    int blocksFor = W;   // wave count the order blocks in `a` were split for
     while (remaining > 0) {
         const int k = std::min(remaining, std::max(1, c.stepsPerLaunch));
+        // launch only the prefix of tiles that still has work (uneven batches are
+        // ordered longest work first); when those tiles cannot give every SIMD a
+        // wave, split each tile over at least four waves
+        int active = b.nTiles;
+        while (active > 0 && b.tileSteps[size_t(active - 1)] <= b.stepsDone) --active;
+        if (active == 0) {
+            b.stepsDone += remaining;
+            break;
+        }
+        int Wl = W;
+        if (c.fm.hasSh && W < 4 && (long long)active * W < 4LL * c.numCU) Wl = 4;
+        if (c.fm.hasSh && Wl != blocksFor) {
+            orderBlocks(c, Wl, a.blockStart);
+            blocksFor = Wl;
+        }
+        a.nTiles = active;
         a.stepBase = b.stepsDone;
         a.steps = k;
-        const int rc = launchStep(a, c.fm, W, c.stream);
+        const int rc = launchStep(a, c.fm, Wl, c.stream);
         if (rc != 0) throw HipError("step kernel launch failed");
         b.stepsDone += k;
         remaining -= k;
@@ -634,6 +662,9 @@ int phx_create(const char* config_json_path, phx_ctx** out) {
             throw HipError(std::string("No usable HIP device: ") + hipGetErrorString(de));
         const char* env = std::getenv("PHX_DATA_DIR");
         c->dataDir = env ? env : libraryDir() + "/data";
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
+        c->numCU = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
         check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         check(hipEventCreate(&c->ev0), "hipEventCreate");
         check(hipEventCreate(&c->ev1), "hipEventCreate");
@@ -641,6 +672,12 @@ int phx_create(const char* config_json_path, phx_ctx** out) {
         if (uploadTideFactors() != 0) throw HipError("tide factor upload failed");
         loadTables(*c);
         if (c->env.hasSh) buildShTables(*c);
+        if (c->fm.drag && c->env.hasWeather) {
+            // weather-slot rows for the uneven-epoch path
+            const int rows = c->env.weather.days * 8;
+            c->nrlSlots.alloc(size_t(rows) * kNrlSlotSize);
+            if (launchNrlSlots(epochArgs(*c), c->nrlSlots.p, rows, c->stream) != 0) throw HipError("weather-slot kernel launch failed");
+        }
         check(hipStreamSynchronize(c->stream), "table upload");
         *out = c.release();
         return PHX_OK;
@@ -666,6 +703,7 @@ void phx_destroy(phx_ctx* ctx) {
     ctx->terms.release();
     ctx->eop.release();
     ctx->weather.release();
+    ctx->nrlSlots.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

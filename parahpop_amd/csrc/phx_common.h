@@ -8,6 +8,7 @@ namespace phx {
 constexpr int kStages = 13;
 constexpr int kTile = 64;          // objects per workgroup tile = one wave64
 constexpr int kMaxBodies = 11;     // Earth, Sun, Moon, 8 planetary system barycenters
+constexpr int kNrlSlotSize = 280;  // doubles per NRLMSISE-00 weather-slot row
 constexpr int kMaxWaves = 8;       // waves per tile that share one object's work
 
 // Per-epoch record: everything the force model needs that depends only on the
@@ -114,6 +115,7 @@ struct StepArgs {
     const long long* slotRecord;
     const double* slotT0;
     const int* slotRows;
+    const double* nrlSlots;      // [wDays * 8][nrlmsise::SLOT_SIZE] weather-slot rows (uneven epochs)
     const double* weather;       // 13 x wDays SoA (CelesTrak components)
     int wDays;
     double wStart;

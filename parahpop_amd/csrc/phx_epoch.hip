@@ -63,7 +63,19 @@ __global__ void __launch_bounds__(64) epochRecordKernel(EpochArgs a, ForceModel 
     if (fm.drag) nrlEpoch(a, et, rec + REC_NRL, rec + REC_NRL_GLOBE);
 }
 
+__global__ void __launch_bounds__(64) nrlSlotKernel(EpochArgs a, double* out, int rows) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    nrlSlotRow(a, i / 8, i % 8, out + size_t(i) * nrlmsise::SLOT_SIZE);
+}
+
 }  // namespace
+
+int launchNrlSlots(const EpochArgs& a, double* out, int rows, hipStream_t stream) {
+    if (rows <= 0) return 0;
+    hipLaunchKernelGGL(nrlSlotKernel, dim3((rows + 63) / 64), dim3(64), 0, stream, a, out, rows);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 int launchEpochRecords(const EpochArgs& a, const ForceModel& fm, hipStream_t stream) {
     if (a.count <= 0) return 0;

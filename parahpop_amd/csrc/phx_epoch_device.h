@@ -399,12 +399,9 @@ __device__ double gastHours(double mjdUtc) {
     return wrapHours((gr + dpsi / 3600.0 * kDeg * cos(eps)) * 24.0 / kTwoPi);
 }
 
-// Writes the NRLMSISE-00 epoch inputs at dst[REC_NRL_* - REC_NRL] and, when
-// globe is given, the epoch-only globe7 terms (nrlmsise::GE_COUNT values).
-__device__ void nrlEpoch(const EpochArgs& a, double et, double* dst, double* globe) {
-    double* rec = dst - REC_NRL;
-    const double mjd = etToMjdUtcNrl(et);
-    // calendar date -> day of year and seconds of day (Nrlmsise00.cu:113-140)
+// Calendar day of year and UTC seconds of day of an MJD UTC
+// (Nrlmsise00.cu:113-140).
+__device__ inline void doySecOf(double mjd, int& doyOut, double& secOut) {
     const long A = long(floor(mjd + 2400001.0));
     long c;
     if (A < 2299161) c = A + 1524;
@@ -423,40 +420,64 @@ __device__ void nrlEpoch(const EpochArgs& a, double et, double* dst, double* glo
     const int hour = int(sod / 3600.0);
     const int minute = int((sod - hour * 3600.0) / 60.0);
     const double second = sod - hour * 3600.0 - minute * 60.0;
-    int dm[12] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
-    if ((year % 4 == 0 && year % 100 != 0) || year % 400 == 0) dm[1] = 29;
+    const int dm[12] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
+    const bool leap = (year % 4 == 0 && year % 100 != 0) || year % 400 == 0;
     int doy = day;
-    for (int m = 1; m < month; ++m) doy += dm[m - 1];
-    rec[REC_NRL_DOY] = double(doy);
-    rec[REC_NRL_SEC] = hour * 3600.0 + minute * 60.0 + second;
-    rec[REC_NRL_GAST] = gastHours(mjd);
-    // weather (adjusted flux; Nrlmsise00.cu:215-241)
-    const int record = int(floor(mjd - a.wStart));
-    bool ok = a.wDays > 0 && record >= 3 && record < a.wDays;
-    if (!ok) {
-        for (int k = 0; k < 10; ++k) rec[REC_NRL_F107 + k] = nan("");
-        rec[REC_NRL_VALID] = 0.0;
-        return;
-    }
-    int slot = int(floor((mjd - floor(mjd)) * 8.0));
+#pragma unroll
+    for (int m = 1; m < 12; ++m)
+        if (m < month) doy += (m == 2 && leap) ? 29 : dm[m - 1];
+    doyOut = doy;
+    secOut = hour * 3600.0 + minute * 60.0 + second;
+}
+
+// Weather-table record and 3-hour slot of an MJD UTC; false outside the
+// table's usable range (three days of history are needed).
+__device__ inline bool weatherSlotOf(const EpochArgs& a, double mjd, int& record, int& slot) {
+    record = int(floor(mjd - a.wStart));
+    slot = int(floor((mjd - floor(mjd)) * 8.0));
     slot = slot < 0 ? 0 : (slot > 7 ? 7 : slot);
+    return a.wDays > 0 && record >= 3 && record < a.wDays;
+}
+
+// Flux and Ap inputs of one weather slot (adjusted flux; Nrlmsise00.cu:215-241):
+// out[0] F10.7 (previous day), out[1] F10.7A, out[2..8] the Ap array.
+__device__ inline void weatherInputs(const EpochArgs& a, int record, int slot, double* out) {
     const double* W = a.weather;
     const int nd = a.wDays;
     auto at = [&](int comp, int r) { return W[comp * nd + r]; };
     auto lag = [&](int l) { const int h = 24 + slot - l; return at(5 + h % 8, record - 3 + h / 8); };
-    rec[REC_NRL_F107] = at(2, record - 1);
-    rec[REC_NRL_F107A] = at(3, record);
-    const double avg = at(4, record);
-    rec[REC_NRL_AP + 0] = avg;
-    rec[REC_NRL_AP + 1] = at(5 + slot, record);
-    rec[REC_NRL_AP + 2] = lag(1);
-    rec[REC_NRL_AP + 3] = lag(2);
-    rec[REC_NRL_AP + 4] = lag(3);
+    out[0] = at(2, record - 1);
+    out[1] = at(3, record);
+    out[2] = at(4, record);
+    out[3] = at(5 + slot, record);
+    out[4] = lag(1);
+    out[5] = lag(2);
+    out[6] = lag(3);
     double s5 = 0.0, s6 = 0.0;
     for (int k = 4; k <= 11; ++k) s5 += lag(k);
     for (int k = 12; k <= 19; ++k) s6 += lag(k);
-    rec[REC_NRL_AP + 5] = s5 / 8.0;
-    rec[REC_NRL_AP + 6] = s6 / 8.0;
+    out[7] = s5 / 8.0;
+    out[8] = s6 / 8.0;
+}
+
+// Writes the NRLMSISE-00 epoch inputs at dst[REC_NRL_* - REC_NRL] and, when
+// globe is given, the epoch-only globe7 terms (nrlmsise::GE_COUNT values).
+__device__ void nrlEpoch(const EpochArgs& a, double et, double* dst, double* globe) {
+    double* rec = dst - REC_NRL;
+    const double mjd = etToMjdUtcNrl(et);
+    int doy;
+    double sec;
+    doySecOf(mjd, doy, sec);
+    rec[REC_NRL_DOY] = double(doy);
+    rec[REC_NRL_SEC] = sec;
+    rec[REC_NRL_GAST] = gastHours(mjd);
+    int record, slot;
+    if (!weatherSlotOf(a, mjd, record, slot)) {
+        for (int k = 0; k < 10; ++k) rec[REC_NRL_F107 + k] = nan("");
+        rec[REC_NRL_VALID] = 0.0;
+        return;
+    }
+    weatherInputs(a, record, slot, rec + REC_NRL_F107);
     rec[REC_NRL_VALID] = 1.0;
     if (globe && a.nrlCoeff) {
         nrlmsise::NrlTables T{a.nrlCoeff};
@@ -464,6 +485,23 @@ __device__ void nrlEpoch(const EpochArgs& a, double et, double* dst, double* glo
     }
 }
 
+// One row of the weather-slot table (3-hour UTC slot `slot` of weather record
+// `record`): the slot's flux/Ap inputs and the globe7 terms that are constant
+// over the slot (nrlmsise::globeSlot).
+__device__ void nrlSlotRow(const EpochArgs& a, int record, int slot, double* row) {
+    using namespace nrlmsise;
+    for (int k = 0; k < SLOT_SIZE; ++k) row[k] = 0.0;
+    if (!(a.wDays > 0 && record >= 3 && record < a.wDays) || !a.nrlCoeff) return;
+    const double mjd = a.wStart + double(record) + (double(slot) + 0.5) / 8.0;
+    int doy;
+    double sec;
+    doySecOf(mjd, doy, sec);
+    weatherInputs(a, record, slot, row + SLOT_F107);
+    NrlTables T{a.nrlCoeff};
+    globeSlot(T, doy, row[SLOT_F107], row[SLOT_F107A], row + SLOT_AP, row);
+    row[SLOT_DOY] = double(doy);
+    row[SLOT_VALID] = 1.0;
+}
 
 }  // namespace
 }  // namespace epochdev

@@ -68,12 +68,13 @@ struct phx_ctx {
     std::string error;
     std::string dataDir;
     // device tables
-    phx::DBuf<double> shTerms, shDiag, nrl, freq, waves, terms, eop, weather;
+    phx::DBuf<double> shTerms, shDiag, nrl, freq, waves, terms, eop, weather, nrlSlots;
     phx::DBuf<float> shTerms32;
     phx::DBuf<int> shOrderStart;
     int freqCount[3] = {0, 0, 0};
     int waveCount = 0;
     // options
+    int numCU = 256;
     int nWaves = 4;
     int stepsPerLaunch = 32;
     double share0 = 0.85, shareCorr = 0.85, shareDrag = 0.35;
@@ -95,6 +96,7 @@ struct phx_batch {
     int nPad = 0;
     std::vector<int> slotOf;     // original index -> padded slot
     int totalSteps = 0;          // steps of the longest group
+    std::vector<int> tileSteps;  // per tile: steps until its last object ends (upper bound)
     int stepsDone = 0;
     phx::DBuf<double> y, cur, end, nextDt, ams, bcd, records, slotT0;
     phx::DBuf<uint8_t> status;

@@ -32,6 +32,10 @@ struct EpochArgs {
 
 int launchEpochRecords(const EpochArgs& a, const ForceModel& fm, hipStream_t stream);
 
+// Weather-slot rows for every 3-hour slot of the weather table (rows =
+// wDays * 8, nrlmsise::SLOT_SIZE doubles each).
+int launchNrlSlots(const EpochArgs& a, double* out, int rows, hipStream_t stream);
+
 // Uploads the degree-6 Legendre recursion factors used by the tide gradient.
 int uploadTideFactors();
 

@@ -12,6 +12,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "phx_common.h"
+
 namespace phx {
 namespace nrlmsise {
 
@@ -1098,7 +1100,9 @@ __device__ inline void globeEpoch(const NrlTables& T, int doy, double sec, doubl
 // expansion is term by term the published one (Nrlmsise00Model.inl:688-850)
 // with cos(x - p) = cos x cos p + sin x sin p for the longitude / local-time
 // phases and powers of the Ap decay factor taken by products.
-__device__ __attribute__((always_inline)) inline double globe7Fast(const State& st, const double* p, const double* E, double dfa, double glat, double& aptOut) {
+__device__ __attribute__((always_inline)) inline double globe7Fast(const State& st, const double* p, const double* E, double dfa, double glat,
+                                                                   double ut71, double ut79c, double ut79s, double ut58,
+                                                                   double& aptOut) {
     const double hr = 0.2618, dgtr = 1.74533E-2;
     const double(&plg)[4][9] = st.plg;
     const double cd14 = E[GE_CD14], cd18 = E[GE_CD18], f2 = E[GE_F2];
@@ -1144,8 +1148,8 @@ __device__ __attribute__((always_inline)) inline double globe7Fast(const State& 
     {
         const double c2g = cg * cg - sg * sg, s2g = 2.0 * sg * cg;
         double t11 = (1.0 + p[95] * plg[0][1]) * (1.0 + p[81] * dfa) * (1.0 + p[119] * plg[0][1] * cd14) *
-                     ((p[68] * plg[0][1] + p[69] * plg[0][3] + p[70] * plg[0][5]) * E[GE_UT71]);
-        t11 += (p[76] * plg[2][3] + p[77] * plg[2][5] + p[78] * plg[2][7]) * (E[GE_UT79C] * c2g - E[GE_UT79S] * s2g) *
+                     ((p[68] * plg[0][1] + p[69] * plg[0][3] + p[70] * plg[0][5]) * ut71);
+        t11 += (p[76] * plg[2][3] + p[77] * plg[2][5] + p[78] * plg[2][7]) * (ut79c * c2g - ut79s * s2g) *
                (1.0 + p[137] * dfa);
         tinf += t11;
     }
@@ -1154,7 +1158,7 @@ __device__ __attribute__((always_inline)) inline double globe7Fast(const State& 
         const double c136 = cg * cos(dgtr * p[136]) + sg * sin(dgtr * p[136]);
         tinf += apt0 * (1. + p[132] * plg[0][1]) * ((p[52] * plg[1][2] + p[98] * plg[1][4] + p[67] * plg[1][6]) * c97) +
                 apt0 * (p[133] * plg[1][1] + p[134] * plg[1][3] + p[135] * plg[1][5]) * cd14 * c136 +
-                apt0 * (p[55] * plg[0][1] + p[56] * plg[0][3] + p[57] * plg[0][5]) * E[GE_UT58];
+                apt0 * (p[55] * plg[0][1] + p[56] * plg[0][3] + p[57] * plg[0][5]) * ut58;
     }
     tinf += f2 * ((p[39] * plg[3][3] + (p[93] * plg[3][4] + p[46] * plg[3][6]) * cd14) * st.s3tloc +
                   (p[40] * plg[3][3] + (p[94] * plg[3][4] + p[48] * plg[3][6]) * cd14) * st.c3tloc);
@@ -1162,53 +1166,114 @@ __device__ __attribute__((always_inline)) inline double globe7Fast(const State& 
     return tinf;
 }
 
+// Globe basis of the fast policies: Legendre functions of the latitude,
+// local-time and longitude trigonometry (double and triple angles by products).
+__device__ __attribute__((always_inline)) inline void fastBasis(State& st, const Input& in, double dfa) {
+    const double dgtr = 1.74533E-2, hr = 0.2618;
+    const double c = sin(in.g_lat * dgtr), s = cos(in.g_lat * dgtr);
+    const double c2 = c * c, c4 = c2 * c2, s2 = s * s;
+    double(&plg)[4][9] = st.plg;
+    plg[0][1] = c;
+    plg[0][2] = 0.5 * (3.0 * c2 - 1.0);
+    plg[0][3] = 0.5 * (5.0 * c * c2 - 3.0 * c);
+    plg[0][4] = (35.0 * c4 - 30.0 * c2 + 3.0) / 8.0;
+    plg[0][5] = (63.0 * c2 * c2 * c - 70.0 * c2 * c + 15.0 * c) / 8.0;
+    plg[0][6] = (11.0 * c * plg[0][5] - 5.0 * plg[0][4]) / 6.0;
+    plg[1][1] = s;
+    plg[1][2] = 3.0 * c * s;
+    plg[1][3] = 1.5 * (5.0 * c2 - 1.0) * s;
+    plg[1][4] = 2.5 * (7.0 * c2 * c - 3.0 * c) * s;
+    plg[1][5] = 1.875 * (21.0 * c4 - 14.0 * c2 + 1.0) * s;
+    plg[1][6] = (11.0 * c * plg[1][5] - 6.0 * plg[1][4]) / 5.0;
+    plg[2][2] = 3.0 * s2;
+    plg[2][3] = 15.0 * s2 * c;
+    plg[2][4] = 7.5 * (7.0 * c2 - 1.0) * s2;
+    plg[2][5] = 3.0 * c * plg[2][4] - 2.0 * plg[2][3];
+    plg[2][6] = (11.0 * c * plg[2][5] - 7.0 * plg[2][4]) / 4.0;
+    plg[2][7] = (13.0 * c * plg[2][6] - 8.0 * plg[2][5]) / 5.0;
+    plg[3][3] = 15.0 * s2 * s;
+    plg[3][4] = 105.0 * s2 * s * c;
+    plg[3][5] = (9.0 * c * plg[3][4] - 7. * plg[3][3]) / 2.0;
+    plg[3][6] = (11.0 * c * plg[3][5] - 8. * plg[3][4]) / 3.0;
+    // local time harmonics from one sin/cos pair (double and triple angles)
+    const double sl = sin(hr * in.lst), cl = cos(hr * in.lst);
+    st.stloc = sl;
+    st.ctloc = cl;
+    st.s2tloc = 2.0 * sl * cl;
+    st.c2tloc = cl * cl - sl * sl;
+    st.s3tloc = sl * (3.0 - 4.0 * sl * sl);
+    st.c3tloc = cl * (4.0 * cl * cl - 3.0);
+    st.cglong = cos(dgtr * in.g_long);
+    st.sglong = sin(dgtr * in.g_long);
+    st.dfa = dfa;
+}
+
 // Fast policy: epoch terms from the per-epoch record.
 struct FastGlobe {
     const double* E;  // GE_COUNT values
-    __device__ __attribute__((always_inline)) void basis(State& st, const Input& in, const Flags&) const {
-        const double dgtr = 1.74533E-2, hr = 0.2618;
-        const double c = sin(in.g_lat * dgtr), s = cos(in.g_lat * dgtr);
-        const double c2 = c * c, c4 = c2 * c2, s2 = s * s;
-        double(&plg)[4][9] = st.plg;
-        plg[0][1] = c;
-        plg[0][2] = 0.5 * (3.0 * c2 - 1.0);
-        plg[0][3] = 0.5 * (5.0 * c * c2 - 3.0 * c);
-        plg[0][4] = (35.0 * c4 - 30.0 * c2 + 3.0) / 8.0;
-        plg[0][5] = (63.0 * c2 * c2 * c - 70.0 * c2 * c + 15.0 * c) / 8.0;
-        plg[0][6] = (11.0 * c * plg[0][5] - 5.0 * plg[0][4]) / 6.0;
-        plg[1][1] = s;
-        plg[1][2] = 3.0 * c * s;
-        plg[1][3] = 1.5 * (5.0 * c2 - 1.0) * s;
-        plg[1][4] = 2.5 * (7.0 * c2 * c - 3.0 * c) * s;
-        plg[1][5] = 1.875 * (21.0 * c4 - 14.0 * c2 + 1.0) * s;
-        plg[1][6] = (11.0 * c * plg[1][5] - 6.0 * plg[1][4]) / 5.0;
-        plg[2][2] = 3.0 * s2;
-        plg[2][3] = 15.0 * s2 * c;
-        plg[2][4] = 7.5 * (7.0 * c2 - 1.0) * s2;
-        plg[2][5] = 3.0 * c * plg[2][4] - 2.0 * plg[2][3];
-        plg[2][6] = (11.0 * c * plg[2][5] - 7.0 * plg[2][4]) / 4.0;
-        plg[2][7] = (13.0 * c * plg[2][6] - 8.0 * plg[2][5]) / 5.0;
-        plg[3][3] = 15.0 * s2 * s;
-        plg[3][4] = 105.0 * s2 * s * c;
-        plg[3][5] = (9.0 * c * plg[3][4] - 7. * plg[3][3]) / 2.0;
-        plg[3][6] = (11.0 * c * plg[3][5] - 8. * plg[3][4]) / 3.0;
-        // local time harmonics from one sin/cos pair (double and triple angles)
-        const double sl = sin(hr * in.lst), cl = cos(hr * in.lst);
-        st.stloc = sl;
-        st.ctloc = cl;
-        st.s2tloc = 2.0 * sl * cl;
-        st.c2tloc = cl * cl - sl * sl;
-        st.s3tloc = sl * (3.0 - 4.0 * sl * sl);
-        st.c3tloc = cl * (4.0 * cl * cl - 3.0);
-        st.cglong = cos(dgtr * in.g_long);
-        st.sglong = sin(dgtr * in.g_long);
-        st.dfa = E[GE_DFA];
-    }
+    __device__ __attribute__((always_inline)) void basis(State& st, const Input& in, const Flags&) const { fastBasis(st, in, E[GE_DFA]); }
     __device__ __attribute__((always_inline)) double operator()(State& st, int j, const double* p, const Input& in, const Flags&) const {
         // globe7 also leaves apt[0] for glob7s (lower thermosphere)
         const double* e = E + GE_STRIDE * j;
         double apt0 = 0.0;
-        const double v = globe7Fast(st, p, e, E[GE_DFA], in.g_lat, apt0);
+        const double v = globe7Fast(st, p, e, E[GE_DFA], in.g_lat, e[GE_UT71], e[GE_UT79C], e[GE_UT79S], e[GE_UT58], apt0);
+        if (p[51] != 0) st.apt[0] = apt0;
+        return v;
+    }
+};
+
+
+// Weather-slot rows (uneven epochs): per 3-hour UTC slot of the weather table,
+// the globe7 terms that are constant over the slot (day-of-year cosines, flux
+// factors, Ap functions) and, per table, cos/sin of the UT phases, so a lane
+// forms cos(sr (sec - p)) = cos(sr sec) cos(sr p) + sin(sr sec) sin(sr p).
+enum : int {
+    GS_C71 = 18, GS_S71, GS_C79, GS_S79, GS_C58, GS_S58, GS_STRIDE = 24,
+    SLOT_DFA = GE_TABLES * GS_STRIDE,
+    SLOT_DOY = SLOT_DFA + 1,
+    SLOT_F107 = SLOT_DFA + 2,    // F10.7 (previous day), then F10.7A, then the 7-entry Ap array
+    SLOT_F107A = SLOT_F107 + 1,
+    SLOT_AP = SLOT_F107 + 2,
+    SLOT_VALID = SLOT_AP + 7,
+    SLOT_SIZE = kNrlSlotSize
+};
+static_assert(SLOT_VALID < SLOT_SIZE, "slot row overflow");
+
+__device__ inline void globeSlot(const NrlTables& T, int doy, double f107, double f107A, const double* ap, double* out) {
+    const double sr = 7.2722E-5;
+    double E[GE_COUNT];
+    globeEpoch(T, doy, 0.0, f107, f107A, ap, E);
+    out[SLOT_DFA] = E[GE_DFA];
+    for (int j = 0; j < GE_TABLES; ++j) {
+        const double* p = globeTable(T, j);
+        double* o = out + GS_STRIDE * j;
+        for (int k = 0; k < GE_STRIDE; ++k) o[k] = E[GE_STRIDE * j + k];
+        o[GE_UT71] = o[GE_UT79C] = o[GE_UT79S] = o[GE_UT58] = 0.0;
+        o[GS_C71] = cos(sr * p[71]);
+        o[GS_S71] = sin(sr * p[71]);
+        o[GS_C79] = cos(sr * p[79]);
+        o[GS_S79] = sin(sr * p[79]);
+        o[GS_C58] = cos(sr * p[58]);
+        o[GS_S58] = sin(sr * p[58]);
+    }
+}
+
+// Slot policy: the lane's weather-slot row S and cos/sin(sr * sec) of its epoch.
+struct SlotGlobe {
+    const double* S;
+    double cs, ss;
+    __device__ __attribute__((always_inline)) void basis(State& st, const Input& in, const Flags&) const {
+        fastBasis(st, in, S[SLOT_DFA]);
+    }
+    __device__ __attribute__((always_inline)) double operator()(State& st, int j, const double* p, const Input& in,
+                                                               const Flags&) const {
+        const double* e = S + GS_STRIDE * j;
+        const double ut71 = cs * e[GS_C71] + ss * e[GS_S71];
+        const double ut79c = cs * e[GS_C79] + ss * e[GS_S79];
+        const double ut79s = ss * e[GS_C79] - cs * e[GS_S79];
+        const double ut58 = cs * e[GS_C58] + ss * e[GS_S58];
+        double apt0 = 0.0;
+        const double v = globe7Fast(st, p, e, S[SLOT_DFA], in.g_lat, ut71, ut79c, ut79s, ut58, apt0);
         if (p[51] != 0) st.apt[0] = apt0;
         return v;
     }

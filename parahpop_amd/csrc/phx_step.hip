@@ -406,11 +406,20 @@ __device__ inline nrlmsise::Flags nrlFlags() {
     return fl;
 }
 
+// NRLMSISE-00 epoch terms of a lane: MODE 1 reads the globe block of the
+// lane's epoch record (shared-epoch batches), MODE 2 the weather-slot row of
+// its epoch (uneven epochs), MODE 0 evaluates globe7 as published.
+struct NrlSlot {
+    const double* row;   // weather-slot row (MODE 2)
+    double cs, ss;       // cos/sin(sr * seconds of day)
+};
+
 // Species k0, k0 + stride, ... of the thermospheric model for this lane
 // (cgs number densities before the metric factor) into spec[k][lane].
-template <bool FAST>
-__device__ __attribute__((always_inline)) inline void nrlSpecies(const NrlLane& L, const double* nrl, const double* __restrict__ coeff, int k0, int stride,
-                           double (*spec)[kTile], int lane) {
+template <int MODE>
+__device__ __attribute__((always_inline)) inline void nrlSpecies(const NrlLane& L, const double* nrl, const NrlSlot& slot,
+                                                                 const double* __restrict__ coeff, int k0, int stride,
+                                                                 double (*spec)[kTile], int lane) {
     nrlmsise::Input in = L.in;
     const nrlmsise::Flags fl = nrlFlags();
     nrlmsise::State st;
@@ -418,8 +427,12 @@ __device__ __attribute__((always_inline)) inline void nrlSpecies(const NrlLane& 
     nrlmsise::Pre pre;
     // gtd7's gravity/radius at this latitude (switch 2 on), Nrlmsise00Model.inl:976-980
     nrlmsise::glatf(in.g_lat, st.gsurf, st.re);
-    if (FAST) {
+    if (MODE == 1) {
         const nrlmsise::FastGlobe G{nrl + (REC_NRL_GLOBE - REC_NRL)};
+        nrlmsise::gts7Prefix(st, T, in, fl, G, pre);
+        for (int k = k0; k < 8; k += stride) spec[k][lane] = nrlmsise::gts7Species(k, st, T, in, fl, G, pre);
+    } else if (MODE == 2) {
+        const nrlmsise::SlotGlobe G{slot.row, slot.cs, slot.ss};
         nrlmsise::gts7Prefix(st, T, in, fl, G, pre);
         for (int k = k0; k < 8; k += stride) spec[k][lane] = nrlmsise::gts7Species(k, st, T, in, fl, G, pre);
     } else {
@@ -430,7 +443,7 @@ __device__ __attribute__((always_inline)) inline void nrlSpecies(const NrlLane& 
 }
 
 // Full model for one lane (serial gtd7d), kg/km^3.
-template <bool FAST>
+template <int MODE>
 __device__ __noinline__ double nrlDensity(const ForceModel& fm, const double* nrl, const double* __restrict__ coeff,
                                           nrlmsise::Input in) {
     nrlmsise::Flags fl = nrlFlags();
@@ -439,7 +452,7 @@ __device__ __noinline__ double nrlDensity(const ForceModel& fm, const double* nr
     nrlmsise::Output out;
     nrlmsise::State st;
     nrlmsise::NrlTables T{coeff};
-    if (FAST) nrlmsise::gtd7d(st, T, in, fl, out, nrlmsise::FastGlobe{nrl + (REC_NRL_GLOBE - REC_NRL)});
+    if (MODE == 1) nrlmsise::gtd7d(st, T, in, fl, out, nrlmsise::FastGlobe{nrl + (REC_NRL_GLOBE - REC_NRL)});
     else nrlmsise::gtd7d(st, T, in, fl, out, nrlmsise::GenericGlobe{});
     if (!isfinite(out.d[5]) || out.d[5] < 0.0) return nan("");
     return (out.d[5] * 1.0e9) * fm.densityScale;
@@ -457,13 +470,13 @@ __device__ inline double nrlRhoFromSpecies(const ForceModel& fm, double (*spec)[
 }
 
 // Drag of one lane with the serial full model (diagnostic probe).
-template <bool FAST, class Rec>
+template <int MODE, class Rec>
 __device__ void dragAccel(const ForceModel& fm, const Rec& rec, const double* nrl, const double* __restrict__ coeff, double px,
                           double py, double pz, double vx, double vy, double vz, double bcd, double& ax, double& ay,
                           double& az) {
     const NrlLane L = nrlLane(fm, rec, nrl, px, py, pz);
     double rho = 0.0;
-    if (L.state == 1 || L.state == 2) rho = nrlDensity<FAST>(fm, nrl, coeff, L.in);
+    if (L.state == 1 || L.state == 2) rho = nrlDensity<MODE>(fm, nrl, coeff, L.in);
     else if (L.state == 3) rho = nan("");
     const double om0 = rec(REC_OMEGA), om1 = rec(REC_OMEGA + 1), om2 = rec(REC_OMEGA + 2);
     const double rx = vx - (om1 * pz - om2 * py);
@@ -618,8 +631,9 @@ __constant__ double kRkC[12] = {2. / 27, 1. / 9, 1. / 6, 5. / 12, 1. / 2, 5. / 6
 
 // One stage's force evaluation for the tile: SH order block of this wave,
 // the extra families assigned to this wave, then (wave 0) the sum.
-template <int W, bool FAST, class Rec>
-__device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, const Rec& rec, const double* nrl, int wave,
+template <int W, int MODE, class Rec>
+__device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, const Rec& rec, const double* nrl,
+                                   const NrlSlot& slot, int wave,
                                    int lane, int m0, int m1, double ams, double bcd, double (*xch)[kTile],
                                    double (*part)[3][kTile], double (*extra)[3][kTile], double (*spec)[kTile],
                                    double out[3]) {
@@ -641,7 +655,7 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
     nl.state = 0;
     if (fm.drag) {
         nl = nrlLane(fm, rec, nrl, px, py, pz);
-        if (__ballot(nl.state == 1) != 0ull) nrlSpecies<FAST>(nl, nrl, a.nrlCoeff, wave, W, spec, lane);
+        if (__ballot(nl.state == 1) != 0ull) nrlSpecies<MODE>(nl, nrl, slot, a.nrlCoeff, wave, W, spec, lane);
     }
     if (wave == corrWave(W)) {
         double ax = 0.0, ay = 0.0, az = 0.0;
@@ -678,7 +692,7 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
         if (fm.drag && nl.state != 0) {
             double rho;
             if (nl.state == 1) rho = nrlRhoFromSpecies(fm, spec, lane);
-            else if (nl.state == 2) rho = nrlDensity<FAST>(fm, nrl, a.nrlCoeff, nl.in);
+            else if (nl.state == 2) rho = nrlDensity<MODE>(fm, nrl, a.nrlCoeff, nl.in);
             else rho = nan("");
             // Drag.h:67-73 with v_rel = v - omega x r (Earth is the centre)
             const double vx = xch[3][lane], vy = xch[4][lane], vz = xch[5][lane];
@@ -774,12 +788,31 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
                 const double ts = xch[6][lane];
                 const GridRec rec = gridRec(gridBase, gridRows, gridT0, a.gridInvH, ts);
                 double nrl[13];
-                if (fm.drag) epochdev::nrlEpoch(wa, ts, nrl, nullptr);
-                stageForces<W, false>(a, fm, rec, nrl, wave, lane, m0, m1, ams, bcd, xch, part, extra, spec, acc);
+                NrlSlot slot{a.nrlSlots, 0.0, 0.0};
+                if (fm.drag) {
+                    // this lane's NRLMSISE-00 time inputs and weather-slot row
+                    const double mjd = epochdev::etToMjdUtcNrl(ts);
+                    int doy, record, wslot;
+                    double sec;
+                    epochdev::doySecOf(mjd, doy, sec);
+                    nrl[REC_NRL_DOY - REC_NRL] = double(doy);
+                    nrl[REC_NRL_SEC - REC_NRL] = sec;
+                    nrl[REC_NRL_GAST - REC_NRL] = epochdev::gastHours(mjd);
+                    const bool ok = epochdev::weatherSlotOf(wa, mjd, record, wslot) && a.nrlSlots;
+                    const double* row = a.nrlSlots + (ok ? size_t(record * 8 + wslot) * nrlmsise::SLOT_SIZE : 0);
+                    for (int k = 0; k < 9; ++k) nrl[REC_NRL_F107 - REC_NRL + k] = ok ? row[nrlmsise::SLOT_F107 + k] : nan("");
+                    nrl[REC_NRL_VALID - REC_NRL] = ok ? 1.0 : 0.0;
+                    const double sr = 7.2722E-5;
+                    slot.row = row;
+                    slot.cs = cos(sr * sec);
+                    slot.ss = sin(sr * sec);
+                }
+                stageForces<W, 2>(a, fm, rec, nrl, slot, wave, lane, m0, m1, ams, bcd, xch, part, extra, spec, acc);
             } else {
                 const ExactRec rec{recStep + s * REC_SIZE};
-                stageForces<W, true>(a, fm, rec, recStep + s * REC_SIZE + REC_NRL, wave, lane, m0, m1, ams, bcd, xch, part, extra,
-                                     spec, acc);
+                const NrlSlot noSlot{nullptr, 0.0, 0.0};
+                stageForces<W, 1>(a, fm, rec, recStep + s * REC_SIZE + REC_NRL, noSlot, wave, lane, m0, m1, ams, bcd, xch, part,
+                                  extra, spec, acc);
             }
             if (wave == 0) {
                 kst[s][0][lane] = xch[3][lane];
@@ -856,7 +889,7 @@ __global__ void __launch_bounds__(64) forceProbeKernel(int n, const double* pos,
         ax += bx, ay += by, az += bz;
     }
     if ((which < 0 || which == 4) && fm.drag) {
-        dragAccel<true>(fm, rec, recp + REC_NRL, nrlCoeff, px, py, pz, vx, vy, vz, bcd[i], bx, by, bz);
+        dragAccel<1>(fm, rec, recp + REC_NRL, nrlCoeff, px, py, pz, vx, vy, vz, bcd[i], bx, by, bz);
         ax += bx, ay += by, az += bz;
     }
     if ((which < 0 || which == 5) && (fm.tides || fm.relativity)) {
