@@ -91,7 +91,7 @@ void phx_batch_destroy(phx_batch* b);
 
 /* Tuning options (no reference counterpart; the reference reads
  * PARAHPOP_PERF_* environment overrides, AccumulateBodies.cu:20-48):
- *   "waves"            1, 2, 4 or 8 waves per 64-object tile
+ *   "waves"            1, 2 or 4 waves per 64-object tile
  *   "steps_per_launch" integration steps per kernel launch
  *   "share0", "share_corr", "share_drag"  relative SH order-block shares
  *   "sort_altitude"    1: group objects below the drag cutoff into common tiles */

@@ -715,7 +715,7 @@ int phx_set_option(phx_ctx* c, const char* key, double v) {
     const std::string k = key;
     if (k == "waves") {
         const int w = int(v);
-        if (w != 1 && w != 2 && w != 4 && w != 8) { c->error = "waves must be 1, 2, 4 or 8"; return PHX_EINVAL; }
+        if (w != 1 && w != 2 && w != 4) { c->error = "waves must be 1, 2 or 4"; return PHX_EINVAL; }
         c->nWaves = w;
     } else if (k == "steps_per_launch") {
         if (!(v >= 1)) { c->error = "steps_per_launch must be >= 1"; return PHX_EINVAL; }
@@ -814,7 +814,7 @@ int phx_batch_autotune(phx_batch* b, int trialSteps, phx_stats* best) {
         };
         std::vector<Cand> cands;
         if (c.fm.hasSh) {
-            for (int w : {1, 2, 4, 8})
+            for (int w : {1, 2, 4})
                 for (double s0 : {0.7, 0.85, 1.0}) {
                     if (w == 1 && s0 != 1.0) continue;
                     cands.push_back({w, s0});

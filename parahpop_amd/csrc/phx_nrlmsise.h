@@ -867,18 +867,152 @@ static __constant__ SpeciesPar kSpeciesPar[7] = {
     {7, 9, 6, 14.0, 0.0, 450.0, 2, 1, 0, 5, 4, 3},         // N
 };
 
+// Quantities that every densu call of a lane's species shares (same tinf, tlb,
+// s and lower-boundary nodes), each computed once with densu's own statements
+// (Nrlmsise00Model.inl:476-576), so a cached call returns the same bits as a
+// full one:
+//  * at the lane altitude z >= za: zeta(z, zlb) and the Bates temperature;
+//  * below za (z clamped to za): the temperature at za, the lower-thermosphere
+//    node update (tn1[0], tgn1[0]) and the five-node spline of 1/T;
+//  * the N2 diffusive-equilibrium factor of the mixing densities at z
+//    (xm = xmm, alpha = 0), shared by every species' mixed density;
+//  * the N2 density at its turbopause height (b28) and at Zlb (db28).
+struct DensuShared {
+    double re, za, zlb, s2, tinf, tlb, glb, xmm;
+    double zLane, zg2L, ttL;
+    bool laneUpper;
+    double zg2A, ttA, z1, z2, t1, zgdif, glb1;
+    double xs[5], ys[5], y2out[5];
+    double pmm, emm;   // pow(tlb/tt, 1 + gamma(xmm)) and its exp factor at the lane altitude
+    double db28, b28;
+};
+
+// zeta(zz, z1) with the lane's Earth radius
+__device__ inline double zeta0(const DensuShared& h, double zz) { return ((zz - h.z1) * (h.re + h.z1) / (h.re + zz)); }
+
+__device__ inline double densuUpperFrom(const DensuShared& h, double zg2, double tt, double dlb, double xm, double alpha) {
+    const double rgas = 831.4;
+    const double gamma = xm * h.glb / (h.s2 * rgas * h.tinf);
+    double expl = exp(-h.s2 * gamma * zg2);
+    if (expl > 50.0) expl = 50.0;
+    if (tt <= 0) expl = 50.0;
+    return dlb * pow((h.tlb / tt), ((1.0 + alpha + gamma))) * expl;
+}
+
+__device__ inline void densuShare(DensuShared& h, State& st, double alt, double tinf, double tlb, double s2, double zlb,
+                                  double xmm, const double* zn1, double* tn1, double* tgn1) {
+    constexpr int MN = 5;
+    const double rgas = 831.4;
+    h.re = st.re;
+    h.za = zn1[0];
+    h.zlb = zlb;
+    h.s2 = s2;
+    h.tinf = tinf;
+    h.tlb = tlb;
+    h.xmm = xmm;
+    h.glb = st.gsurf / pow((1.0 + zlb / st.re), 2.0);
+    h.zLane = alt;
+    h.laneUpper = !(alt < h.za);
+    const double zL = alt > h.za ? alt : h.za;
+    h.zg2L = zeta(st, zL, zlb);
+    h.ttL = tinf - (tinf - tlb) * exp(-s2 * h.zg2L);
+    // below za: z = za
+    h.zg2A = zeta(st, h.za, zlb);
+    h.ttA = tinf - (tinf - tlb) * exp(-s2 * h.zg2A);
+    const double dta = (tinf - h.ttA) * s2 * pow(((st.re + zlb) / (st.re + h.za)), 2.0);
+    tgn1[0] = dta;
+    tn1[0] = h.ttA;
+    h.z1 = zn1[0];
+    h.z2 = zn1[MN - 1];
+    h.t1 = tn1[0];
+    const double t2 = tn1[MN - 1];
+    h.zgdif = zeta(st, h.z2, h.z1);
+#pragma unroll
+    for (int k = 0; k < MN; ++k) {
+        h.xs[k] = zeta(st, zn1[k], h.z1) / h.zgdif;
+        h.ys[k] = 1.0 / tn1[k];
+    }
+    const double yd1 = -tgn1[0] / (h.t1 * h.t1) * h.zgdif;
+    const double yd2 = -tgn1[1] / (t2 * t2) * h.zgdif * pow(((st.re + h.z2) / (st.re + h.z1)), 2.0);
+    splineN(h.xs, h.ys, yd1, yd2, h.y2out);
+    h.glb1 = st.gsurf / pow((1.0 + h.z1 / st.re), 2.0);
+    // mixing-density factor at the lane altitude
+    const double gamma = xmm * h.glb / (s2 * rgas * tinf);
+    double expl = exp(-s2 * gamma * h.zg2L);
+    if (expl > 50.0) expl = 50.0;
+    if (h.ttL <= 0) expl = 50.0;
+    h.pmm = pow((tlb / h.ttL), ((1.0 + 0. + gamma)));
+    h.emm = expl;
+    h.db28 = h.b28 = 0.0;
+}
+
+// densu at an altitude below za from the shared lower-thermosphere profile.
+__device__ inline double densuBelow(const DensuShared& h, double alt, double dlb, double xm, double alpha) {
+    const double rgas = 831.4;
+    const double z = alt > h.z2 ? alt : h.z2;
+    const double x = zeta0(h, z) / h.zgdif;
+    double y;
+    splintN(h.xs, h.ys, h.y2out, x, &y);
+    const double tz = 1.0 / y;
+    if (xm == 0) return tz;
+    const double densa = densuUpperFrom(h, h.zg2A, h.ttA, dlb, xm, alpha);
+    const double gamm = xm * h.glb1 * h.zgdif / rgas;
+    double yi;
+    spliniN(h.xs, h.ys, h.y2out, x, &yi);
+    double expl = gamm * yi;
+    if (expl > 50.0) expl = 50.0;
+    if (tz <= 0) expl = 50.0;
+    return densa * pow((h.t1 / tz), (1.0 + alpha)) * exp(-expl);
+}
+
+// densu at the lane altitude, or at another altitude through the cached
+// profiles where they apply (full densu otherwise).
+__device__ inline double densuLane(const DensuShared& h, State& st, double dlb, double xm, double alpha, const double* zn1,
+                                   double* tn1, double* tgn1) {
+    double tz;
+    if (h.laneUpper) return densuUpperFrom(h, h.zg2L, h.ttL, dlb, xm, alpha);
+    return densu(st, h.zLane, dlb, h.tinf, h.tlb, xm, alpha, &tz, h.zlb, h.s2, 5, zn1, tn1, tgn1);
+}
+
+__device__ inline double densuAt(const DensuShared& h, State& st, double alt, double dlb, double xm, double alpha,
+                                 const double* zn1, double* tn1, double* tgn1) {
+    double tz;
+    if (alt < h.za) return densuBelow(h, alt, dlb, xm, alpha);
+    return densu(st, alt, dlb, h.tinf, h.tlb, xm, alpha, &tz, h.zlb, h.s2, 5, zn1, tn1, tgn1);
+}
+
 // Species k (0 N2, 1 He, 2 O, 3 O2, 4 Ar, 5 H, 6 N, 7 anomalous O): number
 // density in the model's cgs units (before the metric factor). One code path
 // for species 0-6 driven by kSpeciesPar (k is uniform across the wave, so the
 // parameters are scalar loads); each species performs the same operations in
-// the same order as its block of gts7. Densu's side effects (lower-thermosphere
-// nodes, tz) depend only on (tinf, tlb, s), so the N2 density at Zlb (b28) may
-// be evaluated before the species' own density.
+// the same order as its block of gts7, with the densu calls served from the
+// lane's DensuShared (gts7SpeciesShare). Densu's side effects
+// (lower-thermosphere nodes, tz) depend only on (tinf, tlb, s), so the shared
+// quantities may be evaluated before any species.
 template <class Globe>
-__device__ __attribute__((always_inline)) inline double gts7Species(int k, State& st, const NrlTables& T, Input& in,
-                                                                    const Flags& fl, const Globe& G, const Pre& pre) {
+__device__ __attribute__((always_inline)) inline void gts7SpeciesShare(DensuShared& h, bool needDb28, State& st,
+                                                                       const NrlTables& T, Input& in, const Flags& fl,
+                                                                       const Globe& G, const Pre& pre) {
+    double zn1[5] = {pre.zn1[0], pre.zn1[1], pre.zn1[2], pre.zn1[3], pre.zn1[4]};
+    const double* pdm = T.base + OFF_PDM;
+    const double z = in.alt;
+    densuShare(h, st, z, pre.tinf, pre.tlb, pre.s, (T.base + OFF_PTM)[5], pre.xmm, zn1, st.meso_tn1, st.meso_tgn1);
+    const bool wantB28 = !(z > 450.0 && z >= zn1[0]);
+    if (wantB28 || needDb28) {
+        const double g28 = fl.sw[21] * G(st, 4, (T.base + OFF_PD + 150 * (2)), in, fl);
+        h.db28 = pdm[20] * exp(g28) * (T.base + OFF_PD + 150 * (2))[0];
+    }
+    if (wantB28) {
+        const double zh28 = pdm[22] * pre.zhf;
+        h.b28 = densuAt(h, st, zh28, h.db28, 28.0 - pre.xmm, (0.0 - 1.0), zn1, st.meso_tn1, st.meso_tgn1);
+    }
+}
+
+template <class Globe>
+__device__ __attribute__((always_inline)) inline double gts7Species(int k, const DensuShared& h, State& st, const NrlTables& T,
+                                                                    Input& in, const Flags& fl, const Globe& G, const Pre& pre) {
     const int mn1 = 5;
-    const double tinf = pre.tinf, tlb = pre.tlb, s = pre.s, zhf = pre.zhf, xmm = pre.xmm;
+    const double s = pre.s, xmm = pre.xmm;
     double zn1[5] = {pre.zn1[0], pre.zn1[1], pre.zn1[2], pre.zn1[3], pre.zn1[4]};
     const double z = in.alt;
     double tz;
@@ -900,35 +1034,24 @@ __device__ __attribute__((always_inline)) inline double gts7Species(int k, State
     const SpeciesPar P = kSpeciesPar[k];
     const double* pdk = T.base + OFF_PD + 150 * P.pd;
     const double* pdmk = pdm + 10 * P.row;
-    const double alpha2 = 0.0;   // alpha[2] (N2)
-    // N2 mixed density at Zlb: N2's own mixing and the corrections of
-    // He, O2, Ar, H and N consume it; O never does
-    const bool wantB28 = k != 2 && !(z > 450.0 && z >= zn1[0]);
-    double db28 = 0.0, b28 = 0.0;
-    if (wantB28 || k == 0) {
-        const double g28 = fl.sw[21] * G(st, 4, (T.base + OFF_PD + 150 * (2)), in, fl);
-        db28 = pdm[20] * exp(g28) * (T.base + OFF_PD + 150 * (2))[0];
-    }
-    if (wantB28) {
-        const double zh28 = pdm[22] * zhf;
-        b28 = densu(st, zh28, db28, tinf, tlb, 28.0 - xmm, (alpha2 - 1.0), &tz, zlbT, s, mn1, zn1, tn1, tgn1);
-    }
-    double dbk = db28;
+    const double b28 = h.b28;
+    double dbk = h.db28;
     if (k != 0) {
         const double gk = fl.sw[21] * G(st, P.globe, pdk, in, fl);
         dbk = pdmk[0] * exp(gk) * pdk[0];
     }
-    double d = densu(st, z, dbk, tinf, tlb, P.xm, P.alpha, &tz, zlbT, s, mn1, zn1, tn1, tgn1);
+    double d = densuLane(h, st, dbk, P.xm, P.alpha, zn1, tn1, tgn1);
     const double zhm28 = pdm[23] * pdl[25 + 5];
     const bool mix = fl.sw[15] && (k == 1 ? z < P.altl : z <= P.altl);
     if (mix) {
+        // mixed density at the lane altitude: densu(z, b, ..., xmm, 0)
         if (k == 0) {
-            const double dm28 = densu(st, z, b28, tinf, tlb, xmm, alpha2, &tz, zlbT, s, mn1, zn1, tn1, tgn1);
+            const double dm28 = h.laneUpper ? b28 * h.pmm * h.emm : densuLane(h, st, b28, xmm, 0.0, zn1, tn1, tgn1);
             d = dnet(d, dm28, zhm28, xmm, 28.0);
         } else {
             const double zh = pdmk[2];
-            const double bk = densu(st, zh, dbk, tinf, tlb, P.xm - xmm, P.alpha - 1.0, &tz, zlbT, s, mn1, zn1, tn1, tgn1);
-            const double dmk = densu(st, z, bk, tinf, tlb, xmm, 0., &tz, zlbT, s, mn1, zn1, tn1, tgn1);
+            const double bk = densuAt(h, st, zh, dbk, P.xm - xmm, P.alpha - 1.0, zn1, tn1, tgn1);
+            const double dmk = h.laneUpper ? bk * h.pmm * h.emm : densuLane(h, st, bk, xmm, 0., zn1, tn1, tgn1);
             d = dnet(d, dmk, zhm28, xmm, P.xm);
             if (k == 2) {
                 const double rl = pdmk[1] * pdl[25 + 16] * (1.0 + fl.sw[1] * pdl[23] * (in.f107A - 150.0));

@@ -406,6 +406,12 @@ __device__ inline nrlmsise::Flags nrlFlags() {
     return fl;
 }
 
+// Species order for the wave split (wave w evaluates entries w, w + W, ...):
+// with four waves each wave gets at most one of O, H and N, the species whose
+// mixing corrections run at 300-450 km, and the wave that also adds tides and
+// SRP (the third) gets Ar and anomalous O, which never mix there.
+__constant__ int kSpeciesOrder[8] = {0, 1, 4, 3, 2, 5, 7, 6};
+
 // NRLMSISE-00 epoch terms of a lane: MODE 1 reads the globe block of the
 // lane's epoch record (shared-epoch batches), MODE 2 the weather-slot row of
 // its epoch (uneven epochs), MODE 0 evaluates globe7 as published.
@@ -430,15 +436,30 @@ __device__ __attribute__((always_inline)) inline void nrlSpecies(const NrlLane& 
     if (MODE == 1) {
         const nrlmsise::FastGlobe G{nrl + (REC_NRL_GLOBE - REC_NRL)};
         nrlmsise::gts7Prefix(st, T, in, fl, G, pre);
-        for (int k = k0; k < 8; k += stride) spec[k][lane] = nrlmsise::gts7Species(k, st, T, in, fl, G, pre);
+        nrlmsise::DensuShared h;
+        nrlmsise::gts7SpeciesShare(h, kSpeciesOrder[k0] == 0, st, T, in, fl, G, pre);
+        for (int i = k0; i < 8; i += stride) {
+            const int k = kSpeciesOrder[i];
+            spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
+        }
     } else if (MODE == 2) {
         const nrlmsise::SlotGlobe G{slot.row, slot.cs, slot.ss};
         nrlmsise::gts7Prefix(st, T, in, fl, G, pre);
-        for (int k = k0; k < 8; k += stride) spec[k][lane] = nrlmsise::gts7Species(k, st, T, in, fl, G, pre);
+        nrlmsise::DensuShared h;
+        nrlmsise::gts7SpeciesShare(h, kSpeciesOrder[k0] == 0, st, T, in, fl, G, pre);
+        for (int i = k0; i < 8; i += stride) {
+            const int k = kSpeciesOrder[i];
+            spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
+        }
     } else {
         const nrlmsise::GenericGlobe G{};
         nrlmsise::gts7Prefix(st, T, in, fl, G, pre);
-        for (int k = k0; k < 8; k += stride) spec[k][lane] = nrlmsise::gts7Species(k, st, T, in, fl, G, pre);
+        nrlmsise::DensuShared h;
+        nrlmsise::gts7SpeciesShare(h, kSpeciesOrder[k0] == 0, st, T, in, fl, G, pre);
+        for (int i = k0; i < 8; i += stride) {
+            const int k = kSpeciesOrder[i];
+            spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
+        }
     }
 }
 
@@ -957,7 +978,6 @@ int launchStep(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t s
             case 1: hipLaunchKernelGGL((rkf78Kernel<1, true>), grid, dim3(64), 0, stream, a, fm); break;
             case 2: hipLaunchKernelGGL((rkf78Kernel<2, true>), grid, dim3(128), 0, stream, a, fm); break;
             case 4: hipLaunchKernelGGL((rkf78Kernel<4, true>), grid, dim3(256), 0, stream, a, fm); break;
-            case 8: hipLaunchKernelGGL((rkf78Kernel<8, true>), grid, dim3(512), 0, stream, a, fm); break;
             default: return -2;
         }
     } else {
@@ -965,7 +985,6 @@ int launchStep(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t s
             case 1: hipLaunchKernelGGL((rkf78Kernel<1, false>), grid, dim3(64), 0, stream, a, fm); break;
             case 2: hipLaunchKernelGGL((rkf78Kernel<2, false>), grid, dim3(128), 0, stream, a, fm); break;
             case 4: hipLaunchKernelGGL((rkf78Kernel<4, false>), grid, dim3(256), 0, stream, a, fm); break;
-            case 8: hipLaunchKernelGGL((rkf78Kernel<8, false>), grid, dim3(512), 0, stream, a, fm); break;
             default: return -2;
         }
     }

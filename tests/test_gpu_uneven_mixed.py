@@ -65,7 +65,7 @@ def test_autotune_keeps_results(full_workload):
         b = ctx.batch(t, 600.0)
         best = b.autotune(2)
         assert best["object_steps"] == 2 * t["n"]
-        assert ctx.option("waves") in (1, 2, 4, 8)
+        assert ctx.option("waves") in (1, 2, 4)
         b.run()
         _, got = b.download()
     rel = np.max(np.linalg.norm(got[:3] - ref[:3], axis=0) / np.linalg.norm(ref[:3], axis=0))
