@@ -117,6 +117,7 @@ void loadTables(phx_ctx& c) {
         }
         if (v.size() != 3300) throw std::runtime_error("NRLMSISE-00 coefficient table must hold 3300 values");
         c.nrl.upload(v.data(), v.size(), c.stream);
+        if (uploadGlobeTrig(v.data()) != 0) throw HipError("globe phase upload failed");
         const WeatherTable& wt = c.env.weather;
         c.weather.upload(wt.data.data(), wt.data.size(), c.stream);
     }

@@ -39,6 +39,10 @@ int launchNrlSlots(const EpochArgs& a, double* out, int rows, hipStream_t stream
 // Uploads the degree-6 Legendre recursion factors used by the tide gradient.
 int uploadTideFactors();
 
+// Uploads the constant longitude / local-time phases of the NRLMSISE-00
+// globe7 tables, computed from the host copy of the coefficient table.
+int uploadGlobeTrig(const double* nrlCoeffHost);
+
 // RKF78 step kernel: `waves` waves per 64-object tile (1, 2, 4 or 8).
 int launchStep(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t stream);
 

@@ -1218,6 +1218,11 @@ __device__ inline void globeEpoch(const NrlTables& T, int doy, double sec, doubl
     }
 }
 
+// Constant phases of a coefficient table's longitude / local-time terms:
+// cos and sin of hr p[131], dgtr p[97] and dgtr p[136] (GT_COUNT per table,
+// tables in GE order), computed once on the host from the coefficient table.
+enum : int { GT_C131 = 0, GT_S131, GT_C97, GT_S97, GT_C136, GT_S136, GT_COUNT };
+
 // Per-object globe7 with the epoch terms E (this table's GE_STRIDE block) and
 // the basis of State (plg, local-time and longitude trigonometry). The
 // expansion is term by term the published one (Nrlmsise00Model.inl:688-850)
@@ -1225,7 +1230,7 @@ __device__ inline void globeEpoch(const NrlTables& T, int doy, double sec, doubl
 // phases and powers of the Ap decay factor taken by products.
 __device__ __attribute__((always_inline)) inline double globe7Fast(const State& st, const double* p, const double* E, double dfa, double glat,
                                                                    double ut71, double ut79c, double ut79s, double ut58,
-                                                                   double& aptOut) {
+                                                                   const double* tr, double& aptOut) {
     const double hr = 0.2618, dgtr = 1.74533E-2;
     const double(&plg)[4][9] = st.plg;
     const double cd14 = E[GE_CD14], cd18 = E[GE_CD18], f2 = E[GE_F2];
@@ -1257,7 +1262,7 @@ __device__ __attribute__((always_inline)) inline double globe7Fast(const State& 
             const double sx = 1.0 + (1.0 - e19) / (1.0 - ex) * sqrt(ex);
             apt0 = (g[0] + (g[1] * ex + g[2] * e2 + g[3] * e3 + (g[4] * e4 + g[5] * e12) * (1.0 - e8) / (1.0 - ex))) / sx;
         }
-        const double c131 = st.ctloc * cos(hr * p[131]) + st.stloc * sin(hr * p[131]);
+        const double c131 = st.ctloc * tr[GT_C131] + st.stloc * tr[GT_S131];
         tinf += apt0 * (p[50] + p[96] * plg[0][2] + p[54] * plg[0][4] +
                         (p[125] * plg[0][1] + p[126] * plg[0][3] + p[127] * plg[0][5]) * cd14 +
                         (p[128] * plg[1][1] + p[129] * plg[1][3] + p[130] * plg[1][5]) * c131);
@@ -1277,8 +1282,8 @@ __device__ __attribute__((always_inline)) inline double globe7Fast(const State& 
         tinf += t11;
     }
     if (p[51] != 0) {
-        const double c97 = cg * cos(dgtr * p[97]) + sg * sin(dgtr * p[97]);
-        const double c136 = cg * cos(dgtr * p[136]) + sg * sin(dgtr * p[136]);
+        const double c97 = cg * tr[GT_C97] + sg * tr[GT_S97];
+        const double c136 = cg * tr[GT_C136] + sg * tr[GT_S136];
         tinf += apt0 * (1. + p[132] * plg[0][1]) * ((p[52] * plg[1][2] + p[98] * plg[1][4] + p[67] * plg[1][6]) * c97) +
                 apt0 * (p[133] * plg[1][1] + p[134] * plg[1][3] + p[135] * plg[1][5]) * cd14 * c136 +
                 apt0 * (p[55] * plg[0][1] + p[56] * plg[0][3] + p[57] * plg[0][5]) * ut58;
@@ -1334,12 +1339,14 @@ __device__ __attribute__((always_inline)) inline void fastBasis(State& st, const
 // Fast policy: epoch terms from the per-epoch record.
 struct FastGlobe {
     const double* E;  // GE_COUNT values
+    const double* T;  // GE_TABLES x GT_COUNT table phases
     __device__ __attribute__((always_inline)) void basis(State& st, const Input& in, const Flags&) const { fastBasis(st, in, E[GE_DFA]); }
     __device__ __attribute__((always_inline)) double operator()(State& st, int j, const double* p, const Input& in, const Flags&) const {
         // globe7 also leaves apt[0] for glob7s (lower thermosphere)
         const double* e = E + GE_STRIDE * j;
         double apt0 = 0.0;
-        const double v = globe7Fast(st, p, e, E[GE_DFA], in.g_lat, e[GE_UT71], e[GE_UT79C], e[GE_UT79S], e[GE_UT58], apt0);
+        const double v = globe7Fast(st, p, e, E[GE_DFA], in.g_lat, e[GE_UT71], e[GE_UT79C], e[GE_UT79S], e[GE_UT58],
+                                    T + GT_COUNT * j, apt0);
         if (p[51] != 0) st.apt[0] = apt0;
         return v;
     }
@@ -1384,6 +1391,7 @@ __device__ inline void globeSlot(const NrlTables& T, int doy, double f107, doubl
 // Slot policy: the lane's weather-slot row S and cos/sin(sr * sec) of its epoch.
 struct SlotGlobe {
     const double* S;
+    const double* T;  // GE_TABLES x GT_COUNT table phases
     double cs, ss;
     __device__ __attribute__((always_inline)) void basis(State& st, const Input& in, const Flags&) const {
         fastBasis(st, in, S[SLOT_DFA]);
@@ -1396,7 +1404,7 @@ struct SlotGlobe {
         const double ut79s = ss * e[GS_C79] - cs * e[GS_S79];
         const double ut58 = cs * e[GS_C58] + ss * e[GS_S58];
         double apt0 = 0.0;
-        const double v = globe7Fast(st, p, e, S[SLOT_DFA], in.g_lat, ut71, ut79c, ut79s, ut58, apt0);
+        const double v = globe7Fast(st, p, e, S[SLOT_DFA], in.g_lat, ut71, ut79c, ut79s, ut58, T + GT_COUNT * j, apt0);
         if (p[51] != 0) st.apt[0] = apt0;
         return v;
     }

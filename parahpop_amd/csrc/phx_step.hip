@@ -412,6 +412,9 @@ __device__ inline nrlmsise::Flags nrlFlags() {
 // SRP (the third) gets Ar and anomalous O, which never mix there.
 __constant__ int kSpeciesOrder[8] = {0, 1, 4, 3, 2, 5, 7, 6};
 
+// Constant phases of the eleven globe7 coefficient tables (nrlmsise::GT_*).
+__constant__ double kGlobeTrig[nrlmsise::GE_TABLES * nrlmsise::GT_COUNT];
+
 // NRLMSISE-00 epoch terms of a lane: MODE 1 reads the globe block of the
 // lane's epoch record (shared-epoch batches), MODE 2 the weather-slot row of
 // its epoch (uneven epochs), MODE 0 evaluates globe7 as published.
@@ -434,7 +437,7 @@ __device__ __attribute__((always_inline)) inline void nrlSpecies(const NrlLane& 
     // gtd7's gravity/radius at this latitude (switch 2 on), Nrlmsise00Model.inl:976-980
     nrlmsise::glatf(in.g_lat, st.gsurf, st.re);
     if (MODE == 1) {
-        const nrlmsise::FastGlobe G{nrl + (REC_NRL_GLOBE - REC_NRL)};
+        const nrlmsise::FastGlobe G{nrl + (REC_NRL_GLOBE - REC_NRL), kGlobeTrig};
         nrlmsise::gts7Prefix(st, T, in, fl, G, pre);
         nrlmsise::DensuShared h;
         nrlmsise::gts7SpeciesShare(h, kSpeciesOrder[k0] == 0, st, T, in, fl, G, pre);
@@ -443,7 +446,7 @@ __device__ __attribute__((always_inline)) inline void nrlSpecies(const NrlLane& 
             spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
         }
     } else if (MODE == 2) {
-        const nrlmsise::SlotGlobe G{slot.row, slot.cs, slot.ss};
+        const nrlmsise::SlotGlobe G{slot.row, kGlobeTrig, slot.cs, slot.ss};
         nrlmsise::gts7Prefix(st, T, in, fl, G, pre);
         nrlmsise::DensuShared h;
         nrlmsise::gts7SpeciesShare(h, kSpeciesOrder[k0] == 0, st, T, in, fl, G, pre);
@@ -473,7 +476,7 @@ __device__ __noinline__ double nrlDensity(const ForceModel& fm, const double* nr
     nrlmsise::Output out;
     nrlmsise::State st;
     nrlmsise::NrlTables T{coeff};
-    if (MODE == 1) nrlmsise::gtd7d(st, T, in, fl, out, nrlmsise::FastGlobe{nrl + (REC_NRL_GLOBE - REC_NRL)});
+    if (MODE == 1) nrlmsise::gtd7d(st, T, in, fl, out, nrlmsise::FastGlobe{nrl + (REC_NRL_GLOBE - REC_NRL), kGlobeTrig});
     else nrlmsise::gtd7d(st, T, in, fl, out, nrlmsise::GenericGlobe{});
     if (!isfinite(out.d[5]) || out.d[5] < 0.0) return nan("");
     return (out.d[5] * 1.0e9) * fm.densityScale;
@@ -968,6 +971,22 @@ int uploadTideFactors() {
     if (hipMemcpyToSymbol(HIP_SYMBOL(kTideD), D, sizeof D) != hipSuccess) return -1;
     if (hipMemcpyToSymbol(HIP_SYMBOL(kTideS), S, sizeof S) != hipSuccess) return -1;
     return 0;
+}
+
+int uploadGlobeTrig(const double* nrlCoeffHost) {
+    const double hr = 0.2618, dgtr = 1.74533E-2;
+    double t[nrlmsise::GE_TABLES * nrlmsise::GT_COUNT];
+    for (int j = 0; j < nrlmsise::GE_TABLES; ++j) {
+        const double* p = nrlCoeffHost + (j == 0 ? nrlmsise::OFF_PT : (j == 1 ? nrlmsise::OFF_PS : nrlmsise::OFF_PD + 150 * (j - 2)));
+        double* o = t + nrlmsise::GT_COUNT * j;
+        o[nrlmsise::GT_C131] = cos(hr * p[131]);
+        o[nrlmsise::GT_S131] = sin(hr * p[131]);
+        o[nrlmsise::GT_C97] = cos(dgtr * p[97]);
+        o[nrlmsise::GT_S97] = sin(dgtr * p[97]);
+        o[nrlmsise::GT_C136] = cos(dgtr * p[136]);
+        o[nrlmsise::GT_S136] = sin(dgtr * p[136]);
+    }
+    return hipMemcpyToSymbol(HIP_SYMBOL(kGlobeTrig), t, sizeof t) == hipSuccess ? 0 : -1;
 }
 
 int launchStep(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t stream) {
