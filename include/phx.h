@@ -93,7 +93,8 @@ void phx_batch_destroy(phx_batch* b);
  * PARAHPOP_PERF_* environment overrides, AccumulateBodies.cu:20-48):
  *   "waves"            1, 2 or 4 waves per 64-object tile
  *   "steps_per_launch" integration steps per kernel launch
- *   "share0", "share_corr", "share_drag"  relative SH order-block shares
+ *   "share0", "share_corr"  relative SH order-block shares of wave 0 and of the
+ *                      wave that also evaluates tides, relativity and SRP
  *   "sort_altitude"    1: group objects below the drag cutoff into common tiles */
 int phx_set_option(phx_ctx* ctx, const char* key, double value);
 double phx_get_option(const phx_ctx* ctx, const char* key);

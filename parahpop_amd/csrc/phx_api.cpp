@@ -723,7 +723,6 @@ int phx_set_option(phx_ctx* c, const char* key, double v) {
         c->stepsPerLaunch = int(v);
     } else if (k == "share0") c->share0 = v;
     else if (k == "share_corr") c->shareCorr = v;
-    else if (k == "share_drag") c->shareDrag = v;
     else if (k == "sort_altitude") c->sortAltitude = v != 0.0;
     else if (k == "exact_groups_max") c->exactGroupsMax = int(v);
     else if (k == "grid_step") {
@@ -749,7 +748,6 @@ double phx_get_option(const phx_ctx* c, const char* key) {
     if (k == "steps_per_launch") return c->stepsPerLaunch;
     if (k == "share0") return c->share0;
     if (k == "share_corr") return c->shareCorr;
-    if (k == "share_drag") return c->shareDrag;
     if (k == "sort_altitude") return c->sortAltitude;
     if (k == "exact_groups_max") return c->exactGroupsMax;
     if (k == "grid_step") return c->gridStep;

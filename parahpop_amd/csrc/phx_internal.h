@@ -77,7 +77,7 @@ struct phx_ctx {
     int numCU = 256;
     int nWaves = 4;
     int stepsPerLaunch = 32;
-    double share0 = 0.85, shareCorr = 0.85, shareDrag = 0.35;
+    double share0 = 0.85, shareCorr = 0.85;   // SH order-block shares of wave 0 and of the corrections wave
     int sortAltitude = 1;
     int exactGroupsMax = 16;     // more distinct (start, end) groups than this -> grid epochs
     double gridStep = 60.0;      // s, spacing of the epoch grid

@@ -17,6 +17,11 @@
 namespace phx {
 namespace nrlmsise {
 
+// Coefficient tables and per-epoch records are read-only during a kernel: the
+// fast paths read them through constant-address-space pointers, so uniform
+// indices become scalar loads into SGPR operands.
+typedef const __attribute__((address_space(4))) double* NrlCD;
+
 enum : int {
     OFF_PT = 0, OFF_PD = 150, OFF_PS = 1500, OFF_PDL = 1650, OFF_PTM = 1700, OFF_PDM = 1710,
     OFF_PTL = 1790, OFF_PMA = 2190, OFF_SAM = 3190, OFF_PAVGM = 3290, COEFF_COUNT = 3300
@@ -813,37 +818,37 @@ template <class Globe>
 __device__ inline void gts7Prefix(State& st, const NrlTables& T, Input& in, const Flags& fl, const Globe& G, Pre& pre) {
     double zn1[5] = {120.0, 110.0, 100.0, 90.0, 72.5};
     const double dgtr = 1.74533E-2, dr = 1.72142E-2;
-    const double za = (T.base+OFF_PDL+25*(1))[15];
+    const double za = ((NrlCD)(T.base+OFF_PDL+25*(1)))[15];
     zn1[0] = za;
     st.g0Valid = false;
     G.basis(st, in, fl);
     double tinf;
-    if (in.alt > zn1[0]) tinf = (T.base+OFF_PTM)[0] * (T.base+OFF_PT)[0] * (1.0 + fl.sw[16] * G(st, 0, (T.base+OFF_PT), in, fl));
-    else tinf = (T.base+OFF_PTM)[0] * (T.base+OFF_PT)[0];
+    if (in.alt > zn1[0]) tinf = ((NrlCD)(T.base+OFF_PTM))[0] * ((NrlCD)(T.base+OFF_PT))[0] * (1.0 + fl.sw[16] * G(st, 0, (T.base+OFF_PT), in, fl));
+    else tinf = ((NrlCD)(T.base+OFF_PTM))[0] * ((NrlCD)(T.base+OFF_PT))[0];
     double g0v;
-    if (in.alt > zn1[4]) g0v = (T.base+OFF_PTM)[3] * (T.base+OFF_PS)[0] * (1.0 + fl.sw[19] * G(st, 1, (T.base+OFF_PS), in, fl));
-    else g0v = (T.base+OFF_PTM)[3] * (T.base+OFF_PS)[0];
-    const double tlb = (T.base+OFF_PTM)[1] * (1.0 + fl.sw[17] * G(st, 5, (T.base+OFF_PD+150*(3)), in, fl)) * (T.base+OFF_PD+150*(3))[0];
+    if (in.alt > zn1[4]) g0v = ((NrlCD)(T.base+OFF_PTM))[3] * ((NrlCD)(T.base+OFF_PS))[0] * (1.0 + fl.sw[19] * G(st, 1, (T.base+OFF_PS), in, fl));
+    else g0v = ((NrlCD)(T.base+OFF_PTM))[3] * ((NrlCD)(T.base+OFF_PS))[0];
+    const double tlb = ((NrlCD)(T.base+OFF_PTM))[1] * (1.0 + fl.sw[17] * G(st, 5, (T.base+OFF_PD+150*(3)), in, fl)) * ((NrlCD)(T.base+OFF_PD+150*(3)))[0];
     const double s = g0v / (tinf - tlb);
     if (in.alt < 300.0) {
-        st.meso_tn1[1] = (T.base+OFF_PTM)[6] * (T.base+OFF_PTL+100*(0))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(0)), in, fl));
-        st.meso_tn1[2] = (T.base+OFF_PTM)[2] * (T.base+OFF_PTL+100*(1))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(1)), in, fl));
-        st.meso_tn1[3] = (T.base+OFF_PTM)[7] * (T.base+OFF_PTL+100*(2))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(2)), in, fl));
-        st.meso_tn1[4] = (T.base+OFF_PTM)[4] * (T.base+OFF_PTL+100*(3))[0] / (1.0 - fl.sw[18] * fl.sw[20] * glob7s(st, (T.base+OFF_PTL+100*(3)), in, fl));
-        st.meso_tgn1[1] = (T.base+OFF_PTM)[8] * (T.base+OFF_PMA+100*(8))[0] * (1.0 + fl.sw[18] * fl.sw[20] * glob7s(st, (T.base+OFF_PMA+100*(8)), in, fl)) * st.meso_tn1[4] *
-                          st.meso_tn1[4] / (pow(((T.base+OFF_PTM)[4] * (T.base+OFF_PTL+100*(3))[0]), 2.0));
+        st.meso_tn1[1] = ((NrlCD)(T.base+OFF_PTM))[6] * ((NrlCD)(T.base+OFF_PTL+100*(0)))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(0)), in, fl));
+        st.meso_tn1[2] = ((NrlCD)(T.base+OFF_PTM))[2] * ((NrlCD)(T.base+OFF_PTL+100*(1)))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(1)), in, fl));
+        st.meso_tn1[3] = ((NrlCD)(T.base+OFF_PTM))[7] * ((NrlCD)(T.base+OFF_PTL+100*(2)))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(2)), in, fl));
+        st.meso_tn1[4] = ((NrlCD)(T.base+OFF_PTM))[4] * ((NrlCD)(T.base+OFF_PTL+100*(3)))[0] / (1.0 - fl.sw[18] * fl.sw[20] * glob7s(st, (T.base+OFF_PTL+100*(3)), in, fl));
+        st.meso_tgn1[1] = ((NrlCD)(T.base+OFF_PTM))[8] * ((NrlCD)(T.base+OFF_PMA+100*(8)))[0] * (1.0 + fl.sw[18] * fl.sw[20] * glob7s(st, (T.base+OFF_PMA+100*(8)), in, fl)) * st.meso_tn1[4] *
+                          st.meso_tn1[4] / (pow((((NrlCD)(T.base+OFF_PTM))[4] * ((NrlCD)(T.base+OFF_PTL+100*(3)))[0]), 2.0));
     } else {
-        st.meso_tn1[1] = (T.base+OFF_PTM)[6] * (T.base+OFF_PTL+100*(0))[0];
-        st.meso_tn1[2] = (T.base+OFF_PTM)[2] * (T.base+OFF_PTL+100*(1))[0];
-        st.meso_tn1[3] = (T.base+OFF_PTM)[7] * (T.base+OFF_PTL+100*(2))[0];
-        st.meso_tn1[4] = (T.base+OFF_PTM)[4] * (T.base+OFF_PTL+100*(3))[0];
-        st.meso_tgn1[1] = (T.base+OFF_PTM)[8] * (T.base+OFF_PMA+100*(8))[0] * st.meso_tn1[4] * st.meso_tn1[4] / (pow(((T.base+OFF_PTM)[4] * (T.base+OFF_PTL+100*(3))[0]), 2.0));
+        st.meso_tn1[1] = ((NrlCD)(T.base+OFF_PTM))[6] * ((NrlCD)(T.base+OFF_PTL+100*(0)))[0];
+        st.meso_tn1[2] = ((NrlCD)(T.base+OFF_PTM))[2] * ((NrlCD)(T.base+OFF_PTL+100*(1)))[0];
+        st.meso_tn1[3] = ((NrlCD)(T.base+OFF_PTM))[7] * ((NrlCD)(T.base+OFF_PTL+100*(2)))[0];
+        st.meso_tn1[4] = ((NrlCD)(T.base+OFF_PTM))[4] * ((NrlCD)(T.base+OFF_PTL+100*(3)))[0];
+        st.meso_tgn1[1] = ((NrlCD)(T.base+OFF_PTM))[8] * ((NrlCD)(T.base+OFF_PMA+100*(8)))[0] * st.meso_tn1[4] * st.meso_tn1[4] / (pow((((NrlCD)(T.base+OFF_PTM))[4] * ((NrlCD)(T.base+OFF_PTL+100*(3)))[0]), 2.0));
     }
     pre.tinf = tinf;
     pre.tlb = tlb;
     pre.s = s;
-    pre.zhf = (T.base+OFF_PDL+25*(1))[24] * (1.0 + fl.sw[5] * (T.base+OFF_PDL+25*(0))[24] * sin(dgtr * in.g_lat) * cos(dr * (in.doy - (T.base+OFF_PT)[13])));
-    pre.xmm = (T.base+OFF_PDM+10*(2))[4];
+    pre.zhf = ((NrlCD)(T.base+OFF_PDL+25*(1)))[24] * (1.0 + fl.sw[5] * ((NrlCD)(T.base+OFF_PDL+25*(0)))[24] * sin(dgtr * in.g_lat) * cos(dr * (in.doy - ((NrlCD)(T.base+OFF_PT))[13])));
+    pre.xmm = ((NrlCD)(T.base+OFF_PDM+10*(2)))[4];
     for (int k = 0; k < 5; ++k) pre.zn1[k] = zn1[k];
 }
 
@@ -994,13 +999,13 @@ __device__ __attribute__((always_inline)) inline void gts7SpeciesShare(DensuShar
                                                                        const NrlTables& T, Input& in, const Flags& fl,
                                                                        const Globe& G, const Pre& pre) {
     double zn1[5] = {pre.zn1[0], pre.zn1[1], pre.zn1[2], pre.zn1[3], pre.zn1[4]};
-    const double* pdm = T.base + OFF_PDM;
+    const NrlCD pdm = (NrlCD)(T.base + OFF_PDM);
     const double z = in.alt;
-    densuShare(h, st, z, pre.tinf, pre.tlb, pre.s, (T.base + OFF_PTM)[5], pre.xmm, zn1, st.meso_tn1, st.meso_tgn1);
+    densuShare(h, st, z, pre.tinf, pre.tlb, pre.s, ((NrlCD)(T.base + OFF_PTM))[5], pre.xmm, zn1, st.meso_tn1, st.meso_tgn1);
     const bool wantB28 = !(z > 450.0 && z >= zn1[0]);
     if (wantB28 || needDb28) {
         const double g28 = fl.sw[21] * G(st, 4, (T.base + OFF_PD + 150 * (2)), in, fl);
-        h.db28 = pdm[20] * exp(g28) * (T.base + OFF_PD + 150 * (2))[0];
+        h.db28 = pdm[20] * exp(g28) * ((NrlCD)(T.base + OFF_PD + 150 * (2)))[0];
     }
     if (wantB28) {
         const double zh28 = pdm[22] * pre.zhf;
@@ -1018,12 +1023,12 @@ __device__ __attribute__((always_inline)) inline double gts7Species(int k, const
     double tz;
     double* tn1 = st.meso_tn1;
     double* tgn1 = st.meso_tgn1;
-    const double* pdm = T.base + OFF_PDM;   // pdm[row][i] = pdm[10 row + i]
-    const double* pdl = T.base + OFF_PDL;   // pdl[blk][i] = pdl[25 blk + i]
-    const double zlbT = (T.base + OFF_PTM)[5];
+    const NrlCD pdm = (NrlCD)(T.base + OFF_PDM);   // pdm[row][i] = pdm[10 row + i]
+    const NrlCD pdl = (NrlCD)(T.base + OFF_PDL);   // pdl[blk][i] = pdl[25 blk + i]
+    const double zlbT = ((NrlCD)(T.base + OFF_PTM))[5];
     if (k == 7) {  // anomalous O (d[8])
         const double g16h = fl.sw[21] * G(st, 10, (T.base + OFF_PD + 150 * (8)), in, fl);
-        const double db16h = pdm[70] * exp(g16h) * (T.base + OFF_PD + 150 * (8))[0];
+        const double db16h = pdm[70] * exp(g16h) * ((NrlCD)(T.base + OFF_PD + 150 * (8)))[0];
         const double tho = pdm[79] * pdl[6];
         const double ddo = densu(st, z, db16h, tho, tho, 16., 0.0, &tz, zlbT, s, mn1, zn1, tn1, tgn1);
         const double zsht = pdm[75];
@@ -1032,12 +1037,13 @@ __device__ __attribute__((always_inline)) inline double gts7Species(int k, const
         return ddo * exp(-zsht / zsho * (exp(-(z - zmho) / zsht) - 1.));
     }
     const SpeciesPar P = kSpeciesPar[k];
-    const double* pdk = T.base + OFF_PD + 150 * P.pd;
-    const double* pdmk = pdm + 10 * P.row;
+    const double* pdkG = T.base + OFF_PD + 150 * P.pd;
+    const NrlCD pdk = (NrlCD)pdkG;
+    const NrlCD pdmk = pdm + 10 * P.row;
     const double b28 = h.b28;
     double dbk = h.db28;
     if (k != 0) {
-        const double gk = fl.sw[21] * G(st, P.globe, pdk, in, fl);
+        const double gk = fl.sw[21] * G(st, P.globe, pdkG, in, fl);
         dbk = pdmk[0] * exp(gk) * pdk[0];
     }
     double d = densuLane(h, st, dbk, P.xm, P.alpha, zn1, tn1, tgn1);
@@ -1228,9 +1234,10 @@ enum : int { GT_C131 = 0, GT_S131, GT_C97, GT_S97, GT_C136, GT_S136, GT_COUNT };
 // expansion is term by term the published one (Nrlmsise00Model.inl:688-850)
 // with cos(x - p) = cos x cos p + sin x sin p for the longitude / local-time
 // phases and powers of the Ap decay factor taken by products.
-__device__ __attribute__((always_inline)) inline double globe7Fast(const State& st, const double* p, const double* E, double dfa, double glat,
+template <class EP>
+__device__ __attribute__((always_inline)) inline double globe7Fast(const State& st, NrlCD p, EP E, double dfa, double glat,
                                                                    double ut71, double ut79c, double ut79s, double ut58,
-                                                                   const double* tr, double& aptOut) {
+                                                                   NrlCD tr, double& aptOut) {
     const double hr = 0.2618, dgtr = 1.74533E-2;
     const double(&plg)[4][9] = st.plg;
     const double cd14 = E[GE_CD14], cd18 = E[GE_CD18], f2 = E[GE_F2];
@@ -1257,7 +1264,7 @@ __device__ __attribute__((always_inline)) inline double globe7Fast(const State& 
         } else {
             double ex = exp(-10800.0 * sqrt(p[51] * p[51]) / (1.0 + p[138] * (45.0 - sqrt(glat * glat))));
             if (ex > 0.99999) ex = 0.99999;
-            const double* g = E + GE_G0;
+            const auto g = E + GE_G0;
             const double e2 = ex * ex, e3 = e2 * ex, e4 = e2 * e2, e8 = e4 * e4, e12 = e8 * e4, e16 = e8 * e8, e19 = e16 * e3;
             const double sx = 1.0 + (1.0 - e19) / (1.0 - ex) * sqrt(ex);
             apt0 = (g[0] + (g[1] * ex + g[2] * e2 + g[3] * e3 + (g[4] * e4 + g[5] * e12) * (1.0 - e8) / (1.0 - ex))) / sx;
@@ -1345,8 +1352,9 @@ struct FastGlobe {
         // globe7 also leaves apt[0] for glob7s (lower thermosphere)
         const double* e = E + GE_STRIDE * j;
         double apt0 = 0.0;
-        const double v = globe7Fast(st, p, e, E[GE_DFA], in.g_lat, e[GE_UT71], e[GE_UT79C], e[GE_UT79S], e[GE_UT58],
-                                    T + GT_COUNT * j, apt0);
+        const NrlCD ec = (NrlCD)e;
+        const double v = globe7Fast(st, (NrlCD)p, ec, ((NrlCD)E)[GE_DFA], in.g_lat, ec[GE_UT71], ec[GE_UT79C], ec[GE_UT79S],
+                                    ec[GE_UT58], (NrlCD)(T + GT_COUNT * j), apt0);
         if (p[51] != 0) st.apt[0] = apt0;
         return v;
     }
@@ -1404,7 +1412,8 @@ struct SlotGlobe {
         const double ut79s = ss * e[GS_C79] - cs * e[GS_S79];
         const double ut58 = cs * e[GS_C58] + ss * e[GS_S58];
         double apt0 = 0.0;
-        const double v = globe7Fast(st, p, e, S[SLOT_DFA], in.g_lat, ut71, ut79c, ut79s, ut58, T + GT_COUNT * j, apt0);
+        const double v = globe7Fast(st, (NrlCD)p, e, S[SLOT_DFA], in.g_lat, ut71, ut79c, ut79s, ut58, (NrlCD)(T + GT_COUNT * j),
+                                    apt0);
         if (p[51] != 0) st.apt[0] = apt0;
         return v;
     }

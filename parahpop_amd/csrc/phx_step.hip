@@ -66,7 +66,7 @@ enum : uint8_t { ST_TERMINATED = 1, ST_EOS = 2, ST_PAD = 4 };
 // combined with cubic Lagrange weights (uneven epochs).
 struct ExactRec {
     const double* __restrict__ p;
-    __device__ double operator()(int f) const { return p[f]; }
+    __device__ double operator()(int f) const { return ((const __attribute__((address_space(4))) double*)p)[f]; }
 };
 struct GridRec {
     const double* __restrict__ p;  // row k-1
