@@ -127,6 +127,20 @@ __device__ inline double dnet(double dd, double dm, double zhm, double xmm, doub
     return a;
 }
 
+// dnet with (1 + e^ylog)^(1/a) taken as exp(log(1 + e^ylog) / a) (species fast path)
+__device__ inline double dnetFast(double dd, double dm, double zhm, double xmm, double xm) {
+    double a = zhm / (xmm - xm);
+    if (!((dm > 0) && (dd > 0))) {
+        if (dd == 0 && dm == 0) dd = 1;
+        if (dm == 0) return dd;
+        if (dd == 0) return dm;
+    }
+    const double ylog = a * log(dm / dd);
+    if (ylog < -10) return dd;
+    if (ylog > 10) return dm;
+    return dd * exp(log(1.0 + exp(ylog)) * (1.0 / a));
+}
+
 // :223-252
 __device__ inline void splini(const double* xa, const double* ya, const double* y2a, int n, double x, double* y) {
     double yi = 0;
@@ -884,9 +898,9 @@ static __constant__ SpeciesPar kSpeciesPar[7] = {
 //  * the N2 density at its turbopause height (b28) and at Zlb (db28).
 struct DensuShared {
     double re, za, zlb, s2, tinf, tlb, glb, xmm;
-    double zLane, zg2L, ttL;
+    double zLane, zg2L, ttL, lnL;   // lnL = log(tlb / ttL)
     bool laneUpper;
-    double zg2A, ttA, z1, z2, t1, zgdif, glb1;
+    double zg2A, ttA, lnA, z1, z2, t1, zgdif, glb1;
     double xs[5], ys[5], y2out[5];
     double pmm, emm;   // pow(tlb/tt, 1 + gamma(xmm)) and its exp factor at the lane altitude
     double db28, b28;
@@ -895,13 +909,17 @@ struct DensuShared {
 // zeta(zz, z1) with the lane's Earth radius
 __device__ inline double zeta0(const DensuShared& h, double zz) { return ((zz - h.z1) * (h.re + h.z1) / (h.re + zz)); }
 
-__device__ inline double densuUpperFrom(const DensuShared& h, double zg2, double tt, double dlb, double xm, double alpha) {
+// densu's diffusive-equilibrium density above za. The power (tlb/tt)^(1+alpha+gamma)
+// is taken as exp((1+alpha+gamma) log(tlb/tt)) with the lane's shared logarithm
+// (relative difference from pow ~1e-15).
+__device__ inline double densuUpperFrom(const DensuShared& h, double zg2, double tt, double lnT, double dlb, double xm,
+                                        double alpha) {
     const double rgas = 831.4;
     const double gamma = xm * h.glb / (h.s2 * rgas * h.tinf);
     double expl = exp(-h.s2 * gamma * zg2);
     if (expl > 50.0) expl = 50.0;
     if (tt <= 0) expl = 50.0;
-    return dlb * pow((h.tlb / tt), ((1.0 + alpha + gamma))) * expl;
+    return dlb * exp(((1.0 + alpha + gamma)) * lnT) * expl;
 }
 
 __device__ inline void densuShare(DensuShared& h, State& st, double alt, double tinf, double tlb, double s2, double zlb,
@@ -921,9 +939,11 @@ __device__ inline void densuShare(DensuShared& h, State& st, double alt, double 
     const double zL = alt > h.za ? alt : h.za;
     h.zg2L = zeta(st, zL, zlb);
     h.ttL = tinf - (tinf - tlb) * exp(-s2 * h.zg2L);
+    h.lnL = log(tlb / h.ttL);
     // below za: z = za
     h.zg2A = zeta(st, h.za, zlb);
     h.ttA = tinf - (tinf - tlb) * exp(-s2 * h.zg2A);
+    h.lnA = log(tlb / h.ttA);
     const double dta = (tinf - h.ttA) * s2 * pow(((st.re + zlb) / (st.re + h.za)), 2.0);
     tgn1[0] = dta;
     tn1[0] = h.ttA;
@@ -946,7 +966,7 @@ __device__ inline void densuShare(DensuShared& h, State& st, double alt, double 
     double expl = exp(-s2 * gamma * h.zg2L);
     if (expl > 50.0) expl = 50.0;
     if (h.ttL <= 0) expl = 50.0;
-    h.pmm = pow((tlb / h.ttL), ((1.0 + 0. + gamma)));
+    h.pmm = exp(((1.0 + 0. + gamma)) * h.lnL);
     h.emm = expl;
     h.db28 = h.b28 = 0.0;
 }
@@ -960,14 +980,14 @@ __device__ inline double densuBelow(const DensuShared& h, double alt, double dlb
     splintN(h.xs, h.ys, h.y2out, x, &y);
     const double tz = 1.0 / y;
     if (xm == 0) return tz;
-    const double densa = densuUpperFrom(h, h.zg2A, h.ttA, dlb, xm, alpha);
+    const double densa = densuUpperFrom(h, h.zg2A, h.ttA, h.lnA, dlb, xm, alpha);
     const double gamm = xm * h.glb1 * h.zgdif / rgas;
     double yi;
     spliniN(h.xs, h.ys, h.y2out, x, &yi);
     double expl = gamm * yi;
     if (expl > 50.0) expl = 50.0;
     if (tz <= 0) expl = 50.0;
-    return densa * pow((h.t1 / tz), (1.0 + alpha)) * exp(-expl);
+    return densa * exp((1.0 + alpha) * log(h.t1 / tz)) * exp(-expl);
 }
 
 // densu at the lane altitude, or at another altitude through the cached
@@ -975,7 +995,7 @@ __device__ inline double densuBelow(const DensuShared& h, double alt, double dlb
 __device__ inline double densuLane(const DensuShared& h, State& st, double dlb, double xm, double alpha, const double* zn1,
                                    double* tn1, double* tgn1) {
     double tz;
-    if (h.laneUpper) return densuUpperFrom(h, h.zg2L, h.ttL, dlb, xm, alpha);
+    if (h.laneUpper) return densuUpperFrom(h, h.zg2L, h.ttL, h.lnL, dlb, xm, alpha);
     return densu(st, h.zLane, dlb, h.tinf, h.tlb, xm, alpha, &tz, h.zlb, h.s2, 5, zn1, tn1, tgn1);
 }
 
@@ -1053,12 +1073,12 @@ __device__ __attribute__((always_inline)) inline double gts7Species(int k, const
         // mixed density at the lane altitude: densu(z, b, ..., xmm, 0)
         if (k == 0) {
             const double dm28 = h.laneUpper ? b28 * h.pmm * h.emm : densuLane(h, st, b28, xmm, 0.0, zn1, tn1, tgn1);
-            d = dnet(d, dm28, zhm28, xmm, 28.0);
+            d = dnetFast(d, dm28, zhm28, xmm, 28.0);
         } else {
             const double zh = pdmk[2];
             const double bk = densuAt(h, st, zh, dbk, P.xm - xmm, P.alpha - 1.0, zn1, tn1, tgn1);
             const double dmk = h.laneUpper ? bk * h.pmm * h.emm : densuLane(h, st, bk, xmm, 0., zn1, tn1, tgn1);
-            d = dnet(d, dmk, zhm28, xmm, P.xm);
+            d = dnetFast(d, dmk, zhm28, xmm, P.xm);
             if (k == 2) {
                 const double rl = pdmk[1] * pdl[25 + 16] * (1.0 + fl.sw[1] * pdl[23] * (in.f107A - 150.0));
                 d = d * ccor2(z, rl, pdmk[5] * pdl[P.hIdx], pdmk[4] * pdl[P.zIdx], pdmk[5] * pdl[25 + 4]);
