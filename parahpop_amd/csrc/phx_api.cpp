@@ -532,7 +532,9 @@ StepArgs contextStepArgs(const phx_ctx& c, int& W) {
     a.wStart = c.env.hasWeather ? c.env.weather.startMjdUtc : 0.0;
     a.nrlSlots = c.nrlSlots.p;
     a.gridInvH = 1.0 / c.gridStep;
-    W = c.fm.hasSh ? c.nWaves : 1;
+    // drag models run at most four waves per tile (the NRLMSISE-00 path needs
+    // the register budget of one wave per SIMD)
+    W = c.fm.hasSh ? (c.fm.drag ? std::min(c.nWaves, 4) : c.nWaves) : 1;
     if (c.fm.hasSh) orderBlocks(c, W, a.blockStart);
     else
         for (int w = 0; w <= kMaxWaves; ++w) a.blockStart[w] = 0;
@@ -716,7 +718,7 @@ int phx_set_option(phx_ctx* c, const char* key, double v) {
     const std::string k = key;
     if (k == "waves") {
         const int w = int(v);
-        if (w != 1 && w != 2 && w != 4) { c->error = "waves must be 1, 2 or 4"; return PHX_EINVAL; }
+        if (w != 1 && w != 2 && w != 4 && w != 8 && w != 16) { c->error = "waves must be 1, 2, 4, 8 or 16"; return PHX_EINVAL; }
         c->nWaves = w;
     } else if (k == "steps_per_launch") {
         if (!(v >= 1)) { c->error = "steps_per_launch must be >= 1"; return PHX_EINVAL; }
@@ -813,9 +815,10 @@ int phx_batch_autotune(phx_batch* b, int trialSteps, phx_stats* best) {
         };
         std::vector<Cand> cands;
         if (c.fm.hasSh) {
-            for (int w : {1, 2, 4})
+            for (int w : {1, 2, 4, 8, 16})
                 for (double s0 : {0.7, 0.85, 1.0}) {
                     if (w == 1 && s0 != 1.0) continue;
+                    if (c.fm.drag && w > 4) continue;
                     cands.push_back({w, s0});
                 }
         } else {

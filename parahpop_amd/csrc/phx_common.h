@@ -9,7 +9,7 @@ constexpr int kStages = 13;
 constexpr int kTile = 64;          // objects per workgroup tile = one wave64
 constexpr int kMaxBodies = 11;     // Earth, Sun, Moon, 8 planetary system barycenters
 constexpr int kNrlSlotSize = 280;  // doubles per NRLMSISE-00 weather-slot row
-constexpr int kMaxWaves = 8;       // waves per tile that share one object's work
+constexpr int kMaxWaves = 16;      // waves per tile that share one object's work
 
 // Per-epoch record: everything the force model needs that depends only on the
 // stage epoch, computed once per distinct epoch and read by every object.

@@ -655,7 +655,7 @@ __constant__ double kRkC[12] = {2. / 27, 1. / 9, 1. / 6, 5. / 12, 1. / 2, 5. / 6
 
 // One stage's force evaluation for the tile: SH order block of this wave,
 // the extra families assigned to this wave, then (wave 0) the sum.
-template <int W, int MODE, class Rec>
+template <int W, int MODE, bool DRAG, class Rec>
 __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, const Rec& rec, const double* nrl,
                                    const NrlSlot& slot, int wave,
                                    int lane, int m0, int m1, double ams, double bcd, double (*xch)[kTile],
@@ -677,7 +677,7 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
     // lanes below the cutoff; wave 0 assembles the density after the barrier.
     NrlLane nl;
     nl.state = 0;
-    if (fm.drag) {
+    if (DRAG && fm.drag) {
         nl = nrlLane(fm, rec, nrl, px, py, pz);
         if (__ballot(nl.state == 1) != 0ull) nrlSpecies<MODE>(nl, nrl, slot, a.nrlCoeff, wave, W, spec, lane);
     }
@@ -713,7 +713,7 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
             az += hz;
         }
         double dx = 0.0, dy = 0.0, dz = 0.0;
-        if (fm.drag && nl.state != 0) {
+        if (DRAG && fm.drag && nl.state != 0) {
             double rho;
             if (nl.state == 1) rho = nrlRhoFromSpecies(fm, spec, lane);
             else if (nl.state == 2) rho = nrlDensity<MODE>(fm, nrl, a.nrlCoeff, nl.in);
@@ -735,7 +735,10 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
     }
 }
 
-template <int W, bool GRID>
+// DRAG = false: the kernel for force models without NRLMSISE-00 drag, which
+// carries none of the drag code and so keeps a small register budget (more
+// waves per SIMD, up to 16 waves per tile).
+template <int W, bool GRID, bool DRAG>
 __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm) {
     __shared__ double kst[kStages][6][kTile];
     __shared__ double xch[7][kTile];          // staged position, velocity, stage epoch
@@ -813,7 +816,7 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
                 const GridRec rec = gridRec(gridBase, gridRows, gridT0, a.gridInvH, ts);
                 double nrl[13];
                 NrlSlot slot{a.nrlSlots, 0.0, 0.0};
-                if (fm.drag) {
+                if (DRAG && fm.drag) {
                     // this lane's NRLMSISE-00 time inputs and weather-slot row
                     const double mjd = epochdev::etToMjdUtcNrl(ts);
                     int doy, record, wslot;
@@ -831,11 +834,11 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
                     slot.cs = cos(sr * sec);
                     slot.ss = sin(sr * sec);
                 }
-                stageForces<W, 2>(a, fm, rec, nrl, slot, wave, lane, m0, m1, ams, bcd, xch, part, extra, spec, acc);
+                stageForces<W, 2, DRAG>(a, fm, rec, nrl, slot, wave, lane, m0, m1, ams, bcd, xch, part, extra, spec, acc);
             } else {
                 const ExactRec rec{recStep + s * REC_SIZE};
                 const NrlSlot noSlot{nullptr, 0.0, 0.0};
-                stageForces<W, 1>(a, fm, rec, recStep + s * REC_SIZE + REC_NRL, noSlot, wave, lane, m0, m1, ams, bcd, xch, part,
+                stageForces<W, 1, DRAG>(a, fm, rec, recStep + s * REC_SIZE + REC_NRL, noSlot, wave, lane, m0, m1, ams, bcd, xch, part,
                                   extra, spec, acc);
             }
             if (wave == 0) {
@@ -989,24 +992,34 @@ int uploadGlobeTrig(const double* nrlCoeffHost) {
     return hipMemcpyToSymbol(HIP_SYMBOL(kGlobeTrig), t, sizeof t) == hipSuccess ? 0 : -1;
 }
 
+namespace {
+template <bool GRID, bool DRAG>
+int launchShape(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t stream) {
+    const dim3 grid(a.nTiles);
+    switch (waves) {
+        case 1: hipLaunchKernelGGL((rkf78Kernel<1, GRID, DRAG>), grid, dim3(64), 0, stream, a, fm); break;
+        case 2: hipLaunchKernelGGL((rkf78Kernel<2, GRID, DRAG>), grid, dim3(128), 0, stream, a, fm); break;
+        case 4: hipLaunchKernelGGL((rkf78Kernel<4, GRID, DRAG>), grid, dim3(256), 0, stream, a, fm); break;
+        case 8:
+            if (DRAG) return -2;
+            hipLaunchKernelGGL((rkf78Kernel<8, GRID, false>), grid, dim3(512), 0, stream, a, fm);
+            break;
+        case 16:
+            if (DRAG) return -2;
+            hipLaunchKernelGGL((rkf78Kernel<16, GRID, false>), grid, dim3(1024), 0, stream, a, fm);
+            break;
+        default: return -2;
+    }
+    return 0;
+}
+}  // namespace
+
 int launchStep(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t stream) {
     if (a.nTiles <= 0) return 0;
-    const dim3 grid(a.nTiles);
-    if (a.grid) {
-        switch (waves) {
-            case 1: hipLaunchKernelGGL((rkf78Kernel<1, true>), grid, dim3(64), 0, stream, a, fm); break;
-            case 2: hipLaunchKernelGGL((rkf78Kernel<2, true>), grid, dim3(128), 0, stream, a, fm); break;
-            case 4: hipLaunchKernelGGL((rkf78Kernel<4, true>), grid, dim3(256), 0, stream, a, fm); break;
-            default: return -2;
-        }
-    } else {
-        switch (waves) {
-            case 1: hipLaunchKernelGGL((rkf78Kernel<1, false>), grid, dim3(64), 0, stream, a, fm); break;
-            case 2: hipLaunchKernelGGL((rkf78Kernel<2, false>), grid, dim3(128), 0, stream, a, fm); break;
-            case 4: hipLaunchKernelGGL((rkf78Kernel<4, false>), grid, dim3(256), 0, stream, a, fm); break;
-            default: return -2;
-        }
-    }
+    int rc;
+    if (a.grid) rc = fm.drag ? launchShape<true, true>(a, fm, waves, stream) : launchShape<true, false>(a, fm, waves, stream);
+    else rc = fm.drag ? launchShape<false, true>(a, fm, waves, stream) : launchShape<false, false>(a, fm, waves, stream);
+    if (rc != 0) return rc;
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
