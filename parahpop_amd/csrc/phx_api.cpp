@@ -582,6 +582,39 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
     check(hipEventRecord(c.ev0, c.stream), "hipEventRecord");
     int launches = 0;
     int blocksFor = W;   // wave count the order blocks in `a` were split for
+    const bool split = c.split && !b.grid && c.fm.drag && c.fm.hasSh;
+    if (split) {
+        const size_t np = size_t(b.nPad);
+        if (!b.ks.p) {
+            b.ks.alloc(size_t(kStages) * 6 * np);
+            b.ysBuf.alloc(7 * np);
+            b.parts.alloc(6 * np);
+            b.spec.alloc(8 * np);
+        }
+        const int Ws = c.splitWaves;
+        orderBlocks(c, Ws, a.blockStart);
+        SplitArgs sp{b.ks.p, b.ysBuf.p, b.parts.p, b.spec.p, 0, 0, c.splitSpecies};
+        while (remaining > 0) {
+            int active = b.nTiles;
+            while (active > 0 && b.tileSteps[size_t(active - 1)] <= b.stepsDone) --active;
+            if (active == 0) {
+                b.stepsDone += remaining;
+                break;
+            }
+            a.nTiles = active;
+            sp.step = b.stepsDone;
+            for (int s = 0; s < kStages; ++s) {
+                sp.stage = s;
+                if (launchSplitStage(a, c.fm, sp, Ws, c.stream) != 0) throw HipError("split stage launch failed");
+                launches += 2;
+            }
+            if (launchSplitCombine(a, sp, c.stream) != 0) throw HipError("split combine launch failed");
+            ++launches;
+            ++b.stepsDone;
+            --remaining;
+        }
+        W = Ws;
+    }
     while (remaining > 0) {
         const int k = std::min(remaining, std::max(1, c.stepsPerLaunch));
         // launch only the prefix of tiles that still has work (uneven batches are
@@ -621,7 +654,7 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
         stats->epoch_records = double(b.epochRecords);
         stats->launches = launches;
         stats->waves = W;
-        stats->steps_per_launch = c.stepsPerLaunch;
+        stats->steps_per_launch = split ? 1 : c.stepsPerLaunch;
         stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
     }
     return 0;
@@ -731,6 +764,15 @@ int phx_set_option(phx_ctx* c, const char* key, double v) {
         if (!(v > 0 && v <= 600)) { c->error = "grid_step must be in (0, 600] s"; return PHX_EINVAL; }
         c->gridStep = v;
     } else if (k == "force_grid") c->forceGrid = v != 0.0;
+    else if (k == "split") c->split = v != 0.0;
+    else if (k == "split_species") {
+        if (v != 0 && v != 1 && v != 2 && v != 4) { c->error = "split_species must be 0, 1, 2 or 4"; return PHX_EINVAL; }
+        c->splitSpecies = int(v);
+    }
+    else if (k == "split_waves") {
+        if (v != 4 && v != 8 && v != 16) { c->error = "split_waves must be 4, 8 or 16"; return PHX_EINVAL; }
+        c->splitWaves = int(v);
+    }
     else if (k == "serve_steps_per_tick") {
         if (!(v >= 1)) { c->error = "serve_steps_per_tick must be >= 1"; return PHX_EINVAL; }
         c->serveStepsPerTick = int(v);
@@ -754,6 +796,9 @@ double phx_get_option(const phx_ctx* c, const char* key) {
     if (k == "exact_groups_max") return c->exactGroupsMax;
     if (k == "grid_step") return c->gridStep;
     if (k == "force_grid") return c->forceGrid;
+    if (k == "split") return c->split;
+    if (k == "split_species") return c->splitSpecies;
+    if (k == "split_waves") return c->splitWaves;
     if (k == "step_seconds") return c->run.stepSeconds;
     if (k == "duration_seconds") return c->run.durationSeconds;
     if (k == "max_num_steps") return double(c->run.maxNumSteps);

@@ -82,6 +82,9 @@ struct phx_ctx {
     int exactGroupsMax = 16;     // more distinct (start, end) groups than this -> grid epochs
     double gridStep = 60.0;      // s, spacing of the epoch grid
     int forceGrid = 0;
+    int split = 1;               // shared-epoch batches with drag: split stages
+    int splitWaves = 16;         // waves per tile of the drag-free split kernel
+    int splitSpecies = 1;        // species per drag workgroup (0: one four-wave workgroup per tile)
     // serving options
     int serveStepsPerTick = 16;
     long long serveSlots = 262144;      // object slots of the device pool
@@ -99,6 +102,7 @@ struct phx_batch {
     std::vector<int> tileSteps;  // per tile: steps until its last object ends (upper bound)
     int stepsDone = 0;
     phx::DBuf<double> y, cur, end, nextDt, ams, bcd, records, slotT0;
+    phx::DBuf<double> ks, ysBuf, parts, spec;   // split stages (allocated on first use)
     phx::DBuf<uint8_t> status;
     phx::DBuf<long long> tileRecord, counters, slotRecord;
     phx::DBuf<int> slotRows;

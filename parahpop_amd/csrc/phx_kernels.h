@@ -43,8 +43,24 @@ int uploadTideFactors();
 // globe7 tables, computed from the host copy of the coefficient table.
 int uploadGlobeTrig(const double* nrlCoeffHost);
 
-// RKF78 step kernel: `waves` waves per 64-object tile (1, 2, 4 or 8).
+// RKF78 step kernel: `waves` waves per 64-object tile (1, 2 or 4 with drag;
+// also 8 or 16 without).
 int launchStep(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t stream);
+
+// Split stages (shared-epoch batches): k-stages in HBM, one drag-free launch
+// (`shWaves` = 4, 8 or 16 waves per tile) and one drag launch per stage, and a
+// combine launch per step.
+struct SplitArgs {
+    double* ks;      // [13][6][nPad] k-stages
+    double* ys;      // [7][nPad] stage position, velocity, epoch
+    double* parts;   // [6][nPad] point-mass + SH part and corrections part of the stage acceleration
+    double* spec;    // [8][nPad] species densities (drag spread over several workgroups per tile)
+    int stage;       // 0 .. 12
+    int step;        // absolute step index (epoch record of (step, stage))
+    int speciesPerGroup;  // 0: one four-wave drag workgroup per tile; 1, 2, 4: species per workgroup
+};
+int launchSplitStage(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, int shWaves, hipStream_t stream);
+int launchSplitCombine(const StepArgs& a, const SplitArgs& sp, hipStream_t stream);
 
 // Diagnostics used by the unit tests: one force evaluation per object at the
 // given states and epoch record (acc out [3][n]); `which` selects a force family

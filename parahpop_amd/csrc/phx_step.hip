@@ -738,6 +738,64 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
 // DRAG = false: the kernel for force models without NRLMSISE-00 drag, which
 // carries none of the drag code and so keeps a small register budget (more
 // waves per SIMD, up to 16 waves per tile).
+// Drag-free stage forces of the split path: acc = ax + (extra + 0) as in
+// stageForces, and the parts ax (point masses + SH) for the drag kernel; the
+// corrections wave leaves its sum (SRP, tides, relativity) in extra[1].
+template <int W, class Rec>
+__device__ inline void stageForcesParts(const StepArgs& a, const ForceModel& fm, const Rec& rec, int wave, int lane, int m0,
+                                        int m1, double ams, double (*xch)[kTile], double (*part)[3][kTile],
+                                        double (*extra)[3][kTile], double acc[3], double ax[3]) {
+    const double px = xch[0][lane], py = xch[1][lane], pz = xch[2][lane];
+    ShGeom g;
+    if (fm.hasSh) {
+        double sr = 0.0, sp = 0.0, sl = 0.0;
+        g = shGeometry(rec, px, py, pz, fm.shRadius);
+        if (m0 < m1)
+            shOrderBlock(g, m0, m1, fm.shDegree, a.shTerms, a.shTerms32, a.fp64Degree, a.shOrderStart, a.shDiag, fm.shNormalized, sr,
+                         sp, sl);
+        part[wave][0][lane] = sr;
+        part[wave][1][lane] = sp;
+        part[wave][2][lane] = sl;
+    }
+    if (wave == corrWave(W)) {
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        if (fm.tides || fm.relativity) correctionsAccel(fm, rec, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], cx, cy, cz);
+        if (fm.srp) {
+            double bx, by, bz;
+            srpAccel(fm, rec, px, py, pz, ams, bx, by, bz);
+            cx += bx;
+            cy += by;
+            cz += bz;
+        }
+        extra[1][0][lane] = cx;
+        extra[1][1][lane] = cy;
+        extra[1][2][lane] = cz;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        double hx = 0.0, hy = 0.0, hz = 0.0, gx, gy, gz;
+        pointGravity(fm, rec, px, py, pz, gx, gy, gz);
+        if (fm.hasSh) {
+            double r = 0.0, p = 0.0, l = 0.0;
+            for (int w = 0; w < W; ++w) {
+                r += part[w][0][lane];
+                p += part[w][1][lane];
+                l += part[w][2][lane];
+            }
+            shToInertial(rec, g, r, p, l, fm.shGm, hx, hy, hz);
+            gx += hx;
+            gy += hy;
+            gz += hz;
+        }
+        ax[0] = gx;
+        ax[1] = gy;
+        ax[2] = gz;
+        acc[0] = gx + (extra[1][0][lane] + 0.0);
+        acc[1] = gy + (extra[1][1][lane] + 0.0);
+        acc[2] = gz + (extra[1][2][lane] + 0.0);
+    }
+}
+
 template <int W, bool GRID, bool DRAG>
 __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm) {
     __shared__ double kst[kStages][6][kTile];
@@ -882,6 +940,206 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split stages (shared-epoch batches with drag). The fused kernel above holds
+// the NRLMSISE-00 path, which needs ~500 registers per lane, so every tile runs
+// one wave per SIMD and the SH stream's scalar loads go unhidden. Here each
+// stage is two launches: a drag-free kernel (SH, point masses, SRP, tides,
+// relativity; two waves per SIMD at eight waves per tile) and a drag kernel for
+// the tiles with lanes below the cutoff, with the k-stages in HBM; a third
+// kernel completes the step. The arithmetic and its order are those of the
+// fused kernel, so both give the same bits.
+namespace {
+
+__device__ inline size_t ksAt(int s, int d, int nPad, int obj) { return (size_t(s) * 6 + d) * size_t(nPad) + obj; }
+
+template <int W>
+__global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
+    __shared__ double xch[7][kTile];
+    __shared__ double part[W][3][kTile];
+    __shared__ double extra[2][3][kTile];
+    __shared__ double spec[1][kTile];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tile = int(blockIdx.x);
+    const int obj = tile * kTile + lane;
+    const int nPad = a.nPad;
+    const int s = sp.stage;
+    if (wave == 0) {
+        // stage state y_s = y + sum_j (dt a_sj) k_j, t_s = t + dt c_s (as the fused kernel)
+        const double dt = a.nextDt[obj];
+        double ys[6];
+        for (int d = 0; d < 6; ++d) ys[d] = a.y[d * nPad + obj];
+        if (s >= 1) {
+            for (int j = 0; j < s; ++j) {
+                const double aij = kRkA[s - 1][j];
+                if (aij != 0.0) {
+                    const double coef = dt * aij;
+                    for (int d = 0; d < 6; ++d) ys[d] += coef * sp.ks[ksAt(j, d, nPad, obj)];
+                }
+            }
+        }
+        double ts = a.cur[obj];
+        if (s >= 1) ts += dt * kRkC[s - 1];
+        for (int d = 0; d < 6; ++d) {
+            xch[d][lane] = ys[d];
+            sp.ys[size_t(d) * nPad + obj] = ys[d];
+        }
+        xch[6][lane] = ts;
+        sp.ys[size_t(6) * nPad + obj] = ts;
+        for (int d = 0; d < 3; ++d) sp.ks[ksAt(s, d, nPad, obj)] = ys[3 + d];
+    }
+    __syncthreads();
+    const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
+    const ExactRec rec{recp};
+    const NrlSlot noSlot{nullptr, 0.0, 0.0};
+    const int m0 = a.blockStart[wave], m1 = a.blockStart[wave + 1];
+    double acc[3], ax[3];
+    stageForcesParts<W>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, acc, ax);
+    (void)noSlot;
+    (void)spec;
+    if (wave == 0) {
+        for (int d = 0; d < 3; ++d) {
+            sp.ks[ksAt(s, 3 + d, nPad, obj)] = acc[d];
+            sp.parts[size_t(d) * nPad + obj] = ax[d];
+            sp.parts[size_t(3 + d) * nPad + obj] = extra[1][d][lane];
+        }
+    }
+}
+
+// Drag of one stage for the tiles with lanes below the cutoff: the species split
+// over the tile's four waves, then k_s = ax + (extra + drag) as the fused kernel.
+__global__ void __launch_bounds__(256) splitDragKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
+    constexpr int W = 4;
+    __shared__ double spec[8][kTile];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tile = int(blockIdx.x);
+    const int obj = tile * kTile + lane;
+    const int nPad = a.nPad;
+    const int s = sp.stage;
+    const double px = sp.ys[obj], py = sp.ys[size_t(nPad) + obj], pz = sp.ys[size_t(2) * nPad + obj];
+    const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
+    const ExactRec rec{recp};
+    const double* nrl = recp + REC_NRL;
+    const NrlLane nl = nrlLane(fm, rec, nrl, px, py, pz);
+    // the same ballot in every wave: the whole workgroup leaves together
+    if (__ballot(nl.state != 0) == 0ull) return;
+    const NrlSlot noSlot{nullptr, 0.0, 0.0};
+    if (__ballot(nl.state == 1) != 0ull) nrlSpecies<1>(nl, nrl, noSlot, a.nrlCoeff, wave, W, spec, lane);
+    __syncthreads();
+    if (wave == 0 && nl.state != 0) {
+        double rho;
+        if (nl.state == 1) rho = nrlRhoFromSpecies(fm, spec, lane);
+        else if (nl.state == 2) rho = nrlDensity<1>(fm, nrl, a.nrlCoeff, nl.in);
+        else rho = nan("");
+        const double vx = sp.ys[size_t(3) * nPad + obj], vy = sp.ys[size_t(4) * nPad + obj], vz = sp.ys[size_t(5) * nPad + obj];
+        const double om0 = rec(REC_OMEGA), om1 = rec(REC_OMEGA + 1), om2 = rec(REC_OMEGA + 2);
+        const double rx = vx - (om1 * pz - om2 * py);
+        const double ry = vy - (om2 * px - om0 * pz);
+        const double rz = vz - (om0 * py - om1 * px);
+        const double f = -0.5 * rho * a.bcd[obj] * sqrt(rx * rx + ry * ry + rz * rz);
+        const double dd[3] = {f * rx, f * ry, f * rz};
+        for (int d = 0; d < 3; ++d)
+            sp.ks[ksAt(s, 3 + d, nPad, obj)] = sp.parts[size_t(d) * nPad + obj] + (sp.parts[size_t(3 + d) * nPad + obj] + dd[d]);
+    }
+}
+
+// Drag spread over more CUs: workgroup (tile, group) of SPW waves evaluates
+// SPW of the eight species for the tile's lanes below the cutoff and stores
+// them in HBM; splitDragFinishKernel assembles the density and the drag.
+template <int SPW>
+__global__ void __launch_bounds__(64 * SPW) splitSpeciesKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
+    constexpr int G = 8 / SPW;
+    __shared__ double spec[8][kTile];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tile = int(blockIdx.x) / G;
+    const int group = int(blockIdx.x) % G;
+    const int obj = tile * kTile + lane;
+    const int nPad = a.nPad;
+    const int s = sp.stage;
+    const double px = sp.ys[obj], py = sp.ys[size_t(nPad) + obj], pz = sp.ys[size_t(2) * nPad + obj];
+    const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
+    const ExactRec rec{recp};
+    const double* nrl = recp + REC_NRL;
+    const NrlLane nl = nrlLane(fm, rec, nrl, px, py, pz);
+    if (__ballot(nl.state == 1) == 0ull) return;
+    const NrlSlot noSlot{nullptr, 0.0, 0.0};
+    const int i = group * SPW + wave;   // entry of kSpeciesOrder
+    nrlSpecies<1>(nl, nrl, noSlot, a.nrlCoeff, i, 8, spec, lane);
+    const int k = kSpeciesOrder[i];
+    sp.spec[size_t(k) * nPad + obj] = spec[k][lane];
+}
+
+__global__ void __launch_bounds__(64) splitDragFinishKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
+    __shared__ double spec[8][kTile];
+    const int lane = threadIdx.x & 63;
+    const int tile = int(blockIdx.x);
+    const int obj = tile * kTile + lane;
+    const int nPad = a.nPad;
+    const int s = sp.stage;
+    const double px = sp.ys[obj], py = sp.ys[size_t(nPad) + obj], pz = sp.ys[size_t(2) * nPad + obj];
+    const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
+    const ExactRec rec{recp};
+    const double* nrl = recp + REC_NRL;
+    const NrlLane nl = nrlLane(fm, rec, nrl, px, py, pz);
+    if (nl.state == 0) return;
+    double rho;
+    if (nl.state == 1) {
+        for (int k = 0; k < 8; ++k) spec[k][lane] = sp.spec[size_t(k) * nPad + obj];
+        rho = nrlRhoFromSpecies(fm, spec, lane);
+    } else if (nl.state == 2) {
+        rho = nrlDensity<1>(fm, nrl, a.nrlCoeff, nl.in);
+    } else {
+        rho = nan("");
+    }
+    const double vx = sp.ys[size_t(3) * nPad + obj], vy = sp.ys[size_t(4) * nPad + obj], vz = sp.ys[size_t(5) * nPad + obj];
+    const double om0 = rec(REC_OMEGA), om1 = rec(REC_OMEGA + 1), om2 = rec(REC_OMEGA + 2);
+    const double rx = vx - (om1 * pz - om2 * py);
+    const double ry = vy - (om2 * px - om0 * pz);
+    const double rz = vz - (om0 * py - om1 * px);
+    const double f = -0.5 * rho * a.bcd[obj] * sqrt(rx * rx + ry * ry + rz * rz);
+    const double dd[3] = {f * rx, f * ry, f * rz};
+    for (int d = 0; d < 3; ++d)
+        sp.ks[ksAt(s, 3 + d, nPad, obj)] = sp.parts[size_t(d) * nPad + obj] + (sp.parts[size_t(3 + d) * nPad + obj] + dd[d]);
+}
+
+// End of step: y += sum (dt b_s) k_s and the bookkeeping of FixedStep.h:30-37.
+__global__ void __launch_bounds__(64) splitCombineKernel(StepArgs a, SplitArgs sp) {
+    const int lane = threadIdx.x & 63;
+    const int obj = int(blockIdx.x) * kTile + lane;
+    const int nPad = a.nPad;
+    uint8_t st = a.status[obj];
+    unsigned long long completed = 0;
+    if ((st & (ST_TERMINATED | ST_PAD)) == 0) {
+        double dt = a.nextDt[obj], cur = a.cur[obj];
+        double y[6];
+        for (int d = 0; d < 6; ++d) y[d] = a.y[d * nPad + obj];
+        for (int s = 0; s < kStages; ++s) {
+            const double bs = kRkB[s];
+            if (bs != 0.0) {
+                const double cb = dt * bs;
+                for (int d = 0; d < 6; ++d) y[d] += cb * sp.ks[ksAt(s, d, nPad, obj)];
+            }
+        }
+        cur = cur + dt;
+        const double remaining = a.end[obj] - cur;
+        const bool finished = remaining <= 0.0;
+        st = finished ? (ST_TERMINATED | ST_EOS) : 0;
+        dt = finished ? 0.0 : fmin(a.step, remaining);
+        for (int d = 0; d < 6; ++d) a.y[d * nPad + obj] = y[d];
+        a.status[obj] = st;
+        a.nextDt[obj] = dt;
+        a.cur[obj] = cur;
+        completed = 1;
+    }
+    for (int off = 32; off > 0; off >>= 1) completed += __shfl_xor(completed, off);
+    if (lane == 0 && a.counters && completed) atomicAdd((unsigned long long*)a.counters, completed);
+}
+
+}  // namespace
+
 __global__ void __launch_bounds__(64) forceProbeKernel(int n, const double* pos, const double* vel, const double* ams,
                                                        const double* bcd, const double* recp, int which, ForceModel fm,
                                                        const double* shTerms, const int* shOrderStart, const double* shDiag,
@@ -1013,6 +1271,42 @@ int launchShape(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t 
     return 0;
 }
 }  // namespace
+
+int launchSplitStage(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, int shWaves, hipStream_t stream) {
+    if (a.nTiles <= 0) return 0;
+    const dim3 grid(a.nTiles);
+    switch (shWaves) {
+        case 4: hipLaunchKernelGGL((splitShKernel<4>), grid, dim3(256), 0, stream, a, fm, sp); break;
+        case 8: hipLaunchKernelGGL((splitShKernel<8>), grid, dim3(512), 0, stream, a, fm, sp); break;
+        case 16: hipLaunchKernelGGL((splitShKernel<16>), grid, dim3(1024), 0, stream, a, fm, sp); break;
+        default: return -2;
+    }
+    if (fm.drag) {
+        switch (sp.speciesPerGroup) {
+            case 0: hipLaunchKernelGGL(splitDragKernel, grid, dim3(256), 0, stream, a, fm, sp); break;
+            case 1:
+                hipLaunchKernelGGL((splitSpeciesKernel<1>), dim3(a.nTiles * 8), dim3(64), 0, stream, a, fm, sp);
+                hipLaunchKernelGGL(splitDragFinishKernel, grid, dim3(64), 0, stream, a, fm, sp);
+                break;
+            case 2:
+                hipLaunchKernelGGL((splitSpeciesKernel<2>), dim3(a.nTiles * 4), dim3(128), 0, stream, a, fm, sp);
+                hipLaunchKernelGGL(splitDragFinishKernel, grid, dim3(64), 0, stream, a, fm, sp);
+                break;
+            case 4:
+                hipLaunchKernelGGL((splitSpeciesKernel<4>), dim3(a.nTiles * 2), dim3(256), 0, stream, a, fm, sp);
+                hipLaunchKernelGGL(splitDragFinishKernel, grid, dim3(64), 0, stream, a, fm, sp);
+                break;
+            default: return -2;
+        }
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launchSplitCombine(const StepArgs& a, const SplitArgs& sp, hipStream_t stream) {
+    if (a.nTiles <= 0) return 0;
+    hipLaunchKernelGGL(splitCombineKernel, dim3(a.nTiles), dim3(64), 0, stream, a, sp);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
 
 int launchStep(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t stream) {
     if (a.nTiles <= 0) return 0;

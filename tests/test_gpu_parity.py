@@ -210,3 +210,21 @@ def test_uneven_epochs_and_durations_match_oracle(workdir):
         np.testing.assert_array_equal(e_g, e_o)
         _compare(s_g, s_o)
     del dur
+
+
+def test_split_stages_match_fused_kernel(full_workload):
+    """Split stages (k-stages in HBM, drag in its own launch) against the fused
+    multi-step kernel: the same arithmetic in the same order, so with the same
+    SH order blocks (four waves) the results are identical; with eight and
+    sixteen waves only the SH summation order changes."""
+    cfg, t = full_workload
+    fused = _ctx(cfg, split=0, waves=4).propagate(t, 1800.0)
+    for sw, spg in ((4, 0), (4, 1), (4, 2), (4, 4), (8, 0), (16, 1)):
+        e, s, st = _ctx(cfg, split=1, split_waves=sw, split_species=spg).propagate(t, 1800.0)
+        assert st["steps_per_launch"] == 1
+        np.testing.assert_array_equal(e, fused[0])
+        if sw == 4:
+            np.testing.assert_array_equal(s, fused[1])
+        else:
+            rel = np.max(np.linalg.norm(s[:3] - fused[1][:3], axis=0) / np.linalg.norm(fused[1][:3], axis=0))
+            assert rel < 1e-12
