@@ -77,7 +77,8 @@ def run_steps(b, total, chunk):
         st = b.run(k)
         for key in acc:
             acc[key] += st[key]
-        acc["waves"], acc["steps_per_launch"] = st["waves"], st["steps_per_launch"]
+        for key in ("waves", "steps_per_launch", "kernels_per_step"):
+            acc[key] = st[key]
         done += k
     return acc
 
@@ -91,6 +92,8 @@ def leg_main(args, work):
     ctx = parahpop_amd.Context(cfg)
     if args.waves:
         ctx.set_option("waves", args.waves)
+    if args.split >= 0:
+        ctx.set_option("split", args.split)
     spl = args.steps_per_launch or int(ctx.option("steps_per_launch"))
     ctx.set_option("steps_per_launch", spl)
     b = ctx.batch(t, duration)
@@ -121,18 +124,30 @@ def leg_main(args, work):
         "data": "synthetic (seeded population, synthetic 70x70 field, constant space weather/EOP, analytic ephemeris)",
         "config": {"workload": "full-force 70x70, 16384 LEO/MEO objects, 60 s step, 1 day",
                    "objects": args.objects, "degree": args.degree, "waves": st["waves"],
-                   "steps_per_launch": st["steps_per_launch"]},
+                   "steps_per_launch": st["steps_per_launch"],
+                   "path": "split-stage" if st["kernels_per_step"] else "fused multi-step"},
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel_ms_avg": 1e3 * kern / max(st["launches"], 1), "launches": st["launches"],
-                     "flops_per_object_step": fps},
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "flops_per_object_step": fps},
     }
-    traffic = os.path.join(ROOT, "profiles", "rkf78_full_force_70x70_traffic.json")
+    kps = st["kernels_per_step"]
+    if kps:
+        # split-stage path: the unit of work the roofline is priced on is one step
+        # (13 stages of SH, species and finish kernels, then the combine kernel)
+        sw, ss = int(ctx.option("split_waves")), int(ctx.option("split_species"))
+        kern_desc = ("step of %d kernels: 13 x (splitShKernel<%d>, %s) + splitCombineKernel"
+                     % (kps, sw, ("splitSpeciesKernel<%d>, splitDragFinishKernel" % ss) if ss else "splitDragKernel"))
+        per, shape = steps, {"path": "split", "split_waves": sw, "split_species": ss}
+    else:
+        kern_desc = "launch of rkf78Kernel<%d> (%d steps)" % (st["waves"], st["steps_per_launch"])
+        per, shape = st["launches"], {"path": "fused", "steps_per_launch": st["steps_per_launch"]}
+    line["roofline"].update({"kernel": kern_desc, "kernel_ms_avg": 1e3 * kern / max(per, 1), "launches": per,
+                             "object_steps_per_launch": obj_steps / max(per, 1)})
+    traffic = os.path.join(ROOT, "profiles", "step_full_force_70x70_traffic.json")
     if os.path.exists(traffic):
-        # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE counter runs (scaled by a
-        # device copy of known size), valid for the same launch shape
+        # HBM bytes per launch unit from the FETCH_SIZE / WRITE_SIZE counter runs (scaled
+        # by a device copy of known size), valid for the same path and launch shape
         tr = json.load(open(traffic))
-        if tr["steps_per_launch"] == st["steps_per_launch"] and args.objects == 16384 and args.degree == 70:
+        if all(tr.get(k) == v for k, v in shape.items()) and args.objects == 16384 and args.degree == 70:
             line["roofline"]["traffic"] = tr["hbm_bytes_per_launch"]
             line["roofline"]["traffic_source"] = os.path.relpath(traffic, ROOT)
     if not args.no_cpu:
@@ -213,6 +228,7 @@ def leg_uneven(args, work):
             "expected_object_steps": float(np.ceil(dur / 60.0).sum()), "wall_seconds": wall,
             "kernel_seconds": st["kernel_seconds"], "launches": st["launches"], "epoch_records": st["epoch_records"],
             "tuned": {"waves": int(ctx.option("waves")), "share0": ctx.option("share0"),
+                      "path": "split-stage" if st["kernels_per_step"] else "fused multi-step",
                       "trial_ms_per_step": 1e3 * tune["kernel_seconds"] / 4},
             "max_end_epoch_error_s": end_err}
 
@@ -288,6 +304,7 @@ def main():
     ap.add_argument("--degree", type=int, default=70)
     ap.add_argument("--waves", type=int, default=0, help="waves per 64-object tile (0: library default)")
     ap.add_argument("--steps-per-launch", type=int, default=0)
+    ap.add_argument("--split", type=int, default=-1, help="1: split-stage path, 0: fused multi-step kernel (-1: library default)")
     ap.add_argument("--legs", default="main,mixed,uneven,serve")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-objects", type=int, default=1024)

@@ -34,9 +34,10 @@ typedef struct phx_stats {
     double epoch_seconds;      /* device time of the epoch-record kernel */
     double wall_seconds;       /* host wall time of the call */
     double epoch_records;      /* distinct stage epochs evaluated */
-    double launches;           /* step-kernel launches */
+    double launches;           /* kernel launches of the step path */
     int waves;                 /* waves per 64-object tile used */
-    int steps_per_launch;
+    int steps_per_launch;      /* steps per launch of the fused step kernel; 1 on the split-stage path */
+    int kernels_per_step;      /* kernels per step on the split-stage path; 0 on the fused path */
 } phx_stats;
 
 /* Creates a context from a ParaHPOP CONFIG.json: parses and validates the
@@ -136,7 +137,7 @@ void phx_destroy(phx_ctx* ctx);
 int phx_force_probe(phx_ctx* ctx, size_t n, const double* pos, const double* vel, const double* mass,
                     const double* area_km2, const double* cr_eff, const double* cd, double et, int which,
                     double* acc);
-/* Epoch record (REC_SIZE doubles) for one ET epoch. */
+/* Epoch record (352 doubles, record_len >= 352) for one ET epoch. */
 int phx_epoch_record(phx_ctx* ctx, double et, double* record, size_t record_len);
 /* NRLMSISE-00 gtd7d on the device for test-driver style inputs (rows of 16:
  * doy, sec, alt, lat, lon, lst, f107a, f107, ap, ap[7]); out rows of 11. */

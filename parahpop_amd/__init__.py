@@ -13,7 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libphx.so")
-REC_SIZE = 320
+REC_SIZE = 352
 
 PHX_OK, PHX_EINVAL, PHX_EDEVICE, PHX_ENOTDONE, PHX_ENOMEM, PHX_EAGAIN = 0, -1, -2, -3, -4, -5
 
@@ -34,7 +34,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("object_steps", ctypes.c_double), ("kernel_seconds", ctypes.c_double),
                 ("epoch_seconds", ctypes.c_double), ("wall_seconds", ctypes.c_double),
                 ("epoch_records", ctypes.c_double), ("launches", ctypes.c_double),
-                ("waves", ctypes.c_int), ("steps_per_launch", ctypes.c_int)]
+                ("waves", ctypes.c_int), ("steps_per_launch", ctypes.c_int),
+                ("kernels_per_step", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -581,6 +581,7 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
     check(hipMemcpyAsync(&before, b.counters.p, sizeof before, hipMemcpyDeviceToHost, c.stream), "counter read");
     check(hipEventRecord(c.ev0, c.stream), "hipEventRecord");
     int launches = 0;
+    int kernelsPerStep = 0;
     int blocksFor = W;   // wave count the order blocks in `a` were split for
     const bool split = c.split && !b.grid && c.fm.drag && c.fm.hasSh;
     if (split) {
@@ -594,6 +595,8 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
         const int Ws = c.splitWaves;
         orderBlocks(c, Ws, a.blockStart);
         SplitArgs sp{b.ks.p, b.ysBuf.p, b.parts.p, b.spec.p, 0, 0, c.splitSpecies};
+        // per stage: SH kernel, then either the drag kernel or species + finish kernels
+        kernelsPerStep = kStages * (c.splitSpecies ? 3 : 2) + 1;
         while (remaining > 0) {
             int active = b.nTiles;
             while (active > 0 && b.tileSteps[size_t(active - 1)] <= b.stepsDone) --active;
@@ -606,10 +609,9 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
             for (int s = 0; s < kStages; ++s) {
                 sp.stage = s;
                 if (launchSplitStage(a, c.fm, sp, Ws, c.stream) != 0) throw HipError("split stage launch failed");
-                launches += 2;
             }
             if (launchSplitCombine(a, sp, c.stream) != 0) throw HipError("split combine launch failed");
-            ++launches;
+            launches += kernelsPerStep;
             ++b.stepsDone;
             --remaining;
         }
@@ -655,6 +657,7 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
         stats->launches = launches;
         stats->waves = W;
         stats->steps_per_launch = split ? 1 : c.stepsPerLaunch;
+        stats->kernels_per_step = kernelsPerStep;
         stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
     }
     return 0;
@@ -857,6 +860,9 @@ int phx_batch_autotune(phx_batch* b, int trialSteps, phx_stats* best) {
         struct Cand {
             int waves;
             double share0;
+            int split;    // split-stage path (shared-epoch batches with drag)
+            int splitWaves;
+            int splitSpecies;
         };
         std::vector<Cand> cands;
         if (c.fm.hasSh) {
@@ -864,19 +870,27 @@ int phx_batch_autotune(phx_batch* b, int trialSteps, phx_stats* best) {
                 for (double s0 : {0.7, 0.85, 1.0}) {
                     if (w == 1 && s0 != 1.0) continue;
                     if (c.fm.drag && w > 4) continue;
-                    cands.push_back({w, s0});
+                    cands.push_back({w, s0, 0, c.splitWaves, c.splitSpecies});
                 }
+            if (c.fm.drag && !b->grid)
+                for (int sw : {8, 16})
+                    for (int ss : {1, 2}) cands.push_back({c.nWaves, c.share0, 1, sw, ss});
         } else {
-            cands.push_back({1, 1.0});
+            cands.push_back({1, 1.0, 0, c.splitWaves, c.splitSpecies});
         }
-        const int keepWaves = c.nWaves;
-        const double keepShare = c.share0;
+        const Cand keep{c.nWaves, c.share0, c.split ? 1 : 0, c.splitWaves, c.splitSpecies};
+        auto apply = [&c](const Cand& k) {
+            c.nWaves = k.waves;
+            c.share0 = k.share0;
+            c.split = k.split != 0;
+            c.splitWaves = k.splitWaves;
+            c.splitSpecies = k.splitSpecies;
+        };
         double bestT = 0.0;
         int bestI = -1;
         phx_stats st{}, bestSt{};
         for (size_t i = 0; i < cands.size(); ++i) {
-            c.nWaves = cands[i].waves;
-            c.share0 = cands[i].share0;
+            apply(cands[i]);
             int rc = phx_batch_reset(b);
             if (rc) throw std::runtime_error(c.error);
             runBatch(c, *b, trialSteps, &st);
@@ -888,13 +902,7 @@ int phx_batch_autotune(phx_batch* b, int trialSteps, phx_stats* best) {
                 bestSt = st;
             }
         }
-        if (bestI < 0) {
-            c.nWaves = keepWaves;
-            c.share0 = keepShare;
-        } else {
-            c.nWaves = cands[size_t(bestI)].waves;
-            c.share0 = cands[size_t(bestI)].share0;
-        }
+        apply(bestI < 0 ? keep : cands[size_t(bestI)]);
         const int rc = phx_batch_reset(b);
         if (best) *best = bestSt;
         return rc;

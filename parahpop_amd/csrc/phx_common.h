@@ -8,7 +8,7 @@ namespace phx {
 constexpr int kStages = 13;
 constexpr int kTile = 64;          // objects per workgroup tile = one wave64
 constexpr int kMaxBodies = 11;     // Earth, Sun, Moon, 8 planetary system barycenters
-constexpr int kNrlSlotSize = 280;  // doubles per NRLMSISE-00 weather-slot row
+constexpr int kNrlSlotSize = 312;  // doubles per NRLMSISE-00 weather-slot row
 constexpr int kMaxWaves = 16;      // waves per tile that share one object's work
 
 // Per-epoch record: everything the force model needs that depends only on the
@@ -29,9 +29,10 @@ enum EpochRecord : int {
     REC_NRL_F107A = REC_NRL + 4,
     REC_NRL_AP = REC_NRL + 5,     // 7 entries (daily Ap, then the 3-hour history)
     REC_NRL_VALID = REC_NRL + 12,
-    REC_NRL_GLOBE = REC_NRL + 13, // epoch-only globe7 terms of the 11 tables (nrlmsise::GE_COUNT)
-    REC_USED = REC_NRL_GLOBE + 199,
-    REC_SIZE = 320
+    REC_NRL_GLOBE = REC_NRL + 13, // epoch-only globe7 terms of the 11 tables (nrlmsise::GE_COUNT = 199),
+                                  // then the glob7s terms of the 5 lower-thermosphere tables (nrlmsise::GL_COUNT = 30)
+    REC_USED = REC_NRL_GLOBE + 199 + 30,
+    REC_SIZE = 352
 };
 static_assert(REC_USED <= REC_SIZE, "epoch record overflow");
 

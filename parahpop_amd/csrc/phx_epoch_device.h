@@ -461,7 +461,8 @@ __device__ inline void weatherInputs(const EpochArgs& a, int record, int slot, d
 }
 
 // Writes the NRLMSISE-00 epoch inputs at dst[REC_NRL_* - REC_NRL] and, when
-// globe is given, the epoch-only globe7 terms (nrlmsise::GE_COUNT values).
+// globe is given, the epoch-only globe7 terms (nrlmsise::GE_COUNT values)
+// followed by the lower-thermosphere glob7s terms (nrlmsise::GL_COUNT values).
 __device__ void nrlEpoch(const EpochArgs& a, double et, double* dst, double* globe) {
     double* rec = dst - REC_NRL;
     const double mjd = etToMjdUtcNrl(et);
@@ -482,6 +483,7 @@ __device__ void nrlEpoch(const EpochArgs& a, double et, double* dst, double* glo
     if (globe && a.nrlCoeff) {
         nrlmsise::NrlTables T{a.nrlCoeff};
         nrlmsise::globeEpoch(T, doy, rec[REC_NRL_SEC], rec[REC_NRL_F107], rec[REC_NRL_F107A], rec + REC_NRL_AP, globe);
+        nrlmsise::lowerEpoch(T, doy, globe + nrlmsise::GE_COUNT);
     }
 }
 

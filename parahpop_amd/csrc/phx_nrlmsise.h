@@ -845,11 +845,11 @@ __device__ inline void gts7Prefix(State& st, const NrlTables& T, Input& in, cons
     const double tlb = ((NrlCD)(T.base+OFF_PTM))[1] * (1.0 + fl.sw[17] * G(st, 5, (T.base+OFF_PD+150*(3)), in, fl)) * ((NrlCD)(T.base+OFF_PD+150*(3)))[0];
     const double s = g0v / (tinf - tlb);
     if (in.alt < 300.0) {
-        st.meso_tn1[1] = ((NrlCD)(T.base+OFF_PTM))[6] * ((NrlCD)(T.base+OFF_PTL+100*(0)))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(0)), in, fl));
-        st.meso_tn1[2] = ((NrlCD)(T.base+OFF_PTM))[2] * ((NrlCD)(T.base+OFF_PTL+100*(1)))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(1)), in, fl));
-        st.meso_tn1[3] = ((NrlCD)(T.base+OFF_PTM))[7] * ((NrlCD)(T.base+OFF_PTL+100*(2)))[0] / (1.0 - fl.sw[18] * glob7s(st, (T.base+OFF_PTL+100*(2)), in, fl));
-        st.meso_tn1[4] = ((NrlCD)(T.base+OFF_PTM))[4] * ((NrlCD)(T.base+OFF_PTL+100*(3)))[0] / (1.0 - fl.sw[18] * fl.sw[20] * glob7s(st, (T.base+OFF_PTL+100*(3)), in, fl));
-        st.meso_tgn1[1] = ((NrlCD)(T.base+OFF_PTM))[8] * ((NrlCD)(T.base+OFF_PMA+100*(8)))[0] * (1.0 + fl.sw[18] * fl.sw[20] * glob7s(st, (T.base+OFF_PMA+100*(8)), in, fl)) * st.meso_tn1[4] *
+        st.meso_tn1[1] = ((NrlCD)(T.base+OFF_PTM))[6] * ((NrlCD)(T.base+OFF_PTL+100*(0)))[0] / (1.0 - fl.sw[18] * G.lower(st, 0, (T.base+OFF_PTL+100*(0)), in, fl));
+        st.meso_tn1[2] = ((NrlCD)(T.base+OFF_PTM))[2] * ((NrlCD)(T.base+OFF_PTL+100*(1)))[0] / (1.0 - fl.sw[18] * G.lower(st, 1, (T.base+OFF_PTL+100*(1)), in, fl));
+        st.meso_tn1[3] = ((NrlCD)(T.base+OFF_PTM))[7] * ((NrlCD)(T.base+OFF_PTL+100*(2)))[0] / (1.0 - fl.sw[18] * G.lower(st, 2, (T.base+OFF_PTL+100*(2)), in, fl));
+        st.meso_tn1[4] = ((NrlCD)(T.base+OFF_PTM))[4] * ((NrlCD)(T.base+OFF_PTL+100*(3)))[0] / (1.0 - fl.sw[18] * fl.sw[20] * G.lower(st, 3, (T.base+OFF_PTL+100*(3)), in, fl));
+        st.meso_tgn1[1] = ((NrlCD)(T.base+OFF_PTM))[8] * ((NrlCD)(T.base+OFF_PMA+100*(8)))[0] * (1.0 + fl.sw[18] * fl.sw[20] * G.lower(st, 4, (T.base+OFF_PMA+100*(8)), in, fl)) * st.meso_tn1[4] *
                           st.meso_tn1[4] / (pow((((NrlCD)(T.base+OFF_PTM))[4] * ((NrlCD)(T.base+OFF_PTL+100*(3)))[0]), 2.0));
     } else {
         st.meso_tn1[1] = ((NrlCD)(T.base+OFF_PTM))[6] * ((NrlCD)(T.base+OFF_PTL+100*(0)))[0];
@@ -1190,6 +1190,10 @@ struct GenericGlobe {
     __device__ double operator()(State& st, int, const double* p, const Input& in, const Flags& fl) const {
         return globe7(st, p, in, fl);
     }
+    // glob7s of lower-thermosphere table j (lowerTable order)
+    __device__ double lower(State& st, int, const double* p, const Input& in, const Flags& fl) const {
+        return glob7s(st, p, in, fl);
+    }
 };
 
 // Epoch-only parts of globe7 for one coefficient table with the propagator's
@@ -1242,6 +1246,75 @@ __device__ inline void globeEpoch(const NrlTables& T, int doy, double sec, doubl
                          sumex(exp1);
         }
     }
+}
+
+// Lower-thermosphere node tables of gts7 (glob7s, Nrlmsise00Model.inl:858-950):
+// ptl[0..3] and pma[8]. Their day-of-year cosines and the day-of-year factors of
+// the longitudinal term are epoch-only: per table cd14, cd18, cd32, cd39 and
+// LX = p[80] cos(dr (doy - p[81])) + p[85] cos(2 dr (doy - p[86])),
+// LY = p[83] cos(dr (doy - p[84])) + p[87] cos(2 dr (doy - p[88]))
+// with the propagator's switch setting (swc = 1).
+enum : int { GL_CD14 = 0, GL_CD18, GL_CD32, GL_CD39, GL_LX, GL_LY, GL_STRIDE = 6, GL_TABLES = 5, GL_COUNT = GL_TABLES * GL_STRIDE };
+static_assert(GE_COUNT == 199 && GL_COUNT == 30, "epoch record NRLMSISE-00 block layout");
+
+__device__ inline const double* lowerTable(const NrlTables& T, int j) {
+    return T.base + (j < 4 ? OFF_PTL + 100 * j : OFF_PMA + 100 * 8);
+}
+
+__device__ inline void lowerEpoch(const NrlTables& T, int doy, double* out) {
+    const double dr = 1.72142E-2;
+    for (int j = 0; j < GL_TABLES; ++j) {
+        const double* p = lowerTable(T, j);
+        double* e = out + GL_STRIDE * j;
+        e[GL_CD32] = cos(dr * (doy - p[31]));
+        e[GL_CD18] = cos(2.0 * dr * (doy - p[17]));
+        e[GL_CD14] = cos(dr * (doy - p[13]));
+        e[GL_CD39] = cos(2.0 * dr * (doy - p[38]));
+        e[GL_LX] = p[80] * cos(dr * (doy - p[81])) + p[85] * cos(2.0 * dr * (doy - p[86]));
+        e[GL_LY] = p[83] * cos(dr * (doy - p[84])) + p[87] * cos(2.0 * dr * (doy - p[88]));
+    }
+}
+
+// glob7s of table p with its epoch terms e (GL_STRIDE values) and the lane basis
+// of State, for the propagator's switch setting: the published terms in the
+// published summation order (t[9], t[11], t[12] are zero).
+__device__ __attribute__((always_inline)) inline double glob7sFast(const State& st, NrlCD p, NrlCD e) {
+    const double parameterSet = (p[99] == 0.0) ? 2.0 : p[99];
+    if (parameterSet != 2.0) return -1;
+    const double(&plg)[4][9] = st.plg;
+    const double cd14 = e[GL_CD14], cd18 = e[GL_CD18], cd32 = e[GL_CD32], cd39 = e[GL_CD39];
+    const double t0 = p[21] * st.dfa;
+    const double t1 = p[1] * plg[0][2] + p[2] * plg[0][4] + p[22] * plg[0][6] + p[26] * plg[0][1] + p[14] * plg[0][3] + p[59] * plg[0][5];
+    const double t2 = (p[18] + p[47] * plg[0][2] + p[29] * plg[0][4]) * cd32;
+    const double t3 = (p[15] + p[16] * plg[0][2] + p[30] * plg[0][4]) * cd18;
+    const double t4 = (p[9] * plg[0][1] + p[10] * plg[0][3] + p[20] * plg[0][5]) * cd14;
+    const double t5 = (p[37] * plg[0][1]) * cd39;
+    const double t71 = p[11] * plg[1][2] * cd14;
+    const double t72 = p[12] * plg[1][2] * cd14;
+    const double t6 = ((p[3] * plg[1][1] + p[4] * plg[1][3] + t71) * st.ctloc + (p[6] * plg[1][1] + p[7] * plg[1][3] + t72) * st.stloc);
+    const double t81 = (p[23] * plg[2][3] + p[35] * plg[2][5]) * cd14;
+    const double t82 = (p[33] * plg[2][3] + p[36] * plg[2][5]) * cd14;
+    const double t7 = ((p[5] * plg[2][2] + p[41] * plg[2][4] + t81) * st.c2tloc + (p[8] * plg[2][2] + p[42] * plg[2][4] + t82) * st.s2tloc);
+    const double t13 = p[39] * plg[3][3] * st.s3tloc + p[40] * plg[3][3] * st.c3tloc;
+    const double t8 = (p[50] * st.apt[0] + p[96] * plg[0][2] * st.apt[0]);
+    const double t10 = (1.0 + plg[0][1] * e[GL_LX] + e[GL_LY]) *
+                       ((p[64] * plg[1][2] + p[65] * plg[1][4] + p[66] * plg[1][6] + p[74] * plg[1][1] + p[75] * plg[1][3] + p[76] * plg[1][5]) *
+                            st.cglong +
+                        (p[90] * plg[1][2] + p[91] * plg[1][4] + p[92] * plg[1][6] + p[77] * plg[1][1] + p[78] * plg[1][3] + p[79] * plg[1][5]) *
+                            st.sglong);
+    double tt = 0;
+    tt += t0;
+    tt += t1;
+    tt += t2;
+    tt += t3;
+    tt += t4;
+    tt += t5;
+    tt += t6;
+    tt += t7;
+    tt += t8;
+    tt += t10;
+    tt += t13;
+    return tt;
 }
 
 // Constant phases of a coefficient table's longitude / local-time terms:
@@ -1378,6 +1451,9 @@ struct FastGlobe {
         if (p[51] != 0) st.apt[0] = apt0;
         return v;
     }
+    __device__ __attribute__((always_inline)) double lower(State& st, int j, const double* p, const Input&, const Flags&) const {
+        return glob7sFast(st, (NrlCD)p, (NrlCD)(E + GE_COUNT + GL_STRIDE * j));
+    }
 };
 
 
@@ -1393,15 +1469,17 @@ enum : int {
     SLOT_F107A = SLOT_F107 + 1,
     SLOT_AP = SLOT_F107 + 2,
     SLOT_VALID = SLOT_AP + 7,
+    SLOT_LOWER = SLOT_VALID + 1,   // lowerEpoch terms (GL_COUNT)
     SLOT_SIZE = kNrlSlotSize
 };
-static_assert(SLOT_VALID < SLOT_SIZE, "slot row overflow");
+static_assert(SLOT_LOWER + GL_COUNT <= SLOT_SIZE, "slot row overflow");
 
 __device__ inline void globeSlot(const NrlTables& T, int doy, double f107, double f107A, const double* ap, double* out) {
     const double sr = 7.2722E-5;
     double E[GE_COUNT];
     globeEpoch(T, doy, 0.0, f107, f107A, ap, E);
     out[SLOT_DFA] = E[GE_DFA];
+    lowerEpoch(T, doy, out + SLOT_LOWER);
     for (int j = 0; j < GE_TABLES; ++j) {
         const double* p = globeTable(T, j);
         double* o = out + GS_STRIDE * j;
@@ -1436,6 +1514,9 @@ struct SlotGlobe {
                                     apt0);
         if (p[51] != 0) st.apt[0] = apt0;
         return v;
+    }
+    __device__ __attribute__((always_inline)) double lower(State& st, int j, const double* p, const Input&, const Flags&) const {
+        return glob7sFast(st, (NrlCD)p, (NrlCD)(S + SLOT_LOWER + GL_STRIDE * j));
     }
 };
 

@@ -466,9 +466,11 @@ __device__ __attribute__((always_inline)) inline void nrlSpecies(const NrlLane& 
     }
 }
 
-// Full model for one lane (serial gtd7d), kg/km^3.
+// Full model for one lane (serial gtd7d), kg/km^3. Takes the density scale
+// rather than the ForceModel: a reference to a kernel argument passed to a call
+// makes every caller copy the whole argument to scratch memory.
 template <int MODE>
-__device__ __noinline__ double nrlDensity(const ForceModel& fm, const double* nrl, const double* __restrict__ coeff,
+__device__ __noinline__ double nrlDensity(double densityScale, const double* nrl, const double* __restrict__ coeff,
                                           nrlmsise::Input in) {
     nrlmsise::Flags fl = nrlFlags();
     for (int k = 0; k < 24; ++k) fl.switches[k] = 1;
@@ -479,7 +481,7 @@ __device__ __noinline__ double nrlDensity(const ForceModel& fm, const double* nr
     if (MODE == 1) nrlmsise::gtd7d(st, T, in, fl, out, nrlmsise::FastGlobe{nrl + (REC_NRL_GLOBE - REC_NRL), kGlobeTrig});
     else nrlmsise::gtd7d(st, T, in, fl, out, nrlmsise::GenericGlobe{});
     if (!isfinite(out.d[5]) || out.d[5] < 0.0) return nan("");
-    return (out.d[5] * 1.0e9) * fm.densityScale;
+    return (out.d[5] * 1.0e9) * densityScale;
 }
 
 // Total density (kg/km^3) from the species: gtd7d's sum and units.
@@ -500,7 +502,7 @@ __device__ void dragAccel(const ForceModel& fm, const Rec& rec, const double* nr
                           double& az) {
     const NrlLane L = nrlLane(fm, rec, nrl, px, py, pz);
     double rho = 0.0;
-    if (L.state == 1 || L.state == 2) rho = nrlDensity<MODE>(fm, nrl, coeff, L.in);
+    if (L.state == 1 || L.state == 2) rho = nrlDensity<MODE>(fm.densityScale, nrl, coeff, L.in);
     else if (L.state == 3) rho = nan("");
     const double om0 = rec(REC_OMEGA), om1 = rec(REC_OMEGA + 1), om2 = rec(REC_OMEGA + 2);
     const double rx = vx - (om1 * pz - om2 * py);
@@ -716,7 +718,7 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
         if (DRAG && fm.drag && nl.state != 0) {
             double rho;
             if (nl.state == 1) rho = nrlRhoFromSpecies(fm, spec, lane);
-            else if (nl.state == 2) rho = nrlDensity<MODE>(fm, nrl, a.nrlCoeff, nl.in);
+            else if (nl.state == 2) rho = nrlDensity<MODE>(fm.densityScale, nrl, a.nrlCoeff, nl.in);
             else rho = nan("");
             // Drag.h:67-73 with v_rel = v - omega x r (Earth is the centre)
             const double vx = xch[3][lane], vy = xch[4][lane], vz = xch[5][lane];
@@ -1031,7 +1033,7 @@ __global__ void __launch_bounds__(256) splitDragKernel(StepArgs a, ForceModel fm
     if (wave == 0 && nl.state != 0) {
         double rho;
         if (nl.state == 1) rho = nrlRhoFromSpecies(fm, spec, lane);
-        else if (nl.state == 2) rho = nrlDensity<1>(fm, nrl, a.nrlCoeff, nl.in);
+        else if (nl.state == 2) rho = nrlDensity<1>(fm.densityScale, nrl, a.nrlCoeff, nl.in);
         else rho = nan("");
         const double vx = sp.ys[size_t(3) * nPad + obj], vy = sp.ys[size_t(4) * nPad + obj], vz = sp.ys[size_t(5) * nPad + obj];
         const double om0 = rec(REC_OMEGA), om1 = rec(REC_OMEGA + 1), om2 = rec(REC_OMEGA + 2);
@@ -1090,7 +1092,7 @@ __global__ void __launch_bounds__(64) splitDragFinishKernel(StepArgs a, ForceMod
         for (int k = 0; k < 8; ++k) spec[k][lane] = sp.spec[size_t(k) * nPad + obj];
         rho = nrlRhoFromSpecies(fm, spec, lane);
     } else if (nl.state == 2) {
-        rho = nrlDensity<1>(fm, nrl, a.nrlCoeff, nl.in);
+        rho = nrlDensity<1>(fm.densityScale, nrl, a.nrlCoeff, nl.in);
     } else {
         rho = nan("");
     }
