@@ -955,6 +955,31 @@ namespace {
 
 __device__ inline size_t ksAt(int s, int d, int nPad, int obj) { return (size_t(s) * 6 + d) * size_t(nPad) + obj; }
 
+// Component d of the stage state y_s = y + sum_j (dt a_sj) k_j, in the fused
+// kernel's arithmetic and order (so every kernel that forms it gets the same
+// bits). The k-stage loads do not depend on the coefficients: all are issued
+// at once and the terms of zero coefficients (and of j >= s) are not added.
+__device__ __attribute__((always_inline)) inline double stageComponent(const StepArgs& a, const SplitArgs& sp, int s, int obj,
+                                                                       int d, double dt) {
+    const int nPad = a.nPad;
+    double k[kStages - 1];
+#pragma unroll
+    for (int j = 0; j < kStages - 1; ++j) k[j] = sp.ks[ksAt(j, d, nPad, obj)];
+    double y = a.y[d * nPad + obj];
+    const double* arow = kRkA[s >= 1 ? s - 1 : 0];
+#pragma unroll
+    for (int j = 0; j < kStages - 1; ++j) {
+        const double aij = arow[j];
+        const double coef = dt * aij;
+        const double v = y + coef * k[j];
+        y = (j < s && aij != 0.0) ? v : y;
+    }
+    return y;
+}
+
+// Stage s of the split path for one tile: the stage state, then the forces
+// without drag with the SH order blocks over W waves; k_s holds them until the
+// finish kernel adds the drag.
 template <int W>
 __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
     __shared__ double xch[7][kTile];
@@ -967,29 +992,20 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
-    if (wave == 0) {
-        // stage state y_s = y + sum_j (dt a_sj) k_j, t_s = t + dt c_s (as the fused kernel)
+    // stage state y_s = y + sum_j (dt a_sj) k_j, t_s = t + dt c_s (as the fused
+    // kernel), one component per wave
+    for (int d = wave; d < 7; d += W) {
         const double dt = a.nextDt[obj];
-        double ys[6];
-        for (int d = 0; d < 6; ++d) ys[d] = a.y[d * nPad + obj];
-        if (s >= 1) {
-            for (int j = 0; j < s; ++j) {
-                const double aij = kRkA[s - 1][j];
-                if (aij != 0.0) {
-                    const double coef = dt * aij;
-                    for (int d = 0; d < 6; ++d) ys[d] += coef * sp.ks[ksAt(j, d, nPad, obj)];
-                }
-            }
+        double v;
+        if (d < 6) {
+            v = stageComponent(a, sp, s, obj, d, dt);
+            if (d >= 3) sp.ks[ksAt(s, d - 3, nPad, obj)] = v;
+        } else {
+            v = a.cur[obj];
+            if (s >= 1) v += dt * kRkC[s - 1];
         }
-        double ts = a.cur[obj];
-        if (s >= 1) ts += dt * kRkC[s - 1];
-        for (int d = 0; d < 6; ++d) {
-            xch[d][lane] = ys[d];
-            sp.ys[size_t(d) * nPad + obj] = ys[d];
-        }
-        xch[6][lane] = ts;
-        sp.ys[size_t(6) * nPad + obj] = ts;
-        for (int d = 0; d < 3; ++d) sp.ks[ksAt(s, d, nPad, obj)] = ys[3 + d];
+        xch[d][lane] = v;
+        sp.ys[size_t(d) * nPad + obj] = v;
     }
     __syncthreads();
     const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
@@ -1116,14 +1132,22 @@ __global__ void __launch_bounds__(64) splitCombineKernel(StepArgs a, SplitArgs s
     unsigned long long completed = 0;
     if ((st & (ST_TERMINATED | ST_PAD)) == 0) {
         double dt = a.nextDt[obj], cur = a.cur[obj];
+        // y += sum_s (dt b_s) k_s over the nonzero b_s, in stage order; the
+        // k-stage loads are issued before the sums
         double y[6];
-        for (int d = 0; d < 6; ++d) y[d] = a.y[d * nPad + obj];
-        for (int s = 0; s < kStages; ++s) {
-            const double bs = kRkB[s];
-            if (bs != 0.0) {
+        for (int d = 0; d < 6; ++d) {
+            double k[kStages];
+#pragma unroll
+            for (int s = 0; s < kStages; ++s) k[s] = sp.ks[ksAt(s, d, nPad, obj)];
+            double v = a.y[d * nPad + obj];
+#pragma unroll
+            for (int s = 0; s < kStages; ++s) {
+                const double bs = kRkB[s];
                 const double cb = dt * bs;
-                for (int d = 0; d < 6; ++d) y[d] += cb * sp.ks[ksAt(s, d, nPad, obj)];
+                const double w = v + cb * k[s];
+                v = bs != 0.0 ? w : v;
             }
+            y[d] = v;
         }
         cur = cur + dt;
         const double remaining = a.end[obj] - cur;
@@ -1274,7 +1298,21 @@ int launchShape(const StepArgs& a, const ForceModel& fm, int waves, hipStream_t 
 }
 }  // namespace
 
+static int launchSplitSh(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, int shWaves, hipStream_t stream);
+static int launchSplitSpecies(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, hipStream_t stream);
+static int launchSplitFinish(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, hipStream_t stream);
+
+// One stage of the split path: the SH kernel, then the species kernel (one
+// workgroup per tile and species group) and the finish kernel, or the
+// four-wave drag kernel.
 int launchSplitStage(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, int shWaves, hipStream_t stream) {
+    int rc = launchSplitSh(a, fm, sp, shWaves, stream);
+    if (rc == 0 && sp.speciesPerGroup != 0) rc = launchSplitSpecies(a, fm, sp, stream);
+    if (rc == 0) rc = launchSplitFinish(a, fm, sp, stream);
+    return rc;
+}
+
+static int launchSplitSh(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, int shWaves, hipStream_t stream) {
     if (a.nTiles <= 0) return 0;
     const dim3 grid(a.nTiles);
     switch (shWaves) {
@@ -1283,24 +1321,24 @@ int launchSplitStage(const StepArgs& a, const ForceModel& fm, const SplitArgs& s
         case 16: hipLaunchKernelGGL((splitShKernel<16>), grid, dim3(1024), 0, stream, a, fm, sp); break;
         default: return -2;
     }
-    if (fm.drag) {
-        switch (sp.speciesPerGroup) {
-            case 0: hipLaunchKernelGGL(splitDragKernel, grid, dim3(256), 0, stream, a, fm, sp); break;
-            case 1:
-                hipLaunchKernelGGL((splitSpeciesKernel<1>), dim3(a.nTiles * 8), dim3(64), 0, stream, a, fm, sp);
-                hipLaunchKernelGGL(splitDragFinishKernel, grid, dim3(64), 0, stream, a, fm, sp);
-                break;
-            case 2:
-                hipLaunchKernelGGL((splitSpeciesKernel<2>), dim3(a.nTiles * 4), dim3(128), 0, stream, a, fm, sp);
-                hipLaunchKernelGGL(splitDragFinishKernel, grid, dim3(64), 0, stream, a, fm, sp);
-                break;
-            case 4:
-                hipLaunchKernelGGL((splitSpeciesKernel<4>), dim3(a.nTiles * 2), dim3(256), 0, stream, a, fm, sp);
-                hipLaunchKernelGGL(splitDragFinishKernel, grid, dim3(64), 0, stream, a, fm, sp);
-                break;
-            default: return -2;
-        }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+static int launchSplitSpecies(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, hipStream_t stream) {
+    if (a.nTiles <= 0 || !fm.drag) return 0;
+    switch (sp.speciesPerGroup) {
+        case 1: hipLaunchKernelGGL((splitSpeciesKernel<1>), dim3(a.nTiles * 8), dim3(64), 0, stream, a, fm, sp); break;
+        case 2: hipLaunchKernelGGL((splitSpeciesKernel<2>), dim3(a.nTiles * 4), dim3(128), 0, stream, a, fm, sp); break;
+        case 4: hipLaunchKernelGGL((splitSpeciesKernel<4>), dim3(a.nTiles * 2), dim3(256), 0, stream, a, fm, sp); break;
+        default: return -2;   // the four-wave drag kernel needs the SH kernel's stage state
     }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+static int launchSplitFinish(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, hipStream_t stream) {
+    if (a.nTiles <= 0 || !fm.drag) return 0;
+    if (sp.speciesPerGroup == 0) hipLaunchKernelGGL(splitDragKernel, dim3(a.nTiles), dim3(256), 0, stream, a, fm, sp);
+    else hipLaunchKernelGGL(splitDragFinishKernel, dim3(a.nTiles), dim3(64), 0, stream, a, fm, sp);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
