@@ -248,11 +248,18 @@ EpochArgs epochArgs(const phx_ctx& c) {
     return a;
 }
 
-void orderBlocks(const phx_ctx& c, int W, int out[kMaxWaves + 1]) {
+void orderBlocks(const phx_ctx& c, int W, int out[kMaxWaves + 1], bool split) {
     const int L = c.fm.shDegree, M = c.fm.shOrder;
     std::vector<double> share(static_cast<size_t>(W), 1.0);
     const int corrW = W >= 3 ? W - 2 : W - 1;
-    if (W > 1) {
+    if (split && W >= 8) {
+        // split SH kernel: point gravity on wave W - 1, tides and relativity on
+        // W - 2, SRP on W - 3, the sums after the barrier on wave 0
+        share[0] *= c.splitShare0;
+        share[size_t(W - 1)] *= c.splitSharePg;
+        if (c.fm.tides || c.fm.relativity) share[size_t(W - 2)] *= c.splitShareCorr;
+        if (c.fm.srp) share[size_t(W - 3)] *= c.splitShareSrp;
+    } else if (W > 1) {
         share[0] *= c.share0;
         if (c.fm.tides || c.fm.relativity || c.fm.srp) share[size_t(corrW)] *= c.shareCorr;
     }
@@ -535,7 +542,7 @@ StepArgs contextStepArgs(const phx_ctx& c, int& W) {
     // drag models run at most four waves per tile (the NRLMSISE-00 path needs
     // the register budget of one wave per SIMD)
     W = c.fm.hasSh ? (c.fm.drag ? std::min(c.nWaves, 4) : c.nWaves) : 1;
-    if (c.fm.hasSh) orderBlocks(c, W, a.blockStart);
+    if (c.fm.hasSh) orderBlocks(c, W, a.blockStart, false);
     else
         for (int w = 0; w <= kMaxWaves; ++w) a.blockStart[w] = 0;
     return a;
@@ -593,7 +600,7 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
             b.spec.alloc(8 * np);
         }
         const int Ws = c.splitWaves;
-        orderBlocks(c, Ws, a.blockStart);
+        orderBlocks(c, Ws, a.blockStart, true);
         SplitArgs sp{b.ks.p, b.ysBuf.p, b.parts.p, b.spec.p, 0, 0, c.splitSpecies};
         // per stage: SH kernel, then either the drag kernel or species + finish kernels
         kernelsPerStep = kStages * (c.splitSpecies ? 3 : 2) + 1;
@@ -631,7 +638,7 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
         int Wl = W;
         if (c.fm.hasSh && W < 4 && (long long)active * W < 4LL * c.numCU) Wl = 4;
         if (c.fm.hasSh && Wl != blocksFor) {
-            orderBlocks(c, Wl, a.blockStart);
+            orderBlocks(c, Wl, a.blockStart, false);
             blocksFor = Wl;
         }
         a.nTiles = active;
@@ -772,7 +779,11 @@ int phx_set_option(phx_ctx* c, const char* key, double v) {
         if (v != 0 && v != 1 && v != 2 && v != 4) { c->error = "split_species must be 0, 1, 2 or 4"; return PHX_EINVAL; }
         c->splitSpecies = int(v);
     }
-    else if (k == "split_waves") {
+    else if (k == "split_share0" || k == "split_share_pg" || k == "split_share_corr" || k == "split_share_srp") {
+        if (!(v >= 0.0 && v <= 4.0)) { c->error = k + " must be in [0, 4]"; return PHX_EINVAL; }
+        (k == "split_share0" ? c->splitShare0 : k == "split_share_pg" ? c->splitSharePg
+                                              : k == "split_share_corr" ? c->splitShareCorr : c->splitShareSrp) = v;
+    } else if (k == "split_waves") {
         if (v != 4 && v != 8 && v != 16) { c->error = "split_waves must be 4, 8 or 16"; return PHX_EINVAL; }
         c->splitWaves = int(v);
     }
@@ -802,6 +813,10 @@ double phx_get_option(const phx_ctx* c, const char* key) {
     if (k == "split") return c->split;
     if (k == "split_species") return c->splitSpecies;
     if (k == "split_waves") return c->splitWaves;
+    if (k == "split_share0") return c->splitShare0;
+    if (k == "split_share_pg") return c->splitSharePg;
+    if (k == "split_share_corr") return c->splitShareCorr;
+    if (k == "split_share_srp") return c->splitShareSrp;
     if (k == "step_seconds") return c->run.stepSeconds;
     if (k == "duration_seconds") return c->run.durationSeconds;
     if (k == "max_num_steps") return double(c->run.maxNumSteps);

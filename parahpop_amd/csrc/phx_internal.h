@@ -85,6 +85,8 @@ struct phx_ctx {
     int split = 1;               // shared-epoch batches with drag: split stages
     int splitWaves = 16;         // waves per tile of the drag-free split kernel
     int splitSpecies = 1;        // species per drag workgroup (0: one four-wave workgroup per tile)
+    // SH shares of the split kernel's waves with extra work (8 and 16 waves)
+    double splitShare0 = 1.0, splitSharePg = 0.5, splitShareCorr = 0.5, splitShareSrp = 0.5;
     // serving options
     int serveStepsPerTick = 16;
     long long serveSlots = 262144;      // object slots of the device pool
@@ -118,7 +120,10 @@ namespace phx {
 
 EpochArgs epochArgs(const phx_ctx& c);
 // Order blocks per wave, balanced on term counts with relative shares.
-void orderBlocks(const phx_ctx& c, int W, int out[kMaxWaves + 1]);
+// SH order blocks [out[w], out[w + 1]) of the W waves of a tile, balanced by
+// terms with the per-wave shares of the fused (split = false) or the split SH
+// kernel.
+void orderBlocks(const phx_ctx& c, int W, int out[kMaxWaves + 1], bool split);
 // Target validation of driver.cpp:99-112 (throws std::invalid_argument).
 void validateTargets(size_t n, const int64_t* id, const double* epoch, const double* state6, const double* mass,
                      const double* drag, const double* srp, const double* cd, const double* cr);

@@ -632,18 +632,21 @@ __device__ void pointGravity(const ForceModel& fm, const Rec& rec, double px, do
         ay = (-fm.gmEarth * py) * rc;
         az = (-fm.gmEarth * pz) * rc;
     }
+    // every body slot is evaluated (the record holds all kMaxBodies slots) and
+    // the configured ones are added in config order: no branch between the
+    // bodies, so their square roots and divisions overlap
 #pragma unroll
     for (int b = 0; b < kMaxBodies; ++b) {
-        if (b >= fm.nBodies) break;
-        if (!fm.bodyPointGravity[b] || fm.bodyNaif[b] == 399) continue;
+        const bool on = b < fm.nBodies && fm.bodyPointGravity[b] && fm.bodyNaif[b] != 399;
         const int q = REC_BODY + 3 * b;
         const double dx = px - rec(q), dy = py - rec(q + 1), dz = pz - rec(q + 2);
         const double s2 = dx * dx + dy * dy + dz * dz;
         const double rc = 1.0 / (s2 * sqrt(s2));
         const double g = fm.bodyGm[b];
-        ax += (-g * dx) * rc;
-        ay += (-g * dy) * rc;
-        az += (-g * dz) * rc;
+        const double tx = ax + (-g * dx) * rc, ty = ay + (-g * dy) * rc, tz = az + (-g * dz) * rc;
+        ax = on ? tx : ax;
+        ay = on ? ty : ay;
+        az = on ? tz : az;
     }
     ax += rec(REC_INDIRECT + 0);
     ay += rec(REC_INDIRECT + 1);
@@ -746,8 +749,31 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
 template <int W, class Rec>
 __device__ inline void stageForcesParts(const StepArgs& a, const ForceModel& fm, const Rec& rec, int wave, int lane, int m0,
                                         int m1, double ams, double (*xch)[kTile], double (*part)[3][kTile],
-                                        double (*extra)[3][kTile], double acc[3], double ax[3]) {
+                                        double (*extra)[3][kTile], double (*pg)[kTile], double acc[3], double ax[3],
+                                        double ex[3]) {
+    static_assert(W >= 4, "the split kernel spreads the extra families over waves 1 .. 3");
     const double px = xch[0][lane], py = xch[1][lane], pz = xch[2][lane];
+    // the families besides SH, each on a wave of its own before the barrier:
+    // point gravity on wave W - 1, tides and relativity on W - 2, SRP on W - 3
+    if (wave == W - 1) {
+        double gx, gy, gz;
+        pointGravity(fm, rec, px, py, pz, gx, gy, gz);
+        pg[0][lane] = gx;
+        pg[1][lane] = gy;
+        pg[2][lane] = gz;
+    } else if (wave == W - 2) {
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        if (fm.tides || fm.relativity) correctionsAccel(fm, rec, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], cx, cy, cz);
+        extra[1][0][lane] = cx;
+        extra[1][1][lane] = cy;
+        extra[1][2][lane] = cz;
+    } else if (wave == W - 3) {
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        if (fm.srp) srpAccel(fm, rec, px, py, pz, ams, bx, by, bz);
+        extra[0][0][lane] = bx;
+        extra[0][1][lane] = by;
+        extra[0][2][lane] = bz;
+    }
     ShGeom g;
     if (fm.hasSh) {
         double sr = 0.0, sp = 0.0, sl = 0.0;
@@ -759,24 +785,12 @@ __device__ inline void stageForcesParts(const StepArgs& a, const ForceModel& fm,
         part[wave][1][lane] = sp;
         part[wave][2][lane] = sl;
     }
-    if (wave == corrWave(W)) {
-        double cx = 0.0, cy = 0.0, cz = 0.0;
-        if (fm.tides || fm.relativity) correctionsAccel(fm, rec, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], cx, cy, cz);
-        if (fm.srp) {
-            double bx, by, bz;
-            srpAccel(fm, rec, px, py, pz, ams, bx, by, bz);
-            cx += bx;
-            cy += by;
-            cz += bz;
-        }
-        extra[1][0][lane] = cx;
-        extra[1][1][lane] = cy;
-        extra[1][2][lane] = cz;
-    }
     __syncthreads();
     if (wave == 0) {
-        double hx = 0.0, hy = 0.0, hz = 0.0, gx, gy, gz;
-        pointGravity(fm, rec, px, py, pz, gx, gy, gz);
+        // the fused kernel's sums in its order: ax = point gravity + SH,
+        // acc = ax + ((tides + relativity) + SRP)
+        double hx = 0.0, hy = 0.0, hz = 0.0;
+        double gx = pg[0][lane], gy = pg[1][lane], gz = pg[2][lane];
         if (fm.hasSh) {
             double r = 0.0, p = 0.0, l = 0.0;
             for (int w = 0; w < W; ++w) {
@@ -789,12 +803,21 @@ __device__ inline void stageForcesParts(const StepArgs& a, const ForceModel& fm,
             gy += hy;
             gz += hz;
         }
+        double cx = extra[1][0][lane], cy = extra[1][1][lane], cz = extra[1][2][lane];
+        if (fm.srp) {
+            cx += extra[0][0][lane];
+            cy += extra[0][1][lane];
+            cz += extra[0][2][lane];
+        }
         ax[0] = gx;
         ax[1] = gy;
         ax[2] = gz;
-        acc[0] = gx + (extra[1][0][lane] + 0.0);
-        acc[1] = gy + (extra[1][1][lane] + 0.0);
-        acc[2] = gz + (extra[1][2][lane] + 0.0);
+        ex[0] = cx;
+        ex[1] = cy;
+        ex[2] = cz;
+        acc[0] = gx + (cx + 0.0);
+        acc[1] = gy + (cy + 0.0);
+        acc[2] = gz + (cz + 0.0);
     }
 }
 
@@ -985,7 +1008,7 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
     __shared__ double xch[7][kTile];
     __shared__ double part[W][3][kTile];
     __shared__ double extra[2][3][kTile];
-    __shared__ double spec[1][kTile];
+    __shared__ double pg[3][kTile];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int tile = int(blockIdx.x);
@@ -1012,15 +1035,14 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
     const ExactRec rec{recp};
     const NrlSlot noSlot{nullptr, 0.0, 0.0};
     const int m0 = a.blockStart[wave], m1 = a.blockStart[wave + 1];
-    double acc[3], ax[3];
-    stageForcesParts<W>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, acc, ax);
+    double acc[3], ax[3], ex[3];
+    stageForcesParts<W>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex);
     (void)noSlot;
-    (void)spec;
     if (wave == 0) {
         for (int d = 0; d < 3; ++d) {
             sp.ks[ksAt(s, 3 + d, nPad, obj)] = acc[d];
             sp.parts[size_t(d) * nPad + obj] = ax[d];
-            sp.parts[size_t(3 + d) * nPad + obj] = extra[1][d][lane];
+            sp.parts[size_t(3 + d) * nPad + obj] = ex[d];
         }
     }
 }
