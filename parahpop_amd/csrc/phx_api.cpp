@@ -598,10 +598,11 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
             b.ysBuf.alloc(7 * np);
             b.parts.alloc(6 * np);
             b.spec.alloc(8 * np);
+            b.lane.alloc(5 * np);
         }
         const int Ws = c.splitWaves;
         orderBlocks(c, Ws, a.blockStart, true);
-        SplitArgs sp{b.ks.p, b.ysBuf.p, b.parts.p, b.spec.p, 0, 0, c.splitSpecies};
+        SplitArgs sp{b.ks.p, b.ysBuf.p, b.parts.p, b.spec.p, b.lane.p, 0, 0, c.splitSpecies};
         // per stage: SH kernel, then either the drag kernel or species + finish kernels
         kernelsPerStep = kStages * (c.splitSpecies ? 3 : 2) + 1;
         while (remaining > 0) {

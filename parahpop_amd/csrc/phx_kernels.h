@@ -55,6 +55,7 @@ struct SplitArgs {
     double* ys;      // [7][nPad] stage position, velocity, epoch
     double* parts;   // [6][nPad] point-mass + SH part and corrections part of the stage acceleration
     double* spec;    // [8][nPad] species densities (drag spread over several workgroups per tile)
+    double* lane;    // [5][nPad] NRLMSISE-00 lane inputs of the stage: altitude, latitude, longitude, local time, state
     int stage;       // 0 .. 12
     int step;        // absolute step index (epoch record of (step, stage))
     int speciesPerGroup;  // 0: one four-wave drag workgroup per tile; 1, 2, 4: species per workgroup

@@ -398,6 +398,35 @@ __device__ inline NrlLane nrlLane(const ForceModel& fm, const Rec& rec, const do
     return L;
 }
 
+// The lane inputs nrlLane derives from the stage position (altitude, geodetic
+// latitude and longitude in degrees, local solar time, state), kept in HBM by
+// the split SH kernel for the species and finish kernels of the stage.
+__device__ inline void nrlLaneStore(const NrlLane& L, double* lane, int nPad, int obj) {
+    lane[obj] = L.in.alt;
+    lane[size_t(nPad) + obj] = L.in.g_lat;
+    lane[size_t(2) * nPad + obj] = L.in.g_long;
+    lane[size_t(3) * nPad + obj] = L.in.lst;
+    lane[size_t(4) * nPad + obj] = double(L.state);
+}
+
+// nrlLane's result from the kept lane inputs and the stage's epoch record.
+__device__ inline NrlLane nrlLaneLoad(const double* lane, int nPad, int obj, const double* nrl) {
+    const double* r = nrl - REC_NRL;
+    NrlLane L;
+    L.state = int(lane[size_t(4) * nPad + obj]);
+    L.in.doy = int(r[REC_NRL_DOY]);
+    L.in.sec = r[REC_NRL_SEC];
+    L.in.alt = lane[obj];
+    L.in.g_lat = lane[size_t(nPad) + obj];
+    L.in.g_long = lane[size_t(2) * nPad + obj];
+    L.in.lst = lane[size_t(3) * nPad + obj];
+    L.in.f107A = r[REC_NRL_F107A];
+    L.in.f107 = r[REC_NRL_F107];
+    L.in.ap = r[REC_NRL_AP];
+    for (int k = 0; k < 7; ++k) L.in.apa[k] = r[REC_NRL_AP + k];
+    return L;
+}
+
 __device__ inline nrlmsise::Flags nrlFlags() {
     nrlmsise::Flags fl;
     for (int k = 0; k < 24; ++k) fl.switches[k] = 1;
@@ -750,7 +779,7 @@ template <int W, class Rec>
 __device__ inline void stageForcesParts(const StepArgs& a, const ForceModel& fm, const Rec& rec, int wave, int lane, int m0,
                                         int m1, double ams, double (*xch)[kTile], double (*part)[3][kTile],
                                         double (*extra)[3][kTile], double (*pg)[kTile], double acc[3], double ax[3],
-                                        double ex[3]) {
+                                        double ex[3], int tile, double* laneOut) {
     static_assert(W >= 4, "the split kernel spreads the extra families over waves 1 .. 3");
     const double px = xch[0][lane], py = xch[1][lane], pz = xch[2][lane];
     // the families besides SH, each on a wave of its own before the barrier:
@@ -773,6 +802,9 @@ __device__ inline void stageForcesParts(const StepArgs& a, const ForceModel& fm,
         extra[0][0][lane] = bx;
         extra[0][1][lane] = by;
         extra[0][2][lane] = bz;
+    } else if (wave == W - 4 && fm.drag && laneOut) {
+        // the NRLMSISE-00 lane inputs of this stage for the drag kernels
+        nrlLaneStore(nrlLane(fm, rec, rec.p + REC_NRL, px, py, pz), laneOut, a.nPad, tile * kTile + lane);
     }
     ShGeom g;
     if (fm.hasSh) {
@@ -1036,7 +1068,7 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
     const NrlSlot noSlot{nullptr, 0.0, 0.0};
     const int m0 = a.blockStart[wave], m1 = a.blockStart[wave + 1];
     double acc[3], ax[3], ex[3];
-    stageForcesParts<W>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex);
+    stageForcesParts<W>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex, tile, sp.lane);
     (void)noSlot;
     if (wave == 0) {
         for (int d = 0; d < 3; ++d) {
@@ -1099,11 +1131,9 @@ __global__ void __launch_bounds__(64 * SPW) splitSpeciesKernel(StepArgs a, Force
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
-    const double px = sp.ys[obj], py = sp.ys[size_t(nPad) + obj], pz = sp.ys[size_t(2) * nPad + obj];
     const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
-    const ExactRec rec{recp};
     const double* nrl = recp + REC_NRL;
-    const NrlLane nl = nrlLane(fm, rec, nrl, px, py, pz);
+    const NrlLane nl = nrlLaneLoad(sp.lane, nPad, obj, nrl);
     if (__ballot(nl.state == 1) == 0ull) return;
     const NrlSlot noSlot{nullptr, 0.0, 0.0};
     const int i = group * SPW + wave;   // entry of kSpeciesOrder
@@ -1119,12 +1149,12 @@ __global__ void __launch_bounds__(64) splitDragFinishKernel(StepArgs a, ForceMod
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
-    const double px = sp.ys[obj], py = sp.ys[size_t(nPad) + obj], pz = sp.ys[size_t(2) * nPad + obj];
     const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
     const ExactRec rec{recp};
     const double* nrl = recp + REC_NRL;
-    const NrlLane nl = nrlLane(fm, rec, nrl, px, py, pz);
+    const NrlLane nl = nrlLaneLoad(sp.lane, nPad, obj, nrl);
     if (nl.state == 0) return;
+    const double px = sp.ys[obj], py = sp.ys[size_t(nPad) + obj], pz = sp.ys[size_t(2) * nPad + obj];
     double rho;
     if (nl.state == 1) {
         for (int k = 0; k < 8; ++k) spec[k][lane] = sp.spec[size_t(k) * nPad + obj];
