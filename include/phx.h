@@ -77,9 +77,11 @@ int phx_batch_create_uneven(phx_ctx* ctx, size_t n, const int64_t* id, const dou
                             phx_batch** out);
 /* Autotuner (no reference counterpart; the reference reads fixed
  * PARAHPOP_PERF_* overrides, AccumulateBodies.cu:20-48): times trial_steps
- * steps of this batch for each candidate tile shape (waves per 64-object tile,
- * share of the SH order blocks on wave 0), keeps the fastest in the context's
- * options, resets the batch and reports the winner's trial statistics. */
+ * steps of this batch for each candidate shape (fused kernel: waves per
+ * 64-object tile and the SH share of wave 0; split stages for shared-epoch
+ * batches with drag: SH waves and species per workgroup), keeps the fastest in
+ * the context's options, resets the batch and reports the winner's trial
+ * statistics. */
 int phx_batch_autotune(phx_batch* b, int trial_steps, phx_stats* best);
 /* Runs up to max_steps further integration steps (<= 0: to completion). */
 int phx_batch_run(phx_batch* b, int max_steps, phx_stats* stats);
@@ -92,11 +94,26 @@ void phx_batch_destroy(phx_batch* b);
 
 /* Tuning options (no reference counterpart; the reference reads
  * PARAHPOP_PERF_* environment overrides, AccumulateBodies.cu:20-48):
- *   "waves"            1, 2 or 4 waves per 64-object tile
- *   "steps_per_launch" integration steps per kernel launch
- *   "share0", "share_corr"  relative SH order-block shares of wave 0 and of the
- *                      wave that also evaluates tides, relativity and SRP
- *   "sort_altitude"    1: group objects below the drag cutoff into common tiles */
+ *   "split"            1 (default): shared-epoch batches with drag run as split
+ *                      stages (SH, species and finish kernels per stage, k-stages
+ *                      in device memory); 0: the fused multi-step kernel
+ *   "split_waves"      4, 8 or 16 (default) waves per tile of the split SH kernel
+ *   "split_species"    1 (default), 2 or 4 species per drag workgroup; 0: one
+ *                      four-wave drag workgroup per tile
+ *   "split_share0", "split_share_pg", "split_share_corr", "split_share_srp"
+ *                      SH order-block shares (8 and 16 waves) of wave 0 and of the
+ *                      waves that also evaluate point gravity, tides and
+ *                      relativity, and SRP (defaults 1, 0.5, 0.5, 0.5)
+ *   "waves"            fused kernel: 1, 2 or 4 waves per 64-object tile (8 or 16
+ *                      without drag)
+ *   "steps_per_launch" fused kernel: integration steps per kernel launch
+ *   "share0", "share_corr"  fused kernel: relative SH order-block shares of wave 0
+ *                      and of the wave that also evaluates tides, relativity, SRP
+ *   "sort_altitude"    1: group objects below the drag cutoff into common tiles
+ *   "exact_groups_max", "grid_step", "force_grid"  epoch-record mode (exact
+ *                      per-stage records up to this many (start, end) groups,
+ *                      else a uniform grid of grid_step seconds)
+ *   "serve_slots", "serve_record_mib", "serve_steps_per_tick"  serving pool */
 int phx_set_option(phx_ctx* ctx, const char* key, double value);
 double phx_get_option(const phx_ctx* ctx, const char* key);
 
