@@ -145,7 +145,16 @@ void buildShTables(phx_ctx& c) {
         diag[size_t(m)] = f.factors[2 * f.index(m, m)];
         start[size_t(m)] = k;
         if (m > M) continue;
-        for (int n = m; n <= L; ++n, ++k) {
+        // each order's run is padded with zero terms to a multiple of four
+        // (the kernel's FP64 loop reads four-term batches)
+        for (int n = m; n < m + ((L - m + 4) & ~3); ++n, ++k) {
+            if (n > L) {
+                for (int p = 0; p < SHT_STRIDE; ++p) {
+                    terms.push_back(0.0);
+                    terms32.push_back(0.0f);
+                }
+                continue;
+            }
             const double C = f.C[f.index(n, m)], S = m > 0 ? f.S[f.index(n, m)] : 0.0;
             double e;
             if (f.normalized) e = n == m ? 0.0 : std::sqrt((2.0 * n + 1.0) * (double(n) * n - double(m) * m) / (2.0 * n - 1.0));
@@ -169,6 +178,13 @@ void buildShTables(phx_ctx& c) {
     c.shTerms.upload(terms.data(), terms.size(), c.stream);
     c.shTerms32.upload(terms32.data(), terms32.size(), c.stream);
     c.shOrderStart.upload(start.data(), start.size(), c.stream);
+    // then the cumulative products prod_{k=1..m} c_k (the seed of an order block)
+    double prod = 1.0;
+    diag.push_back(1.0);
+    for (int m = 1; m <= L; ++m) {
+        prod *= diag[size_t(m)];
+        diag.push_back(prod);
+    }
     c.shDiag.upload(diag.data(), diag.size(), c.stream);
 }
 
@@ -250,16 +266,54 @@ EpochArgs epochArgs(const phx_ctx& c) {
 
 void orderBlocks(const phx_ctx& c, int W, int out[kMaxWaves + 1], bool split) {
     const int L = c.fm.shDegree, M = c.fm.shOrder;
+    if (split && W >= 8) {
+        // split SH kernel: costs in units of one FP64 term of one wave. Order m
+        // runs its padded term count (order 0 with its derivative recursion
+        // costs more per term) plus a fixed per-order cost; the waves that carry
+        // point gravity (W - 1), tides and relativity (W - 2), SRP (W - 3) and
+        // the NRLMSISE-00 lane inputs (W - 4) start with that cost. Every wave
+        // gets the same total where it can (water filling).
+        std::vector<double> cost(static_cast<size_t>(M) + 1);
+        double sum = 0.0;
+        for (int m = 0; m <= M; ++m) {
+            cost[size_t(m)] = double((L - m + 4) & ~3) * (m == 0 ? c.splitCostOrder0 : 1.0) + c.splitCostOrder;
+            sum += cost[size_t(m)];
+        }
+        std::vector<double> extra(static_cast<size_t>(W), 0.0);
+        extra[size_t(W - 1)] = c.splitCostPg;
+        if (c.fm.tides || c.fm.relativity) extra[size_t(W - 2)] = c.splitCostCorr;
+        if (c.fm.srp) extra[size_t(W - 3)] = c.splitCostSrp;
+        if (c.fm.drag) extra[size_t(W - 4)] = c.splitCostLane;
+        // level T with sum_w max(0, T - extra_w) = sum (bisection)
+        double lo = 0.0, hi = sum + *std::max_element(extra.begin(), extra.end());
+        for (int it = 0; it < 100; ++it) {
+            const double T = 0.5 * (lo + hi);
+            double got = 0.0;
+            for (double e : extra) got += std::max(0.0, T - e);
+            (got < sum ? lo : hi) = T;
+        }
+        const double T = hi;
+        out[0] = 0;
+        int m = 0;
+        double acc = 0.0, target = 0.0;
+        for (int w = 0; w < W; ++w) {
+            target += std::max(0.0, T - extra[size_t(w)]);
+            if (w == W - 1) {
+                m = M + 1;
+            } else {
+                while (m <= M && acc + 0.5 * cost[size_t(m)] <= target) {
+                    acc += cost[size_t(m)];
+                    ++m;
+                }
+            }
+            out[w + 1] = m;
+        }
+        for (int w = W + 1; w <= kMaxWaves; ++w) out[w] = out[W];
+        return;
+    }
     std::vector<double> share(static_cast<size_t>(W), 1.0);
     const int corrW = W >= 3 ? W - 2 : W - 1;
-    if (split && W >= 8) {
-        // split SH kernel: point gravity on wave W - 1, tides and relativity on
-        // W - 2, SRP on W - 3, the sums after the barrier on wave 0
-        share[0] *= c.splitShare0;
-        share[size_t(W - 1)] *= c.splitSharePg;
-        if (c.fm.tides || c.fm.relativity) share[size_t(W - 2)] *= c.splitShareCorr;
-        if (c.fm.srp) share[size_t(W - 3)] *= c.splitShareSrp;
-    } else if (W > 1) {
+    if (W > 1) {
         share[0] *= c.share0;
         if (c.fm.tides || c.fm.relativity || c.fm.srp) share[size_t(corrW)] *= c.shareCorr;
     }
@@ -780,10 +834,15 @@ int phx_set_option(phx_ctx* c, const char* key, double v) {
         if (v != 0 && v != 1 && v != 2 && v != 4) { c->error = "split_species must be 0, 1, 2 or 4"; return PHX_EINVAL; }
         c->splitSpecies = int(v);
     }
-    else if (k == "split_share0" || k == "split_share_pg" || k == "split_share_corr" || k == "split_share_srp") {
-        if (!(v >= 0.0 && v <= 4.0)) { c->error = k + " must be in [0, 4]"; return PHX_EINVAL; }
-        (k == "split_share0" ? c->splitShare0 : k == "split_share_pg" ? c->splitSharePg
-                                              : k == "split_share_corr" ? c->splitShareCorr : c->splitShareSrp) = v;
+    else if (k == "split_cost_pg" || k == "split_cost_corr" || k == "split_cost_srp" || k == "split_cost_lane" ||
+             k == "split_cost_order" || k == "split_cost_order0") {
+        if (!(v >= 0.0 && v <= 1e4)) { c->error = k + " must be in [0, 10000]"; return PHX_EINVAL; }
+        (k == "split_cost_pg"      ? c->splitCostPg
+         : k == "split_cost_corr"  ? c->splitCostCorr
+         : k == "split_cost_srp"   ? c->splitCostSrp
+         : k == "split_cost_lane"  ? c->splitCostLane
+         : k == "split_cost_order" ? c->splitCostOrder
+                                   : c->splitCostOrder0) = v;
     } else if (k == "split_waves") {
         if (v != 4 && v != 8 && v != 16) { c->error = "split_waves must be 4, 8 or 16"; return PHX_EINVAL; }
         c->splitWaves = int(v);
@@ -814,10 +873,12 @@ double phx_get_option(const phx_ctx* c, const char* key) {
     if (k == "split") return c->split;
     if (k == "split_species") return c->splitSpecies;
     if (k == "split_waves") return c->splitWaves;
-    if (k == "split_share0") return c->splitShare0;
-    if (k == "split_share_pg") return c->splitSharePg;
-    if (k == "split_share_corr") return c->splitShareCorr;
-    if (k == "split_share_srp") return c->splitShareSrp;
+    if (k == "split_cost_pg") return c->splitCostPg;
+    if (k == "split_cost_corr") return c->splitCostCorr;
+    if (k == "split_cost_srp") return c->splitCostSrp;
+    if (k == "split_cost_lane") return c->splitCostLane;
+    if (k == "split_cost_order") return c->splitCostOrder;
+    if (k == "split_cost_order0") return c->splitCostOrder0;
     if (k == "step_seconds") return c->run.stepSeconds;
     if (k == "duration_seconds") return c->run.durationSeconds;
     if (k == "max_num_steps") return double(c->run.maxNumSteps);

@@ -102,7 +102,7 @@ struct StepArgs {
     const double* shTerms;       // packed term stream (FP64)
     const float* shTerms32;      // packed term stream (FP32) for mixed precision
     const int* shOrderStart;     // [L+2] first term of order m
-    const double* shDiag;        // [L+1] sectoral factors c_m
+    const double* shDiag;        // [L+1] sectoral factors c_m, then [L+1] products c_1 ... c_m
     int blockStart[kMaxWaves + 1];  // order blocks per wave: [blockStart[w], blockStart[w+1])
     int fp64Degree;              // mixed precision: degrees above this use FP32 terms (-1: all FP64)
     const double* nrlCoeff;      // NRLMSISE-00 coefficient blocks (device)

@@ -85,8 +85,10 @@ struct phx_ctx {
     int split = 1;               // shared-epoch batches with drag: split stages
     int splitWaves = 16;         // waves per tile of the drag-free split kernel
     int splitSpecies = 1;        // species per drag workgroup (0: one four-wave workgroup per tile)
-    // SH shares of the split kernel's waves with extra work (8 and 16 waves)
-    double splitShare0 = 1.0, splitSharePg = 0.5, splitShareCorr = 0.5, splitShareSrp = 0.5;
+    // split SH kernel (8 and 16 waves): costs of the waves' extra families and of
+    // an order, in FP64 terms of one wave (balancing of the order blocks)
+    double splitCostPg = 105.0, splitCostCorr = 246.0, splitCostSrp = 83.0, splitCostLane = 104.0;
+    double splitCostOrder = 2.0, splitCostOrder0 = 1.8;
     // serving options
     int serveStepsPerTick = 16;
     long long serveSlots = 262144;      // object slots of the device pool

@@ -143,6 +143,7 @@ __device__ __attribute__((always_inline)) inline void shRun(CP t, int count, T u
 template <class T, class CP>
 __device__ __attribute__((always_inline)) inline void shRun0(CP t, int count, T u, T v, T w, T& P0, T& Pm1, T& dP0, T& dPm1,
                                                              T& X1, T& D1) {
+#pragma unroll 4
     for (int i = 0; i < count; ++i, t += SHT_STRIDE) {
         X1 = fma(P0, t[SHT_C1], X1);
         D1 = fma(dP0, t[SHT_C], D1);
@@ -181,16 +182,94 @@ __device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const doubl
     const double w = g.alpha * g.alpha;
     const float uf = float(u), vf = float(v), wf = float(w);
     double ar = 0.0, aphiDirect = 0.0, aphiNum = 0.0, alamNum = 0.0;
-    // Seed P~_{m0,m0} = prod c_k (alpha cos phi)^m0 and cos/sin(m0 lambda).
+    // Seed P~_{m0,m0} = (prod_{k <= m0} c_k) (alpha cos phi)^m0 and
+    // cos/sin(m0 lambda): the power and the phasor by binary powering.
     double Pmm = 1.0, cosML = 1.0, sinML = 0.0;
-    for (int k = 1; k <= m0; ++k) {
-        Pmm = diag[k] * v * Pmm;
-        const double s = g.cosL * sinML + g.sinL * cosML;
-        const double c = g.cosL * cosML - g.sinL * sinML;
-        sinML = s;
-        cosML = c;
+    if (m0 > 0) {
+        double vp = 1.0, bv = v, bc = g.cosL, bs = g.sinL;
+        for (int e = m0; e > 0; e >>= 1) {
+            if (e & 1) {
+                vp *= bv;
+                const double c = cosML * bc - sinML * bs;
+                const double s = sinML * bc + cosML * bs;
+                cosML = c;
+                sinML = s;
+            }
+            bv *= bv;
+            const double c2 = bc * bc - bs * bs;
+            const double s2 = 2.0 * bc * bs;
+            bc = c2;
+            bs = s2;
+        }
+        Pmm = diag[L + 1 + m0] * vp;
     }
-    for (int m = m0; m < m1; ++m) {
+    if (nSplit > L && m0 < m1) {
+        // FP64 stream: order 0 by its own recursion, orders >= 1 as one flat loop
+        // over four-term batches (each order's run is padded with zero terms to a
+        // multiple of four), so an order change costs no extra round trip for its
+        // start, its remainder or its next diagonal factor
+        int m = m0;
+        if (m == 0) {
+            double P0 = Pmm, Pm1 = 0.0, dP0 = 0.0, dPm1 = 0.0;
+            double X1 = 0.0, D1 = 0.0;
+            // the padded run (zero terms leave the sums unchanged)
+            shRun0<double>(terms + SHT_STRIDE * orderStart[0], (L + 4) & ~3, u, v, w, P0, Pm1, dP0, dPm1, X1, D1);
+            ar -= X1;
+            aphiDirect += D1;
+            if (1 <= L) Pmm = diag[1] * v * Pmm;
+            const double s = g.cosL * sinML + g.sinL * cosML;
+            const double c = g.cosL * cosML - g.sinL * sinML;
+            sinML = s;
+            cosML = c;
+            m = 1;
+        }
+        if (m < m1) {
+            ConstD t = terms + SHT_STRIDE * orderStart[m];
+            int left = (L - m + 4) >> 2;
+            // the next diagonal factor is prefetched an order ahead
+            double dNext = diag[min(m + 1, L)];
+            double P0 = Pmm, Pm1 = 0.0;
+            double A1 = 0.0, A2 = 0.0, A5 = 0.0, A6 = 0.0, B1 = 0.0, B2 = 0.0;
+            while (true) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    ConstD q = t + j * SHT_STRIDE;
+                    A1 = fma(P0, q[SHT_C1], A1);
+                    A2 = fma(P0, q[SHT_S1], A2);
+                    A5 = fma(P0, q[SHT_C], A5);
+                    A6 = fma(P0, q[SHT_S], A6);
+                    B1 = fma(Pm1, q[SHT_EC], B1);
+                    B2 = fma(Pm1, q[SHT_ES], B2);
+                    const double Pn = fma(q[SHT_A] * u, P0, -(q[SHT_B] * w) * Pm1);
+                    Pm1 = P0;
+                    P0 = Pn;
+                }
+                t += 4 * SHT_STRIDE;
+                if (--left == 0) {
+                    const double X1 = cosML * A1 + sinML * A2;
+                    const double X5 = cosML * A5 + sinML * A6;
+                    const double XE = cosML * B1 + sinML * B2;
+                    const double X6 = sinML * A5 - cosML * A6;
+                    ar -= X1;
+                    aphiNum += g.alpha * XE - g.sinPhi * (X1 - X5);
+                    alamNum -= double(m) * X6;
+                    if (m + 1 >= m1) break;
+                    if (m + 1 <= L) Pmm = dNext * v * Pmm;
+                    const double s = g.cosL * sinML + g.sinL * cosML;
+                    const double c = g.cosL * cosML - g.sinL * sinML;
+                    sinML = s;
+                    cosML = c;
+                    ++m;
+                    dNext = diag[min(m + 1, L)];
+                    left = (L - m + 4) >> 2;
+                    P0 = Pmm;
+                    Pm1 = 0.0;
+                    A1 = A2 = A5 = A6 = B1 = B2 = 0.0;
+                }
+            }
+        }
+    }
+    for (int m = (nSplit > L ? m1 : m0); m < m1; ++m) {
         const int first = orderStart[m];
         const int count = L - m + 1;
         const int c64 = max(0, min(count, nSplit - m));   // degrees m .. m + c64 - 1 in FP64
