@@ -100,10 +100,13 @@ void phx_batch_destroy(phx_batch* b);
  *   "split_waves"      4, 8 or 16 (default) waves per tile of the split SH kernel
  *   "split_species"    1 (default), 2 or 4 species per drag workgroup; 0: one
  *                      four-wave drag workgroup per tile
- *   "split_share0", "split_share_pg", "split_share_corr", "split_share_srp"
- *                      SH order-block shares (8 and 16 waves) of wave 0 and of the
- *                      waves that also evaluate point gravity, tides and
- *                      relativity, and SRP (defaults 1, 0.5, 0.5, 0.5)
+ *   "split_cost_pg", "split_cost_corr", "split_cost_srp", "split_cost_lane",
+ *   "split_cost_order", "split_cost_order0"  balancing of the split SH kernel's
+ *                      order blocks (8 and 16 waves), in FP64 terms of one wave:
+ *                      the cost of point gravity, tides and relativity, SRP and
+ *                      the NRLMSISE-00 lane inputs on the waves that carry them,
+ *                      the fixed cost of an order and the per-term factor of
+ *                      order 0 (defaults 105, 246, 83, 104, 2, 1.8)
  *   "waves"            fused kernel: 1, 2 or 4 waves per 64-object tile (8 or 16
  *                      without drag)
  *   "steps_per_launch" fused kernel: integration steps per kernel launch
