@@ -134,8 +134,8 @@ def leg_main(args, work):
         # split-stage path: the unit of work the roofline is priced on is one step
         # (13 stages of SH, species and finish kernels, then the combine kernel)
         sw, ss = int(ctx.option("split_waves")), int(ctx.option("split_species"))
-        kern_desc = ("step of %d kernels: 13 x (splitShKernel<%d>, %s) + splitCombineKernel"
-                     % (kps, sw, ("splitSpeciesKernel<%d>, splitDragFinishKernel" % ss) if ss else "splitDragKernel"))
+        drag = {0: "splitDragKernel", 8: "splitDragTileKernel"}.get(ss, "splitSpeciesKernel<%d>, splitDragFinishKernel" % ss)
+        kern_desc = "step of %d kernels: 13 x (splitShKernel<%d>, %s) + splitCombineKernel" % (kps, sw, drag)
         per, shape = steps, {"path": "split", "split_waves": sw, "split_species": ss}
     else:
         kern_desc = "launch of rkf78Kernel<%d> (%d steps)" % (st["waves"], st["steps_per_launch"])

@@ -98,8 +98,9 @@ void phx_batch_destroy(phx_batch* b);
  *                      stages (SH, species and finish kernels per stage, k-stages
  *                      in device memory); 0: the fused multi-step kernel
  *   "split_waves"      4, 8 or 16 (default) waves per tile of the split SH kernel
- *   "split_species"    1 (default), 2 or 4 species per drag workgroup; 0: one
- *                      four-wave drag workgroup per tile
+ *   "split_species"    8 (default): one eight-wave drag workgroup per tile (the
+ *                      species and the drag); 1, 2 or 4: species per workgroup and
+ *                      a finish kernel; 0: one four-wave drag workgroup per tile
  *   "split_cost_pg", "split_cost_corr", "split_cost_srp", "split_cost_lane",
  *   "split_cost_order", "split_cost_order0"  balancing of the split SH kernel's
  *                      order blocks (8 and 16 waves), in FP64 terms of one wave:

@@ -658,7 +658,7 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
         orderBlocks(c, Ws, a.blockStart, true);
         SplitArgs sp{b.ks.p, b.ysBuf.p, b.parts.p, b.spec.p, b.lane.p, 0, 0, c.splitSpecies};
         // per stage: SH kernel, then either the drag kernel or species + finish kernels
-        kernelsPerStep = kStages * (c.splitSpecies ? 3 : 2) + 1;
+        kernelsPerStep = kStages * (c.splitSpecies == 0 || c.splitSpecies == 8 ? 2 : 3) + 1;
         while (remaining > 0) {
             int active = b.nTiles;
             while (active > 0 && b.tileSteps[size_t(active - 1)] <= b.stepsDone) --active;
@@ -831,7 +831,7 @@ int phx_set_option(phx_ctx* c, const char* key, double v) {
     } else if (k == "force_grid") c->forceGrid = v != 0.0;
     else if (k == "split") c->split = v != 0.0;
     else if (k == "split_species") {
-        if (v != 0 && v != 1 && v != 2 && v != 4) { c->error = "split_species must be 0, 1, 2 or 4"; return PHX_EINVAL; }
+        if (v != 0 && v != 1 && v != 2 && v != 4 && v != 8) { c->error = "split_species must be 0, 1, 2, 4 or 8"; return PHX_EINVAL; }
         c->splitSpecies = int(v);
     }
     else if (k == "split_cost_pg" || k == "split_cost_corr" || k == "split_cost_srp" || k == "split_cost_lane" ||
@@ -951,7 +951,7 @@ int phx_batch_autotune(phx_batch* b, int trialSteps, phx_stats* best) {
                 }
             if (c.fm.drag && !b->grid)
                 for (int sw : {8, 16})
-                    for (int ss : {1, 2}) cands.push_back({c.nWaves, c.share0, 1, sw, ss});
+                    for (int ss : {1, 8}) cands.push_back({c.nWaves, c.share0, 1, sw, ss});
         } else {
             cands.push_back({1, 1.0, 0, c.splitWaves, c.splitSpecies});
         }

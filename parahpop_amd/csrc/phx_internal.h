@@ -84,7 +84,8 @@ struct phx_ctx {
     int forceGrid = 0;
     int split = 1;               // shared-epoch batches with drag: split stages
     int splitWaves = 16;         // waves per tile of the drag-free split kernel
-    int splitSpecies = 1;        // species per drag workgroup (0: one four-wave workgroup per tile)
+    int splitSpecies = 8;        // 8: one eight-wave drag workgroup per tile (species and drag); 1, 2, 4:
+                                 // species per workgroup, then a finish kernel; 0: one four-wave workgroup per tile
     // split SH kernel (8 and 16 waves): costs of the waves' extra families and of
     // an order, in FP64 terms of one wave (balancing of the order blocks)
     double splitCostPg = 105.0, splitCostCorr = 246.0, splitCostSrp = 83.0, splitCostLane = 104.0;

@@ -1441,13 +1441,17 @@ struct FastGlobe {
     const double* E;  // GE_COUNT values
     const double* T;  // GE_TABLES x GT_COUNT table phases
     __device__ __attribute__((always_inline)) void basis(State& st, const Input& in, const Flags&) const { fastBasis(st, in, E[GE_DFA]); }
+    // globe7 of table j without side effects; apt0 is the table's Ap function
+    __device__ __attribute__((always_inline)) double value(const State& st, int j, const double* p, const Input& in,
+                                                          double& apt0) const {
+        const NrlCD ec = (NrlCD)(E + GE_STRIDE * j);
+        return globe7Fast(st, (NrlCD)p, ec, ((NrlCD)E)[GE_DFA], in.g_lat, ec[GE_UT71], ec[GE_UT79C], ec[GE_UT79S], ec[GE_UT58],
+                          (NrlCD)(T + GT_COUNT * j), apt0);
+    }
     __device__ __attribute__((always_inline)) double operator()(State& st, int j, const double* p, const Input& in, const Flags&) const {
         // globe7 also leaves apt[0] for glob7s (lower thermosphere)
-        const double* e = E + GE_STRIDE * j;
         double apt0 = 0.0;
-        const NrlCD ec = (NrlCD)e;
-        const double v = globe7Fast(st, (NrlCD)p, ec, ((NrlCD)E)[GE_DFA], in.g_lat, ec[GE_UT71], ec[GE_UT79C], ec[GE_UT79S],
-                                    ec[GE_UT58], (NrlCD)(T + GT_COUNT * j), apt0);
+        const double v = value(st, j, p, in, apt0);
         if (p[51] != 0) st.apt[0] = apt0;
         return v;
     }

@@ -1221,6 +1221,109 @@ __global__ void __launch_bounds__(64 * SPW) splitSpeciesKernel(StepArgs a, Force
     sp.spec[size_t(k) * nPad + obj] = spec[k][lane];
 }
 
+// Globe policy serving precomputed values: globe7 of table j from row j
+// (with the Ap function of pt, ps and pd[3] in rows 11-13 for the apt[0] that
+// the serial calls leave) and glob7s of lower-thermosphere table j from row
+// 14 + j of the workgroup's exchange array.
+struct SharedGlobe {
+    double (*v)[kTile];
+    int lane;
+    __device__ __attribute__((always_inline)) void basis(nrlmsise::State&, const nrlmsise::Input&, const nrlmsise::Flags&) const {}
+    __device__ __attribute__((always_inline)) double operator()(nrlmsise::State& st, int j, const double* p,
+                                                               const nrlmsise::Input&, const nrlmsise::Flags&) const {
+        if (((nrlmsise::NrlCD)p)[51] != 0) {
+            if (j == 0) st.apt[0] = v[11][lane];
+            else if (j == 1) st.apt[0] = v[12][lane];
+            else if (j == 5) st.apt[0] = v[13][lane];
+        }
+        return v[j][lane];
+    }
+    __device__ __attribute__((always_inline)) double lower(nrlmsise::State&, int j, const double*, const nrlmsise::Input&,
+                                                          const nrlmsise::Flags&) const {
+        return v[14 + j][lane];
+    }
+};
+
+// Drag of one tile and stage in one eight-wave workgroup. The waves first
+// evaluate the eleven globe7 tables the tile's lanes need (two per wave at
+// most) and, when a lane is below 300 km, the five glob7s node tables; from
+// those values each wave forms gts7's prefix and the shared densu quantities
+// and one species (kSpeciesOrder[wave]); wave 0 then assembles the density
+// and the drag, k_s = ax + (extra + drag), as the fused kernel. Every value is
+// the one the serial evaluation forms.
+__global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
+    __shared__ double gv[19][kTile];
+    __shared__ double spec[8][kTile];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int tile = int(blockIdx.x);
+    const int obj = tile * kTile + lane;
+    const int nPad = a.nPad;
+    const int s = sp.stage;
+    const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
+    const ExactRec rec{recp};
+    const double* nrl = recp + REC_NRL;
+    const NrlLane nl = nrlLaneLoad(sp.lane, nPad, obj, nrl);
+    // the same lanes in every wave: the branch is uniform across the workgroup
+    if (__ballot(nl.state == 1) != 0ull) {
+        const nrlmsise::Input in0 = nl.in;
+        const nrlmsise::Flags fl = nrlFlags();
+        const nrlmsise::NrlTables T{a.nrlCoeff};
+        const nrlmsise::FastGlobe F{nrl + (REC_NRL_GLOBE - REC_NRL), kGlobeTrig};
+        {
+            nrlmsise::State st;
+            F.basis(st, in0, fl);
+            for (int j = wave; j < 11; j += 8) {
+                double apt = 0.0;
+                gv[j][lane] = F.value(st, j, nrlmsise::globeTable(T, j), in0, apt);
+                if (j == 0) gv[11][lane] = apt;
+                else if (j == 1) gv[12][lane] = apt;
+                else if (j == 5) gv[13][lane] = apt;
+            }
+            if (__syncthreads_or(in0.alt < 300.0)) {
+                if (wave >= 3) {
+                    // apt[0] as gts7's globe7 calls of pt, ps and pd[3] leave it
+                    const double za = ((nrlmsise::NrlCD)(T.base + nrlmsise::OFF_PDL + 25 * 1))[15];
+                    double apt = 0.0;
+                    if (in0.alt > za && ((nrlmsise::NrlCD)nrlmsise::globeTable(T, 0))[51] != 0) apt = gv[11][lane];
+                    if (in0.alt > 72.5 && ((nrlmsise::NrlCD)nrlmsise::globeTable(T, 1))[51] != 0) apt = gv[12][lane];
+                    if (((nrlmsise::NrlCD)nrlmsise::globeTable(T, 5))[51] != 0) apt = gv[13][lane];
+                    st.apt[0] = apt;
+                    const int j = wave - 3;
+                    gv[14 + j][lane] = F.lower(st, j, nrlmsise::lowerTable(T, j), in0, fl);
+                }
+                __syncthreads();
+            }
+        }
+        nrlmsise::Input in = in0;
+        nrlmsise::State st;
+        nrlmsise::glatf(in.g_lat, st.gsurf, st.re);
+        const SharedGlobe G{gv, lane};
+        nrlmsise::Pre pre;
+        nrlmsise::gts7Prefix(st, T, in, fl, G, pre);
+        nrlmsise::DensuShared h;
+        const int k = kSpeciesOrder[wave];
+        nrlmsise::gts7SpeciesShare(h, k == 0, st, T, in, fl, G, pre);
+        spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
+    }
+    __syncthreads();
+    if (wave != 0 || nl.state == 0) return;
+    double rho;
+    if (nl.state == 1) rho = nrlRhoFromSpecies(fm, spec, lane);
+    else if (nl.state == 2) rho = nrlDensity<1>(fm.densityScale, nrl, a.nrlCoeff, nl.in);
+    else rho = nan("");
+    const double px = sp.ys[obj], py = sp.ys[size_t(nPad) + obj], pz = sp.ys[size_t(2) * nPad + obj];
+    const double vx = sp.ys[size_t(3) * nPad + obj], vy = sp.ys[size_t(4) * nPad + obj], vz = sp.ys[size_t(5) * nPad + obj];
+    const double om0 = rec(REC_OMEGA), om1 = rec(REC_OMEGA + 1), om2 = rec(REC_OMEGA + 2);
+    const double rx = vx - (om1 * pz - om2 * py);
+    const double ry = vy - (om2 * px - om0 * pz);
+    const double rz = vz - (om0 * py - om1 * px);
+    const double f = -0.5 * rho * a.bcd[obj] * sqrt(rx * rx + ry * ry + rz * rz);
+    const double dd[3] = {f * rx, f * ry, f * rz};
+    for (int d = 0; d < 3; ++d)
+        sp.ks[ksAt(s, 3 + d, nPad, obj)] = sp.parts[size_t(d) * nPad + obj] + (sp.parts[size_t(3 + d) * nPad + obj] + dd[d]);
+}
+
 __global__ void __launch_bounds__(64) splitDragFinishKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
     __shared__ double spec[8][kTile];
     const int lane = threadIdx.x & 63;
@@ -1438,7 +1541,12 @@ static int launchSplitFinish(const StepArgs& a, const ForceModel& fm, const Spli
 // four-wave drag kernel.
 int launchSplitStage(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, int shWaves, hipStream_t stream) {
     int rc = launchSplitSh(a, fm, sp, shWaves, stream);
-    if (rc == 0 && sp.speciesPerGroup != 0) rc = launchSplitSpecies(a, fm, sp, stream);
+    if (rc != 0 || a.nTiles <= 0 || !fm.drag) return rc;
+    if (sp.speciesPerGroup == 8) {
+        hipLaunchKernelGGL(splitDragTileKernel, dim3(a.nTiles), dim3(512), 0, stream, a, fm, sp);
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
+    if (sp.speciesPerGroup != 0) rc = launchSplitSpecies(a, fm, sp, stream);
     if (rc == 0) rc = launchSplitFinish(a, fm, sp, stream);
     return rc;
 }

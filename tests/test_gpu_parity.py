@@ -219,9 +219,9 @@ def test_split_stages_match_fused_kernel(full_workload):
     sixteen waves only the SH summation order changes."""
     cfg, t = full_workload
     fused = _ctx(cfg, split=0, waves=4).propagate(t, 1800.0)
-    for sw, spg in ((4, 0), (4, 1), (4, 2), (4, 4), (8, 0), (16, 1)):
+    for sw, spg in ((4, 0), (4, 1), (4, 2), (4, 4), (4, 8), (8, 0), (16, 1), (16, 8)):
         e, s, st = _ctx(cfg, split=1, split_waves=sw, split_species=spg).propagate(t, 1800.0)
-        assert st["steps_per_launch"] == 1 and st["kernels_per_step"] == 13 * (3 if spg else 2) + 1
+        assert st["steps_per_launch"] == 1 and st["kernels_per_step"] == 13 * (2 if spg in (0, 8) else 3) + 1
         assert st["launches"] % st["kernels_per_step"] == 0
         np.testing.assert_array_equal(e, fused[0])
         if sw == 4:
@@ -239,17 +239,20 @@ def test_split_stages_uneven_durations(full_workload):
     # exact per-stage records, the path the split stages serve)
     dur = np.array([307.25, 611.5, 1207.0, 1500.0])[np.arange(t["n"]) % 4]
     out = []
-    for opt in ({"split": 0, "waves": 4}, {"split": 1, "split_waves": 4, "split_species": 1}):
+    for opt in ({"split": 0, "waves": 4}, {"split": 1, "split_waves": 4, "split_species": 1},
+                {"split": 1, "split_waves": 4, "split_species": 8}):
         ctx = _ctx(cfg, **opt)
         b = ctx.batch(t, dur)
         st = b.run()
         assert b.done()
         out.append(b.download() + (st,))
         b.close()
-    np.testing.assert_array_equal(out[1][0], out[0][0])
-    np.testing.assert_array_equal(out[1][1], out[0][1])
-    assert out[1][2]["object_steps"] == out[0][2]["object_steps"]
-    assert out[1][2]["kernels_per_step"] == 40 and out[0][2]["kernels_per_step"] == 0
+    for o in out[1:]:
+        np.testing.assert_array_equal(o[0], out[0][0])
+        np.testing.assert_array_equal(o[1], out[0][1])
+        assert o[2]["object_steps"] == out[0][2]["object_steps"]
+    assert out[1][2]["kernels_per_step"] == 40 and out[2][2]["kernels_per_step"] == 27
+    assert out[0][2]["kernels_per_step"] == 0
 
 
 def test_low_altitude_drag_paths_match_oracle(full_workload):
@@ -271,7 +274,7 @@ def test_low_altitude_drag_paths_match_oracle(full_workload):
     low["state"] = np.concatenate([u * r, w * np.sqrt(398600.4415 / r)])
     low["n"] = n
     e_o, s_o, _ = _oracle(cfg).propagate(low, 600.0, threads=8)
-    for opt in ({}, {"split": 0}, {"force_grid": 1}):
+    for opt in ({}, {"split_species": 1}, {"split_species": 8}, {"split": 0}, {"force_grid": 1}):
         e_g, s_g, st = _ctx(cfg, **opt).propagate(low, 600.0)
         np.testing.assert_array_equal(e_g, e_o)
         if not opt:
