@@ -36,6 +36,8 @@
 // reference form with the analytic m = 1 limit.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "phx_common.h"
 #include "phx_kernels.h"
 #include "phx_epoch_device.h"
@@ -1221,6 +1223,20 @@ __global__ void __launch_bounds__(64 * SPW) splitSpeciesKernel(StepArgs a, Force
     sp.spec[size_t(k) * nPad + obj] = spec[k][lane];
 }
 
+// Calls f on every field of the densu quantities a lane's species share.
+template <class H, class F>
+__device__ __attribute__((always_inline)) inline void densuSharedFields(H& h, F&& f) {
+    f(h.re), f(h.za), f(h.zlb), f(h.s2), f(h.tinf), f(h.tlb), f(h.glb), f(h.xmm);
+    f(h.zLane), f(h.zg2L), f(h.ttL), f(h.lnL), f(h.laneUpper);
+    f(h.zg2A), f(h.ttA), f(h.lnA), f(h.z1), f(h.z2), f(h.t1), f(h.zgdif), f(h.glb1);
+    for (int k = 0; k < 5; ++k) f(h.xs[k]);
+    for (int k = 0; k < 5; ++k) f(h.ys[k]);
+    for (int k = 0; k < 5; ++k) f(h.y2out[k]);
+    f(h.pmm), f(h.emm), f(h.db28), f(h.b28);
+}
+constexpr int kDensuSharedFields = 40;
+static_assert(sizeof(nrlmsise::DensuShared) == kDensuSharedFields * sizeof(double), "every densu field is exchanged");
+
 // Globe policy serving precomputed values: globe7 of table j from row j
 // (with the Ap function of pt, ps and pd[3] in rows 11-13 for the apt[0] that
 // the serial calls leave) and glob7s of lower-thermosphere table j from row
@@ -1254,6 +1270,7 @@ struct SharedGlobe {
 __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
     __shared__ double gv[19][kTile];
     __shared__ double spec[8][kTile];
+    __shared__ double hx[kDensuSharedFields + 2][kTile];   // wave 0's densu quantities and lower-boundary node
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int tile = int(blockIdx.x);
@@ -1301,9 +1318,27 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
         const SharedGlobe G{gv, lane};
         nrlmsise::Pre pre;
         nrlmsise::gts7Prefix(st, T, in, fl, G, pre);
+        // the densu quantities every species shares: formed once (wave 0, with
+        // the N2 turbopause density) and handed to the other waves
         nrlmsise::DensuShared h;
+        if (wave == 0) {
+            nrlmsise::gts7SpeciesShare(h, true, st, T, in, fl, G, pre);
+            int i = 0;
+            densuSharedFields(h, [&](auto& x) { hx[i++][lane] = double(x); });
+            hx[kDensuSharedFields][lane] = st.meso_tn1[0];
+            hx[kDensuSharedFields + 1][lane] = st.meso_tgn1[0];
+        }
+        __syncthreads();
+        if (wave != 0) {
+            int i = 0;
+            densuSharedFields(h, [&](auto& x) {
+                using T = typename std::remove_reference<decltype(x)>::type;
+                x = T(hx[i++][lane]);
+            });
+            st.meso_tn1[0] = hx[kDensuSharedFields][lane];
+            st.meso_tgn1[0] = hx[kDensuSharedFields + 1][lane];
+        }
         const int k = kSpeciesOrder[wave];
-        nrlmsise::gts7SpeciesShare(h, k == 0, st, T, in, fl, G, pre);
         spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
     }
     __syncthreads();
