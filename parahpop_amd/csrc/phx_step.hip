@@ -1287,15 +1287,17 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
         const nrlmsise::Flags fl = nrlFlags();
         const nrlmsise::NrlTables T{a.nrlCoeff};
         const nrlmsise::FastGlobe F{nrl + (REC_NRL_GLOBE - REC_NRL), kGlobeTrig};
+        const int k = kSpeciesOrder[wave];
         {
+            // globe7 of the prefix tables pt, ps, pd[3] and of pd[2] (N2), one per
+            // wave; with a lane below 300 km, then the five glob7s node tables
             nrlmsise::State st;
             F.basis(st, in0, fl);
-            for (int j = wave; j < 11; j += 8) {
+            if (wave < 4) {
+                const int j = wave == 2 ? 5 : (wave == 3 ? 4 : wave);
                 double apt = 0.0;
                 gv[j][lane] = F.value(st, j, nrlmsise::globeTable(T, j), in0, apt);
-                if (j == 0) gv[11][lane] = apt;
-                else if (j == 1) gv[12][lane] = apt;
-                else if (j == 5) gv[13][lane] = apt;
+                if (wave < 3) gv[11 + wave][lane] = apt;
             }
             if (__syncthreads_or(in0.alt < 300.0)) {
                 if (wave >= 3) {
@@ -1318,18 +1320,28 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
         const SharedGlobe G{gv, lane};
         nrlmsise::Pre pre;
         nrlmsise::gts7Prefix(st, T, in, fl, G, pre);
-        // the densu quantities every species shares: formed once (wave 0, with
-        // the N2 turbopause density) and handed to the other waves
         nrlmsise::DensuShared h;
         if (wave == 0) {
+            // the densu quantities every species shares, formed once (with the N2
+            // turbopause density) and handed to the other waves; then N2
             nrlmsise::gts7SpeciesShare(h, true, st, T, in, fl, G, pre);
             int i = 0;
             densuSharedFields(h, [&](auto& x) { hx[i++][lane] = double(x); });
             hx[kDensuSharedFields][lane] = st.meso_tn1[0];
             hx[kDensuSharedFields + 1][lane] = st.meso_tgn1[0];
+            spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
+        } else {
+            // meanwhile each other wave evaluates its species' globe7 table; the
+            // anomalous oxygen (its own densu profile) needs nothing else
+            nrlmsise::State sb;
+            F.basis(sb, in, fl);
+            const int j = k == 7 ? 10 : nrlmsise::kSpeciesPar[k].globe;
+            double apt = 0.0;
+            gv[j][lane] = F.value(sb, j, nrlmsise::globeTable(T, j), in, apt);
+            if (k == 7) spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
         }
         __syncthreads();
-        if (wave != 0) {
+        if (wave != 0 && k != 7) {
             int i = 0;
             densuSharedFields(h, [&](auto& x) {
                 using T = typename std::remove_reference<decltype(x)>::type;
@@ -1337,9 +1349,8 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
             });
             st.meso_tn1[0] = hx[kDensuSharedFields][lane];
             st.meso_tgn1[0] = hx[kDensuSharedFields + 1][lane];
+            spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
         }
-        const int k = kSpeciesOrder[wave];
-        spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
     }
     __syncthreads();
     if (wave != 0 || nl.state == 0) return;
