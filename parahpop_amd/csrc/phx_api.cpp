@@ -644,7 +644,9 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
     int launches = 0;
     int kernelsPerStep = 0;
     int blocksFor = W;   // wave count the order blocks in `a` were split for
-    const bool split = c.split && !b.grid && c.fm.drag && c.fm.hasSh;
+    // split stages: shared-epoch batches with drag, and per-lane epochs with the
+    // eight-wave drag kernel
+    const bool split = c.split && c.fm.drag && c.fm.hasSh && (!b.grid || c.splitSpecies == 8);
     if (split) {
         const size_t np = size_t(b.nPad);
         if (!b.ks.p) {
@@ -652,7 +654,7 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
             b.ysBuf.alloc(7 * np);
             b.parts.alloc(6 * np);
             b.spec.alloc(8 * np);
-            b.lane.alloc(5 * np);
+            b.lane.alloc(8 * np);
         }
         const int Ws = c.splitWaves;
         orderBlocks(c, Ws, a.blockStart, true);
@@ -949,9 +951,10 @@ int phx_batch_autotune(phx_batch* b, int trialSteps, phx_stats* best) {
                     if (c.fm.drag && w > 4) continue;
                     cands.push_back({w, s0, 0, c.splitWaves, c.splitSpecies});
                 }
-            if (c.fm.drag && !b->grid)
+            if (c.fm.drag)
                 for (int sw : {8, 16})
-                    for (int ss : {1, 8}) cands.push_back({c.nWaves, c.share0, 1, sw, ss});
+                    for (int ss : {1, 8})
+                        if (!b->grid || ss == 8) cands.push_back({c.nWaves, c.share0, 1, sw, ss});
         } else {
             cands.push_back({1, 1.0, 0, c.splitWaves, c.splitSpecies});
         }

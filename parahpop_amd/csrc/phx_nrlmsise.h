@@ -1506,16 +1506,20 @@ struct SlotGlobe {
     __device__ __attribute__((always_inline)) void basis(State& st, const Input& in, const Flags&) const {
         fastBasis(st, in, S[SLOT_DFA]);
     }
-    __device__ __attribute__((always_inline)) double operator()(State& st, int j, const double* p, const Input& in,
-                                                               const Flags&) const {
+    // globe7 of table j without side effects; apt0 is the table's Ap function
+    __device__ __attribute__((always_inline)) double value(const State& st, int j, const double* p, const Input& in,
+                                                          double& apt0) const {
         const double* e = S + GS_STRIDE * j;
         const double ut71 = cs * e[GS_C71] + ss * e[GS_S71];
         const double ut79c = cs * e[GS_C79] + ss * e[GS_S79];
         const double ut79s = ss * e[GS_C79] - cs * e[GS_S79];
         const double ut58 = cs * e[GS_C58] + ss * e[GS_S58];
+        return globe7Fast(st, (NrlCD)p, e, S[SLOT_DFA], in.g_lat, ut71, ut79c, ut79s, ut58, (NrlCD)(T + GT_COUNT * j), apt0);
+    }
+    __device__ __attribute__((always_inline)) double operator()(State& st, int j, const double* p, const Input& in,
+                                                               const Flags&) const {
         double apt0 = 0.0;
-        const double v = globe7Fast(st, (NrlCD)p, e, S[SLOT_DFA], in.g_lat, ut71, ut79c, ut79s, ut58, (NrlCD)(T + GT_COUNT * j),
-                                    apt0);
+        const double v = value(st, j, p, in, apt0);
         if (p[51] != 0) st.apt[0] = apt0;
         return v;
     }

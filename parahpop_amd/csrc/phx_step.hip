@@ -508,6 +508,46 @@ __device__ inline NrlLane nrlLaneLoad(const double* lane, int nPad, int obj, con
     return L;
 }
 
+// This lane's NRLMSISE-00 time inputs (in the epoch record's REC_NRL block
+// layout) and weather-slot row at its stage epoch ts, for per-lane epochs (the
+// fused kernel's statements); returns the row index, sec is the UT second.
+__device__ inline int laneNrlInputs(const StepArgs& a, double ts, double (&nrl)[13], double& sec) {
+    EpochArgs wa{};
+    wa.weather = a.weather;
+    wa.wDays = a.wDays;
+    wa.wStart = a.wStart;
+    const double mjd = epochdev::etToMjdUtcNrl(ts);
+    int doy, record, wslot;
+    epochdev::doySecOf(mjd, doy, sec);
+    nrl[REC_NRL_DOY - REC_NRL] = double(doy);
+    nrl[REC_NRL_SEC - REC_NRL] = sec;
+    nrl[REC_NRL_GAST - REC_NRL] = epochdev::gastHours(mjd);
+    const bool ok = epochdev::weatherSlotOf(wa, mjd, record, wslot) && a.nrlSlots;
+    const int row = ok ? record * 8 + wslot : 0;
+    const double* r = a.nrlSlots + size_t(row) * nrlmsise::SLOT_SIZE;
+    for (int k = 0; k < 9; ++k) nrl[REC_NRL_F107 - REC_NRL + k] = ok ? r[nrlmsise::SLOT_F107 + k] : nan("");
+    nrl[REC_NRL_VALID - REC_NRL] = ok ? 1.0 : 0.0;
+    return row;
+}
+
+// Per-lane epochs: the lane's UT second, day of year and weather-slot row after
+// the five nrlLaneStore fields, and nrlLane's result rebuilt from them.
+__device__ inline void nrlLaneStoreGrid(double sec, int doy, int row, double* lane, int nPad, int obj) {
+    lane[size_t(5) * nPad + obj] = sec;
+    lane[size_t(6) * nPad + obj] = double(doy);
+    lane[size_t(7) * nPad + obj] = double(row);
+}
+
+__device__ inline NrlLane nrlLaneLoadGrid(const double* lane, int nPad, int obj, const double* slots, int& row,
+                                          double (&nrl)[13]) {
+    row = int(lane[size_t(7) * nPad + obj]);
+    nrl[REC_NRL_DOY - REC_NRL] = lane[size_t(6) * nPad + obj];
+    nrl[REC_NRL_SEC - REC_NRL] = lane[size_t(5) * nPad + obj];
+    for (int k = 0; k < 9; ++k)
+        nrl[REC_NRL_F107 - REC_NRL + k] = slots ? slots[size_t(row) * nrlmsise::SLOT_SIZE + nrlmsise::SLOT_F107 + k] : nan("");
+    return nrlLaneLoad(lane, nPad, obj, nrl);
+}
+
 __device__ inline nrlmsise::Flags nrlFlags() {
     nrlmsise::Flags fl;
     for (int k = 0; k < 24; ++k) fl.switches[k] = 1;
@@ -860,7 +900,7 @@ template <int W, class Rec>
 __device__ inline void stageForcesParts(const StepArgs& a, const ForceModel& fm, const Rec& rec, int wave, int lane, int m0,
                                         int m1, double ams, double (*xch)[kTile], double (*part)[3][kTile],
                                         double (*extra)[3][kTile], double (*pg)[kTile], double acc[3], double ax[3],
-                                        double ex[3], int tile, double* laneOut) {
+                                        double ex[3]) {
     static_assert(W >= 4, "the split kernel spreads the extra families over waves 1 .. 3");
     const double px = xch[0][lane], py = xch[1][lane], pz = xch[2][lane];
     // the families besides SH, each on a wave of its own before the barrier:
@@ -883,9 +923,6 @@ __device__ inline void stageForcesParts(const StepArgs& a, const ForceModel& fm,
         extra[0][0][lane] = bx;
         extra[0][1][lane] = by;
         extra[0][2][lane] = bz;
-    } else if (wave == W - 4 && fm.drag && laneOut) {
-        // the NRLMSISE-00 lane inputs of this stage for the drag kernels
-        nrlLaneStore(nrlLane(fm, rec, rec.p + REC_NRL, px, py, pz), laneOut, a.nPad, tile * kTile + lane);
     }
     ShGeom g;
     if (fm.hasSh) {
@@ -1116,7 +1153,7 @@ __device__ __attribute__((always_inline)) inline double stageComponent(const Ste
 // Stage s of the split path for one tile: the stage state, then the forces
 // without drag with the SH order blocks over W waves; k_s holds them until the
 // finish kernel adds the drag.
-template <int W>
+template <int W, bool GRID>
 __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
     __shared__ double xch[7][kTile];
     __shared__ double part[W][3][kTile];
@@ -1144,13 +1181,28 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
         sp.ys[size_t(d) * nPad + obj] = v;
     }
     __syncthreads();
-    const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
-    const ExactRec rec{recp};
-    const NrlSlot noSlot{nullptr, 0.0, 0.0};
     const int m0 = a.blockStart[wave], m1 = a.blockStart[wave + 1];
     double acc[3], ax[3], ex[3];
-    stageForcesParts<W>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex, tile, sp.lane);
-    (void)noSlot;
+    const double px = xch[0][lane], py = xch[1][lane], pz = xch[2][lane];
+    if (GRID) {
+        // per-lane epochs: records interpolated at the lane's stage epoch
+        const double ts = xch[6][lane];
+        const GridRec rec = gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH, ts);
+        if (fm.drag && wave == W - 4) {
+            // the NRLMSISE-00 lane inputs of this stage for the drag kernel
+            double nrl[13], sec;
+            const int row = laneNrlInputs(a, ts, nrl, sec);
+            const NrlLane L = nrlLane(fm, rec, nrl, px, py, pz);
+            nrlLaneStore(L, sp.lane, nPad, obj);
+            nrlLaneStoreGrid(sec, L.in.doy, row, sp.lane, nPad, obj);
+        }
+        stageForcesParts<W>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex);
+    } else {
+        const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
+        const ExactRec rec{recp};
+        if (fm.drag && wave == W - 4) nrlLaneStore(nrlLane(fm, rec, recp + REC_NRL, px, py, pz), sp.lane, nPad, obj);
+        stageForcesParts<W>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex);
+    }
     if (wave == 0) {
         for (int d = 0; d < 3; ++d) {
             sp.ks[ksAt(s, 3 + d, nPad, obj)] = acc[d];
@@ -1267,6 +1319,7 @@ struct SharedGlobe {
 // and one species (kSpeciesOrder[wave]); wave 0 then assembles the density
 // and the drag, k_s = ax + (extra + drag), as the fused kernel. Every value is
 // the one the serial evaluation forms.
+template <bool GRID>
 __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
     __shared__ double gv[19][kTile];
     __shared__ double spec[8][kTile];
@@ -1277,16 +1330,27 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
-    const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
-    const ExactRec rec{recp};
-    const double* nrl = recp + REC_NRL;
-    const NrlLane nl = nrlLaneLoad(sp.lane, nPad, obj, nrl);
+    // shared epochs: the tile's record; per-lane epochs: the lane's time inputs
+    // and weather-slot row (the globe terms of its slot)
+    const double* recp = GRID ? nullptr : a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
+    double nrlLocal[13];
+    int row = 0;
+    const double* nrl = GRID ? nrlLocal : recp + REC_NRL;
+    const NrlLane nl = GRID ? nrlLaneLoadGrid(sp.lane, nPad, obj, a.nrlSlots, row, nrlLocal) : nrlLaneLoad(sp.lane, nPad, obj, nrl);
+    using Globe = typename std::conditional<GRID, nrlmsise::SlotGlobe, nrlmsise::FastGlobe>::type;
     // the same lanes in every wave: the branch is uniform across the workgroup
     if (__ballot(nl.state == 1) != 0ull) {
         const nrlmsise::Input in0 = nl.in;
         const nrlmsise::Flags fl = nrlFlags();
         const nrlmsise::NrlTables T{a.nrlCoeff};
-        const nrlmsise::FastGlobe F{nrl + (REC_NRL_GLOBE - REC_NRL), kGlobeTrig};
+        Globe F;
+        if constexpr (GRID) {
+            const double sr = 7.2722E-5;
+            const double sec = nrl[REC_NRL_SEC - REC_NRL];
+            F = nrlmsise::SlotGlobe{a.nrlSlots + size_t(row) * nrlmsise::SLOT_SIZE, kGlobeTrig, cos(sr * sec), sin(sr * sec)};
+        } else {
+            F = nrlmsise::FastGlobe{nrl + (REC_NRL_GLOBE - REC_NRL), kGlobeTrig};
+        }
         const int k = kSpeciesOrder[wave];
         {
             // globe7 of the prefix tables pt, ps, pd[3] and of pd[2] (N2), one per
@@ -1356,11 +1420,19 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
     if (wave != 0 || nl.state == 0) return;
     double rho;
     if (nl.state == 1) rho = nrlRhoFromSpecies(fm, spec, lane);
-    else if (nl.state == 2) rho = nrlDensity<1>(fm.densityScale, nrl, a.nrlCoeff, nl.in);
+    else if (nl.state == 2) rho = nrlDensity<GRID ? 2 : 1>(fm.densityScale, nrl, a.nrlCoeff, nl.in);
     else rho = nan("");
     const double px = sp.ys[obj], py = sp.ys[size_t(nPad) + obj], pz = sp.ys[size_t(2) * nPad + obj];
     const double vx = sp.ys[size_t(3) * nPad + obj], vy = sp.ys[size_t(4) * nPad + obj], vz = sp.ys[size_t(5) * nPad + obj];
-    const double om0 = rec(REC_OMEGA), om1 = rec(REC_OMEGA + 1), om2 = rec(REC_OMEGA + 2);
+    double om0, om1, om2;
+    if constexpr (GRID) {
+        const GridRec rec = gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH,
+                                    sp.ys[size_t(6) * nPad + obj]);
+        om0 = rec(REC_OMEGA), om1 = rec(REC_OMEGA + 1), om2 = rec(REC_OMEGA + 2);
+    } else {
+        const ExactRec rec{recp};
+        om0 = rec(REC_OMEGA), om1 = rec(REC_OMEGA + 1), om2 = rec(REC_OMEGA + 2);
+    }
     const double rx = vx - (om1 * pz - om2 * py);
     const double ry = vy - (om2 * px - om0 * pz);
     const double rz = vz - (om0 * py - om1 * px);
@@ -1589,9 +1661,11 @@ int launchSplitStage(const StepArgs& a, const ForceModel& fm, const SplitArgs& s
     int rc = launchSplitSh(a, fm, sp, shWaves, stream);
     if (rc != 0 || a.nTiles <= 0 || !fm.drag) return rc;
     if (sp.speciesPerGroup == 8) {
-        hipLaunchKernelGGL(splitDragTileKernel, dim3(a.nTiles), dim3(512), 0, stream, a, fm, sp);
+        if (a.grid) hipLaunchKernelGGL((splitDragTileKernel<true>), dim3(a.nTiles), dim3(512), 0, stream, a, fm, sp);
+        else hipLaunchKernelGGL((splitDragTileKernel<false>), dim3(a.nTiles), dim3(512), 0, stream, a, fm, sp);
         return hipGetLastError() == hipSuccess ? 0 : -1;
     }
+    if (a.grid) return -2;   // the per-species and four-wave drag kernels read shared-epoch records
     if (sp.speciesPerGroup != 0) rc = launchSplitSpecies(a, fm, sp, stream);
     if (rc == 0) rc = launchSplitFinish(a, fm, sp, stream);
     return rc;
@@ -1600,11 +1674,20 @@ int launchSplitStage(const StepArgs& a, const ForceModel& fm, const SplitArgs& s
 static int launchSplitSh(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, int shWaves, hipStream_t stream) {
     if (a.nTiles <= 0) return 0;
     const dim3 grid(a.nTiles);
-    switch (shWaves) {
-        case 4: hipLaunchKernelGGL((splitShKernel<4>), grid, dim3(256), 0, stream, a, fm, sp); break;
-        case 8: hipLaunchKernelGGL((splitShKernel<8>), grid, dim3(512), 0, stream, a, fm, sp); break;
-        case 16: hipLaunchKernelGGL((splitShKernel<16>), grid, dim3(1024), 0, stream, a, fm, sp); break;
-        default: return -2;
+    if (a.grid) {
+        switch (shWaves) {
+            case 4: hipLaunchKernelGGL((splitShKernel<4, true>), grid, dim3(256), 0, stream, a, fm, sp); break;
+            case 8: hipLaunchKernelGGL((splitShKernel<8, true>), grid, dim3(512), 0, stream, a, fm, sp); break;
+            case 16: hipLaunchKernelGGL((splitShKernel<16, true>), grid, dim3(1024), 0, stream, a, fm, sp); break;
+            default: return -2;
+        }
+    } else {
+        switch (shWaves) {
+            case 4: hipLaunchKernelGGL((splitShKernel<4, false>), grid, dim3(256), 0, stream, a, fm, sp); break;
+            case 8: hipLaunchKernelGGL((splitShKernel<8, false>), grid, dim3(512), 0, stream, a, fm, sp); break;
+            case 16: hipLaunchKernelGGL((splitShKernel<16, false>), grid, dim3(1024), 0, stream, a, fm, sp); break;
+            default: return -2;
+        }
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

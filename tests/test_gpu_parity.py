@@ -274,7 +274,8 @@ def test_low_altitude_drag_paths_match_oracle(full_workload):
     low["state"] = np.concatenate([u * r, w * np.sqrt(398600.4415 / r)])
     low["n"] = n
     e_o, s_o, _ = _oracle(cfg).propagate(low, 600.0, threads=8)
-    for opt in ({}, {"split_species": 1}, {"split_species": 8}, {"split": 0}, {"force_grid": 1}):
+    for opt in ({}, {"split_species": 1}, {"split_species": 8}, {"split": 0}, {"force_grid": 1},
+                {"force_grid": 1, "split": 0}):
         e_g, s_g, st = _ctx(cfg, **opt).propagate(low, 600.0)
         np.testing.assert_array_equal(e_g, e_o)
         if not opt:
@@ -282,3 +283,22 @@ def test_low_altitude_drag_paths_match_oracle(full_workload):
         # ten minutes of drag at 150 km: the interpolated records stay within
         # the grid test's micrometre level, everything within 1 mm / 1 um/s
         _compare(s_g, s_o)
+
+
+def test_split_stages_per_lane_epochs_match_fused_kernel(full_workload):
+    """Per-lane (interpolated) epoch records: the split stages with the
+    eight-wave drag kernel against the fused kernel, bit for bit at equal order
+    blocks (four waves), within 1e-12 relative at sixteen waves."""
+    cfg, t = full_workload
+    t = dict(t)
+    t["epoch"] = t["epoch"] + np.arange(t["n"]) * 0.37 / 86400.0   # distinct epochs
+    fused = _ctx(cfg, force_grid=1, split=0, waves=4).propagate(t, 1800.0)
+    for sw in (4, 16):
+        e, s, st = _ctx(cfg, force_grid=1, split=1, split_waves=sw, split_species=8).propagate(t, 1800.0)
+        assert st["kernels_per_step"] == 27
+        np.testing.assert_array_equal(e, fused[0])
+        if sw == 4:
+            np.testing.assert_array_equal(s, fused[1])
+        else:
+            rel = np.max(np.linalg.norm(s[:3] - fused[1][:3], axis=0) / np.linalg.norm(fused[1][:3], axis=0))
+            assert rel < 1e-12
