@@ -306,6 +306,20 @@ void Server::tick() {
     int W = 1;
     StepArgs a = contextStepArgs(c_, W);
     const int K = std::max(1, c_.serveStepsPerTick);
+    // split stages (per-lane epochs with the eight-wave drag kernel) when the
+    // context selects them, as for device batches
+    const bool split = c_.split && c_.fm.drag && c_.fm.hasSh && c_.splitSpecies == 8;
+    if (split) {
+        W = c_.splitWaves;
+        orderBlocks(c_, W, a.blockStart, true);
+        if (!ks_.p) {
+            const size_t S = size_t(slots_);
+            ks_.alloc(size_t(kStages) * 6 * S);
+            ys_.alloc(7 * S);
+            parts_.alloc(6 * S);
+            lane_.alloc(8 * S);
+        }
+    }
     a.nTiles = activeTiles_;
     a.nPad = int(slots_);
     a.y = y_.p;
@@ -326,13 +340,27 @@ void Server::tick() {
     a.slotRows = slotRows_.p;
     a.stepBase = 0;
     a.steps = K;
-    if (graphTiles_ != activeTiles_ || graphWaves_ != W || graphSteps_ != K || !exec_) {
+    if (graphTiles_ != activeTiles_ || graphWaves_ != W || graphSteps_ != K || graphSplit_ != int(split) || !exec_) {
         if (exec_) check(hipGraphExecDestroy(exec_), "hipGraphExecDestroy");
         if (graph_) check(hipGraphDestroy(graph_), "hipGraphDestroy");
         exec_ = nullptr;
         graph_ = nullptr;
         check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        const int rc = launchStep(a, c_.fm, W, stream_);
+        int rc = 0;
+        if (split) {
+            // K steps of 13 stages (SH kernel, drag kernel) and the end-of-step kernel
+            SplitArgs sp{ks_.p, ys_.p, parts_.p, nullptr, lane_.p, 0, 0, 8};
+            for (int k = 0; k < K && rc == 0; ++k) {
+                sp.step = k;
+                for (int s = 0; s < kStages && rc == 0; ++s) {
+                    sp.stage = s;
+                    rc = launchSplitStage(a, c_.fm, sp, W, stream_);
+                }
+                if (rc == 0) rc = launchSplitCombine(a, sp, stream_);
+            }
+        } else {
+            rc = launchStep(a, c_.fm, W, stream_);
+        }
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(stream_, &g);
         if (rc != 0 || e != hipSuccess) {
@@ -344,6 +372,7 @@ void Server::tick() {
         graphTiles_ = activeTiles_;
         graphWaves_ = W;
         graphSteps_ = K;
+        graphSplit_ = int(split);
         std::lock_guard<std::mutex> lk(mu_);
         ++captures_;
     }

@@ -106,7 +106,9 @@ private:
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
-    int graphTiles_ = -1, graphWaves_ = -1, graphSteps_ = -1;
+    int graphTiles_ = -1, graphWaves_ = -1, graphSteps_ = -1, graphSplit_ = -1;
+    // split-stage buffers over the pool (allocated on first use)
+    DBuf<double> ks_, ys_, parts_, lane_;
 
     // shared with callers
     mutable std::mutex mu_;

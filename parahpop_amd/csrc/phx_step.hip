@@ -1161,7 +1161,7 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
     __shared__ double pg[3][kTile];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int tile = int(blockIdx.x);
+    const int tile = __builtin_amdgcn_readfirstlane(a.tileList ? a.tileList[blockIdx.x] : int(blockIdx.x));
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
@@ -1326,7 +1326,7 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
     __shared__ double hx[kDensuSharedFields + 2][kTile];   // wave 0's densu quantities and lower-boundary node
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int tile = int(blockIdx.x);
+    const int tile = __builtin_amdgcn_readfirstlane(a.tileList ? a.tileList[blockIdx.x] : int(blockIdx.x));
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
@@ -1478,7 +1478,8 @@ __global__ void __launch_bounds__(64) splitDragFinishKernel(StepArgs a, ForceMod
 // End of step: y += sum (dt b_s) k_s and the bookkeeping of FixedStep.h:30-37.
 __global__ void __launch_bounds__(64) splitCombineKernel(StepArgs a, SplitArgs sp) {
     const int lane = threadIdx.x & 63;
-    const int obj = int(blockIdx.x) * kTile + lane;
+    const int tile = __builtin_amdgcn_readfirstlane(a.tileList ? a.tileList[blockIdx.x] : int(blockIdx.x));
+    const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     uint8_t st = a.status[obj];
     unsigned long long completed = 0;
@@ -1665,7 +1666,9 @@ int launchSplitStage(const StepArgs& a, const ForceModel& fm, const SplitArgs& s
         else hipLaunchKernelGGL((splitDragTileKernel<false>), dim3(a.nTiles), dim3(512), 0, stream, a, fm, sp);
         return hipGetLastError() == hipSuccess ? 0 : -1;
     }
-    if (a.grid) return -2;   // the per-species and four-wave drag kernels read shared-epoch records
+    // the per-species and four-wave drag kernels read shared-epoch records of
+    // contiguous tiles
+    if (a.grid || a.tileList) return -2;
     if (sp.speciesPerGroup != 0) rc = launchSplitSpecies(a, fm, sp, stream);
     if (rc == 0) rc = launchSplitFinish(a, fm, sp, stream);
     return rc;
