@@ -94,9 +94,10 @@ void phx_batch_destroy(phx_batch* b);
 
 /* Tuning options (no reference counterpart; the reference reads
  * PARAHPOP_PERF_* environment overrides, AccumulateBodies.cu:20-48):
- *   "split"            1 (default): shared-epoch batches with drag run as split
- *                      stages (SH, species and finish kernels per stage, k-stages
- *                      in device memory); 0: the fused multi-step kernel
+ *   "split"            1 (default): batches and serving with drag run as split
+ *                      stages (SH kernel and drag kernel per stage, k-stages in
+ *                      device memory; per-lane epochs need split_species 8);
+ *                      0: the fused multi-step kernel
  *   "split_waves"      4, 8 or 16 (default) waves per tile of the split SH kernel
  *   "split_species"    8 (default): one eight-wave drag workgroup per tile (the
  *                      species and the drag); 1, 2 or 4: species per workgroup and
