@@ -36,6 +36,10 @@ def library():
         lib.orc_nrlmsise.argtypes = [ctypes.c_char_p, i] + [d] * 8 + [vp, i, vp, vp]
         lib.orc_body_position.argtypes = [i, d, vp]
         lib.orc_earth_orientation.argtypes = [d, vp, vp]
+        lib.orc_geodetic_altitude.argtypes = [vp, vp, d, d]
+        lib.orc_geodetic_altitude.restype = d
+        lib.orc_exp_density.argtypes = [vp, d]
+        lib.orc_exp_density.restype = d
         lib.orc_last_error.restype = ctypes.c_char_p
         lib.orc_destroy.argtypes = [vp]
         lib.orc_destroy.restype = None
@@ -83,6 +87,15 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(self.lib.orc_last_error().decode())
         return out
+
+    def exp_density(self, h):
+        return self.lib.orc_exp_density(self.h, float(h))
+
+
+def geodetic_altitude(r, pole, req, flattening):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    pole = np.ascontiguousarray(pole, dtype=np.float64)
+    return library().orc_geodetic_altitude(_p(r), _p(pole), float(req), float(flattening))
 
 
 def nrlmsise(doy, sec, alt, lat, lon, lst, f107a, f107, ap, apa=None):

@@ -196,6 +196,8 @@ V3 acceleration(const orc_ctx& c, V3 pos, V3 vel, double et, const Params& p, in
         }
         if (b.shape && c.hasSh && (only < 0 || only == 1))
             acc = acc + sphericalHarmonics(pos, orientation().R, c.sh);
+        else if (b.oblateness && !b.shape && (only < 0 || only == 1))
+            acc = acc + j2Gravity(pos, orientation().R, b.gm, b.j2, b.radius, b.soi);
         if (b.radiation && (only < 0 || only == 3)) {
             const V3 d = pos - bodyPos;
             if (!c.anyOcculter) {
@@ -220,6 +222,21 @@ V3 acceleration(const orc_ctx& c, V3 pos, V3 vel, double et, const Params& p, in
             const double bc = p.cd * p.area / p.mass;
             const double factor = -0.5 * rho * bc * norm(vrel);
             acc = acc + V3{factor * vrel.x, factor * vrel.y, factor * vrel.z};
+        } else if (b.atmosphere && !b.expSegments.empty() && (only < 0 || only == 4)) {
+            // PrivateAccelerations.h:167-202: the pole is the body z axis in ICRF
+            const Orientation& o = orientation();
+            const V3 pole{o.R.m[6], o.R.m[7], o.R.m[8]};
+            const double h = geodeticAltitude(pos, pole, b.radius, b.flattening);
+            const phx::ExpSegment* bands = b.expSegments.data();
+            const int n = int(b.expSegments.size());
+            const int band = expBand(bands, n, h);
+            if (h <= bands[band].hCutoff) {
+                const V3 vrel = vel - cross(o.omega, pos);
+                const double rho = expDensity(bands, n, h);
+                const double bc = p.cd * p.area / p.mass;
+                const double factor = -0.5 * rho * bc * norm(vrel);
+                acc = acc + V3{factor * vrel.x, factor * vrel.y, factor * vrel.z};
+            }
         }
     }
     if (m.anyCorrections() && (only < 0 || only == 5)) {
@@ -386,4 +403,15 @@ int orc_earth_orientation(double et, double* R9, double* omega3) {
     for (int i = 0; i < 9; ++i) R9[i] = o.R.m[i];
     omega3[0] = o.omega.x; omega3[1] = o.omega.y; omega3[2] = o.omega.z;
     return 0;
+}
+
+double orc_geodetic_altitude(const double* r3, const double* pole3, double Req, double f) {
+    return geodeticAltitude(V3{r3[0], r3[1], r3[2]}, V3{pole3[0], pole3[1], pole3[2]}, Req, f);
+}
+
+double orc_exp_density(const orc_ctx* ctx, double h) {
+    for (const auto& b : ctx->env.model.bodies)
+        if (b.naif == phx::NAIF_EARTH && b.atmosphere && !b.expSegments.empty())
+            return expDensity(b.expSegments.data(), int(b.expSegments.size()), h);
+    return std::nan("");
 }

@@ -42,6 +42,12 @@ int orc_nrlmsise(const char* data_dir, int doy, double sec, double alt, double g
  * with zero polar motion and UT1 = UTC = TT - 69.184 s. */
 int orc_body_position(int naif, double et, double* pos3);
 int orc_earth_orientation(double et, double* R9, double* omega3);
+/* Geodetic altitude (km) of r above the ellipsoid (equatorial radius Req km,
+ * flattening f) whose polar axis is the unit vector pole, r and pole in one frame. */
+double orc_geodetic_altitude(const double* r3, const double* pole3, double Req, double f);
+/* Density (kg/km^3) of the configured Earth exponential atmosphere at geodetic
+ * altitude h km; NaN when the configuration has none. */
+double orc_exp_density(const orc_ctx* ctx, double h);
 double orc_step_seconds(const orc_ctx* ctx);
 double orc_duration_seconds(const orc_ctx* ctx);
 const char* orc_last_error(void);

@@ -413,6 +413,65 @@ inline V3 sphericalHarmonics(V3 pos, const M3& R, const ShTables& f) {
 }
 
 // ---------------------------------------------------------------------------
+// J2-only gravity — accelerations/J2.h:17-92 (Wakker eq. 20.6): skipped beyond
+// the body's sphere of influence, evaluated in body-fixed axes.
+// ---------------------------------------------------------------------------
+inline V3 j2BodyFixed(V3 p, double gm, double j2, double radius) {
+    const double invr = 1.0 / std::sqrt(squaredNorm(p));
+    const double invr2 = invr * invr;
+    const double invr5 = invr2 * invr2 * invr;
+    const double v = 1.0 - 5.0 * (p.z * p.z) * invr2;
+    const double c = -1.5 * gm * j2 * invr5 * radius * radius;
+    return {c * p.x * v, c * p.y * v, c * p.z * (v + 2.0)};
+}
+
+inline V3 j2Gravity(V3 pos, const M3& R, double gm, double j2, double radius, double soi) {
+    if (norm(pos) > soi) return V3{};
+    return mulT(R, j2BodyFixed(mul(R, pos), gm, j2, radius));
+}
+
+// ---------------------------------------------------------------------------
+// Exponential atmosphere — environment/atmosphere/Geodetic.h:40-72 (geodetic
+// altitude by fixed-point iteration on the polar coordinate; ||r|| - R_eq when
+// the flattening is 0) and PiecewiseExponentialAtmosphere.h:100-170 (band =
+// number of bands s >= 1 with h0_s <= h; zero above the band's cutoff).
+// The iteration is capped at kGeodeticMaxIter (it converges in 2-4 steps).
+// ---------------------------------------------------------------------------
+constexpr int kGeodeticMaxIter = 32;
+
+inline double geodeticAltitude(V3 r, V3 pole, double Req, double f) {
+    if (f == 0.0) return norm(r) - Req;
+    const double r2 = dot(r, r);
+    const double z = dot(r, pole);
+    const double xy = r2 - z * z;
+    const double e2 = (2.0 - f) * f;
+    double d0 = 0.0, d = 1e10, sfq = 0.0, zd = z;
+    for (int it = 0; it < kGeodeticMaxIter && std::fabs(d - d0) > 1e-9; ++it) {
+        d0 = d;
+        const double z2 = zd * zd;
+        sfq = z2 / (xy + z2);
+        d = Req * e2 * std::sqrt(sfq / (1.0 - e2 * sfq));
+        zd = z + std::copysign(d, z);
+    }
+    return std::sqrt(xy + zd * zd) - Req / std::sqrt(1.0 - e2 * sfq);
+}
+
+// S: a band with fields rho0, h0, scaleHeight, hCutoff
+template <class S>
+inline int expBand(const S* b, int n, double h) {
+    int band = 0;
+    for (int s = 1; s < n; ++s) band += b[s].h0 <= h ? 1 : 0;
+    return band;
+}
+
+template <class S>
+inline double expDensity(const S* b, int n, double h) {
+    const S& s = b[expBand(b, n, h)];
+    if (h > s.hCutoff) return 0.0;
+    return s.rho0 * std::exp(-(h - s.h0) / s.scaleHeight);
+}
+
+// ---------------------------------------------------------------------------
 // SRP and dual-cone shadow — accelerations/SRP.h:18,62-69;
 // environment/eclipse/Shadow.h:84-142
 // ---------------------------------------------------------------------------

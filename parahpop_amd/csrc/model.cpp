@@ -6,6 +6,9 @@
 //   CelesTrak SW-All                   model/environment/atmosphere/Nrlmsise00Atmosphere.h:146-236
 //   HPOP EOP-All                       model/environment/earthcorrections/EarthCorrections.h:141-177
 //   constants.json (SOIOld -> soi)     third_party/brie/brie/constants/ConstantsJson.h:51-74
+//   gravity.oblateness, inline j2      interface/config/model/environment/bodies/BodyConfig.h:440-515,
+//                                      interface/config/model/environment/Bodies.h:700-750
+//   atmosphere.exponential blocks      interface/config/model/environment/Atmosphere.h:130-160,326-350,430-510
 #include "model.h"
 
 #include <algorithm>
@@ -129,8 +132,7 @@ void parseGravity(BodySpec& b, const json::Value& g, const std::string& base) {
                 b.shape = activationValue(kv.second);
                 if (b.shape) parseShape(b, kv.second, base);
             } else if (k == "oblateness") {
-                if (activationValue(kv.second))
-                    throw std::runtime_error("J2-only gravity (oblateness) is not provided; configure gravity.shape.");
+                b.oblateness = activationValue(kv.second);
             } else {
                 throw std::runtime_error("Unsupported gravity feature: " + kv.first);
             }
@@ -138,6 +140,7 @@ void parseGravity(BodySpec& b, const json::Value& g, const std::string& base) {
     } else if (g.isArray()) {
         for (const auto& item : *g.arr) {
             const std::string k = lower(item.asString());
+            if (k == "oblateness") { b.oblateness = true; continue; }
             if (k == "shape") throw std::runtime_error("gravity shape requires a coefficient-file object.");
             throw std::runtime_error("Unsupported gravity feature: " + item.str);
         }
@@ -146,17 +149,66 @@ void parseGravity(BodySpec& b, const json::Value& g, const std::string& base) {
     }
 }
 
+// One {rho0, h0, scaleHeight, hCutoff} block: all four required, no other key
+// (Atmosphere.h:461-505).
+ExpSegment parseExpSegment(const json::Value& v) {
+    if (!v.isObject()) throw std::runtime_error("Each exponential atmosphere block must be an object.");
+    ExpSegment s;
+    bool rho0 = false, h0 = false, scale = false, cutoff = false;
+    for (const auto& kv : *v.obj) {
+        const std::string k = lower(kv.first);
+        if (k == "rho0") { s.rho0 = kv.second.asDouble(); rho0 = true; }
+        else if (k == "h0") { s.h0 = kv.second.asDouble(); h0 = true; }
+        else if (k == "scaleheight") { s.scaleHeight = kv.second.asDouble(); scale = true; }
+        else if (k == "hcutoff") { s.hCutoff = kv.second.asDouble(); cutoff = true; }
+        else throw std::runtime_error("Invalid atmosphere field: '" + kv.first +
+                                      "'. Valid fields are: [h0, hcutoff, rho0, scaleheight] (case insensitive)");
+    }
+    if (!rho0) throw std::runtime_error("Atmosphere rho0 not specified.");
+    if (!h0) throw std::runtime_error("Atmosphere h0 not specified.");
+    if (!scale) throw std::runtime_error("Atmosphere scaleHeight not specified.");
+    if (!cutoff) throw std::runtime_error("Atmosphere hCutoff not specified.");
+    if (!(s.rho0 >= 0.0)) throw std::runtime_error("Atmosphere rho0 must be non-negative.");
+    if (!(s.scaleHeight > 0.0)) throw std::runtime_error("Atmosphere scaleHeight must be positive.");
+    return s;
+}
+
+// A single block or a list of blocks, sorted ascending by h0; a band may not
+// start below the previous band's cutoff (a gap reads as vacuum) (Atmosphere.h:326-350,430-459).
+void parseExponential(BodySpec& b, const json::Value& v) {
+    std::vector<ExpSegment> segs;
+    if (v.isArray()) {
+        if (v.size() == 0) throw std::runtime_error("Piecewise atmosphere list must contain at least one block.");
+        for (const auto& e : *v.arr) segs.push_back(parseExpSegment(e));
+    } else if (v.isObject()) {
+        segs.push_back(parseExpSegment(v));
+    } else {
+        throw std::runtime_error("Exponential atmosphere must be an object (single block) or a list of objects (piecewise).");
+    }
+    std::stable_sort(segs.begin(), segs.end(), [](const ExpSegment& x, const ExpSegment& y) { return x.h0 < y.h0; });
+    for (size_t i = 1; i < segs.size(); ++i)
+        if (!(segs[i].h0 >= segs[i - 1].hCutoff))
+            throw std::runtime_error("Overlapping atmosphere segments: a band with h0=" + std::to_string(segs[i].h0) +
+                                     " km starts below the previous band's hCutoff=" + std::to_string(segs[i - 1].hCutoff) + " km.");
+    b.expSegments = std::move(segs);
+}
+
 void parseAtmosphere(BodySpec& b, const json::Value& a, const std::string& base) {
     b.atmosphere = activationValue(a);
     if (!a.isObject()) {
-        if (b.atmosphere) throw std::runtime_error("atmosphere requires an nrlmsise00 model block.");
+        if (b.atmosphere) throw std::runtime_error("atmosphere requires a model block (exponential or nrlmsise00).");
         return;
     }
+    if (a.size() != 1)
+        throw std::runtime_error("Atmosphere block must be a single-key dispatch dict naming the model (exponential or nrlmsise00).");
     for (const auto& kv : *a.obj) {
         const std::string k = lower(kv.first);
-        if (k == "exponential")
-            throw std::runtime_error("Exponential atmospheres are not provided; configure nrlmsise00.");
-        if (k != "nrlmsise00") throw std::runtime_error("Unsupported atmosphere model: " + kv.first);
+        if (k == "exponential") {
+            parseExponential(b, kv.second);
+            continue;
+        }
+        if (k != "nrlmsise00" && k != "nrlmsise-00")
+            throw std::runtime_error("Unknown atmosphere model: '" + kv.first + "'.  Supported: exponential, nrlmsise00.");
         b.nrlmsise = true;
         const json::Value& n = kv.second;
         if (!n.isObject()) throw std::runtime_error("nrlmsise00 must be an object.");
@@ -175,7 +227,7 @@ void parseAtmosphere(BodySpec& b, const json::Value& a, const std::string& base)
     }
 }
 
-struct ConstEntry { int naif; double gm, r, soi; bool hasGm; };
+struct ConstEntry { int naif; double gm, r, soi, j2; bool hasGm; };
 
 void loadConstants(const std::string& path, ModelConfig& m, std::vector<ConstEntry>& out) {
     json::Value root = json::readFile(path);
@@ -187,10 +239,11 @@ void loadConstants(const std::string& path, ModelConfig& m, std::vector<ConstEnt
         if (const auto* v = e.find("EMRAT")) m.emrat = v->asDouble();
         const auto* id = e.find("NAIFID");
         if (!id) continue;
-        ConstEntry c{int(id->asDouble()), 0.0, 0.0, 1e20, false};
+        ConstEntry c{int(id->asDouble()), 0.0, 0.0, 1e20, 0.0, false};
         if (const auto* v = e.find("GM")) { c.gm = v->asDouble(); c.hasGm = true; }
         if (const auto* v = e.find("R")) c.r = v->asDouble();
         if (const auto* v = e.find("SOIOld")) c.soi = v->asDouble();
+        if (const auto* v = e.find("J2")) c.j2 = v->asDouble();
         out.push_back(c);
     }
 }
@@ -279,7 +332,7 @@ void parseConfig(const json::Value& root, const std::string& configDir,
         b.name = kv.first;
         b.naif = naifFromName(kv.first);
         for (const auto& c : consts)
-            if (c.naif == b.naif) { if (c.hasGm) b.gm = c.gm; b.radius = c.r; b.soi = c.soi; }
+            if (c.naif == b.naif) { if (c.hasGm) b.gm = c.gm; b.radius = c.r; b.soi = c.soi; b.j2 = c.j2; }
         const json::Value& f = kv.second;
         if (f.isObject()) {
             for (const auto& p : *f.obj) {
@@ -288,10 +341,16 @@ void parseConfig(const json::Value& root, const std::string& configDir,
                 else if (q == "radiation") b.radiation = activationValue(p.second);
                 else if (q == "atmosphere") parseAtmosphere(b, p.second, configDir);
                 else if (q == "occulting") b.occulting = activationValue(p.second);
-                else if (q == "flattening") b.flattening = p.second.asDouble();
+                else if (q == "flattening") {
+                    b.flattening = p.second.asDouble();
+                    if (!(b.flattening >= 0.0 && b.flattening < 1.0))
+                        throw std::runtime_error("flattening must lie in [0, 1) for body " + b.name);
+                }
                 else if (q == "gm") b.gm = p.second.asDouble();
                 else if (q == "radius") b.radius = p.second.asDouble();
                 else if (q == "soi") b.soi = p.second.asDouble();
+                else if (q == "j2") b.j2 = p.second.asDouble();
+                else if (q == "amean") (void)p.second.asDouble();  // accepted; not used by these forces
                 else throw std::runtime_error("Unsupported body option: " + p.first);
             }
         } else if (f.isArray()) {
@@ -319,6 +378,10 @@ void parseConfig(const json::Value& root, const std::string& configDir,
         if (b.radiation && b.naif != NAIF_SUN) throw std::runtime_error("Radiation is supported for the Sun only.");
         if (b.atmosphere && b.naif != NAIF_EARTH) throw std::runtime_error("Atmospheres are supported for Earth only.");
         if (b.shape && b.naif != NAIF_EARTH) throw std::runtime_error("Spherical harmonics are supported for Earth only.");
+        if (b.oblateness && !b.shape && b.naif != NAIF_EARTH)
+            throw std::runtime_error("J2 (oblateness) is supported for Earth only.");
+        if (b.atmosphere && !b.nrlmsise && !(b.radius > 0))
+            throw std::runtime_error("Exponential atmosphere needs the body radius: " + b.name);
         if (b.occulting && !(b.radius > 0)) throw std::runtime_error("Occulting body needs a radius: " + b.name);
         if (b.radiation && !(b.radius > 0)) throw std::runtime_error("Sun radius missing for shadow geometry.");
     }

@@ -62,6 +62,12 @@ struct NrlSettings {
     double densityScale = 1.0;
 };
 
+// One block of an exponential atmosphere: rho0 * exp(-(h - h0) / scaleHeight)
+// for altitudes of its band, zero above hCutoff (rho0 in kg/km^3, lengths in km).
+struct ExpSegment {
+    double rho0 = 0.0, h0 = 0.0, scaleHeight = 1.0, hCutoff = 0.0;
+};
+
 struct BodySpec {
     std::string name;
     int naif = -1;
@@ -80,9 +86,14 @@ struct BodySpec {
     int shHeaderLines = 0;
     int shDegree = -1;
     int shOrder = -1;
+    // gravity.oblateness (J2 only; a shape field takes precedence)
+    bool oblateness = false;
+    double j2 = 0.0;          // unnormalized J2 (constants.json "J2" or inline "j2")
     // atmosphere.nrlmsise00
     bool nrlmsise = false;
     NrlSettings nrl;
+    // atmosphere.exponential: segments sorted ascending by h0
+    std::vector<ExpSegment> expSegments;
 };
 
 struct Precision {

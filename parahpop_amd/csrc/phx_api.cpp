@@ -209,6 +209,26 @@ void buildForceModel(phx_ctx& c) {
                 fm.densityScale = b.nrl.densityScale;
                 fm.earthRadius = b.radius;
                 fm.flattening = b.flattening;
+            } else if (b.atmosphere && !b.expSegments.empty()) {
+                if (b.expSegments.size() > size_t(kMaxExpSegments))
+                    throw std::runtime_error("at most " + std::to_string(kMaxExpSegments) + " exponential atmosphere bands are supported");
+                fm.expDrag = 1;
+                fm.nExpSeg = int(b.expSegments.size());
+                for (size_t s = 0; s < b.expSegments.size(); ++s) {
+                    fm.expSeg[s][0] = b.expSegments[s].rho0;
+                    fm.expSeg[s][1] = b.expSegments[s].h0;
+                    fm.expSeg[s][2] = b.expSegments[s].scaleHeight;
+                    fm.expSeg[s][3] = b.expSegments[s].hCutoff;
+                }
+                fm.earthRadius = b.radius;
+                fm.flattening = b.flattening;
+            }
+            if (b.oblateness && !b.shape) {
+                fm.hasJ2 = 1;
+                fm.j2Gm = b.gm;
+                fm.j2Coef = b.j2;
+                fm.j2Radius = b.radius;
+                fm.j2Soi = b.soi;
             }
         }
         if (b.radiation) {
@@ -759,6 +779,7 @@ int phx_create(const char* config_json_path, phx_ctx** out) {
         json::Value root = json::readFile(path);
         parseConfig(root, dirName(path), c->run, c->env.model, false);
         loadEnvironment(c->env);
+        buildForceModel(*c);  // host-only; rejects what the kernels cannot hold before any device call
         int ndev = 0;
         const hipError_t de = hipGetDeviceCount(&ndev);
         if (de != hipSuccess || ndev <= 0)
@@ -771,7 +792,6 @@ int phx_create(const char* config_json_path, phx_ctx** out) {
         check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         check(hipEventCreate(&c->ev0), "hipEventCreate");
         check(hipEventCreate(&c->ev1), "hipEventCreate");
-        buildForceModel(*c);
         if (uploadTideFactors() != 0) throw HipError("tide factor upload failed");
         loadTables(*c);
         if (c->env.hasSh) buildShTables(*c);

@@ -10,6 +10,7 @@ constexpr int kTile = 64;          // objects per workgroup tile = one wave64
 constexpr int kMaxBodies = 11;     // Earth, Sun, Moon, 8 planetary system barycenters
 constexpr int kNrlSlotSize = 312;  // doubles per NRLMSISE-00 weather-slot row
 constexpr int kMaxWaves = 16;      // waves per tile that share one object's work
+constexpr int kMaxExpSegments = 32; // bands of a piecewise exponential atmosphere
 
 // Per-epoch record: everything the force model needs that depends only on the
 // stage epoch, computed once per distinct epoch and read by every object.
@@ -73,14 +74,26 @@ struct ForceModel {
     int nOccluders;
     int occBody[4];               // body indices of occulting bodies (Sun excluded)
     double occRadius[4];
+    // J2-only Earth gravity (when no shape field is configured)
+    int hasJ2;
+    double j2Gm, j2Coef, j2Radius, j2Soi;
     // drag (Earth, NRLMSISE-00)
     int drag;
     double hCutoff, densityScale, earthRadius, flattening;
+    // drag (Earth, piecewise exponential atmosphere; shares earthRadius and
+    // flattening): band s is {rho0 kg/km^3, h0 km, scale height km, cutoff km},
+    // sorted ascending by h0
+    int expDrag;
+    int nExpSeg;
+    double expSeg[kMaxExpSegments][4];
     // Earth corrections
     int tides;                    // solid or ocean active
     int relativity;
     double muEarthSI, radiusSI, clightSI;
 };
+
+// kernel arguments travel by value (StepArgs + ForceModel + SplitArgs per launch)
+static_assert(sizeof(ForceModel) <= 2048, "force model too large for a kernel argument");
 
 // Step-kernel arguments.
 struct StepArgs {

@@ -803,6 +803,86 @@ __device__ void pointGravity(const ForceModel& fm, const Rec& rec, double px, do
     az += rec(REC_INDIRECT + 2);
 }
 
+// J2-only gravity (J2.h:17-92, Wakker eq. 20.6): evaluated in ITRF axes and
+// rotated back; zero beyond the sphere of influence.
+template <class Rec>
+__device__ inline void j2Accel(const ForceModel& fm, const Rec& rec, double px, double py, double pz, double& ax, double& ay,
+                               double& az) {
+    double E[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) E[k] = rec(REC_R + k);
+    const double fx = E[0] * px + E[1] * py + E[2] * pz;
+    const double fy = E[3] * px + E[4] * py + E[5] * pz;
+    const double fz = E[6] * px + E[7] * py + E[8] * pz;
+    const double invr = 1.0 / sqrt(fx * fx + fy * fy + fz * fz);
+    const double invr2 = invr * invr;
+    const double invr5 = invr2 * invr2 * invr;
+    const double v = 1.0 - 5.0 * (fz * fz) * invr2;
+    const double c = -1.5 * fm.j2Gm * fm.j2Coef * invr5 * fm.j2Radius * fm.j2Radius;
+    const double bx = c * fx * v, by = c * fy * v, bz = c * fz * (v + 2.0);
+    const bool in = !(sqrt(px * px + py * py + pz * pz) > fm.j2Soi);
+    ax = in ? E[0] * bx + E[3] * by + E[6] * bz : 0.0;
+    ay = in ? E[1] * bx + E[4] * by + E[7] * bz : 0.0;
+    az = in ? E[2] * bx + E[5] * by + E[8] * bz : 0.0;
+}
+
+// Geodetic altitude above the oblate Earth (Geodetic.h:40-72): fixed-point
+// iteration on the polar coordinate, at most kGeodeticIter steps (it converges
+// in 2-4); ||r|| - R_eq when the flattening is 0.
+constexpr int kGeodeticIter = 32;
+
+__device__ inline double geodeticAltitude(double px, double py, double pz, double k0, double k1, double k2, double req,
+                                          double f) {
+    if (f == 0.0) return sqrt(px * px + py * py + pz * pz) - req;
+    const double r2 = px * px + py * py + pz * pz;
+    const double z = px * k0 + py * k1 + pz * k2;
+    const double xy = r2 - z * z;
+    const double e2 = (2.0 - f) * f;
+    double d0 = 0.0, d = 1e10, sfq = 0.0, zd = z;
+    for (int it = 0; it < kGeodeticIter && fabs(d - d0) > 1e-9; ++it) {
+        d0 = d;
+        const double z2 = zd * zd;
+        sfq = z2 / (xy + z2);
+        d = req * e2 * sqrt(sfq / (1.0 - e2 * sfq));
+        zd = z + copysign(d, z);
+    }
+    return sqrt(xy + zd * zd) - req / sqrt(1.0 - e2 * sfq);
+}
+
+// Drag in a piecewise exponential atmosphere (PrivateAccelerations.h:167-202,
+// PiecewiseExponentialAtmosphere.h:100-170): the band is the last one whose
+// base h0 lies at or below the geodetic altitude (band 0 below them all), the
+// density rho0 exp(-(h - h0)/H) up to the band's cutoff, zero above it.
+template <class Rec>
+__device__ inline void expDragAccel(const ForceModel& fm, const Rec& rec, double px, double py, double pz, double vx,
+                                    double vy, double vz, double bcd, double& dx, double& dy, double& dz) {
+    const double h = geodeticAltitude(px, py, pz, rec(REC_R + 6), rec(REC_R + 7), rec(REC_R + 8), fm.earthRadius,
+                                      fm.flattening);
+    // bands are sorted by h0, so the last band with h0 <= h is the count of them
+    double rho0 = fm.expSeg[0][0], h0 = fm.expSeg[0][1], sh = fm.expSeg[0][2], cut = fm.expSeg[0][3];
+#pragma unroll
+    for (int s = 1; s < kMaxExpSegments; ++s) {
+        if (s < fm.nExpSeg && fm.expSeg[s][1] <= h) {
+            rho0 = fm.expSeg[s][0];
+            h0 = fm.expSeg[s][1];
+            sh = fm.expSeg[s][2];
+            cut = fm.expSeg[s][3];
+        }
+    }
+    dx = dy = dz = 0.0;
+    if (h <= cut) {
+        const double rho = rho0 * exp(-(h - h0) / sh);
+        const double om0 = rec(REC_OMEGA), om1 = rec(REC_OMEGA + 1), om2 = rec(REC_OMEGA + 2);
+        const double rx = vx - (om1 * pz - om2 * py);
+        const double ry = vy - (om2 * px - om0 * pz);
+        const double rz = vz - (om0 * py - om1 * px);
+        const double f = -0.5 * rho * bcd * sqrt(rx * rx + ry * ry + rz * rz);
+        dx = f * rx;
+        dy = f * ry;
+        dz = f * rz;
+    }
+}
+
 // Which wave carries which extra force family for a W-wave tile.
 __device__ constexpr int corrWave(int W) { return W >= 3 ? W - 2 : W - 1; }
 
@@ -866,8 +946,15 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
             ax += hx;
             ay += hy;
             az += hz;
+        } else if (fm.hasJ2) {
+            double hx, hy, hz;
+            j2Accel(fm, rec, px, py, pz, hx, hy, hz);
+            ax += hx;
+            ay += hy;
+            az += hz;
         }
         double dx = 0.0, dy = 0.0, dz = 0.0;
+        if (fm.expDrag) expDragAccel(fm, rec, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], bcd, dx, dy, dz);
         if (DRAG && fm.drag && nl.state != 0) {
             double rho;
             if (nl.state == 1) rho = nrlRhoFromSpecies(fm, spec, lane);
@@ -1547,6 +1634,9 @@ __global__ void __launch_bounds__(64) forceProbeKernel(int n, const double* pos,
         shOrderBlock(g, 0, fm.shOrder + 1, fm.shDegree, shTerms, nullptr, -1, shOrderStart, shDiag, fm.shNormalized, r, p, l);
         shToInertial(rec, g, r, p, l, fm.shGm, bx, by, bz);
         ax += bx, ay += by, az += bz;
+    } else if ((which < 0 || which == 1) && fm.hasJ2) {
+        j2Accel(fm, rec, px, py, pz, bx, by, bz);
+        ax += bx, ay += by, az += bz;
     }
     if ((which < 0 || which == 3) && fm.srp) {
         srpAccel(fm, rec, px, py, pz, ams[i], bx, by, bz);
@@ -1554,6 +1644,10 @@ __global__ void __launch_bounds__(64) forceProbeKernel(int n, const double* pos,
     }
     if ((which < 0 || which == 4) && fm.drag) {
         dragAccel<1>(fm, rec, recp + REC_NRL, nrlCoeff, px, py, pz, vx, vy, vz, bcd[i], bx, by, bz);
+        ax += bx, ay += by, az += bz;
+    }
+    if ((which < 0 || which == 4) && fm.expDrag) {
+        expDragAccel(fm, rec, px, py, pz, vx, vy, vz, bcd[i], bx, by, bz);
         ax += bx, ay += by, az += bz;
     }
     if ((which < 0 || which == 5) && (fm.tides || fm.relativity)) {

@@ -193,6 +193,57 @@ def full_model(degree, gravity_file, start_mjd_utc, end_mjd_utc):
                              "oceanTides": True, "relativity": True, "eopFile": "EOP-All.txt"}}}
 
 
+# Exponential atmosphere bands (base altitude km, base density kg/m^3, scale
+# height km): Vallado, Fundamentals of Astrodynamics and Applications, Table 8-4.
+EXPONENTIAL_BANDS = [
+    (0, 1.225, 7.249), (25, 3.899e-2, 6.349), (30, 1.774e-2, 6.682), (40, 3.972e-3, 7.554),
+    (50, 1.057e-3, 8.382), (60, 3.206e-4, 7.714), (70, 8.770e-5, 6.549), (80, 1.905e-5, 5.799),
+    (90, 3.396e-6, 5.382), (100, 5.297e-7, 5.877), (110, 9.661e-8, 7.263), (120, 2.438e-8, 9.473),
+    (130, 8.484e-9, 12.636), (140, 3.845e-9, 16.149), (150, 2.070e-9, 22.523), (180, 5.464e-10, 29.740),
+    (200, 2.789e-10, 37.105), (250, 7.248e-11, 45.546), (300, 2.418e-11, 53.628), (350, 9.518e-12, 53.298),
+    (400, 3.725e-12, 58.515), (450, 1.585e-12, 60.828), (500, 6.967e-13, 63.822), (600, 1.454e-13, 71.835),
+    (700, 3.614e-14, 88.667), (800, 1.170e-14, 124.64), (900, 5.245e-15, 181.05), (1000, 3.019e-15, 268.00),
+]
+
+
+def exponential_blocks(bands=EXPONENTIAL_BANDS, top=1500.0):
+    """Atmosphere blocks of the config schema: rho0 in kg/km^3, each band's cutoff
+    at the next band's base (the last at `top`)."""
+    out = []
+    for k, (h0, rho, scale) in enumerate(bands):
+        cut = bands[k + 1][0] if k + 1 < len(bands) else top
+        out.append({"rho0": rho * 1e9, "h0": float(h0), "scaleHeight": float(scale), "hCutoff": float(cut)})
+    return out
+
+
+def alternate_model(atmosphere=None, flattening=0.0033528106647474805, third_bodies=True):
+    """J2-only Earth gravity with an exponential atmosphere (piecewise by default),
+    Sun and Moon point masses, SRP with Earth shadow and relativity."""
+    earth = {"gravity": {"oblateness": True}, "flattening": flattening, "occulting": True,
+             "atmosphere": {"exponential": exponential_blocks() if atmosphere is None else atmosphere}}
+    bodies = {"Earth": earth}
+    if third_bodies:
+        bodies["Sun"] = {"gravity": True, "radiation": True}
+        bodies["Moon"] = {"gravity": True}
+    return {"environment": {
+        "bodies": bodies, "ephemeris": "analytic", "orientations": "analytic", "constants": "constants.json",
+        "preferSSB": True, "earthCorrections": {"model": "iers2010_fes2004", "relativity": True}}}
+
+
+def make_alternate_workload(out, n=16384, duration=86400.0, step=60.0, seed=2, model=None):
+    """Workload-2 population (80 % LEO 300-2000 km, 20 % MEO) under the alternate
+    force model; returns the CONFIG.json and TARGETS.json paths."""
+    os.makedirs(out, exist_ok=True)
+    with open(os.path.join(out, "constants.json"), "w") as f:
+        json.dump(CONSTANTS, f, indent=1)
+    pop = population(n, seed, mix=(0.8, 0.2, 0.0))
+    epochs = np.full(n, EPOCH0)
+    max_steps = int(math.ceil(duration / step)) + 2
+    write_config(os.path.join(out, "CONFIG.json"), model or alternate_model(), duration, step, max_steps)
+    write_targets(os.path.join(out, "TARGETS.json"), pop, epochs)
+    return os.path.join(out, "CONFIG.json"), os.path.join(out, "TARGETS.json")
+
+
 def central_model():
     return {"environment": {"bodies": {"Earth": {"gravity": True}}, "ephemeris": "analytic",
                             "constants": "constants.json", "preferSSB": True}}

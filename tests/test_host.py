@@ -65,6 +65,31 @@ def _create_error(path):
     ({"environment": {"bodies": {"Earth": {"gravity": True}, "Vulcan": {"gravity": True}}, "constants": "constants.json"}}, "Unsupported body"),
     ({"environment": {"bodies": {"Earth": {"gravity": True}}, "constants": "constants.json",
                       "earthCorrections": {"model": "iers2010_fes2004", "solidEarthTides": True}}}, "eopFile"),
+    ({"environment": {"bodies": {"Earth": {"atmosphere": {"exponential": [
+        {"rho0": 1.0, "h0": 100.0, "scaleHeight": 10.0, "hCutoff": 300.0},
+        {"rho0": 1.0, "h0": 200.0, "scaleHeight": 10.0, "hCutoff": 400.0}]}}}, "constants": "constants.json"}},
+     "Overlapping atmosphere segments"),
+    ({"environment": {"bodies": {"Earth": {"atmosphere": {"exponential": {"rho0": 1.0, "h0": 100.0, "scaleHeight": 10.0}}}},
+                      "constants": "constants.json"}}, "hCutoff not specified"),
+    ({"environment": {"bodies": {"Earth": {"atmosphere": {"exponential": {"rho0": 1.0, "h0": 100.0, "scaleHeight": 10.0,
+                                                                          "hCutoff": 200.0, "T0": 1.0}}}},
+                      "constants": "constants.json"}}, "Invalid atmosphere field"),
+    ({"environment": {"bodies": {"Earth": {"atmosphere": {"exponential": {"rho0": -1.0, "h0": 100.0, "scaleHeight": 10.0,
+                                                                          "hCutoff": 200.0}}}},
+                      "constants": "constants.json"}}, "rho0 must be non-negative"),
+    ({"environment": {"bodies": {"Earth": {"atmosphere": {"exponential": [], "nrlmsise00": {}}}},
+                      "constants": "constants.json"}}, "single-key dispatch"),
+    ({"environment": {"bodies": {"Earth": {"atmosphere": {"jacchia": {}}}}, "constants": "constants.json"}},
+     "Unknown atmosphere model"),
+    ({"environment": {"bodies": {"Earth": {"gravity": True}, "Moon": {"gravity": {"oblateness": True}}},
+                      "constants": "constants.json"}}, "J2 (oblateness) is supported for Earth only"),
+    ({"environment": {"bodies": {"Earth": {"gravity": ["oblateness", "rings"]}}, "constants": "constants.json"}},
+     "Unsupported gravity feature"),
+    ({"environment": {"bodies": {"Earth": {"gravity": True, "flattening": 1.0}}, "constants": "constants.json"}},
+     "flattening must lie in [0, 1)"),
+    ({"environment": {"bodies": {"Earth": {"atmosphere": {"exponential": [
+        {"rho0": 1.0, "h0": 10.0 * k, "scaleHeight": 10.0, "hCutoff": 10.0 * k + 10.0} for k in range(33)]}}},
+        "constants": "constants.json"}}, "at most 32 exponential atmosphere bands"),
 ])
 def test_config_rejections(tmp_path, model, expect):
     rc, msg = _create_error(_write_cfg(str(tmp_path), model))
