@@ -614,8 +614,9 @@ StepArgs contextStepArgs(const phx_ctx& c, int& W) {
     a.nrlSlots = c.nrlSlots.p;
     a.gridInvH = 1.0 / c.gridStep;
     // drag models run at most four waves per tile (the NRLMSISE-00 path needs
-    // the register budget of one wave per SIMD)
-    W = c.fm.hasSh ? (c.fm.drag ? std::min(c.nWaves, 4) : c.nWaves) : 1;
+    // the register budget of one wave per SIMD); without a field the force
+    // families spread over at most four waves
+    W = c.fm.hasSh ? (c.fm.drag ? std::min(c.nWaves, 4) : c.nWaves) : std::min(c.nWaves, 4);
     if (c.fm.hasSh) orderBlocks(c, W, a.blockStart, false);
     else
         for (int w = 0; w <= kMaxWaves; ++w) a.blockStart[w] = 0;
@@ -976,7 +977,7 @@ int phx_batch_autotune(phx_batch* b, int trialSteps, phx_stats* best) {
                     for (int ss : {1, 8})
                         if (!b->grid || ss == 8) cands.push_back({c.nWaves, c.share0, 1, sw, ss});
         } else {
-            cands.push_back({1, 1.0, 0, c.splitWaves, c.splitSpecies});
+            for (int w : {1, 2, 4}) cands.push_back({w, 1.0, 0, c.splitWaves, c.splitSpecies});
         }
         const Cand keep{c.nWaves, c.share0, c.split ? 1 : 0, c.splitWaves, c.splitSpecies};
         auto apply = [&c](const Cand& k) {

@@ -769,6 +769,28 @@ __device__ void correctionsAccel(const ForceModel& fm, const Rec& rec, double px
     az = 1.0e-3 * sz;
 }
 
+// Body slots [0, N) of the point-gravity sum, added in config order; the
+// unconfigured ones are evaluated and discarded: no branch between the bodies,
+// so their square roots and divisions overlap. The fused multiply-adds are
+// written out, so every kernel that inlines this forms the same values.
+template <int N, class Rec>
+__device__ inline void bodySlots(const ForceModel& fm, const Rec& rec, double px, double py, double pz, double& ax, double& ay,
+                                 double& az) {
+#pragma unroll
+    for (int b = 0; b < N; ++b) {
+        const bool on = b < fm.nBodies && fm.bodyPointGravity[b] && fm.bodyNaif[b] != 399;
+        const int q = REC_BODY + 3 * b;
+        const double dx = px - rec(q), dy = py - rec(q + 1), dz = pz - rec(q + 2);
+        const double s2 = fma(dz, dz, fma(dy, dy, dx * dx));
+        const double rc = 1.0 / (s2 * sqrt(s2));
+        const double g = fm.bodyGm[b];
+        const double tx = fma(-g * dx, rc, ax), ty = fma(-g * dy, rc, ay), tz = fma(-g * dz, rc, az);
+        ax = on ? tx : ax;
+        ay = on ? ty : ay;
+        az = on ? tz : az;
+    }
+}
+
 // Central and third-body point gravity (PointGravity.h:16-81); the indirect
 // third-body term is an epoch-only sum taken from the record.
 template <class Rec>
@@ -776,28 +798,15 @@ __device__ void pointGravity(const ForceModel& fm, const Rec& rec, double px, do
                              double& az) {
     ax = ay = az = 0.0;
     if (fm.earthPointGravity) {
-        const double s2 = px * px + py * py + pz * pz;
+        const double s2 = fma(pz, pz, fma(py, py, px * px));
         const double rc = 1.0 / (s2 * sqrt(s2));
         ax = (-fm.gmEarth * px) * rc;
         ay = (-fm.gmEarth * py) * rc;
         az = (-fm.gmEarth * pz) * rc;
     }
-    // every body slot is evaluated (the record holds all kMaxBodies slots) and
-    // the configured ones are added in config order: no branch between the
-    // bodies, so their square roots and divisions overlap
-#pragma unroll
-    for (int b = 0; b < kMaxBodies; ++b) {
-        const bool on = b < fm.nBodies && fm.bodyPointGravity[b] && fm.bodyNaif[b] != 399;
-        const int q = REC_BODY + 3 * b;
-        const double dx = px - rec(q), dy = py - rec(q + 1), dz = pz - rec(q + 2);
-        const double s2 = dx * dx + dy * dy + dz * dz;
-        const double rc = 1.0 / (s2 * sqrt(s2));
-        const double g = fm.bodyGm[b];
-        const double tx = ax + (-g * dx) * rc, ty = ay + (-g * dy) * rc, tz = az + (-g * dz) * rc;
-        ax = on ? tx : ax;
-        ay = on ? ty : ay;
-        az = on ? tz : az;
-    }
+    // a few bodies (Earth, Sun, Moon, one more) or every slot the record holds
+    if (fm.nBodies <= 4) bodySlots<4>(fm, rec, px, py, pz, ax, ay, az);
+    else bodySlots<kMaxBodies>(fm, rec, px, py, pz, ax, ay, az);
     ax += rec(REC_INDIRECT + 0);
     ay += rec(REC_INDIRECT + 1);
     az += rec(REC_INDIRECT + 2);
@@ -861,12 +870,16 @@ __device__ inline void expDragAccel(const ForceModel& fm, const Rec& rec, double
     // bands are sorted by h0, so the last band with h0 <= h is the count of them
     double rho0 = fm.expSeg[0][0], h0 = fm.expSeg[0][1], sh = fm.expSeg[0][2], cut = fm.expSeg[0][3];
 #pragma unroll
-    for (int s = 1; s < kMaxExpSegments; ++s) {
-        if (s < fm.nExpSeg && fm.expSeg[s][1] <= h) {
-            rho0 = fm.expSeg[s][0];
-            h0 = fm.expSeg[s][1];
-            sh = fm.expSeg[s][2];
-            cut = fm.expSeg[s][3];
+    for (int s0 = 0; s0 < kMaxExpSegments; s0 += 8) {
+        if (s0 >= fm.nExpSeg) break;
+#pragma unroll
+        for (int s = (s0 == 0 ? 1 : s0); s < s0 + 8; ++s) {
+            if (s < fm.nExpSeg && fm.expSeg[s][1] <= h) {
+                rho0 = fm.expSeg[s][0];
+                h0 = fm.expSeg[s][1];
+                sh = fm.expSeg[s][2];
+                cut = fm.expSeg[s][3];
+            }
         }
     }
     dx = dy = dz = 0.0;
@@ -916,6 +929,31 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
         nl = nrlLane(fm, rec, nrl, px, py, pz);
         if (__ballot(nl.state == 1) != 0ull) nrlSpecies<MODE>(nl, nrl, slot, a.nrlCoeff, wave, W, spec, lane);
     }
+    // Without a field the families take waves of their own before the barrier
+    // (point masses and J2 on the last wave, exponential drag on wave 1 from
+    // four waves on), so wave 0 only sums; the values are those of one wave.
+    const bool spread = W >= 2 && !fm.hasSh;
+    if (spread && wave == W - 1) {
+        double gx, gy, gz;
+        pointGravity(fm, rec, px, py, pz, gx, gy, gz);
+        if (fm.hasJ2) {
+            double jx, jy, jz;
+            j2Accel(fm, rec, px, py, pz, jx, jy, jz);
+            gx += jx;
+            gy += jy;
+            gz += jz;
+        }
+        part[0][0][lane] = gx;
+        part[0][1][lane] = gy;
+        part[0][2][lane] = gz;
+    }
+    if (spread && W >= 4 && fm.expDrag && wave == 1) {
+        double dx, dy, dz;
+        expDragAccel(fm, rec, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], bcd, dx, dy, dz);
+        extra[0][0][lane] = dx;
+        extra[0][1][lane] = dy;
+        extra[0][2][lane] = dz;
+    }
     if (wave == corrWave(W)) {
         double ax = 0.0, ay = 0.0, az = 0.0;
         if (fm.tides || fm.relativity) correctionsAccel(fm, rec, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], ax, ay, az);
@@ -933,7 +971,13 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
     __syncthreads();
     if (wave == 0) {
         double ax, ay, az;
-        pointGravity(fm, rec, px, py, pz, ax, ay, az);
+        if (spread) {
+            ax = part[0][0][lane];
+            ay = part[0][1][lane];
+            az = part[0][2][lane];
+        } else {
+            pointGravity(fm, rec, px, py, pz, ax, ay, az);
+        }
         if (fm.hasSh) {
             double r = 0.0, p = 0.0, l = 0.0;
             for (int w = 0; w < W; ++w) {
@@ -946,7 +990,7 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
             ax += hx;
             ay += hy;
             az += hz;
-        } else if (fm.hasJ2) {
+        } else if (fm.hasJ2 && !spread) {
             double hx, hy, hz;
             j2Accel(fm, rec, px, py, pz, hx, hy, hz);
             ax += hx;
@@ -954,7 +998,15 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
             az += hz;
         }
         double dx = 0.0, dy = 0.0, dz = 0.0;
-        if (fm.expDrag) expDragAccel(fm, rec, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], bcd, dx, dy, dz);
+        if (fm.expDrag) {
+            if (spread && W >= 4) {
+                dx = extra[0][0][lane];
+                dy = extra[0][1][lane];
+                dz = extra[0][2][lane];
+            } else {
+                expDragAccel(fm, rec, px, py, pz, xch[3][lane], xch[4][lane], xch[5][lane], bcd, dx, dy, dz);
+            }
+        }
         if (DRAG && fm.drag && nl.state != 0) {
             double rho;
             if (nl.state == 1) rho = nrlRhoFromSpecies(fm, spec, lane);

@@ -72,3 +72,14 @@ def test_single_block_spherical_altitude_matches_oracle(workdir):
     dp = np.linalg.norm(s_g[:3] - s_o[:3], axis=0) * 1e6
     dv = np.linalg.norm(s_g[3:] - s_o[3:], axis=0) * 1e9
     assert np.max(dp) <= 1.0 and np.max(dv) <= 1.0, (np.max(dp), np.max(dv))
+
+
+@pytest.mark.parametrize("waves", [2, 4])
+def test_alternate_wave_spread_matches_one_wave(alt_workload, waves):
+    """Spreading the families over waves moves work, not arithmetic: the final
+    states equal the one-wave kernel's bit for bit."""
+    cfg, t = alt_workload
+    _, ref, _ = _ctx(cfg, waves=1).propagate(t, 3600.0)
+    _, got, st = _ctx(cfg, waves=waves).propagate(t, 3600.0)
+    assert st["waves"] == waves
+    np.testing.assert_array_equal(got, ref)
