@@ -18,6 +18,10 @@ Further legs (one JSON object each in the same line):
   serve   Poisson arrivals of requests (1 - 50,000 objects log-uniform, 1 h -
           3 d log-uniform, uneven-batch distributions) on a mixed-precision
           70x70 model: p50/p99 latency, object-steps/s, error vs the FP64 path
+  alternate  J2-only Earth gravity and drag in a 28-band exponential
+          atmosphere over the geodetic altitude (Sun/Moon, SRP with shadow,
+          relativity): one day at 60 s for 16,384 and 262,144 objects, the
+          error against the CPU oracle (64 objects) and the oracle's own rate
 
 --gpus N > 1 exits: this build runs on one GPU (the propagation shards by
 object slots with no exchange between GPUs).
@@ -44,6 +48,13 @@ FP64_PEAK_TFLOPS = 78.6
 # NRLMSISE-00 not counted), F_RK = 800.
 F_OTHER = 1465.0
 F_RK = 800.0
+
+
+# Alternate model, FLOPs per force evaluation: central 25, Sun and Moon 2 x 25
+# (direct + indirect), J2 45 (two 3x3 rotations 30 + the Wakker form 15), SRP
+# with the shadow cones 100, relativity 30, drag over the geodetic altitude 85
+# (three fixed-point steps 45, density and v_rel 40).
+F_ALT_RHS = 335.0
 
 
 def flops_per_object_step(L):
@@ -233,6 +244,48 @@ def leg_uneven(args, work):
             "max_end_epoch_error_s": end_err}
 
 
+def leg_alternate(args, work):
+    import parahpop_amd
+    from parahpop_amd import synth
+    duration = 86400.0
+    f = 13.0 * F_ALT_RHS + F_RK
+    out = {"workload": "J2-only gravity, 28-band exponential drag over the geodetic altitude, Sun/Moon, SRP with "
+                       "Earth shadow, relativity; seed-2 population, 60 s step, 1 d",
+           "flops_per_object_step": f}
+    states = {}
+    for n in (args.objects, 262144):
+        cfg, tg = synth.make_alternate_workload(os.path.join(work, "walt%d" % n), n=n, duration=duration)
+        t = synth.load_targets(tg)
+        ctx = parahpop_amd.Context(cfg)
+        b = ctx.batch(t, duration)
+        b.run(2)
+        b.reset()
+        t0 = time.perf_counter()
+        st = b.run()
+        wall = time.perf_counter() - t0
+        states[n] = (cfg, t, b.download()[1])
+        rate = st["object_steps"] / wall
+        out["objects_%d" % n] = {"value": rate, "kernel_seconds": st["kernel_seconds"], "waves": st["waves"],
+                                 "fp64_tflops": rate * f * 1e-12, "fp64_frac": rate * f * 1e-12 / FP64_PEAK_TFLOPS}
+        log("alternate leg: %d objects %.4g object-steps/s" % (n, rate))
+        ctx.close()
+    out["value"] = out["objects_%d" % args.objects]["value"]
+    if not args.no_cpu:
+        import oracle
+        cfg, t, s_g = states[args.objects]
+        idx = np.linspace(0, t["n"] - 1, 64).astype(int)
+        threads = min(16, os.cpu_count() or 1)
+        orc = oracle.Oracle(cfg)
+        _, s_o, ost = orc.propagate(subset(t, idx), duration, threads=threads)
+        rms, mx = pos_err_mm(s_g[:, idx], s_o)
+        out["vs_oracle_pos_mm_64obj"] = {"rms": rms, "max": mx}
+        cidx = np.arange(min(1024, t["n"]))
+        _, _, cst = orc.propagate(subset(t, cidx), 3600.0, threads=threads)
+        out["cpu_baseline"] = {"value": cst["object_steps"] / cst["seconds"], "unit": "object-steps/s", "cores": threads,
+                               "kind": "port", "sample": "%d objects x 1 h on the oracle" % len(cidx)}
+    return out
+
+
 def leg_serve(args, work):
     import parahpop_amd
     from parahpop_amd import synth
@@ -305,7 +358,7 @@ def main():
     ap.add_argument("--waves", type=int, default=0, help="waves per 64-object tile (0: library default)")
     ap.add_argument("--steps-per-launch", type=int, default=0)
     ap.add_argument("--split", type=int, default=-1, help="1: split-stage path, 0: fused multi-step kernel (-1: library default)")
-    ap.add_argument("--legs", default="main,mixed,uneven,serve")
+    ap.add_argument("--legs", default="main,mixed,uneven,serve,alternate")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-objects", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=86400.0)
@@ -348,6 +401,8 @@ def main():
     if "serve" in legs:
         line["serve"] = leg_serve(args, work)
         log("serve: p50 %.2f s p99 %.2f s" % (line["serve"]["latency_p50_s"], line["serve"]["latency_p99_s"]))
+    if "alternate" in legs:
+        line["alternate"] = leg_alternate(args, work)
     print(json.dumps(line))
 
 
