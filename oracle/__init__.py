@@ -36,6 +36,7 @@ def library():
         lib.orc_nrlmsise.argtypes = [ctypes.c_char_p, i] + [d] * 8 + [vp, i, vp, vp]
         lib.orc_body_position.argtypes = [i, d, vp]
         lib.orc_earth_orientation.argtypes = [d, vp, vp]
+        lib.orc_ctx_body_position.argtypes = [vp, i, d, vp]
         lib.orc_geodetic_altitude.argtypes = [vp, vp, d, d]
         lib.orc_geodetic_altitude.restype = d
         lib.orc_exp_density.argtypes = [vp, d]
@@ -87,6 +88,12 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(self.lib.orc_last_error().decode())
         return out
+
+    def body_position(self, naif, et):
+        p = np.zeros(3)
+        if self.lib.orc_ctx_body_position(self.h, int(naif), float(et), _p(p)) != 0:
+            raise RuntimeError(self.lib.orc_last_error().decode())
+        return p
 
     def exp_density(self, h):
         return self.lib.orc_exp_density(self.h, float(h))

@@ -42,7 +42,19 @@ struct orc_ctx {
     NrlmsiseTables nrlTables;
     NrlWeather weather;
     bool anyOcculter = false;
+    std::map<int, std::vector<phx::BrieStep>> ephChains;   // Chebyshev ephemeris chains to the Earth
 };
+
+// Geocentric position (km, ICRF) of a body: the configured Chebyshev
+// ephemeris (brie.h), or the analytic series.
+V3 bodyPosition(const orc_ctx& c, int naif, double et) {
+    if (c.ephChains.empty()) return bodyGeocentric(naif, et);
+    auto it = c.ephChains.find(naif);
+    if (it == c.ephChains.end()) throw std::runtime_error("no ephemeris chain for body " + std::to_string(naif));
+    double p[3];
+    phx::brieChainPosition(c.env.brie, it->second, et, p);
+    return V3{p[0], p[1], p[2]};
+}
 
 static thread_local std::string g_error;
 
@@ -125,6 +137,13 @@ int orc_create(const char* config_json_path, const char* data_dir, orc_ctx** out
         }
         for (const auto& b : ctx->env.model.bodies)
             if (b.occulting && b.naif != phx::NAIF_SUN) ctx->anyOcculter = true;
+        if (!ctx->env.brie.empty()) {
+            for (const auto& b : ctx->env.model.bodies)
+                ctx->ephChains[b.naif] = phx::brieChain(ctx->env.brie, b.naif, phx::NAIF_EARTH);
+            for (int naif : {phx::NAIF_SUN, phx::NAIF_MOON})
+                if (!ctx->ephChains.count(naif) && ctx->env.model.tidesActive())
+                    ctx->ephChains[naif] = phx::brieChain(ctx->env.brie, naif, phx::NAIF_EARTH);
+        }
         *out = ctx.release();
         return 0;
     } catch (const std::exception& e) {
@@ -153,7 +172,7 @@ V3 earthCorrections(const orc_ctx& c, V3 pos, V3 vel, double et, const Orientati
         in.mjdUtc = t.mjdUtc; in.ut1Utc = t.ut1Utc; in.ttUtc = t.ttUtc;
         in.xpArcsec = t.xpArcsec; in.ypArcsec = t.ypArcsec;
         if (!t.valid) return V3{std::nan(""), std::nan(""), std::nan("")};
-        const V3 sun = bodyGeocentric(phx::NAIF_SUN, et), moon = bodyGeocentric(phx::NAIF_MOON, et);
+        const V3 sun = bodyPosition(c, phx::NAIF_SUN, et), moon = bodyPosition(c, phx::NAIF_MOON, et);
         in.sun[0] = 1000.0 * sun.x; in.sun[1] = 1000.0 * sun.y; in.sun[2] = 1000.0 * sun.z;
         in.moon[0] = 1000.0 * moon.x; in.moon[1] = 1000.0 * moon.y; in.moon[2] = 1000.0 * moon.z;
         for (int i = 0; i < 9; ++i) in.E[i] = ori->R.m[i];
@@ -186,7 +205,7 @@ V3 acceleration(const orc_ctx& c, V3 pos, V3 vel, double et, const Params& p, in
         return ori;
     };
     for (const auto& b : m.bodies) {
-        const V3 bodyPos = bodyGeocentric(b.naif, et);
+        const V3 bodyPos = bodyPosition(c, b.naif, et);
         if (b.pointGravity) {
             if (b.naif == phx::NAIF_EARTH) {
                 if (only < 0 || only == 0) acc = acc + twoBody(pos, b.gm);
@@ -206,7 +225,7 @@ V3 acceleration(const orc_ctx& c, V3 pos, V3 vel, double et, const Params& p, in
                 double occ = 1.0;
                 for (const auto& o : m.bodies) {
                     if (!o.occulting || o.naif == phx::NAIF_SUN) continue;
-                    const V3 occPos = bodyGeocentric(o.naif, et);
+                    const V3 occPos = bodyPosition(c, o.naif, et);
                     occ *= shadowFactor(d, pos - occPos, b.radius, o.radius);
                 }
                 const V3 s = srp(d, AMS, m.au);
@@ -334,6 +353,14 @@ int orc_propagate(orc_ctx* ctx, size_t n, const int64_t* id, const double* epoch
             nextDt[i] = std::min(step, span);
             count = std::max(count, (long long)std::ceil(span / step) + 1);
         }
+        // every evaluated epoch must lie in the ephemeris intervals (Interpolators.h:65-76)
+        for (const auto& kv : ctx->ephChains) {
+            double s0, s1;
+            phx::brieCoverage(ctx->env.brie, kv.second, s0, s1);
+            for (size_t i = 0; i < n; ++i)
+                if (cur[i] < s0 || end[i] > s1)
+                    throw std::runtime_error("Epoch not in the available ephemeris range of body " + std::to_string(kv.first));
+        }
         std::vector<double> y(6 * n);
         for (size_t i = 0; i < n; ++i)
             for (int d = 0; d < 6; ++d) y[6 * i + d] = state6[d * n + i];
@@ -414,4 +441,15 @@ double orc_exp_density(const orc_ctx* ctx, double h) {
         if (b.naif == phx::NAIF_EARTH && b.atmosphere && !b.expSegments.empty())
             return expDensity(b.expSegments.data(), int(b.expSegments.size()), h);
     return std::nan("");
+}
+
+int orc_ctx_body_position(const orc_ctx* ctx, int naif, double et, double* pos3) {
+    try {
+        const V3 p = bodyPosition(*ctx, naif, et);
+        pos3[0] = p.x; pos3[1] = p.y; pos3[2] = p.z;
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
 }

@@ -42,6 +42,9 @@ int orc_nrlmsise(const char* data_dir, int doy, double sec, double alt, double g
  * with zero polar motion and UT1 = UTC = TT - 69.184 s. */
 int orc_body_position(int naif, double et, double* pos3);
 int orc_earth_orientation(double et, double* R9, double* omega3);
+/* Geocentric body position (km) from the context's ephemeris: its Chebyshev
+ * tables when the configuration names .brie files, else the analytic series. */
+int orc_ctx_body_position(const orc_ctx* ctx, int naif, double et, double* pos3);
 /* Geodetic altitude (km) of r above the ellipsoid (equatorial radius Req km,
  * flattening f) whose polar axis is the unit vector pole, r and pole in one frame. */
 double orc_geodetic_altitude(const double* r3, const double* pole3, double Req, double f);

@@ -1,0 +1,87 @@
+// Chebyshev ephemeris tables in the reference's .brie format (CBOR), the
+// position chains from a body to the Earth, and their evaluation. Shared by the
+// host library (upload, coverage checks) and the CPU oracle.
+//
+// Reference behaviour followed (third_party/ unless noted):
+//   file layouts 1 and 2                  brie/brie/util/cbor.cpp:200-300,
+//                                         brie/brie/states/EphUnit.h:218-250,508-560
+//   metadata (field-major, one column     brie/brie/states/metadata/RefEphUnit.h:13-44,
+//   per body unit)                        brie/brie/states/metadata/EphUnit.h:50-100
+//   body-unit data: radius, interval      brie/brie/core/BodyInterpolator.h:55-100
+//   starts, coefficients [dim][deg][int]
+//   interval and Chebyshev time           parm/parm/interpolate/detail/Interpolators.h:65-76,
+//                                         parm/parm/interpolate/chebyshev/Interpolator.h:50-56
+//   backward Clenshaw recurrence          parm/parm/interpolate/chebyshev/Interpolator.h:60-90
+//   chains to the highest common center   brie/brie/core/ChainInterpolator.h:34-45,
+//                                         brie/brie/states/metadata/RefEphUnit.h:401-430
+#pragma once
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+namespace phx {
+
+// One body unit: `target` relative to `center` (NAIF ids), km, ICRF axes, ET seconds.
+struct BrieUnit {
+    int frame = 0, dtype = 0, target = 0, center = 0;
+    int nIntervals = 0, pdeg = 0;
+    long long offset = 0, size = 0;   // into BrieTable::data
+    double start = 0.0, final = 0.0;
+};
+
+struct BrieTable {
+    std::vector<BrieUnit> units;
+    std::vector<double> data;
+    bool empty() const { return units.empty(); }
+};
+
+// One step of a chain: add sign * position of unit `unit`.
+struct BrieStep {
+    int unit;
+    double sign;
+};
+
+// Read one .brie file (layout 1 or 2) and append its units to `t`.
+void loadBrie(const std::string& path, BrieTable& t);
+
+// The steps that give body `naif` relative to body `center`: the target's
+// chain up to the highest common center (+), the center's chain up to it (-).
+// Throws when either body has no unit.
+std::vector<BrieStep> brieChain(const BrieTable& t, int naif, int center);
+
+// [start, end] ET seconds that every unit of the chain covers.
+void brieCoverage(const BrieTable& t, const std::vector<BrieStep>& chain, double& start, double& end);
+
+// Position (km) of one unit at `et`, Clenshaw over the interval's coefficients.
+inline void brieUnitPosition(const BrieTable& t, const BrieUnit& u, double et, double out[3]) {
+    const double* d = t.data.data() + u.offset;
+    const double r = d[0];
+    long long idx = (long long)((et - d[1]) / (2.0 * r));
+    if (idx < 0) idx = 0;
+    if (idx > u.nIntervals - 1) idx = u.nIntervals - 1;
+    const double s = (et - d[1 + idx]) / r - 1.0;
+    const double s2 = 2.0 * s;
+    const long long n = u.nIntervals;
+    for (int k = 0; k < 3; ++k) {
+        const double* c = d + 1 + n + (long long)k * n * (u.pdeg + 1) + idx;
+        double w = 0.0, x = 0.0;
+        for (int i = u.pdeg; i >= 1; --i) {
+            const double tmp = c[i * n] - w;
+            w = x;
+            x = std::fma(s2, x, tmp);
+        }
+        out[k] = std::fma(s, x, c[0] - w);
+    }
+}
+
+inline void brieChainPosition(const BrieTable& t, const std::vector<BrieStep>& chain, double et, double out[3]) {
+    out[0] = out[1] = out[2] = 0.0;
+    for (const auto& st : chain) {
+        double p[3];
+        brieUnitPosition(t, t.units[size_t(st.unit)], et, p);
+        for (int k = 0; k < 3; ++k) out[k] += st.sign * p[k];
+    }
+}
+
+}  // namespace phx

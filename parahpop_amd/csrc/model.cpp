@@ -287,14 +287,27 @@ void parseConfig(const json::Value& root, const std::string& configDir,
         if (k == "bodies") continue;
         if (k == "constants") {
             model.constantsFile = resolvePath(kv.second.asString(), configDir);
-        } else if (k == "ephemeris" || k == "orientations") {
+        } else if (k == "ephemeris") {
+            // "analytic", or one or more .brie files (Environment.h:38; EphUnit.h:508-560)
+            std::vector<std::string> v;
+            if (kv.second.isString()) v.push_back(kv.second.str);
+            else if (kv.second.isArray() && kv.second.size() > 0)
+                for (const auto& e : *kv.second.arr) v.push_back(e.asString());
+            else throw std::runtime_error("ephemeris must name \"analytic\" or a list of .brie files.");
+            if (v.size() == 1 && lower(v[0]) == "analytic") {
+                model.ephemeris = "analytic";
+            } else {
+                model.ephemeris = "brie";
+                for (const auto& f : v) model.ephemerisFiles.push_back(resolvePath(f, configDir));
+            }
+        } else if (k == "orientations") {
             std::string v;
             if (kv.second.isString()) v = kv.second.str;
             else if (kv.second.isArray() && kv.second.size() == 1) v = kv.second.at(0).asString();
             else throw std::runtime_error(kv.first + " must name one source.");
             if (lower(v) != "analytic")
                 throw std::runtime_error(kv.first + ": only the \"analytic\" source is provided (got " + v + ")");
-            (k == "ephemeris" ? model.ephemeris : model.orientation) = "analytic";
+            model.orientation = "analytic";
         } else if (k == "prefserssb" || k == "preferssb") {
             model.preferSSB = kv.second.asBool();
         } else if (k == "earthcorrections") {
@@ -577,6 +590,16 @@ void loadEnvironment(Environment& env) {
     if (env.model.tidesActive()) {
         env.eop = loadEop(env.model.eopFile);
         env.hasEop = true;
+    }
+    if (env.model.ephemeris == "brie") {
+        for (const auto& f : env.model.ephemerisFiles) loadBrie(f, env.brie);
+        // every configured body (and the Sun and Moon of the tide model) must
+        // chain to the Earth
+        for (const auto& b : env.model.bodies) (void)brieChain(env.brie, b.naif, NAIF_EARTH);
+        if (env.model.tidesActive()) {
+            (void)brieChain(env.brie, NAIF_SUN, NAIF_EARTH);
+            (void)brieChain(env.brie, NAIF_MOON, NAIF_EARTH);
+        }
     }
 }
 

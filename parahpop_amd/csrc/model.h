@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "brie.h"
 #include "json_lite.h"
 
 namespace phx {
@@ -104,7 +105,8 @@ struct Precision {
 struct ModelConfig {
     std::vector<BodySpec> bodies;
     std::string constantsFile;
-    std::string ephemeris = "analytic";
+    std::string ephemeris = "analytic";        // or "brie": Chebyshev tables from ephemerisFiles
+    std::vector<std::string> ephemerisFiles;
     std::string orientation = "analytic";
     bool preferSSB = true;
     // earthCorrections
@@ -136,6 +138,7 @@ struct RunConfig {
 
 struct Environment {
     ModelConfig model;
+    BrieTable brie;        // Chebyshev ephemeris (empty: analytic series)
     ShField sh;            // Earth field (empty degree 0 when not configured)
     bool hasSh = false;
     WeatherTable weather;

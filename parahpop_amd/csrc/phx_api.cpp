@@ -254,6 +254,22 @@ void buildForceModel(phx_ctx& c) {
         fm.shNormalized = c.env.sh.normalized ? 1 : 0;
     }
     fm.tides = m.tidesActive() ? 1 : 0;
+    // Chebyshev ephemeris: chains of the body slots (bodies, then Sun and Moon)
+    c.ephChains.clear();
+    if (!c.env.brie.empty()) {
+        for (int slot = 0; slot < kEphSlots; ++slot) {
+            int naif;
+            if (slot < int(m.bodies.size())) naif = m.bodies[size_t(slot)].naif;
+            else if (slot == kMaxBodies && m.tidesActive()) naif = NAIF_SUN;
+            else if (slot == kMaxBodies + 1 && m.tidesActive()) naif = NAIF_MOON;
+            else continue;
+            auto ch = brieChain(c.env.brie, naif, NAIF_EARTH);
+            if (ch.size() > size_t(kEphSteps))
+                throw std::runtime_error("ephemeris chain of body " + std::to_string(naif) + " is longer than " +
+                                         std::to_string(kEphSteps) + " links");
+            c.ephChains.push_back(ch);
+        }
+    }
     fm.relativity = m.relativity ? 1 : 0;
     fm.muEarthSI = 398600.4415e9;
     fm.radiusSI = 6378.1363e3;
@@ -281,7 +297,22 @@ EpochArgs epochArgs(const phx_ctx& c) {
     a.weather = c.weather.p;
     a.wDays = c.env.hasWeather ? c.env.weather.days : 0;
     a.wStart = c.env.hasWeather ? c.env.weather.startMjdUtc : 0.0;
+    a.eph = c.eph.p;
+    a.ephChain = c.ephChain.p;
     return a;
+}
+
+void checkEphemerisCoverage(const phx_ctx& c, double lo, double hi) {
+    for (const auto& ch : c.ephChains) {
+        double s0, s1;
+        brieCoverage(c.env.brie, ch, s0, s1);
+        if (lo < s0 || hi > s1) {
+            std::ostringstream m;
+            m.precision(17);
+            m << "Epoch range [" << lo << ", " << hi << "] not in the available ephemeris range [" << s0 << ", " << s1 << "]";
+            throw std::runtime_error(m.str());
+        }
+    }
 }
 
 void orderBlocks(const phx_ctx& c, int W, int out[kMaxWaves + 1], bool split) {
@@ -416,6 +447,14 @@ std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, c
         if (!(span > 0) || !std::isfinite(span) || std::ceil(span / step) >= 2147483647.0)
             throw std::invalid_argument("Duration/step cannot be represented by the integration time/index types.");
         count = std::max(count, (long long)std::ceil(span / step) + 1);
+    }
+    if (!c.ephChains.empty()) {
+        double lo = start[0], hi = endEt[0];
+        for (size_t i = 0; i < n; ++i) {
+            lo = std::min(lo, start[i]);
+            hi = std::max(hi, endEt[i]);
+        }
+        checkEphemerisCoverage(c, lo, hi);
     }
     const long long launches = std::min(count, c.run.maxNumSteps);
     // epoch groups: identical (start, end) share one step sequence
@@ -796,6 +835,29 @@ int phx_create(const char* config_json_path, phx_ctx** out) {
         if (uploadTideFactors() != 0) throw HipError("tide factor upload failed");
         loadTables(*c);
         if (c->env.hasSh) buildShTables(*c);
+        if (!c->env.brie.empty()) {
+            std::vector<double> rows(size_t(kEphSlots) * kEphRow, 0.0);
+            const ModelConfig& m = c->env.model;
+            for (int slot = 0; slot < kEphSlots; ++slot) {
+                int naif;
+                if (slot < int(m.bodies.size())) naif = m.bodies[size_t(slot)].naif;
+                else if (slot == kMaxBodies && m.tidesActive()) naif = NAIF_SUN;
+                else if (slot == kMaxBodies + 1 && m.tidesActive()) naif = NAIF_MOON;
+                else continue;
+                const auto ch = brieChain(c->env.brie, naif, NAIF_EARTH);
+                double* row = rows.data() + size_t(slot) * kEphRow;
+                row[0] = double(ch.size());
+                for (size_t k = 0; k < ch.size(); ++k) {
+                    const BrieUnit& u = c->env.brie.units[size_t(ch[k].unit)];
+                    row[1 + 4 * k] = double(u.offset);
+                    row[2 + 4 * k] = double(u.nIntervals);
+                    row[3 + 4 * k] = double(u.pdeg);
+                    row[4 + 4 * k] = ch[k].sign;
+                }
+            }
+            c->eph.upload(c->env.brie.data.data(), c->env.brie.data.size(), c->stream);
+            c->ephChain.upload(rows.data(), rows.size(), c->stream);
+        }
         if (c->fm.drag && c->env.hasWeather) {
             // weather-slot rows for the uneven-epoch path
             const int rows = c->env.weather.days * 8;

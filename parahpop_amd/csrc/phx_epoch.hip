@@ -15,6 +15,38 @@ namespace phx {
 namespace {
 using namespace epochdev;
 
+// Position (km) of one ephemeris slot relative to the Earth at `et`: the chain's
+// body units, each by backward Clenshaw over the coefficients of the interval
+// holding `et` (BodyInterpolator.h:55-100, chebyshev/Interpolator.h:50-90),
+// the same operations as the host evaluation in brie.h.
+__device__ D3 ephPosition(const double* __restrict__ eph, const double* __restrict__ row, double et) {
+    double out[3] = {0.0, 0.0, 0.0};
+    const int steps = int(row[0]);
+    for (int k = 0; k < steps && k < kEphSteps; ++k) {
+        const double* st = row + 1 + 4 * k;
+        const long long n = (long long)st[1];
+        const int pdeg = int(st[2]);
+        const double sign = st[3];
+        const double* d = eph + (long long)st[0];
+        const double r = d[0];
+        long long idx = (long long)((et - d[1]) / (2.0 * r));
+        idx = idx < 0 ? 0 : (idx > n - 1 ? n - 1 : idx);
+        const double s = (et - d[1 + idx]) / r - 1.0;
+        const double s2 = 2.0 * s;
+        for (int c = 0; c < 3; ++c) {
+            const double* q = d + 1 + n + (long long)c * n * (pdeg + 1) + idx;
+            double w = 0.0, x = 0.0;
+            for (int i = pdeg; i >= 1; --i) {
+                const double tmp = q[i * n] - w;
+                w = x;
+                x = fma(s2, x, tmp);
+            }
+            out[c] += sign * fma(s, x, q[0] - w);
+        }
+    }
+    return D3{out[0], out[1], out[2]};
+}
+
 __global__ void __launch_bounds__(64) epochRecordKernel(EpochArgs a, ForceModel fm) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.count) return;
@@ -27,7 +59,7 @@ __global__ void __launch_bounds__(64) epochRecordKernel(EpochArgs a, ForceModel 
     for (int k = 0; k < 3; ++k) rec[REC_OMEGA + k] = w[k];
     double ind[3] = {0.0, 0.0, 0.0};
     for (int b = 0; b < fm.nBodies; ++b) {
-        const D3 p = bodyPos(fm.bodyNaif[b], et);
+        const D3 p = a.eph ? ephPosition(a.eph, a.ephChain + b * kEphRow, et) : bodyPos(fm.bodyNaif[b], et);
         rec[REC_BODY + 3 * b + 0] = p.x;
         rec[REC_BODY + 3 * b + 1] = p.y;
         rec[REC_BODY + 3 * b + 2] = p.z;
@@ -50,7 +82,9 @@ __global__ void __launch_bounds__(64) epochRecordKernel(EpochArgs a, ForceModel 
         } else {
             if (a.solid) {
                 const double T = et / (86400.0 * 36525.0);
-                solidTide(a, tm, R, sunPos(T), moonPos(T), C, S);
+                const D3 sun = a.eph ? ephPosition(a.eph, a.ephChain + kMaxBodies * kEphRow, et) : sunPos(T);
+                const D3 moon = a.eph ? ephPosition(a.eph, a.ephChain + (kMaxBodies + 1) * kEphRow, et) : moonPos(T);
+                solidTide(a, tm, R, sun, moon, C, S);
             }
             if (a.ocean) oceanTide(a, tm, C, S);
         }

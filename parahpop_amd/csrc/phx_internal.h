@@ -69,6 +69,8 @@ struct phx_ctx {
     std::string dataDir;
     // device tables
     phx::DBuf<double> shTerms, shDiag, nrl, freq, waves, terms, eop, weather, nrlSlots;
+    phx::DBuf<double> eph, ephChain;   // Chebyshev ephemeris data and per-slot chains (empty: analytic)
+    std::vector<std::vector<phx::BrieStep>> ephChains;   // host copy, for coverage checks
     phx::DBuf<float> shTerms32;
     phx::DBuf<int> shOrderStart;
     int freqCount[3] = {0, 0, 0};
@@ -128,6 +130,9 @@ EpochArgs epochArgs(const phx_ctx& c);
 // kernel.
 void orderBlocks(const phx_ctx& c, int W, int out[kMaxWaves + 1], bool split);
 // Target validation of driver.cpp:99-112 (throws std::invalid_argument).
+// Throws unless the Chebyshev ephemeris (when configured) covers ET [lo, hi].
+void checkEphemerisCoverage(const phx_ctx& c, double lo, double hi);
+
 void validateTargets(size_t n, const int64_t* id, const double* epoch, const double* state6, const double* mass,
                      const double* drag, const double* srp, const double* cd, const double* cr);
 // Context-wide step-kernel arguments (tables, weather, wave split); the caller

@@ -28,7 +28,17 @@ struct EpochArgs {
     const double* weather;
     int wDays;
     double wStart;
+    // Chebyshev ephemeris (nullptr: analytic series): the body-unit data and, per
+    // slot (the configured bodies, then the Sun and the Moon for the tides), a
+    // row of kEphRow doubles: step count, then {data offset, intervals, degree,
+    // sign} per step of the chain to the Earth
+    const double* eph;
+    const double* ephChain;
 };
+
+constexpr int kEphSteps = 6;
+constexpr int kEphRow = 1 + 4 * kEphSteps;
+constexpr int kEphSlots = kMaxBodies + 2;
 
 int launchEpochRecords(const EpochArgs& a, const ForceModel& fm, hipStream_t stream);
 

@@ -138,6 +138,7 @@ uint64_t Server::submit(const phx_request& q) {
         lo = i == 0 ? st : std::min(lo, st);
         hi = i == 0 ? en : std::max(hi, en);
     }
+    checkEphemerisCoverage(c_, lo, hi);
     r->steps = int(std::min<long long>(steps, c_.run.maxNumSteps));
     r->gridT0 = lo - 2.0 * h_;
     r->rows = (long long)std::ceil((hi - r->gridT0) / h_) + 3;

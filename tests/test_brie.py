@@ -1,0 +1,129 @@
+"""Chebyshev ephemeris in the reference's .brie format: the CBOR reader (both
+layouts), the Clenshaw evaluation against numpy's Chebyshev series, the chains
+to the Earth through the Earth-Moon and solar-system barycenters, coverage and
+missing-body errors (CPU: oracle and host library)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import brie_writer
+
+DAY = 86400.0
+EPOCH0_ET = 4035.0 * DAY - 43200.0
+T0, T1 = EPOCH0_ET - 2 * DAY, EPOCH0_ET + 12 * DAY
+PLANETS = {"MercuryBarycenter": 1, "VenusBarycenter": 2, "MarsBarycenter": 4, "JupiterBarycenter": 5}
+
+
+def _analytic(naif, et):
+    import oracle
+    return oracle.body_position(naif, et)
+
+
+def _config(out, eph, bodies=("Sun", "Moon") + tuple(PLANETS)):
+    from parahpop_amd import synth
+    os.makedirs(out, exist_ok=True)
+    with open(os.path.join(out, "constants.json"), "w") as f:
+        json.dump(synth.CONSTANTS, f)
+    b = {"Earth": {"gravity": True}}
+    for name in bodies:
+        b[name] = {"gravity": True, "gm": synth.PLANET_GM[name]} if name in synth.PLANET_GM else {"gravity": True}
+    model = {"environment": {"bodies": b, "ephemeris": eph, "orientations": "analytic", "constants": "constants.json"}}
+    p = os.path.join(out, "CONFIG.json")
+    synth.write_config(p, model, 3600.0, 60.0, 100)
+    return p
+
+
+@pytest.fixture(scope="module")
+def eph_dir(tmp_path_factory):
+    d = tmp_path_factory.mktemp("brie")
+    units = brie_writer.ssb_units(_analytic, T0, T1, bodies=(10, 301, 1, 2, 4, 5))
+    built2 = brie_writer.write_brie(str(d / "eph2.brie"), units, layout=2)
+    brie_writer.write_brie(str(d / "eph1.brie"), units, layout=1)
+    return d, built2
+
+
+def test_layouts_read_alike_and_reproduce_the_fitted_functions(eph_dir):
+    """Layout 1 and layout 2 files of the same units give the same positions bit
+    for bit; the chains reproduce the functions the units were fitted to (the
+    analytic series) within the fit error, through the barycenter chains."""
+    import oracle
+    d, _ = eph_dir
+    o2 = oracle.Oracle(_config(str(d / "c2"), ["../eph2.brie"]))
+    o1 = oracle.Oracle(_config(str(d / "c1"), ["../eph1.brie"]))
+    rng = np.random.default_rng(2)
+    for et in rng.uniform(T0, T1 - 1.0, 40):
+        for naif in (10, 301, 1, 2, 4, 5):
+            p2 = o2.body_position(naif, et)
+            np.testing.assert_array_equal(o1.body_position(naif, et), p2)
+            ref = _analytic(naif, et)
+            tol = 1e-6 * max(1.0, np.linalg.norm(ref) / 1e6)   # km: 1 mm per 1e6 km of distance
+            assert np.linalg.norm(p2 - ref) < tol, (naif, np.linalg.norm(p2 - ref))
+
+
+def test_clenshaw_equals_numpy_chebval(eph_dir):
+    """The Moon relative to the Earth from the file equals the difference of the
+    two units' series evaluated with numpy's chebval on the written coefficients
+    (interval = floor((t - t0) / 2r), s = (t - start)/r - 1)."""
+    import oracle
+    d, built = eph_dir
+    o = oracle.Oracle(_config(str(d / "c2"), ["../eph2.brie"]))
+    units = {u["target"]: (data, n, u["pdeg"]) for u, data, n in built}
+
+    def series(target, et):
+        data, n, pdeg = units[target]
+        r, starts = data[0], data[1:1 + n]
+        coef = data[1 + n:].reshape(3, pdeg + 1, n)
+        k = int((et - starts[0]) // (2 * r))
+        s = (et - starts[k]) / r - 1.0
+        return np.array([np.polynomial.chebyshev.chebval(s, coef[c, :, k]) for c in range(3)])
+
+    rng = np.random.default_rng(5)
+    for et in rng.uniform(T0, T1 - 1.0, 50):
+        got = o.body_position(301, et)
+        exp = series(301, et) - series(399, et)
+        assert np.linalg.norm(got - exp) <= 1e-12 * np.linalg.norm(exp), (et, got - exp)
+
+
+def test_shifted_body_flows_into_positions_and_forces(tmp_path):
+    """A file whose Moon is displaced by 100 km moves the Moon by 100 km and
+    changes the third-body force (so the file, not the analytic series, is used)."""
+    import oracle
+    shift = lambda naif, et: np.array([100.0, 0.0, 0.0]) if naif == 301 else 0.0
+    units = brie_writer.ssb_units(_analytic, T0, T1, bodies=(10, 301), shift=shift)
+    brie_writer.write_brie(str(tmp_path / "moved.brie"), units)
+    o = oracle.Oracle(_config(str(tmp_path / "c"), ["../moved.brie"], bodies=("Sun", "Moon")))
+    a = oracle.Oracle(_config(str(tmp_path / "a"), "analytic", bodies=("Sun", "Moon")))
+    et = EPOCH0_ET + 3.3 * DAY
+    np.testing.assert_allclose(o.body_position(301, et) - _analytic(301, et), [100.0, 0.0, 0.0], atol=1e-6)
+    r = np.array([7000.0, 100.0, 50.0])
+    fo = o.force(r, np.zeros(3), et, 100.0, 1e-6, 1.0, 2.2, which=2)
+    fa = a.force(r, np.zeros(3), et, 100.0, 1e-6, 1.0, 2.2, which=2)
+    assert np.linalg.norm(fo - fa) > 1e-4 * np.linalg.norm(fa)
+
+
+def test_missing_body_and_coverage_errors(eph_dir, tmp_path):
+    import oracle
+    d, _ = eph_dir
+    with pytest.raises(RuntimeError, match="no chain connects body 6"):
+        oracle.Oracle(_config(str(tmp_path / "m"), [str(d / "eph2.brie")], bodies=("Sun", "SaturnBarycenter")))
+    o = oracle.Oracle(_config(str(tmp_path / "c"), [str(d / "eph2.brie")]))
+    t = {"id": np.array([1]), "epoch": np.array([4035.0]), "state": np.array([[7000.0], [0], [0], [0], [7.5], [0]]),
+         "mass": np.array([100.0]), "drag_area": np.array([1.0]), "srp_area": np.array([1.0]), "cd": np.array([2.2]),
+         "cr": np.array([1.3]), "n": 1}
+    o.propagate(t, 3600.0, threads=1)
+    with pytest.raises(RuntimeError, match="ephemeris range"):
+        o.propagate(t, 20 * DAY, threads=1)
+
+
+def test_library_rejects_missing_ephemeris_bodies(eph_dir, tmp_path):
+    """The host library applies the same chain rules before touching a device."""
+    import ctypes
+    import parahpop_amd
+    d, _ = eph_dir
+    p = _config(str(tmp_path / "m"), [str(d / "eph2.brie")], bodies=("Sun", "SaturnBarycenter"))
+    lib = parahpop_amd.library()
+    h = ctypes.c_void_p()
+    assert lib.phx_create(p.encode(), ctypes.byref(h)) == -1
+    assert "no chain connects body 6" in lib.phx_last_error(None).decode()

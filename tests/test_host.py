@@ -60,7 +60,10 @@ def _create_error(path):
     ({"environment": {"bodies": {"Earth": {"gravity": True}}, "constants": "constants.json", "bogus": 1}}, "Unsupported model.environment option"),
     ({"environment": {"bodies": {"Sun": {"gravity": True}}, "constants": "constants.json"}}, "Earth must be configured"),
     ({"environment": {"bodies": {"Earth": {"gravity": {"shape": {"file": "missing.tab"}}}}, "constants": "constants.json"}}, "Required data file not found"),
-    ({"environment": {"bodies": {"Earth": {"gravity": True}}, "ephemeris": ["de440s.brie"], "constants": "constants.json"}}, "analytic"),
+    ({"environment": {"bodies": {"Earth": {"gravity": True}}, "ephemeris": ["de440s.brie"], "constants": "constants.json"}},
+     "Required data file not found"),
+    ({"environment": {"bodies": {"Earth": {"gravity": True}}, "orientations": ["itrf.brot2"], "constants": "constants.json"}},
+     "analytic"),
     ({"environment": {"bodies": {"Earth": {"gravity": True}, "Moon": {"radiation": True}}, "constants": "constants.json"}}, "Radiation is supported for the Sun only"),
     ({"environment": {"bodies": {"Earth": {"gravity": True}, "Vulcan": {"gravity": True}}, "constants": "constants.json"}}, "Unsupported body"),
     ({"environment": {"bodies": {"Earth": {"gravity": True}}, "constants": "constants.json",
