@@ -18,6 +18,10 @@ Further legs (one JSON object each in the same line):
   serve   Poisson arrivals of requests (1 - 50,000 objects log-uniform, 1 h -
           3 d log-uniform, uneven-batch distributions) on a mixed-precision
           70x70 model: p50/p99 latency, object-steps/s, error vs the FP64 path
+  ephemeris  the headline workload with its body positions from a .brie
+          Chebyshev ephemeris (synthetic circular orbits in the reference's
+          barycenter chains) beside the analytic series: object-steps/s over
+          128 steps and the epoch-record time of each
   alternate  J2-only Earth gravity and drag in a 28-band exponential
           atmosphere over the geodetic altitude (Sun/Moon, SRP with shadow,
           relativity): one day at 60 s for 16,384 and 262,144 objects, the
@@ -286,6 +290,36 @@ def leg_alternate(args, work):
     return out
 
 
+def leg_ephemeris(args, work):
+    import parahpop_amd
+    from parahpop_amd import brie_io, synth
+    day = 86400.0
+    d = os.path.join(work, "w2eph")
+    cfg, tg = synth.make_workload(d, 2, n=args.objects, duration=day)
+    t = synth.load_targets(tg)
+    et0 = 4035.0 * day - 43200.0
+    brie_io.write_brie(os.path.join(d, "eph.brie"), brie_io.ssb_units(brie_io.circular_geocentric, et0 - 2 * day,
+                                                                          et0 + 3 * day))
+    doc = json.load(open(cfg))
+    doc["model"]["environment"]["ephemeris"] = ["eph.brie"]
+    cfg_brie = os.path.join(d, "CONFIG_brie.json")
+    json.dump(doc, open(cfg_brie, "w"))
+    out = {"workload": "full-force 70x70, %d objects, 60 s step; body positions from a .brie file (12 body units, "
+                       "degree 10-13) vs the analytic series" % t["n"], "steps": args.eph_steps}
+    for label, path in (("analytic", cfg), ("brie", cfg_brie)):
+        ctx = parahpop_amd.Context(path)
+        b = ctx.batch(t, day)
+        b.run(4)
+        st = b.run(args.eph_steps)
+        out[label] = {"value": st["object_steps"] / st["kernel_seconds"], "ms_per_step": 1e3 * st["kernel_seconds"] / args.eph_steps,
+                      "epoch_records": st["epoch_records"], "epoch_seconds": st["epoch_seconds"]}
+        log("ephemeris leg: %s %.4g object-steps/s" % (label, out[label]["value"]))
+        ctx.close()
+    out["value"] = out["brie"]["value"]
+    out["brie_vs_analytic"] = out["brie"]["value"] / out["analytic"]["value"]
+    return out
+
+
 def leg_serve(args, work):
     import parahpop_amd
     from parahpop_amd import synth
@@ -358,7 +392,8 @@ def main():
     ap.add_argument("--waves", type=int, default=0, help="waves per 64-object tile (0: library default)")
     ap.add_argument("--steps-per-launch", type=int, default=0)
     ap.add_argument("--split", type=int, default=-1, help="1: split-stage path, 0: fused multi-step kernel (-1: library default)")
-    ap.add_argument("--legs", default="main,mixed,uneven,serve,alternate")
+    ap.add_argument("--legs", default="main,mixed,uneven,serve,alternate,ephemeris")
+    ap.add_argument("--eph-steps", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-objects", type=int, default=1024)
     ap.add_argument("--cpu-seconds", type=float, default=86400.0)
@@ -403,6 +438,8 @@ def main():
         log("serve: p50 %.2f s p99 %.2f s" % (line["serve"]["latency_p50_s"], line["serve"]["latency_p99_s"]))
     if "alternate" in legs:
         line["alternate"] = leg_alternate(args, work)
+    if "ephemeris" in legs:
+        line["ephemeris"] = leg_ephemeris(args, work)
     print(json.dumps(line))
 
 

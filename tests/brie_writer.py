@@ -1,121 +1,2 @@
-"""Test helper: writes Chebyshev ephemerides in the reference's .brie format
-(CBOR; layout 2 as brie/brie/util/cbor.cpp:200-300 reads it, layout 1 as
-brie/brie/states/metadata/EphUnit.h:50-85 reads it), fitted to given position
-functions, with the body-unit data layout of brie/brie/core/BodyInterpolator.h:55-100
-(radius, interval starts, coefficients [component][degree][interval])."""
-import math
-import struct
-
-import numpy as np
-
-
-def cbor(obj):
-    """Minimal CBOR encoder: maps with text keys, arrays, ints, float64, text."""
-    def head(major, n):
-        if n < 24:
-            return bytes([(major << 5) | n])
-        for info, fmt in ((24, ">B"), (25, ">H"), (26, ">I"), (27, ">Q")):
-            if n < (1 << (8 * struct.calcsize(fmt))):
-                return bytes([(major << 5) | info]) + struct.pack(fmt, n)
-        raise ValueError(n)
-    if isinstance(obj, bool):
-        return bytes([0xf5 if obj else 0xf4])
-    if isinstance(obj, (int, np.integer)):
-        return head(0, int(obj)) if obj >= 0 else head(1, -1 - int(obj))
-    if isinstance(obj, (float, np.floating)):
-        return b"\xfb" + struct.pack(">d", float(obj))
-    if isinstance(obj, str):
-        b = obj.encode()
-        return head(3, len(b)) + b
-    if isinstance(obj, (list, tuple, np.ndarray)):
-        return head(4, len(obj)) + b"".join(cbor(x) for x in obj)
-    if isinstance(obj, dict):
-        return head(5, len(obj)) + b"".join(cbor(k) + cbor(v) for k, v in obj.items())
-    raise TypeError(type(obj))
-
-
-def fit_unit(func, t0, t1, interval, pdeg):
-    """Chebyshev fit of func(et) -> (3,) on intervals [t0 + k h, t0 + (k+1) h).
-    Returns the body-unit data array and the number of intervals."""
-    n = int(math.ceil((t1 - t0) / interval))
-    r = 0.5 * interval
-    starts = t0 + interval * np.arange(n)
-    x = np.cos(np.pi * (np.arange(pdeg + 1) + 0.5) / (pdeg + 1))   # Chebyshev nodes
-    coef = np.zeros((3, pdeg + 1, n))
-    for k in range(n):
-        vals = np.array([func(starts[k] + r * (xi + 1.0)) for xi in x])   # (pdeg+1, 3)
-        for d in range(3):
-            coef[d, :, k] = np.polynomial.chebyshev.chebfit(x, vals[:, d], pdeg)
-    data = np.concatenate([[r], starts, coef.reshape(-1)])
-    return data, n
-
-
-def write_brie(path, units, layout=2):
-    """units: dicts with target, center, t0, t1, interval, pdeg, func."""
-    built = []
-    for u in units:
-        data, n = fit_unit(u["func"], u["t0"], u["t1"], u["interval"], u["pdeg"])
-        built.append((u, data, n))
-    if layout == 2:
-        N = len(built)
-        fields = [[], [], [], [], [], [], [], []]
-        reals = [[], []]
-        offset = 0
-        for u, data, n in built:
-            for f, v in enumerate((1, 2, u["target"], u["center"], n, u["pdeg"], offset, len(data))):
-                fields[f].append(int(v))
-            reals[0].append(float(data[1]))
-            reals[1].append(float(data[1] + 2 * data[0] * n))
-            offset += len(data)
-        doc = {"layout": 2, "core": {
-            "metadata": {"nBodyUnits": N, "intMetadata": [v for f in fields for v in f],
-                         "doubleMetadata": reals[0] + reals[1]},
-            "data": np.concatenate([d for _, d, _ in built])}}
-    else:
-        doc = {"layout": 1, "core": [
-            {"metadata": {"frame": 1, "dtype": 2, "target": u["target"], "center": u["center"], "nintervals": n,
-                          "pdeg": u["pdeg"], "startEpoch": float(d[1]), "finalEpoch": float(d[1] + 2 * d[0] * n)},
-             "data": d} for u, d, n in built]}
-    with open(path, "wb") as f:
-        f.write(cbor(doc))
-    return built
-
-
-def ssb_units(geocentric, t0, t1, emrat=81.30056822149722, bodies=(10, 301, 1, 2, 4, 5, 6, 7, 8, 9),
-              shift=None):
-    """Body units in the reference's chain structure (planetary barycenters,
-    the Sun and the Earth-Moon barycenter about the solar-system barycenter, the
-    Earth and the Moon about the Earth-Moon barycenter) whose geocentric
-    differences reproduce geocentric(naif, et). shift(naif, et) adds a
-    displacement to a body's geocentric position."""
-    mu = 1.0 / (1.0 + emrat)
-    day = 86400.0
-
-    def g(naif, et):
-        p = np.asarray(geocentric(naif, et), dtype=float)
-        return p + (shift(naif, et) if shift else 0.0)
-
-    def wobble(et):   # the Sun about the barycenter: an arbitrary smooth motion
-        a = 2 * np.pi * et / (11.86 * 365.25 * day)
-        return 7.4e5 * np.array([np.cos(a), np.sin(a), 0.02 * np.sin(2 * a)])
-
-    def earth_emb(et):
-        return -mu * g(301, et)
-
-    units = [
-        {"target": 3, "center": 0, "interval": 8 * day, "pdeg": 13,
-         "func": lambda et: wobble(et) - g(10, et) - earth_emb(et)},
-        {"target": 399, "center": 3, "interval": 4 * day, "pdeg": 13, "func": earth_emb},
-        {"target": 10, "center": 0, "interval": 16 * day, "pdeg": 11, "func": wobble},
-    ]
-    if 301 in bodies:
-        units.append({"target": 301, "center": 3, "interval": 4 * day, "pdeg": 13,
-                      "func": lambda et: (1.0 - mu) * g(301, et)})
-    for b in bodies:
-        if b in (10, 301):
-            continue
-        units.append({"target": b, "center": 0, "interval": {1: 8, 2: 16, 4: 16}.get(b, 32) * day, "pdeg": 13,
-                      "func": (lambda b: lambda et: wobble(et) - g(10, et) + g(b, et))(b)})
-    for u in units:
-        u["t0"], u["t1"] = t0, t1
-    return units
+"""Test access to the .brie writer of the package."""
+from parahpop_amd.brie_io import cbor, circular_geocentric, fit_unit, ssb_units, write_brie  # noqa: F401
