@@ -151,7 +151,7 @@ inline TimeData timeAt(const Eop& e, double epochEt) {
 
 // ---------------------------------------------------------------------------
 // Analytic Earth orientation (no reference counterpart: the configurations
-// carry no .brot2 file). ICRF -> ITRF: R = W(xp,yp) * R3(ERA) * C(X,Y,s),
+// name no .brot2 file). ICRF -> ITRF: R = W(xp,yp)^T * R3(ERA) * C(X,Y,s)^T,
 // IERS Conventions 2010 eqs. 5.1, 5.10, 5.15 with the IAU 2006 precession
 // part of the CIP coordinates (Table 5.2a polynomial terms) and no nutation.
 // ---------------------------------------------------------------------------
@@ -201,10 +201,12 @@ inline Orientation earthOrientation(const Eop* eop, double et) {
     const double s = -0.5 * X * Y + (0.000094 + 0.00380865 * t) * kArcsec;
     const double a = 1.0 / (1.0 + std::sqrt(1.0 - X * X - Y * Y));
     const M3 Cb = {{1 - a * X * X, -a * X * Y, -X, -a * X * Y, 1 - a * Y * Y, -Y, X, Y, 1 - a * (X * X + Y * Y)}};
-    const M3 C = matmul(rotZ(s), Cb);
+    // GCRS -> CIRS = R3(-s) R3(-E) R2(d) R3(E) (the transpose of eq. 5.10's Q)
+    const M3 C = matmul(rotZ(-s), Cb);
     const M3 E = matmul(rotZ(earthRotationAngle(mjdUt1)), C);
     const double sp = -47e-6 * t * kArcsec;
-    const M3 W = matmul(rotZ(-sp), matmul(rotY(xp * kArcsec), rotX(yp * kArcsec)));
+    // TIRS -> ITRS = R1(-yp) R2(-xp) R3(s') (the transpose of eq. 5.3's W)
+    const M3 W = matmul(rotX(-yp * kArcsec), matmul(rotY(-xp * kArcsec), rotZ(sp)));
     Orientation o;
     o.R = matmul(W, E);
     o.omega = mulT(o.R, V3{0.0, 0.0, kEarthRate});

@@ -94,7 +94,7 @@ __device__ void rotZ(double a, double* M) {
     M[0] = c; M[1] = s; M[2] = 0; M[3] = -s; M[4] = c; M[5] = 0; M[6] = 0; M[7] = 0; M[8] = 1;
 }
 
-// R = W(xp, yp, s') * R3(ERA) * R3(s) * [CIP matrix], omega = R^T (0, 0, w_E)
+// R = R1(-yp) R2(-xp) R3(s') * R3(ERA) * R3(-s) * [CIP matrix], omega = R^T (0, 0, w_E)
 __device__ void orientation(const EpochArgs& a, double et, double* R, double* omega) {
     double mjdUt1, t, xp, yp;
     if (a.eopDays >= 2) {
@@ -116,19 +116,20 @@ __device__ void orientation(const EpochArgs& a, double et, double* R, double* om
     const double aa = 1.0 / (1.0 + sqrt(1.0 - X * X - Y * Y));
     double C[9] = {1 - aa * X * X, -aa * X * Y, -X, -aa * X * Y, 1 - aa * Y * Y, -Y, X, Y, 1 - aa * (X * X + Y * Y)};
     double M[9];
-    rotZ(s, M);
+    // GCRS -> CIRS = R3(-s) R3(-E) R2(d) R3(E) (the transpose of IERS 2010 eq. 5.10's Q)
+    rotZ(-s, M);
     mat3mul(M, C, C);
     rotZ(earthRotationAngle(mjdUt1), M);
     mat3mul(M, C, C);
-    // polar motion W = R3(-s') R2(xp) R1(yp)
+    // TIRS -> ITRS = R1(-yp) R2(-xp) R3(s') (the transpose of eq. 5.3's W)
     const double sp = -47e-6 * t * kArcsec;
     const double cx = cos(xp * kArcsec), sx = sin(xp * kArcsec), cy = cos(yp * kArcsec), sy = sin(yp * kArcsec);
-    double R2[9] = {cx, 0, -sx, 0, 1, 0, sx, 0, cx};
-    double R1[9] = {1, 0, 0, 0, cy, sy, 0, -sy, cy};
+    double R2[9] = {cx, 0, sx, 0, 1, 0, -sx, 0, cx};
+    double R1[9] = {1, 0, 0, 0, cy, -sy, 0, sy, cy};
     double W[9];
-    mat3mul(R2, R1, W);
-    rotZ(-sp, M);
-    mat3mul(M, W, W);
+    rotZ(sp, M);
+    mat3mul(R2, M, W);
+    mat3mul(R1, W, W);
     mat3mul(W, C, R);
     omega[0] = R[6] * kEarthRate;
     omega[1] = R[7] * kEarthRate;

@@ -223,3 +223,41 @@ def test_analytic_ephemeris_bands():
     R, w = oracle.earth_orientation(EPOCH0_ET)
     np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-15)
     assert abs(np.linalg.norm(w) - 7.292115146706979e-5) < 1e-18
+
+
+def test_analytic_orientation_follows_the_cio_conventions():
+    """ICRF -> ITRF without polar motion equals R3(ERA) R3(-(E + s)) R2(d) R3(E)
+    built from explicit rotations (the reference's IERS2000 precession-nutation
+    form PN = R3(-e - s) R2(d) R3(e), orientations/detail/IERS2000Model.h:220-235),
+    with E = atan2(Y, X), d = atan(sqrt((X^2 + Y^2) / (1 - X^2 - Y^2))), the IAU
+    2006 polynomial X, Y, s (IERS 2010 Table 5.2a) and the ERA of IERS 2010 eq.
+    5.15 at UT1 = UTC = TT - 69.184 s; the pole row is the CIP (X, Y, Z)."""
+    import oracle
+    asec = np.pi / 648000.0
+
+    def r3(a):
+        c, s = np.cos(a), np.sin(a)
+        return np.array([[c, s, 0], [-s, c, 0], [0, 0, 1.0]])
+
+    def r2(a):
+        c, s = np.cos(a), np.sin(a)
+        return np.array([[c, 0, -s], [0, 1.0, 0], [s, 0, c]])
+
+    for days in (-3000.0, 0.0, 4035.25, 9000.7):
+        et = days * 86400.0 + 3600.0 * 7.3
+        R, w = oracle.earth_orientation(et)
+        g = np.radians(357.53 + 0.98560028 * (et / 86400.0))
+        tt = et - (0.001657 * np.sin(g) + 0.000014 * np.sin(2 * g))   # TDB - TT, two terms (< 30 us)
+        t = tt / (86400.0 * 36525.0)
+        X = (-0.016617 + t * (2004.191898 + t * (-0.4297829 + t * (-0.19861834)))) * asec
+        Y = (-0.006951 + t * (-0.025896 + t * (-22.4072747 + t * 0.00190059))) * asec
+        s = -0.5 * X * Y + (0.000094 + 0.00380865 * t) * asec
+        E = np.arctan2(Y, X)
+        d = np.arctan(np.sqrt((X * X + Y * Y) / (1.0 - X * X - Y * Y)))
+        PN = r3(-E - s) @ r2(d) @ r3(E)
+        tu = (tt - 69.184) / 86400.0   # UT1 days from J2000 noon
+        era = 2 * np.pi * (0.7790572732640 + 1.00273781191135448 * tu)
+        exp = r3(era) @ PN
+        np.testing.assert_allclose(R, exp, atol=5e-9, rtol=0)
+        np.testing.assert_allclose(R[2], [X, Y, np.sqrt(1 - X * X - Y * Y)], atol=2e-11, rtol=0)
+        np.testing.assert_allclose(w, R[2] * 7.292115146706979e-5, rtol=1e-12)
