@@ -37,6 +37,7 @@ def library():
         lib.orc_body_position.argtypes = [i, d, vp]
         lib.orc_earth_orientation.argtypes = [d, vp, vp]
         lib.orc_ctx_body_position.argtypes = [vp, i, d, vp]
+        lib.orc_ctx_earth_orientation.argtypes = [vp, d, vp, vp]
         lib.orc_geodetic_altitude.argtypes = [vp, vp, d, d]
         lib.orc_geodetic_altitude.restype = d
         lib.orc_exp_density.argtypes = [vp, d]
@@ -94,6 +95,11 @@ class Oracle:
         if self.lib.orc_ctx_body_position(self.h, int(naif), float(et), _p(p)) != 0:
             raise RuntimeError(self.lib.orc_last_error().decode())
         return p
+
+    def earth_orientation(self, et):
+        R, w = np.zeros(9), np.zeros(3)
+        self.lib.orc_ctx_earth_orientation(self.h, float(et), _p(R), _p(w))
+        return R.reshape(3, 3), w
 
     def exp_density(self, h):
         return self.lib.orc_exp_density(self.h, float(h))

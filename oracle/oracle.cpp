@@ -45,6 +45,16 @@ struct orc_ctx {
     std::map<int, std::vector<phx::BrieStep>> ephChains;   // Chebyshev ephemeris chains to the Earth
 };
 
+// ICRF -> ITRF rotation and Earth rate: the configured IERS2000 unit, or the
+// analytic model (with the EOP table when loaded).
+Orientation contextOrientation(const orc_ctx& c, double et) {
+    if (!c.env.brot.empty()) {
+        const phx::BrotIers& u = c.env.brot.iers[0];
+        return iers2000Orientation(c.env.brot.data.data() + u.offset, u.nNut, u.nErp, et);
+    }
+    return earthOrientation(c.eop.days >= 2 ? &c.eop : nullptr, et);
+}
+
 // Geocentric position (km, ICRF) of a body: the configured Chebyshev
 // ephemeris (brie.h), or the analytic series.
 V3 bodyPosition(const orc_ctx& c, int naif, double et) {
@@ -201,7 +211,7 @@ V3 acceleration(const orc_ctx& c, V3 pos, V3 vel, double et, const Params& p, in
     bool haveOri = false;
     Orientation ori;
     auto orientation = [&]() -> const Orientation& {
-        if (!haveOri) { ori = earthOrientation(c.eop.days >= 2 ? &c.eop : nullptr, et); haveOri = true; }
+        if (!haveOri) { ori = contextOrientation(c, et); haveOri = true; }
         return ori;
     };
     for (const auto& b : m.bodies) {
@@ -353,6 +363,12 @@ int orc_propagate(orc_ctx* ctx, size_t n, const int64_t* id, const double* epoch
             nextDt[i] = std::min(step, span);
             count = std::max(count, (long long)std::ceil(span / step) + 1);
         }
+        if (!ctx->env.brot.empty()) {
+            const phx::BrotIers& u = ctx->env.brot.iers[0];
+            for (size_t i = 0; i < n; ++i)
+                if (cur[i] < u.start || end[i] > u.final)
+                    throw std::runtime_error("Epoch not in the available orientation range");
+        }
         // every evaluated epoch must lie in the ephemeris intervals (Interpolators.h:65-76)
         for (const auto& kv : ctx->ephChains) {
             double s0, s1;
@@ -452,4 +468,11 @@ int orc_ctx_body_position(const orc_ctx* ctx, int naif, double et, double* pos3)
         g_error = e.what();
         return -1;
     }
+}
+
+int orc_ctx_earth_orientation(const orc_ctx* ctx, double et, double* R9, double* omega3) {
+    const Orientation o = contextOrientation(*ctx, et);
+    for (int i = 0; i < 9; ++i) R9[i] = o.R.m[i];
+    omega3[0] = o.omega.x; omega3[1] = o.omega.y; omega3[2] = o.omega.z;
+    return 0;
 }

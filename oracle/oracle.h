@@ -45,6 +45,10 @@ int orc_earth_orientation(double et, double* R9, double* omega3);
 /* Geocentric body position (km) from the context's ephemeris: its Chebyshev
  * tables when the configuration names .brie files, else the analytic series. */
 int orc_ctx_body_position(const orc_ctx* ctx, int naif, double et, double* pos3);
+/* ICRF -> ITRF rotation (row-major) and Earth rate from the context's
+ * orientation: its IERS2000 unit when the configuration names a .brot2 file,
+ * else the analytic model. */
+int orc_ctx_earth_orientation(const orc_ctx* ctx, double et, double* R9, double* omega3);
 /* Geodetic altitude (km) of r above the ellipsoid (equatorial radius Req km,
  * flattening f) whose polar axis is the unit vector pole, r and pole in one frame. */
 double orc_geodetic_altitude(const double* r3, const double* pole3, double Req, double f);

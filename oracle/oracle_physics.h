@@ -27,6 +27,7 @@
 #pragma once
 
 #include <cmath>
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -210,6 +211,87 @@ inline Orientation earthOrientation(const Eop* eop, double et) {
     Orientation o;
     o.R = matmul(W, E);
     o.omega = mulT(o.R, V3{0.0, 0.0, kEarthRate});
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// IERS2000 Earth orientation from a .brot2 unit —
+// brie/orientations/detail/IERS2000Model.h:120-300,380-400,505-560 (TT model
+// time, prelude, R = PM ER PN, omega = vee(Rdot^T R)) and
+// brie/orientations/detail/LagrangeInterp.h:25-150 with
+// parm/interpolate/lagrange/Interpolator.h:45-95,416-432,815-835 (six-node
+// uniform nutation stencil, four-node ERP stencil over the node keys).
+// payload: X,Y,s [n1 each] | xp,yp,dUT1,dX,dY [n2 each] | x0,h,x0e,he | keys[n2]
+// ---------------------------------------------------------------------------
+// value and d/dq of the Lagrange polynomial through nodes (q_m, y_m), m < np
+inline void lagrange(int np, const double* nodes, const double* y, int stride, int nch, double q, double* val, double* der) {
+    for (int c = 0; c < nch; ++c) val[c] = der[c] = 0.0;
+    for (int j = 0; j < np; ++j) {
+        double prod = 1.0, dprod = 0.0, denom = 1.0;
+        for (int m = 0; m < np; ++m) {
+            if (m == j) continue;
+            const double f = q - nodes[m];
+            dprod = std::fma(dprod, f, prod);
+            prod *= f;
+            denom *= nodes[j] - nodes[m];
+        }
+        const double wv = prod * (1.0 / denom), wd = dprod * (1.0 / denom);
+        for (int c = 0; c < nch; ++c) {
+            val[c] = std::fma(wv, y[c * stride + j], val[c]);
+            der[c] = std::fma(wd, y[c * stride + j], der[c]);
+        }
+    }
+}
+
+inline Orientation iers2000Orientation(const double* d, int n1, int n2, double et) {
+    const double day = 86400.0;
+    const double tdbMjd = et / day + 0.5;
+    const double g = kDeg * (357.53 + 0.9856003 * (tdbMjd - 0.5));
+    const double corr = (0.001658 * std::sin(g) + 0.000014 * std::sin(2.0 * g)) / day;
+    double frac = std::fmod(et, day) / day + 0.5;
+    frac -= std::floor(frac);
+    const double ttFrac = frac - corr, tt = tdbMjd - corr;
+    const double* k = d + 3 * n1 + 5 * n2;
+    const double x0 = k[0], h = k[1];
+    long lo = (long)std::floor((tt - x0) / h) - 2;
+    lo = std::max(0L, std::min(lo, (long)n1 - 6));
+    const double localNodes[6] = {0, 1, 2, 3, 4, 5};
+    double nut[3], nutd[3];
+    lagrange(6, localNodes, d + lo, n1, 3, (tt - (x0 + double(lo) * h)) / h, nut, nutd);
+    const double* keys = k + 4;
+    const long last = (long)(std::upper_bound(keys, keys + n2, tt) - keys) - 1;
+    const long st = std::max(0L, std::min(last - 1, (long)n2 - 4));
+    double erp[5], erpd[5];
+    lagrange(4, keys + st, d + 3 * n1 + st, n2, 5, tt, erp, erpd);
+    const double X = nut[0] + erp[3], Y = nut[1] + erp[4], s = nut[2];
+    const double Xd = (nutd[0] / h + erpd[3]) / day, Yd = (nutd[1] / h + erpd[4]) / day, sd = nutd[2] / h / day;
+    const double xp = erp[0], yp = erp[1], dut1 = erp[2];
+    const double xpd = erpd[0] / day, ypd = erpd[1] / day, dut1d = erpd[2] / day;
+    const double r2 = X * X + Y * Y, rr = std::sqrt(r2), fr = 1.0 / std::sqrt(1.0 - r2);
+    const double e = r2 > 0 ? std::atan2(Y, X) : 0.0, dd = std::atan(rr * fr);
+    const double edot = r2 > 0 ? (Yd * X - Xd * Y) / r2 : 0.0, ddot = (X * Xd + Y * Yd) * fr / rr;
+    const double mm = -e - s, mmd = -edot - sd;
+    double theta = kTwoPi * (std::fmod(ttFrac + dut1 - 0.5, 1.0) + 0.7790572732640 + 0.00273781191135448 * (tt + dut1 - 0.5));
+    theta -= kTwoPi * std::floor(theta / kTwoPi);
+    const double thetad = kTwoPi * ((1.0 / day + dut1d) + 0.00273781191135448 / day);
+    const double sp = -6.24e-15 * (tt - 0.5), spd = -7.2205e-20;
+    // PN = R3(mm) R2(dd) R3(e) and its derivative by the product rule
+    const M3 PN = matmul(rotZ(mm), matmul(rotY(dd), rotZ(e)));
+    auto dRotZ = [](double a, double ad) { const double c = std::cos(a), s = std::sin(a); return M3{{-s * ad, c * ad, 0, -c * ad, -s * ad, 0, 0, 0, 0}}; };
+    auto dRotY = [](double a, double ad) { const double c = std::cos(a), s = std::sin(a); return M3{{-s * ad, 0, -c * ad, 0, 0, 0, c * ad, 0, -s * ad}}; };
+    auto add = [](const M3& A, const M3& B) { M3 C; for (int i = 0; i < 9; ++i) C.m[i] = A.m[i] + B.m[i]; return C; };
+    const M3 PNd = add(add(matmul(dRotZ(mm, mmd), matmul(rotY(dd), rotZ(e))), matmul(rotZ(mm), matmul(dRotY(dd, ddot), rotZ(e)))),
+                       matmul(rotZ(mm), matmul(rotY(dd), dRotZ(e, edot))));
+    const M3 ER = rotZ(theta), ERd = dRotZ(theta, thetad);
+    const M3 PM = {{1.0, sp, xp, -sp, 1.0, -yp, -xp, yp, 1.0}};
+    const M3 PMd = {{0.0, spd, xpd, -spd, 0.0, -ypd, -xpd, ypd, 0.0}};
+    Orientation o;
+    o.R = matmul(PM, matmul(ER, PN));
+    const M3 Rd = add(add(matmul(PM, matmul(ERd, PN)), matmul(PMd, matmul(ER, PN))), matmul(PM, matmul(ER, PNd)));
+    M3 M;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) M.m[3 * i + j] = Rd.m[i] * o.R.m[j] + Rd.m[3 + i] * o.R.m[3 + j] + Rd.m[6 + i] * o.R.m[6 + j];
+    o.omega = V3{0.5 * (M.m[7] - M.m[5]), 0.5 * (M.m[2] - M.m[6]), 0.5 * (M.m[3] - M.m[1])};
     return o;
 }
 

@@ -147,3 +147,44 @@ def circular_geocentric(naif, et):
               6: (9.537, 10759.2, 4.0), 7: (19.19, 30688.5, 5.1), 8: (30.07, 60182.0, 2.9), 9: (39.48, 90560.0, 3.7)}
     au, period, phase = orbits[naif]
     return sun + circ(au * 1.495978707e8, period, phase, np.radians(23.44))
+
+
+def write_brot2(path, tt0, tt1, channels, h_nut=0.5, h_erp=1.0, erp_keys=None, target=3000):
+    """IERS2000 Earth orientation (.brot2, layout 2, one unit_type-4 unit) with
+    the dual-table payload of third_party/brie/brie/orientations/detail/LagrangeInterp.h:25-40:
+    channels(tt) -> (X, Y, s, xp, yp, UT1-TT [days], dX, dY) at TT MJD2000 days;
+    nutation nodes every h_nut days from tt0, ERP nodes at erp_keys (default
+    every h_erp days). The envelope (ET seconds) leaves three nodes of margin."""
+    nodes_nut = tt0 + h_nut * np.arange(int(math.ceil((tt1 - tt0) / h_nut)) + 1)
+    keys = np.asarray(erp_keys, dtype=float) if erp_keys is not None else \
+        tt0 + h_erp * np.arange(int(math.ceil((tt1 - tt0) / h_erp)) + 1)
+    nut = np.array([channels(t)[:3] for t in nodes_nut]).T          # (3, n1)
+    erp = np.array([channels(t)[3:] for t in keys]).T               # (5, n2)
+    n1, n2 = nut.shape[1], erp.shape[1]
+    data = np.concatenate([nut.reshape(-1), erp.reshape(-1), [tt0, h_nut, keys[0], h_erp], keys])
+    lo = max(nodes_nut[2], keys[1])
+    hi = min(nodes_nut[-4], keys[-3])
+    ints = [4, target, 0, 1, n1, n2, 5, 3, 0, 0, 0, len(data)]
+    doc = {"layout": 2, "core": {"metadata": {"nBodyUnits": 1, "intMetadata": ints,
+                                              "doubleMetadata": [(lo - 0.5) * 86400.0, (hi - 0.5) * 86400.0]},
+                                 "data": data}}
+    with open(path, "wb") as f:
+        f.write(cbor(doc))
+    return data
+
+
+def iau2006_channels(xp_arcsec=0.0, yp_arcsec=0.0, wobble=False):
+    """Channels of the analytic Earth orientation: IAU 2006 X, Y, s polynomials
+    (IERS 2010 Table 5.2a), UT1 = TT - 69.184 s, optional polar motion (a
+    constant, or a one-year wobble of that amplitude) and zero CIP corrections."""
+    asec = np.pi / 648000.0
+
+    def ch(tt):
+        t = (tt - 0.5) / 36525.0
+        X = (-0.016617 + t * (2004.191898 + t * (-0.4297829 + t * (-0.19861834)))) * asec
+        Y = (-0.006951 + t * (-0.025896 + t * (-22.4072747 + t * 0.00190059))) * asec
+        s = -0.5 * X * Y + (0.000094 + 0.00380865 * t) * asec
+        a = 2 * np.pi * tt / 433.0 if wobble else 0.0
+        return (X, Y, s, xp_arcsec * asec * np.cos(a), yp_arcsec * asec * np.sin(a) if wobble else yp_arcsec * asec,
+                -69.184 / 86400.0, 0.0, 0.0)
+    return ch

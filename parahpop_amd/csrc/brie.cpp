@@ -232,6 +232,72 @@ void loadBrie(const std::string& path, BrieTable& t) {
     }
 }
 
+// Orientation metadata: 12 integer fields per unit (UNITTYPE, TARGET, eight
+// unit-specific fields, DATAOFFSET, DATASIZE) and the envelope
+// (orientations/metadata/Layout.h:12-98).
+void loadBrot2(const std::string& path, BrotTable& t) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("Failed to open file: " + path);
+    std::vector<uint8_t> bytes((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    Cbor c{bytes.data(), bytes.data() + bytes.size(), path};
+    while (c.p < c.end && (*c.p >> 5) == 4 && (*c.p & 31) == 1) ++c.p;
+    const uint64_t pairs = c.head(5);
+    int layout = 0;
+    long long nUnits = -1;
+    std::vector<double> ints, reals, data;
+    for (uint64_t i = 0; i < pairs; ++i) {
+        const std::string k = c.text();
+        if (k == "layout") {
+            layout = int(c.integer());
+        } else if (k == "core") {
+            if (layout != 2) c.fail("only layout-2 orientation files are supported");
+            const uint64_t cp = c.head(5);
+            for (uint64_t j = 0; j < cp; ++j) {
+                const std::string ck = c.text();
+                if (ck == "metadata") {
+                    const uint64_t mp = c.head(5);
+                    for (uint64_t q = 0; q < mp; ++q) {
+                        const std::string mk = c.text();
+                        if (mk == "nBodyUnits") nUnits = c.integer();
+                        else if (mk == "intMetadata") c.numbers(ints);
+                        else if (mk == "doubleMetadata") c.numbers(reals);
+                        else c.skip();
+                    }
+                } else if (ck == "data") {
+                    c.numbers(data);
+                } else {
+                    c.skip();
+                }
+            }
+        } else {
+            c.skip();
+        }
+    }
+    constexpr int kInt = 12;
+    if (nUnits <= 0 || ints.size() != size_t(kInt * nUnits) || reals.size() != size_t(2 * nUnits))
+        c.fail("orientation metadata sizes do not match nBodyUnits");
+    const long long base = (long long)t.data.size();
+    for (long long u = 0; u < nUnits; ++u) {
+        auto f = [&](int field) { return (long long)ints[size_t(field * nUnits + u)]; };
+        if (f(0) != 4 || f(2) != 0) continue;   // unit_type 4, model 0 (IERS2000)
+        BrotIers b;
+        b.target = int(f(1));
+        b.nNut = int(f(4));
+        b.nErp = int(f(5));
+        if (f(6) != 5 || f(7) != 3) c.fail("IERS2000 unit with Lagrange degrees other than 5 and 3");
+        b.offset = base + f(10);
+        const long long size = f(11);
+        b.start = reals[size_t(u)];
+        b.final = reals[size_t(nUnits + u)];
+        const long long need = 3LL * b.nNut + 5LL * b.nErp + 4 + b.nErp;
+        if (b.nNut < 6 || b.nErp < 4 || size < need || f(10) + size > (long long)data.size())
+            c.fail("IERS2000 unit with inconsistent sizes");
+        t.iers.push_back(b);
+    }
+    t.data.insert(t.data.end(), data.begin(), data.end());
+    if (t.iers.empty()) throw std::runtime_error(path + ": no IERS2000 orientation unit");
+}
+
 namespace {
 
 int unitOf(const BrieTable& t, int naif) {

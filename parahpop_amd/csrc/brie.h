@@ -42,6 +42,27 @@ struct BrieStep {
     double sign;
 };
 
+// IERS2000 Earth orientation unit of a .brot2 file (unit_type 4, model 0):
+// dual-table Lagrange payload from `offset` (orientations/detail/LagrangeInterp.h:25-40):
+//   X[nNut] Y[nNut] s[nNut] | xp[nErp] yp[nErp] dUT1[nErp] dX[nErp] dY[nErp] |
+//   x0_nut h_nut x0_erp h_erp | ERP node keys[nErp]     (TT MJD2000 days)
+struct BrotIers {
+    int target = 0;
+    int nNut = 0, nErp = 0;
+    long long offset = 0;
+    double start = 0.0, final = 0.0;   // envelope, ET seconds
+};
+
+struct BrotTable {
+    std::vector<BrotIers> iers;
+    std::vector<double> data;
+    bool empty() const { return iers.empty(); }
+};
+
+// Read one .brot2 file (layout 2) and append its IERS2000 units to `t`; other
+// unit types are skipped.
+void loadBrot2(const std::string& path, BrotTable& t);
+
 // Read one .brie file (layout 1 or 2) and append its units to `t`.
 void loadBrie(const std::string& path, BrieTable& t);
 

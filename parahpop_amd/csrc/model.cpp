@@ -301,13 +301,18 @@ void parseConfig(const json::Value& root, const std::string& configDir,
                 for (const auto& f : v) model.ephemerisFiles.push_back(resolvePath(f, configDir));
             }
         } else if (k == "orientations") {
-            std::string v;
-            if (kv.second.isString()) v = kv.second.str;
-            else if (kv.second.isArray() && kv.second.size() == 1) v = kv.second.at(0).asString();
-            else throw std::runtime_error(kv.first + " must name one source.");
-            if (lower(v) != "analytic")
-                throw std::runtime_error(kv.first + ": only the \"analytic\" source is provided (got " + v + ")");
-            model.orientation = "analytic";
+            // "analytic", or .brot2 files holding an IERS2000 unit (orientations/EphUnit.h:405-445)
+            std::vector<std::string> v;
+            if (kv.second.isString()) v.push_back(kv.second.str);
+            else if (kv.second.isArray() && kv.second.size() > 0)
+                for (const auto& e : *kv.second.arr) v.push_back(e.asString());
+            else throw std::runtime_error("orientations must name \"analytic\" or a list of .brot2 files.");
+            if (v.size() == 1 && lower(v[0]) == "analytic") {
+                model.orientation = "analytic";
+            } else {
+                model.orientation = "brot2";
+                for (const auto& f : v) model.orientationFiles.push_back(resolvePath(f, configDir));
+            }
         } else if (k == "prefserssb" || k == "preferssb") {
             model.preferSSB = kv.second.asBool();
         } else if (k == "earthcorrections") {
@@ -363,6 +368,7 @@ void parseConfig(const json::Value& root, const std::string& configDir,
                 else if (q == "radius") b.radius = p.second.asDouble();
                 else if (q == "soi") b.soi = p.second.asDouble();
                 else if (q == "j2") b.j2 = p.second.asDouble();
+                else if (q == "orientation") (void)p.second.asString();  // frame name; Earth uses the IERS2000 unit
                 else if (q == "amean") (void)p.second.asDouble();  // accepted; not used by these forces
                 else throw std::runtime_error("Unsupported body option: " + p.first);
             }
@@ -590,6 +596,12 @@ void loadEnvironment(Environment& env) {
     if (env.model.tidesActive()) {
         env.eop = loadEop(env.model.eopFile);
         env.hasEop = true;
+    }
+    if (env.model.orientation == "brot2") {
+        for (const auto& f : env.model.orientationFiles) loadBrot2(f, env.brot);
+        if (env.brot.iers.size() != 1)
+            throw std::runtime_error("orientations: expected one IERS2000 Earth orientation unit, found " +
+                                     std::to_string(env.brot.iers.size()));
     }
     if (env.model.ephemeris == "brie") {
         for (const auto& f : env.model.ephemerisFiles) loadBrie(f, env.brie);

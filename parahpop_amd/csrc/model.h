@@ -107,7 +107,8 @@ struct ModelConfig {
     std::string constantsFile;
     std::string ephemeris = "analytic";        // or "brie": Chebyshev tables from ephemerisFiles
     std::vector<std::string> ephemerisFiles;
-    std::string orientation = "analytic";
+    std::string orientation = "analytic";      // or "brot2": IERS2000 tables from orientationFiles
+    std::vector<std::string> orientationFiles;
     bool preferSSB = true;
     // earthCorrections
     bool solidTides = false, oceanTides = false, relativity = false;
@@ -139,6 +140,7 @@ struct RunConfig {
 struct Environment {
     ModelConfig model;
     BrieTable brie;        // Chebyshev ephemeris (empty: analytic series)
+    BrotTable brot;        // IERS2000 Earth orientation (empty: analytic model)
     ShField sh;            // Earth field (empty degree 0 when not configured)
     bool hasSh = false;
     WeatherTable weather;

@@ -299,10 +299,23 @@ EpochArgs epochArgs(const phx_ctx& c) {
     a.wStart = c.env.hasWeather ? c.env.weather.startMjdUtc : 0.0;
     a.eph = c.eph.p;
     a.ephChain = c.ephChain.p;
+    a.iers = c.iers.p;
+    a.iersNut = c.env.brot.empty() ? 0 : c.env.brot.iers[0].nNut;
+    a.iersErp = c.env.brot.empty() ? 0 : c.env.brot.iers[0].nErp;
     return a;
 }
 
 void checkEphemerisCoverage(const phx_ctx& c, double lo, double hi) {
+    if (!c.env.brot.empty()) {
+        const BrotIers& u = c.env.brot.iers[0];
+        if (lo < u.start || hi > u.final) {
+            std::ostringstream m;
+            m.precision(17);
+            m << "Epoch range [" << lo << ", " << hi << "] not in the available orientation range [" << u.start << ", "
+              << u.final << "]";
+            throw std::runtime_error(m.str());
+        }
+    }
     for (const auto& ch : c.ephChains) {
         double s0, s1;
         brieCoverage(c.env.brie, ch, s0, s1);
@@ -448,7 +461,7 @@ std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, c
             throw std::invalid_argument("Duration/step cannot be represented by the integration time/index types.");
         count = std::max(count, (long long)std::ceil(span / step) + 1);
     }
-    if (!c.ephChains.empty()) {
+    if (!c.ephChains.empty() || !c.env.brot.empty()) {
         double lo = start[0], hi = endEt[0];
         for (size_t i = 0; i < n; ++i) {
             lo = std::min(lo, start[i]);
@@ -857,6 +870,11 @@ int phx_create(const char* config_json_path, phx_ctx** out) {
             }
             c->eph.upload(c->env.brie.data.data(), c->env.brie.data.size(), c->stream);
             c->ephChain.upload(rows.data(), rows.size(), c->stream);
+        }
+        if (!c->env.brot.empty()) {
+            const BrotIers& u = c->env.brot.iers[0];
+            const size_t len = size_t(3 * u.nNut + 5 * u.nErp + 4 + u.nErp);
+            c->iers.upload(c->env.brot.data.data() + u.offset, len, c->stream);
         }
         if (c->fm.drag && c->env.hasWeather) {
             // weather-slot rows for the uneven-epoch path

@@ -54,7 +54,8 @@ __global__ void __launch_bounds__(64) epochRecordKernel(EpochArgs a, ForceModel 
     double* rec = a.out + size_t(i) * REC_SIZE;
     for (int k = 0; k < REC_SIZE; ++k) rec[k] = 0.0;
     double R[9], w[3];
-    orientation(a, et, R, w);
+    if (a.iers) iersOrientation(a, et, R, w);
+    else orientation(a, et, R, w);
     for (int k = 0; k < 9; ++k) rec[REC_R + k] = R[k];
     for (int k = 0; k < 3; ++k) rec[REC_OMEGA + k] = w[k];
     double ind[3] = {0.0, 0.0, 0.0};

@@ -136,6 +136,136 @@ __device__ void orientation(const EpochArgs& a, double et, double* R, double* om
     omega[2] = R[8] * kEarthRate;
 }
 
+// ---- IERS2000 orientation from a .brot2 unit ----
+// Lagrange value and derivative sweep over NP stencil nodes (lagrange/Interpolator.h:45-95):
+// uniform grids in the local coordinate (nodes 0..NP-1), non-uniform ones over the keys.
+template <int NP, int NCH, bool UNIFORM>
+__device__ inline void lagrangeSweep(const double* __restrict__ vals, int n, int lo, double q,
+                                     const double* __restrict__ keys, double* val, double* der) {
+    for (int c = 0; c < NCH; ++c) val[c] = der[c] = 0.0;
+    for (int j = 0; j < NP; ++j) {
+        double prod = 1.0, dprod = 0.0, denom = 1.0;
+        for (int m = 0; m < NP; ++m) {
+            if (m == j) continue;
+            double f, dfac;
+            if (UNIFORM) {
+                f = q - double(m);
+                dfac = double(j - m);
+            } else {
+                const double xm = keys[lo + m];
+                f = q - xm;
+                dfac = keys[lo + j] - xm;
+            }
+            dprod = fma(dprod, f, prod);
+            prod *= f;
+            denom *= dfac;
+        }
+        const double invD = 1.0 / denom;
+        const double wv = prod * invD, wd = dprod * invD;
+        for (int c = 0; c < NCH; ++c) {
+            const double y = vals[c * n + lo + j];
+            val[c] = fma(wv, y, val[c]);
+            der[c] = fma(wd, y, der[c]);
+        }
+    }
+}
+
+// ICRF -> ITRF R = PM ER PN and omega = vee(Rdot^T R) from the interpolated
+// channels [X, Y, s, xp, yp, UT1-TT (days), dX, dY] and their per-day rates
+// (orientations/detail/IERS2000Model.h:120-300,380-400), at the TT model time of
+// the TDB epoch (:505-560).
+__device__ void iersOrientation(const EpochArgs& a, double et, double* R, double* omega) {
+    const double* d = a.iers;
+    const int n1 = a.iersNut, n2 = a.iersErp;
+    const double invSec = 1.0 / 86400.0;
+    const double tdbMjd = et * invSec + 0.5;
+    const double g = (kPi / 180.0) * (357.53 + 0.9856003 * (tdbMjd - 0.5));
+    const double ttCorr = (0.001658 * sin(g) + 0.000014 * sin(2.0 * g)) * invSec;
+    double frac = fmod(et, 86400.0) * invSec + 0.5;
+    frac -= floor(frac);
+    const double ttFrac = frac - ttCorr;
+    const double tt = tdbMjd - ttCorr;
+    const double* cst = d + 3 * n1 + 5 * n2;
+    // nutation: uniform grid, six-node stencil centred on the bracketing interval
+    const double x0 = cst[0], h = cst[1];
+    long lo = long(floor((tt - x0) / h)) - 2;
+    lo = lo < 0 ? 0 : (lo > n1 - 6 ? n1 - 6 : lo);
+    const double u = (tt - (x0 + double(lo) * h)) / h;
+    double nv[3], nd[3];
+    lagrangeSweep<6, 3, true>(d, n1, int(lo), u, nullptr, nv, nd);
+    const double invH = 1.0 / h;
+    // ERP: node keys searched (largest key <= tt), four-node stencil
+    const double* keys = cst + 4;
+    int blo = 0, bhi = n2;
+    while (blo < bhi) {
+        const int mid = blo + ((bhi - blo) >> 1);
+        if (keys[mid] <= tt) blo = mid + 1;
+        else bhi = mid;
+    }
+    int st = blo - 1 - 1;
+    st = st < 0 ? 0 : (st > n2 - 4 ? n2 - 4 : st);
+    double ev[5], ed[5];
+    lagrangeSweep<4, 5, false>(d + 3 * n1, n2, st, tt, keys, ev, ed);
+    // prelude (:130-185)
+    const double X = nv[0], Y = nv[1], s = nv[2];
+    const double Xd = nd[0] * invH * invSec, Yd = nd[1] * invH * invSec, sd = nd[2] * invH * invSec;
+    const double xp = ev[0], yp = ev[1], dut1 = ev[2], dX = ev[3], dY = ev[4];
+    const double xpd = ed[0] * invSec, ypd = ed[1] * invSec, dut1d = ed[2] * invSec, dXd = ed[3] * invSec, dYd = ed[4] * invSec;
+    const double x = X + dX, y = Y + dY, xdot = Xd + dXd, ydot = Yd + dYd;
+    const double r2 = x * x + y * y, rr = sqrt(r2), fr = 1.0 / sqrt(1.0 - r2);
+    const double e = r2 > 0.0 ? atan2(y, x) : 0.0;
+    const double dd = atan(rr * fr);
+    const double edot = r2 > 0.0 ? (ydot * x - xdot * y) / r2 : 0.0;
+    const double ddot = (x * xdot + y * ydot) * fr / rr;
+    const double mems = -e - s, memsdot = -edot - sd;
+    const double ut1jd = tt + dut1 - 0.5;
+    const double f = fmod(ttFrac + dut1 - 0.5, 1.0);
+    const double twoPi = 2.0 * kPi;
+    double theta = twoPi * (f + 0.7790572732640 + 0.00273781191135448 * ut1jd);
+    theta = theta - twoPi * floor(theta / twoPi);
+    const double thetadot = twoPi * ((invSec + dut1d) + 0.00273781191135448 * invSec);
+    const double sp = -6.24e-15 * (tt - 0.5), spdot = -7.2205e-20;
+    // matrices (:187-262)
+    const double s1 = sin(e), c1 = cos(e), s2 = sin(dd), c2 = cos(dd), s3 = sin(mems), c3 = cos(mems);
+    const double PN[9] = {c1 * c2 * c3 - s1 * s3, c1 * s3 + s1 * c2 * c3, -c3 * s2,
+                          -s1 * c3 - c1 * c2 * s3, c1 * c3 - s1 * c2 * s3, s2 * s3,
+                          s2 * c1, s2 * s1, c2};
+    double PNd[9];
+    {
+        const double c1d = -s1 * edot, s1d = c1 * edot, c2d = -s2 * ddot, s2d = c2 * ddot, c3d = -s3 * memsdot, s3d = c3 * memsdot;
+        const double c2s3 = c2 * s3, c2c3 = c2 * c3, c2s3d = c2d * s3 + c2 * s3d, c2c3d = c2d * c3 + c2 * c3d;
+        PNd[0] = -s1d * s3 - s1 * s3d + c1d * c2c3 + c1 * c2c3d;
+        PNd[1] = c1d * s3 + c1 * s3d + s1d * c2c3 + s1 * c2c3d;
+        PNd[2] = -c3d * s2 - c3 * s2d;
+        PNd[3] = -s1d * c3 - s1 * c3d - c1d * c2s3 - c1 * c2s3d;
+        PNd[4] = c1d * c3 + c1 * c3d - s1d * c2s3 - s1 * c2s3d;
+        PNd[5] = s2d * s3 + s2 * s3d;
+        PNd[6] = s2d * c1 + s2 * c1d;
+        PNd[7] = s2d * s1 + s2 * s1d;
+        PNd[8] = c2d;
+    }
+    const double ct = cos(theta), stt = sin(theta);
+    const double ER[9] = {ct, stt, 0.0, -stt, ct, 0.0, 0.0, 0.0, 1.0};
+    const double ERd[9] = {-stt * thetadot, ct * thetadot, 0.0, -ct * thetadot, -stt * thetadot, 0.0, 0.0, 0.0, 0.0};
+    const double PM[9] = {1.0, sp, xp, -sp, 1.0, -yp, -xp, yp, 1.0};
+    const double PMd[9] = {0.0, spdot, xpd, -spdot, 0.0, -ypd, -xpd, ypd, 0.0};
+    double ERPN[9], A[9], T1[9], T2[9], T3[9], Rd[9];
+    mat3mul(ER, PN, ERPN);
+    mat3mul(PM, ERPN, R);
+    mat3mul(ERd, PN, A);
+    mat3mul(PM, A, T1);
+    mat3mul(PMd, ERPN, T2);
+    mat3mul(ER, PNd, A);
+    mat3mul(PM, A, T3);
+    for (int i = 0; i < 9; ++i) Rd[i] = T1[i] + T2[i] + T3[i];
+    double M[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) M[3 * i + j] = Rd[i] * R[j] + Rd[3 + i] * R[3 + j] + Rd[6 + i] * R[6 + j];
+    omega[0] = 0.5 * (M[7] - M[5]);
+    omega[1] = 0.5 * (M[2] - M[6]);
+    omega[2] = 0.5 * (M[3] - M[1]);
+}
+
 // ---- analytic ephemeris (geocentric, ICRF-aligned km) ----
 __device__ D3 eclToEq(double x, double y, double z) {
     const double eps = 23.43929111 * kDeg;

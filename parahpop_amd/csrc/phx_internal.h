@@ -70,6 +70,7 @@ struct phx_ctx {
     // device tables
     phx::DBuf<double> shTerms, shDiag, nrl, freq, waves, terms, eop, weather, nrlSlots;
     phx::DBuf<double> eph, ephChain;   // Chebyshev ephemeris data and per-slot chains (empty: analytic)
+    phx::DBuf<double> iers;            // IERS2000 orientation payload (empty: analytic model)
     std::vector<std::vector<phx::BrieStep>> ephChains;   // host copy, for coverage checks
     phx::DBuf<float> shTerms32;
     phx::DBuf<int> shOrderStart;
@@ -130,7 +131,8 @@ EpochArgs epochArgs(const phx_ctx& c);
 // kernel.
 void orderBlocks(const phx_ctx& c, int W, int out[kMaxWaves + 1], bool split);
 // Target validation of driver.cpp:99-112 (throws std::invalid_argument).
-// Throws unless the Chebyshev ephemeris (when configured) covers ET [lo, hi].
+// Throws unless the Chebyshev ephemeris and the IERS2000 orientation (when
+// configured) cover ET [lo, hi].
 void checkEphemerisCoverage(const phx_ctx& c, double lo, double hi);
 
 void validateTargets(size_t n, const int64_t* id, const double* epoch, const double* state6, const double* mass,

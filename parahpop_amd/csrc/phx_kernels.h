@@ -34,6 +34,10 @@ struct EpochArgs {
     // sign} per step of the chain to the Earth
     const double* eph;
     const double* ephChain;
+    // IERS2000 Earth orientation payload of a .brot2 unit (nullptr: analytic
+    // model) and its nutation / ERP node counts
+    const double* iers;
+    int iersNut, iersErp;
 };
 
 constexpr int kEphSteps = 6;

@@ -63,7 +63,7 @@ def _create_error(path):
     ({"environment": {"bodies": {"Earth": {"gravity": True}}, "ephemeris": ["de440s.brie"], "constants": "constants.json"}},
      "Required data file not found"),
     ({"environment": {"bodies": {"Earth": {"gravity": True}}, "orientations": ["itrf.brot2"], "constants": "constants.json"}},
-     "analytic"),
+     "Required data file not found"),
     ({"environment": {"bodies": {"Earth": {"gravity": True}, "Moon": {"radiation": True}}, "constants": "constants.json"}}, "Radiation is supported for the Sun only"),
     ({"environment": {"bodies": {"Earth": {"gravity": True}, "Vulcan": {"gravity": True}}, "constants": "constants.json"}}, "Unsupported body"),
     ({"environment": {"bodies": {"Earth": {"gravity": True}}, "constants": "constants.json",
