@@ -188,6 +188,15 @@ void buildShTables(phx_ctx& c) {
     c.shDiag.upload(diag.data(), diag.size(), c.stream);
 }
 
+// Body of ephemeris slot `slot`: the configured bodies in order, then the Sun
+// and the Moon of the tide model (-1: unused slot).
+int ephSlotBody(const ModelConfig& m, int slot) {
+    if (slot < int(m.bodies.size())) return m.bodies[size_t(slot)].naif;
+    if (slot == kMaxBodies && m.tidesActive()) return NAIF_SUN;
+    if (slot == kMaxBodies + 1 && m.tidesActive()) return NAIF_MOON;
+    return -1;
+}
+
 void buildForceModel(phx_ctx& c) {
     const ModelConfig& m = c.env.model;
     ForceModel& fm = c.fm;
@@ -258,11 +267,8 @@ void buildForceModel(phx_ctx& c) {
     c.ephChains.clear();
     if (!c.env.brie.empty()) {
         for (int slot = 0; slot < kEphSlots; ++slot) {
-            int naif;
-            if (slot < int(m.bodies.size())) naif = m.bodies[size_t(slot)].naif;
-            else if (slot == kMaxBodies && m.tidesActive()) naif = NAIF_SUN;
-            else if (slot == kMaxBodies + 1 && m.tidesActive()) naif = NAIF_MOON;
-            else continue;
+            const int naif = ephSlotBody(m, slot);
+            if (naif < 0) continue;
             auto ch = brieChain(c.env.brie, naif, NAIF_EARTH);
             if (ch.size() > size_t(kEphSteps))
                 throw std::runtime_error("ephemeris chain of body " + std::to_string(naif) + " is longer than " +
@@ -852,11 +858,8 @@ int phx_create(const char* config_json_path, phx_ctx** out) {
             std::vector<double> rows(size_t(kEphSlots) * kEphRow, 0.0);
             const ModelConfig& m = c->env.model;
             for (int slot = 0; slot < kEphSlots; ++slot) {
-                int naif;
-                if (slot < int(m.bodies.size())) naif = m.bodies[size_t(slot)].naif;
-                else if (slot == kMaxBodies && m.tidesActive()) naif = NAIF_SUN;
-                else if (slot == kMaxBodies + 1 && m.tidesActive()) naif = NAIF_MOON;
-                else continue;
+                const int naif = ephSlotBody(m, slot);
+                if (naif < 0) continue;
                 const auto ch = brieChain(c->env.brie, naif, NAIF_EARTH);
                 double* row = rows.data() + size_t(slot) * kEphRow;
                 row[0] = double(ch.size());
