@@ -156,3 +156,28 @@ def test_failure_cases(envisat_like):
     for value in (0, -1):
         c = _variant(d, cfg, "threads_%d" % value, lambda doc: doc["execution"].update(maxThreads=value))
         _launch(c, tp, os.path.join(d, "threads_%d" % value), expect_success=False)
+
+
+def test_process_with_reference_data_formats(tmp_path):
+    """The process reads the reference's data formats end to end: a .brie
+    Chebyshev ephemeris, a .brot2 IERS2000 Earth orientation, J2-only gravity and
+    a piecewise exponential atmosphere with Earth flattening; 32 objects, 2 h,
+    final_states.csv against the oracle at compare.py's tolerances."""
+    import oracle
+    import brie_writer
+    from parahpop_amd import synth
+    d = str(tmp_path)
+    cfg, tg = synth.make_alternate_workload(d, n=32, duration=7200.0)
+    et0 = 4035.0 * 86400.0 - 43200.0
+    brie_writer.write_brie(os.path.join(d, "eph.brie"),
+                           brie_writer.ssb_units(oracle.body_position, et0 - 2 * 86400.0, et0 + 3 * 86400.0, bodies=(10, 301)))
+    brie_writer.write_brot2(os.path.join(d, "itrf.brot2"), 4031.0, 4041.0,
+                            brie_writer.iau2006_channels(xp_arcsec=0.2, yp_arcsec=0.3, wobble=True))
+    doc = json.load(open(cfg))
+    doc["model"]["environment"]["ephemeris"] = ["eph.brie"]
+    doc["model"]["environment"]["orientations"] = ["itrf.brot2"]
+    json.dump(doc, open(cfg, "w"))
+    t = synth.load_targets(tg)
+    out = os.path.join(d, "out")
+    _launch(cfg, tg, out)
+    _compare_with_oracle(cfg, t, out)
