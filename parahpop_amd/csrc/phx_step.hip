@@ -91,6 +91,15 @@ __device__ inline GridRec gridRec(const double* table, int count, double t0, dou
     return r;
 }
 
+// Per-lane records whose rotation entries were interpolated once per stage
+// into LDS by one wave (every other field interpolated on use).
+struct GridRecR {
+    GridRec g;
+    const double (*r)[kTile];
+    int lane;
+    __device__ double operator()(int f) const { return (f >= REC_R && f < REC_R + 9) ? r[f - REC_R][lane] : g(f); }
+};
+
 // Per-lane geometry of the body-fixed position used by the SH evaluation.
 struct ShGeom {
     double sinPhi, cosPhi, sinL, cosL, alpha, r;
@@ -1295,6 +1304,7 @@ __device__ __attribute__((always_inline)) inline double stageComponent(const Ste
 template <int W, bool GRID>
 __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
     __shared__ double xch[7][kTile];
+    __shared__ double recR[GRID ? 9 : 1][kTile];   // per-lane epochs: the stage's rotation
     __shared__ double part[W][3][kTile];
     __shared__ double extra[2][3][kTile];
     __shared__ double pg[3][kTile];
@@ -1315,6 +1325,12 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
         } else {
             v = a.cur[obj];
             if (s >= 1) v += dt * kRkC[s - 1];
+            if (GRID) {
+                // the rotation at the lane's stage epoch, once for every wave
+                const GridRec g = gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH, v);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) recR[GRID ? k : 0][lane] = g(REC_R + k);
+            }
         }
         xch[d][lane] = v;
         sp.ys[size_t(d) * nPad + obj] = v;
@@ -1326,7 +1342,8 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
     if (GRID) {
         // per-lane epochs: records interpolated at the lane's stage epoch
         const double ts = xch[6][lane];
-        const GridRec rec = gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH, ts);
+        const GridRecR rec{gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH, ts), recR,
+                           lane};
         if (fm.drag && wave == W - 4) {
             // the NRLMSISE-00 lane inputs of this stage for the drag kernel
             double nrl[13], sec;
