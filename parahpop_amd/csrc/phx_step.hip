@@ -91,13 +91,16 @@ __device__ inline GridRec gridRec(const double* table, int count, double t0, dou
     return r;
 }
 
-// Per-lane records whose rotation entries were interpolated once per stage
-// into LDS by one wave (every other field interpolated on use).
-struct GridRecR {
+// Per-lane records whose fields below the NRLMSISE-00 block (rotation, Earth
+// rate, indirect term, bodies, tide coefficients) were interpolated once per
+// stage into LDS by the waves idle in the stage-state phase; the rest are
+// interpolated on use.
+constexpr int kRecStaged = REC_NRL;
+struct GridRecF {
     GridRec g;
     const double (*r)[kTile];
     int lane;
-    __device__ double operator()(int f) const { return (f >= REC_R && f < REC_R + 9) ? r[f - REC_R][lane] : g(f); }
+    __device__ double operator()(int f) const { return f < kRecStaged ? r[f][lane] : g(f); }
 };
 
 // Per-lane geometry of the body-fixed position used by the SH evaluation.
@@ -1304,7 +1307,7 @@ __device__ __attribute__((always_inline)) inline double stageComponent(const Ste
 template <int W, bool GRID>
 __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
     __shared__ double xch[7][kTile];
-    __shared__ double recR[GRID ? 9 : 1][kTile];   // per-lane epochs: the stage's rotation
+    __shared__ double recF[GRID ? kRecStaged : 1][kTile];   // per-lane epochs: staged record fields
     __shared__ double part[W][3][kTile];
     __shared__ double extra[2][3][kTile];
     __shared__ double pg[3][kTile];
@@ -1325,15 +1328,20 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
         } else {
             v = a.cur[obj];
             if (s >= 1) v += dt * kRkC[s - 1];
-            if (GRID) {
-                // the rotation at the lane's stage epoch, once for every wave
-                const GridRec g = gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH, v);
-#pragma unroll
-                for (int k = 0; k < 9; ++k) recR[GRID ? k : 0][lane] = g(REC_R + k);
-            }
         }
         xch[d][lane] = v;
         sp.ys[size_t(d) * nPad + obj] = v;
+    }
+    if (GRID) {
+        // record fields at the lane's stage epoch, spread over the waves from
+        // 6 on (wave 6 % W when there are fewer), each field once per lane
+        const int first = W > 6 ? 6 : 6 % W, stagers = W > 6 ? W - 6 : 1;
+        if (wave >= first && (W > 6 || wave == first)) {
+            double ts = a.cur[obj];
+            if (s >= 1) ts += a.nextDt[obj] * kRkC[s - 1];
+            const GridRec g = gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH, ts);
+            for (int f = wave - first; f < kRecStaged; f += stagers) recF[GRID ? f : 0][lane] = g(f);
+        }
     }
     __syncthreads();
     const int m0 = a.blockStart[wave], m1 = a.blockStart[wave + 1];
@@ -1342,7 +1350,7 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
     if (GRID) {
         // per-lane epochs: records interpolated at the lane's stage epoch
         const double ts = xch[6][lane];
-        const GridRecR rec{gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH, ts), recR,
+        const GridRecF rec{gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH, ts), recF,
                            lane};
         if (fm.drag && wave == W - 4) {
             // the NRLMSISE-00 lane inputs of this stage for the drag kernel
