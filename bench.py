@@ -244,6 +244,7 @@ def leg_uneven(args, work):
             "expected_object_steps": float(np.ceil(dur / 60.0).sum()), "wall_seconds": wall,
             "kernel_seconds": st["kernel_seconds"], "launches": st["launches"], "epoch_records": st["epoch_records"],
             "tuned": {"waves": int(ctx.option("waves")), "share0": ctx.option("share0"),
+                      "split_waves": int(ctx.option("split_waves")), "split_species": int(ctx.option("split_species")),
                       "path": "split-stage" if st["kernels_per_step"] else "fused multi-step",
                       "trial_ms_per_step": 1e3 * tune["kernel_seconds"] / 4},
             "max_end_epoch_error_s": end_err}

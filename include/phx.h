@@ -99,6 +99,10 @@ void phx_batch_destroy(phx_batch* b);
  *                      device memory; per-lane epochs need split_species 8);
  *                      0: the fused multi-step kernel
  *   "split_waves"      4, 8 or 16 (default) waves per tile of the split SH kernel
+ *   "split_groups"     1 .. 4 (default 4): with 8 or 16 waves, a batch with fewer
+ *                      active tiles than compute units splits each tile's SH orders
+ *                      over up to this many workgroups (min(groups, CUs / tiles));
+ *                      the last workgroup to finish sums the parts in a fixed order
  *   "split_species"    8 (default): one eight-wave drag workgroup per tile (the
  *                      species and the drag); 1, 2 or 4: species per workgroup and
  *                      a finish kernel; 0: one four-wave drag workgroup per tile

@@ -334,50 +334,56 @@ void checkEphemerisCoverage(const phx_ctx& c, double lo, double hi) {
     }
 }
 
+void splitOrderBlocks(const phx_ctx& c, int W, int G, int* out) {
+    // split SH kernel: costs in units of one FP64 term of one wave. Order m
+    // runs its padded term count (order 0 with its derivative recursion
+    // costs more per term) plus a fixed per-order cost; the waves that carry
+    // point gravity (W - 1), tides and relativity (W - 2), SRP (W - 3) and
+    // the NRLMSISE-00 lane inputs (W - 4) of the first workgroup start with
+    // that cost. Every wave gets the same total where it can (water filling).
+    const int L = c.fm.shDegree, M = c.fm.shOrder;
+    const int Wt = W * G;
+    std::vector<double> cost(static_cast<size_t>(M) + 1);
+    double sum = 0.0;
+    for (int m = 0; m <= M; ++m) {
+        cost[size_t(m)] = double((L - m + 4) & ~3) * (m == 0 ? c.splitCostOrder0 : 1.0) + c.splitCostOrder;
+        sum += cost[size_t(m)];
+    }
+    std::vector<double> extra(static_cast<size_t>(Wt), 0.0);
+    extra[size_t(W - 1)] = c.splitCostPg;
+    if (c.fm.tides || c.fm.relativity) extra[size_t(W - 2)] = c.splitCostCorr;
+    if (c.fm.srp) extra[size_t(W - 3)] = c.splitCostSrp;
+    if (c.fm.drag) extra[size_t(W - 4)] = c.splitCostLane;
+    // level T with sum_w max(0, T - extra_w) = sum (bisection)
+    double lo = 0.0, hi = sum + *std::max_element(extra.begin(), extra.end());
+    for (int it = 0; it < 100; ++it) {
+        const double T = 0.5 * (lo + hi);
+        double got = 0.0;
+        for (double e : extra) got += std::max(0.0, T - e);
+        (got < sum ? lo : hi) = T;
+    }
+    const double T = hi;
+    out[0] = 0;
+    int m = 0;
+    double acc = 0.0, target = 0.0;
+    for (int w = 0; w < Wt; ++w) {
+        target += std::max(0.0, T - extra[size_t(w)]);
+        if (w == Wt - 1) {
+            m = M + 1;
+        } else {
+            while (m <= M && acc + 0.5 * cost[size_t(m)] <= target) {
+                acc += cost[size_t(m)];
+                ++m;
+            }
+        }
+        out[w + 1] = m;
+    }
+}
+
 void orderBlocks(const phx_ctx& c, int W, int out[kMaxWaves + 1], bool split) {
     const int L = c.fm.shDegree, M = c.fm.shOrder;
     if (split && W >= 8) {
-        // split SH kernel: costs in units of one FP64 term of one wave. Order m
-        // runs its padded term count (order 0 with its derivative recursion
-        // costs more per term) plus a fixed per-order cost; the waves that carry
-        // point gravity (W - 1), tides and relativity (W - 2), SRP (W - 3) and
-        // the NRLMSISE-00 lane inputs (W - 4) start with that cost. Every wave
-        // gets the same total where it can (water filling).
-        std::vector<double> cost(static_cast<size_t>(M) + 1);
-        double sum = 0.0;
-        for (int m = 0; m <= M; ++m) {
-            cost[size_t(m)] = double((L - m + 4) & ~3) * (m == 0 ? c.splitCostOrder0 : 1.0) + c.splitCostOrder;
-            sum += cost[size_t(m)];
-        }
-        std::vector<double> extra(static_cast<size_t>(W), 0.0);
-        extra[size_t(W - 1)] = c.splitCostPg;
-        if (c.fm.tides || c.fm.relativity) extra[size_t(W - 2)] = c.splitCostCorr;
-        if (c.fm.srp) extra[size_t(W - 3)] = c.splitCostSrp;
-        if (c.fm.drag) extra[size_t(W - 4)] = c.splitCostLane;
-        // level T with sum_w max(0, T - extra_w) = sum (bisection)
-        double lo = 0.0, hi = sum + *std::max_element(extra.begin(), extra.end());
-        for (int it = 0; it < 100; ++it) {
-            const double T = 0.5 * (lo + hi);
-            double got = 0.0;
-            for (double e : extra) got += std::max(0.0, T - e);
-            (got < sum ? lo : hi) = T;
-        }
-        const double T = hi;
-        out[0] = 0;
-        int m = 0;
-        double acc = 0.0, target = 0.0;
-        for (int w = 0; w < W; ++w) {
-            target += std::max(0.0, T - extra[size_t(w)]);
-            if (w == W - 1) {
-                m = M + 1;
-            } else {
-                while (m <= M && acc + 0.5 * cost[size_t(m)] <= target) {
-                    acc += cost[size_t(m)];
-                    ++m;
-                }
-            }
-            out[w + 1] = m;
-        }
+        splitOrderBlocks(c, W, 1, out);
         for (int w = W + 1; w <= kMaxWaves; ++w) out[w] = out[W];
         return;
     }
@@ -738,6 +744,7 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
         const int Ws = c.splitWaves;
         orderBlocks(c, Ws, a.blockStart, true);
         SplitArgs sp{b.ks.p, b.ysBuf.p, b.parts.p, b.spec.p, b.lane.p, 0, 0, c.splitSpecies};
+        int groupsFor = 1;   // group count the group blocks in `sp` were split for
         // per stage: SH kernel, then either the drag kernel or species + finish kernels
         kernelsPerStep = kStages * (c.splitSpecies == 0 || c.splitSpecies == 8 ? 2 : 3) + 1;
         while (remaining > 0) {
@@ -749,6 +756,26 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
             }
             a.nTiles = active;
             sp.step = b.stepsDone;
+            // while the active tiles leave CUs idle, share each tile's SH over
+            // several workgroups
+            int G = 1;
+            if (Ws >= 8 && c.splitGroups > 1) G = std::max(1, std::min(c.splitGroups, c.numCU / std::max(1, active)));
+            if (G > 1) {
+                if (!b.tileCount.p) {
+                    b.shPart.alloc(size_t(kMaxGroups) * 3 * np);
+                    b.famPart.alloc(6 * np);
+                    b.tileCount.alloc(size_t(b.nTiles));
+                    check(hipMemsetAsync(b.tileCount.p, 0, size_t(b.nTiles) * sizeof(unsigned), c.stream), "hipMemsetAsync");
+                }
+                if (G != groupsFor) {
+                    splitOrderBlocks(c, Ws, G, sp.groupBlockStart);
+                    groupsFor = G;
+                }
+            }
+            sp.groups = G;
+            sp.shPart = b.shPart.p;
+            sp.famPart = b.famPart.p;
+            sp.tileCount = b.tileCount.p;
             for (int s = 0; s < kStages; ++s) {
                 sp.stage = s;
                 if (launchSplitStage(a, c.fm, sp, Ws, c.stream) != 0) throw HipError("split stage launch failed");
@@ -936,6 +963,10 @@ int phx_set_option(phx_ctx* c, const char* key, double v) {
         c->gridStep = v;
     } else if (k == "force_grid") c->forceGrid = v != 0.0;
     else if (k == "split") c->split = v != 0.0;
+    else if (k == "split_groups") {
+        if (!(v >= 1.0 && v <= double(kMaxGroups))) { c->error = "split_groups must be 1 .. 4"; return PHX_EINVAL; }
+        c->splitGroups = int(v);
+    }
     else if (k == "split_species") {
         if (v != 0 && v != 1 && v != 2 && v != 4 && v != 8) { c->error = "split_species must be 0, 1, 2, 4 or 8"; return PHX_EINVAL; }
         c->splitSpecies = int(v);
@@ -976,6 +1007,7 @@ double phx_get_option(const phx_ctx* c, const char* key) {
     if (k == "exact_groups_max") return c->exactGroupsMax;
     if (k == "grid_step") return c->gridStep;
     if (k == "force_grid") return c->forceGrid;
+    if (k == "split_groups") return c->splitGroups;
     if (k == "split") return c->split;
     if (k == "split_species") return c->splitSpecies;
     if (k == "split_waves") return c->splitWaves;

@@ -10,6 +10,8 @@ constexpr int kTile = 64;          // objects per workgroup tile = one wave64
 constexpr int kMaxBodies = 11;     // Earth, Sun, Moon, 8 planetary system barycenters
 constexpr int kNrlSlotSize = 312;  // doubles per NRLMSISE-00 weather-slot row
 constexpr int kMaxWaves = 16;      // waves per tile that share one object's work
+constexpr int kMaxGroups = 4;      // SH workgroups per tile when few tiles are active
+constexpr int kMaxGroupWaves = kMaxGroups * kMaxWaves;
 constexpr int kMaxExpSegments = 32; // bands of a piecewise exponential atmosphere
 
 // Per-epoch record: everything the force model needs that depends only on the

@@ -87,6 +87,7 @@ struct phx_ctx {
     int forceGrid = 0;
     int split = 1;               // shared-epoch batches with drag: split stages
     int splitWaves = 16;         // waves per tile of the drag-free split kernel
+    int splitGroups = 4;         // at most this many SH workgroups per tile while the active tiles leave CUs idle
     int splitSpecies = 8;        // 8: one eight-wave drag workgroup per tile (species and drag); 1, 2, 4:
                                  // species per workgroup, then a finish kernel; 0: one four-wave workgroup per tile
     // split SH kernel (8 and 16 waves): costs of the waves' extra families and of
@@ -111,6 +112,8 @@ struct phx_batch {
     int stepsDone = 0;
     phx::DBuf<double> y, cur, end, nextDt, ams, bcd, records, slotT0;
     phx::DBuf<double> ks, ysBuf, parts, spec, lane;   // split stages (allocated on first use)
+    phx::DBuf<double> shPart, famPart;               // grouped SH workgroups: partial sums
+    phx::DBuf<unsigned> tileCount;                   // grouped SH workgroups: arrivals per tile (zeroed)
     phx::DBuf<uint8_t> status;
     phx::DBuf<long long> tileRecord, counters, slotRecord;
     phx::DBuf<int> slotRows;
@@ -130,6 +133,9 @@ EpochArgs epochArgs(const phx_ctx& c);
 // terms with the per-wave shares of the fused (split = false) or the split SH
 // kernel.
 void orderBlocks(const phx_ctx& c, int W, int out[kMaxWaves + 1], bool split);
+// Split SH order blocks over G workgroups of W waves (out: G W + 1 entries);
+// the other families sit on waves W - 1 .. W - 4 of the first workgroup.
+void splitOrderBlocks(const phx_ctx& c, int W, int G, int* out);
 // Target validation of driver.cpp:99-112 (throws std::invalid_argument).
 // Throws unless the Chebyshev ephemeris and the IERS2000 orientation (when
 // configured) cover ET [lo, hi].

@@ -73,6 +73,16 @@ struct SplitArgs {
     int stage;       // 0 .. 12
     int step;        // absolute step index (epoch record of (step, stage))
     int speciesPerGroup;  // 0: one four-wave drag workgroup per tile; 1, 2, 4: species per workgroup
+    // SH over several workgroups per tile (few active tiles): `groups`
+    // workgroups of the SH kernel's waves share the tile's order blocks
+    // (groupBlockStart over groups * waves); each writes its partial SH sums to
+    // shPart ([groups][3][nPad]) and the first also the other families to
+    // famPart ([6][nPad]); the last to arrive (tileCount, reset by it) forms k_s.
+    int groups;
+    double* shPart;
+    double* famPart;
+    unsigned* tileCount;
+    int groupBlockStart[kMaxGroupWaves + 1];
 };
 int launchSplitStage(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, int shWaves, hipStream_t stream);
 int launchSplitCombine(const StepArgs& a, const SplitArgs& sp, hipStream_t stream);

@@ -1047,16 +1047,22 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
 // Drag-free stage forces of the split path: acc = ax + (extra + 0) as in
 // stageForces, and the parts ax (point masses + SH) for the drag kernel; the
 // corrections wave leaves its sum (SRP, tides, relativity) in extra[1].
-template <int W, class Rec>
+//
+// FINAL = false (SH over several workgroups): only the first workgroup
+// (`extras`) evaluates the other families, and wave 0 returns the raw sums:
+// the workgroup's SH components (radial, latitude, longitude) in rpl, point
+// gravity in ax and the corrections + SRP sum in ex.
+template <int W, bool FINAL = true, class Rec>
 __device__ inline void stageForcesParts(const StepArgs& a, const ForceModel& fm, const Rec& rec, int wave, int lane, int m0,
                                         int m1, double ams, double (*xch)[kTile], double (*part)[3][kTile],
                                         double (*extra)[3][kTile], double (*pg)[kTile], double acc[3], double ax[3],
-                                        double ex[3]) {
+                                        double ex[3], bool extras = true, double rpl[3] = nullptr) {
     static_assert(W >= 4, "the split kernel spreads the extra families over waves 1 .. 3");
     const double px = xch[0][lane], py = xch[1][lane], pz = xch[2][lane];
     // the families besides SH, each on a wave of its own before the barrier:
     // point gravity on wave W - 1, tides and relativity on W - 2, SRP on W - 3
-    if (wave == W - 1) {
+    if (!extras) {
+    } else if (wave == W - 1) {
         double gx, gy, gz;
         pointGravity(fm, rec, px, py, pz, gx, gy, gz);
         pg[0][lane] = gx;
@@ -1087,6 +1093,35 @@ __device__ inline void stageForcesParts(const StepArgs& a, const ForceModel& fm,
         part[wave][2][lane] = sl;
     }
     __syncthreads();
+    if (!FINAL) {
+        if (wave == 0) {
+            double r = 0.0, p = 0.0, l = 0.0;
+            if (fm.hasSh)
+                for (int w = 0; w < W; ++w) {
+                    r += part[w][0][lane];
+                    p += part[w][1][lane];
+                    l += part[w][2][lane];
+                }
+            rpl[0] = r;
+            rpl[1] = p;
+            rpl[2] = l;
+            if (extras) {
+                ax[0] = pg[0][lane];
+                ax[1] = pg[1][lane];
+                ax[2] = pg[2][lane];
+                double cx = extra[1][0][lane], cy = extra[1][1][lane], cz = extra[1][2][lane];
+                if (fm.srp) {
+                    cx += extra[0][0][lane];
+                    cy += extra[0][1][lane];
+                    cz += extra[0][2][lane];
+                }
+                ex[0] = cx;
+                ex[1] = cy;
+                ex[2] = cz;
+            }
+        }
+        return;
+    }
     if (wave == 0) {
         // the fused kernel's sums in its order: ax = point gravity + SH,
         // acc = ax + ((tides + relativity) + SRP)
@@ -1304,6 +1339,40 @@ __device__ __attribute__((always_inline)) inline double stageComponent(const Ste
 // Stage s of the split path for one tile: the stage state, then the forces
 // without drag with the SH order blocks over W waves; k_s holds them until the
 // finish kernel adds the drag.
+// Stage state of a split SH workgroup: y_s = y + sum_j (dt a_sj) k_j and
+// t_s = t + dt c_s (as the fused kernel), one component per wave, into xch;
+// with `write` also into ys and the velocity k-stage slots. Per-lane epochs:
+// the waves from 6 on (wave 6 % W when there are fewer) interpolate every
+// record field below the NRLMSISE-00 block at the lane's stage epoch into recF.
+template <int W, bool GRID>
+__device__ inline void splitStageState(const StepArgs& a, const SplitArgs& sp, int wave, int lane, int obj, bool write,
+                                       double (*xch)[kTile], double (*recF)[kTile]) {
+    const int nPad = a.nPad;
+    const int s = sp.stage;
+    for (int d = wave; d < 7; d += W) {
+        const double dt = a.nextDt[obj];
+        double v;
+        if (d < 6) {
+            v = stageComponent(a, sp, s, obj, d, dt);
+            if (d >= 3 && write) sp.ks[ksAt(s, d - 3, nPad, obj)] = v;
+        } else {
+            v = a.cur[obj];
+            if (s >= 1) v += dt * kRkC[s - 1];
+        }
+        xch[d][lane] = v;
+        if (write) sp.ys[size_t(d) * nPad + obj] = v;
+    }
+    if (GRID) {
+        const int first = W > 6 ? 6 : 6 % W, stagers = W > 6 ? W - 6 : 1;
+        if (wave >= first && (W > 6 || wave == first)) {
+            double ts = a.cur[obj];
+            if (s >= 1) ts += a.nextDt[obj] * kRkC[s - 1];
+            const GridRec g = gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH, ts);
+            for (int f = wave - first; f < kRecStaged; f += stagers) recF[GRID ? f : 0][lane] = g(f);
+        }
+    }
+}
+
 template <int W, bool GRID>
 __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
     __shared__ double xch[7][kTile];
@@ -1317,32 +1386,7 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
-    // stage state y_s = y + sum_j (dt a_sj) k_j, t_s = t + dt c_s (as the fused
-    // kernel), one component per wave
-    for (int d = wave; d < 7; d += W) {
-        const double dt = a.nextDt[obj];
-        double v;
-        if (d < 6) {
-            v = stageComponent(a, sp, s, obj, d, dt);
-            if (d >= 3) sp.ks[ksAt(s, d - 3, nPad, obj)] = v;
-        } else {
-            v = a.cur[obj];
-            if (s >= 1) v += dt * kRkC[s - 1];
-        }
-        xch[d][lane] = v;
-        sp.ys[size_t(d) * nPad + obj] = v;
-    }
-    if (GRID) {
-        // record fields at the lane's stage epoch, spread over the waves from
-        // 6 on (wave 6 % W when there are fewer), each field once per lane
-        const int first = W > 6 ? 6 : 6 % W, stagers = W > 6 ? W - 6 : 1;
-        if (wave >= first && (W > 6 || wave == first)) {
-            double ts = a.cur[obj];
-            if (s >= 1) ts += a.nextDt[obj] * kRkC[s - 1];
-            const GridRec g = gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH, ts);
-            for (int f = wave - first; f < kRecStaged; f += stagers) recF[GRID ? f : 0][lane] = g(f);
-        }
-    }
+    splitStageState<W, GRID>(a, sp, wave, lane, obj, true, xch, recF);
     __syncthreads();
     const int m0 = a.blockStart[wave], m1 = a.blockStart[wave + 1];
     double acc[3], ax[3], ex[3];
@@ -1373,6 +1417,104 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
             sp.parts[size_t(d) * nPad + obj] = ax[d];
             sp.parts[size_t(3 + d) * nPad + obj] = ex[d];
         }
+    }
+}
+
+// Wave 0 of an SH workgroup of a grouped tile: writes the group's partial sums
+// (and, for group 0, point gravity and corrections + SRP); the last group to
+// arrive at the tile's counter sums the partials in group order and writes
+// k_s and the parts as the one-workgroup kernel does, then resets the counter.
+// The fences order the partial stores before the count (release) and the count
+// before the partial loads (acquire) across the chip's L2 caches.
+template <class Rec>
+__device__ inline void splitGroupFinish(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, const Rec& rec, int g,
+                                        int tile, int obj, int lane, double px, double py, double pz, const double rpl[3],
+                                        const double ax[3], const double ex[3]) {
+    const int nPad = a.nPad, G = sp.groups;
+    for (int c = 0; c < 3; ++c) sp.shPart[(size_t(g) * 3 + c) * nPad + obj] = rpl[c];
+    if (g == 0)
+        for (int c = 0; c < 3; ++c) {
+            sp.famPart[size_t(c) * nPad + obj] = ax[c];
+            sp.famPart[size_t(3 + c) * nPad + obj] = ex[c];
+        }
+    __threadfence();
+    unsigned prev = 0;
+    if (lane == 0) prev = atomicAdd(sp.tileCount + tile, 1u);
+    prev = __shfl(prev, 0);
+    if (prev != unsigned(G - 1)) return;
+    __threadfence();
+    double r = 0.0, p = 0.0, l = 0.0;
+    for (int q = 0; q < G; ++q) {
+        r += sp.shPart[(size_t(q) * 3 + 0) * nPad + obj];
+        p += sp.shPart[(size_t(q) * 3 + 1) * nPad + obj];
+        l += sp.shPart[(size_t(q) * 3 + 2) * nPad + obj];
+    }
+    double gx = sp.famPart[obj], gy = sp.famPart[size_t(nPad) + obj], gz = sp.famPart[2 * size_t(nPad) + obj];
+    const double cx = sp.famPart[3 * size_t(nPad) + obj], cy = sp.famPart[4 * size_t(nPad) + obj],
+                 cz = sp.famPart[5 * size_t(nPad) + obj];
+    if (fm.hasSh) {
+        double hx, hy, hz;
+        const ShGeom geo = shGeometry(rec, px, py, pz, fm.shRadius);
+        shToInertial(rec, geo, r, p, l, fm.shGm, hx, hy, hz);
+        gx += hx;
+        gy += hy;
+        gz += hz;
+    }
+    const int s = sp.stage;
+    const double acc[3] = {gx + (cx + 0.0), gy + (cy + 0.0), gz + (cz + 0.0)};
+    const double axs[3] = {gx, gy, gz}, exs[3] = {cx, cy, cz};
+    for (int d = 0; d < 3; ++d) {
+        sp.ks[ksAt(s, 3 + d, nPad, obj)] = acc[d];
+        sp.parts[size_t(d) * nPad + obj] = axs[d];
+        sp.parts[size_t(3 + d) * nPad + obj] = exs[d];
+    }
+    if (lane == 0) sp.tileCount[tile] = 0u;
+}
+
+// SH of a tile over sp.groups workgroups when few tiles are active: block
+// t * G + g is group g of tile t; its waves run the order blocks
+// groupBlockStart[g W + w]; group 0 also carries the other families and the
+// NRLMSISE-00 lane inputs; splitGroupFinish assembles k_s.
+template <int W, bool GRID>
+__global__ void __launch_bounds__(64 * W) splitShGroupKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
+    __shared__ double xch[7][kTile];
+    __shared__ double recF[GRID ? kRecStaged : 1][kTile];
+    __shared__ double part[W][3][kTile];
+    __shared__ double extra[2][3][kTile];
+    __shared__ double pg[3][kTile];
+    const int G = sp.groups;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g = __builtin_amdgcn_readfirstlane(int(blockIdx.x) % G);
+    const int ti = __builtin_amdgcn_readfirstlane(int(blockIdx.x) / G);
+    const int tile = __builtin_amdgcn_readfirstlane(a.tileList ? a.tileList[ti] : ti);
+    const int obj = tile * kTile + lane;
+    const int nPad = a.nPad;
+    const int s = sp.stage;
+    splitStageState<W, GRID>(a, sp, wave, lane, obj, g == 0, xch, recF);
+    __syncthreads();
+    const int m0 = sp.groupBlockStart[g * W + wave], m1 = sp.groupBlockStart[g * W + wave + 1];
+    double acc[3], ax[3] = {0.0, 0.0, 0.0}, ex[3] = {0.0, 0.0, 0.0}, rpl[3] = {0.0, 0.0, 0.0};
+    const double px = xch[0][lane], py = xch[1][lane], pz = xch[2][lane];
+    if (GRID) {
+        const double ts = xch[6][lane];
+        const GridRecF rec{gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH, ts), recF,
+                           lane};
+        if (g == 0 && fm.drag && wave == W - 4) {
+            double nrl[13], sec;
+            const int row = laneNrlInputs(a, ts, nrl, sec);
+            const NrlLane L = nrlLane(fm, rec, nrl, px, py, pz);
+            nrlLaneStore(L, sp.lane, nPad, obj);
+            nrlLaneStoreGrid(sec, L.in.doy, row, sp.lane, nPad, obj);
+        }
+        stageForcesParts<W, false>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex, g == 0, rpl);
+        if (wave == 0) splitGroupFinish(a, fm, sp, rec, g, tile, obj, lane, px, py, pz, rpl, ax, ex);
+    } else {
+        const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
+        const ExactRec rec{recp};
+        if (g == 0 && fm.drag && wave == W - 4) nrlLaneStore(nrlLane(fm, rec, recp + REC_NRL, px, py, pz), sp.lane, nPad, obj);
+        stageForcesParts<W, false>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex, g == 0, rpl);
+        if (wave == 0) splitGroupFinish(a, fm, sp, rec, g, tile, obj, lane, px, py, pz, rpl, ax, ex);
     }
 }
 
@@ -1847,6 +1989,24 @@ int launchSplitStage(const StepArgs& a, const ForceModel& fm, const SplitArgs& s
 
 static int launchSplitSh(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, int shWaves, hipStream_t stream) {
     if (a.nTiles <= 0) return 0;
+    if (sp.groups > 1) {
+        // several SH workgroups per tile (eight or sixteen waves each), counters
+        // and partial buffers supplied by the caller
+        if (sp.groups > kMaxGroups || !sp.shPart || !sp.famPart || !sp.tileCount) return -2;
+        const dim3 grid(a.nTiles * sp.groups);
+        switch (shWaves) {
+            case 8:
+                if (a.grid) hipLaunchKernelGGL((splitShGroupKernel<8, true>), grid, dim3(512), 0, stream, a, fm, sp);
+                else hipLaunchKernelGGL((splitShGroupKernel<8, false>), grid, dim3(512), 0, stream, a, fm, sp);
+                break;
+            case 16:
+                if (a.grid) hipLaunchKernelGGL((splitShGroupKernel<16, true>), grid, dim3(1024), 0, stream, a, fm, sp);
+                else hipLaunchKernelGGL((splitShGroupKernel<16, false>), grid, dim3(1024), 0, stream, a, fm, sp);
+                break;
+            default: return -2;
+        }
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
     const dim3 grid(a.nTiles);
     if (a.grid) {
         switch (shWaves) {
