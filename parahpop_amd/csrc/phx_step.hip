@@ -1363,12 +1363,20 @@ __device__ inline void splitStageState(const StepArgs& a, const SplitArgs& sp, i
         if (write) sp.ys[size_t(d) * nPad + obj] = v;
     }
     if (GRID) {
+        // the idle waves' threads walk (lane, field) items field-fastest, so one
+        // load instruction reads consecutive fields of one lane's four rows (the
+        // lanes' rows lie apart when their epochs differ)
         const int first = W > 6 ? 6 : 6 % W, stagers = W > 6 ? W - 6 : 1;
         if (wave >= first && (W > 6 || wave == first)) {
-            double ts = a.cur[obj];
-            if (s >= 1) ts += a.nextDt[obj] * kRkC[s - 1];
-            const GridRec g = gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH, ts);
-            for (int f = wave - first; f < kRecStaged; f += stagers) recF[GRID ? f : 0][lane] = g(f);
+            const int tileBase = obj - lane;
+            for (int item = (wave - first) * kTile + lane; item < kTile * kRecStaged; item += stagers * kTile) {
+                const int l = item / kRecStaged, f = item - l * kRecStaged;
+                const int o = tileBase + l;
+                double ts = a.cur[o];
+                if (s >= 1) ts += a.nextDt[o] * kRkC[s - 1];
+                const GridRec g = gridRec(a.records + a.slotRecord[o] * REC_SIZE, a.slotRows[o], a.slotT0[o], a.gridInvH, ts);
+                recF[GRID ? f : 0][l] = g(f);
+            }
         }
     }
 }
