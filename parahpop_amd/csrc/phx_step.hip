@@ -466,8 +466,10 @@ __device__ inline NrlLane nrlLane(const ForceModel& fm, const Rec& rec, const do
     const double* r = nrl - REC_NRL;
     L.state = 0;
     if (!geodetic(fx, fy, fz, fm.earthRadius, fm.flattening, lat, lon, alt)) {
+        // invalid position (its density is NaN): stored at 500 km, where the
+        // species waves that evaluate the whole tile take their shortest branches
         L.state = 3;
-        alt = 0.0;
+        alt = 500.0;
         lat = lon = 0.0;
     } else if (alt > fm.hCutoff) {
         L.state = 0;
@@ -642,6 +644,20 @@ __device__ __noinline__ double nrlDensity(double densityScale, const double* nrl
     nrlmsise::NrlTables T{coeff};
     if (MODE == 1) nrlmsise::gtd7d(st, T, in, fl, out, nrlmsise::FastGlobe{nrl + (REC_NRL_GLOBE - REC_NRL), kGlobeTrig});
     else nrlmsise::gtd7d(st, T, in, fl, out, nrlmsise::GenericGlobe{});
+    if (!isfinite(out.d[5]) || out.d[5] < 0.0) return nan("");
+    return (out.d[5] * 1.0e9) * densityScale;
+}
+
+// Full model for one lane with a given globe evaluator (per-lane epochs: the
+// lane's weather-slot row), kg/km^3.
+template <class Globe>
+__device__ __noinline__ double nrlDensityGlobe(double densityScale, const double* __restrict__ coeff, nrlmsise::Input in,
+                                               Globe F) {
+    nrlmsise::Flags fl = nrlFlags();
+    nrlmsise::Output out;
+    nrlmsise::State st;
+    nrlmsise::NrlTables T{coeff};
+    nrlmsise::gtd7d(st, T, in, fl, out, F);
     if (!isfinite(out.d[5]) || out.d[5] < 0.0) return nan("");
     return (out.d[5] * 1.0e9) * densityScale;
 }
@@ -1368,13 +1384,20 @@ __device__ inline void splitStageState(const StepArgs& a, const SplitArgs& sp, i
         // lanes' rows lie apart when their epochs differ)
         const int first = W > 6 ? 6 : 6 % W, stagers = W > 6 ? W - 6 : 1;
         if (wave >= first && (W > 6 || wave == first)) {
-            const int tileBase = obj - lane;
+            // each thread forms its own lane's rows and weights; the items take
+            // them from lane l by cross-lane reads
+            double ts = a.cur[obj];
+            if (s >= 1) ts += a.nextDt[obj] * kRkC[s - 1];
+            const GridRec own = gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH, ts);
+            const long long ownOff = (long long)(own.p - a.records);
             for (int item = (wave - first) * kTile + lane; item < kTile * kRecStaged; item += stagers * kTile) {
                 const int l = item / kRecStaged, f = item - l * kRecStaged;
-                const int o = tileBase + l;
-                double ts = a.cur[o];
-                if (s >= 1) ts += a.nextDt[o] * kRkC[s - 1];
-                const GridRec g = gridRec(a.records + a.slotRecord[o] * REC_SIZE, a.slotRows[o], a.slotT0[o], a.gridInvH, ts);
+                GridRec g;
+                g.p = a.records + __shfl(ownOff, l);
+                g.w0 = __shfl(own.w0, l);
+                g.w1 = __shfl(own.w1, l);
+                g.w2 = __shfl(own.w2, l);
+                g.w3 = __shfl(own.w3, l);
                 recF[GRID ? f : 0][l] = g(f);
             }
         }
@@ -1734,8 +1757,18 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
     if (wave != 0 || nl.state == 0) return;
     double rho;
     if (nl.state == 1) rho = nrlRhoFromSpecies(fm, spec, lane);
-    else if (nl.state == 2) rho = nrlDensity<GRID ? 2 : 1>(fm.densityScale, nrl, a.nrlCoeff, nl.in);
-    else rho = nan("");
+    else if (nl.state == 2) {
+        // below 72.5 km: the serial model, with the lane's weather-slot globe terms
+        // for per-lane epochs
+        if constexpr (GRID) {
+            const double sec = nrl[REC_NRL_SEC - REC_NRL];
+            rho = nrlDensityGlobe(fm.densityScale, a.nrlCoeff, nl.in,
+                                  nrlmsise::SlotGlobe{a.nrlSlots + size_t(row) * nrlmsise::SLOT_SIZE, kGlobeTrig,
+                                                      cos(7.2722E-5 * sec), sin(7.2722E-5 * sec)});
+        } else {
+            rho = nrlDensity<1>(fm.densityScale, nrl, a.nrlCoeff, nl.in);
+        }
+    } else rho = nan("");
     const double px = sp.ys[obj], py = sp.ys[size_t(nPad) + obj], pz = sp.ys[size_t(2) * nPad + obj];
     const double vx = sp.ys[size_t(3) * nPad + obj], vy = sp.ys[size_t(4) * nPad + obj], vz = sp.ys[size_t(5) * nPad + obj];
     double om0, om1, om2;
