@@ -7,7 +7,8 @@
  * is available from phx_last_error(). No C++ exception crosses the ABI.
  * A context owns one device, one HIP stream and its device memory; it is not
  * re-entrant (use one context per host thread or serialize calls), except the
- * serving calls phx_submit/phx_poll, which are thread-safe.
+ * serving calls phx_submit/phx_poll, which are thread-safe; phx_last_error
+ * returns a copy owned by the calling thread (valid until its next call).
  */
 #ifndef PHX_H
 #define PHX_H
@@ -122,7 +123,10 @@ void phx_batch_destroy(phx_batch* b);
  *   "exact_groups_max", "grid_step", "force_grid"  epoch-record mode (exact
  *                      per-stage records up to this many (start, end) groups,
  *                      else a uniform grid of grid_step seconds)
- *   "serve_slots", "serve_record_mib", "serve_steps_per_tick"  serving pool */
+ *   "serve_slots", "serve_record_mib", "serve_steps_per_tick"  serving pool
+ * The serving loop reads the options that shape its steps (the split and
+ * fused kernel shapes, the balancing costs, serve_steps_per_tick) once, at the
+ * first phx_submit; later changes apply to batches only. */
 int phx_set_option(phx_ctx* ctx, const char* key, double value);
 double phx_get_option(const phx_ctx* ctx, const char* key);
 

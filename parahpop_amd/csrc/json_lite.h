@@ -1,12 +1,15 @@
 // Minimal JSON reader/writer for CONFIG.json, TARGETS.json, constants.json and
 // summary.json. Objects keep their key order (body order in the force sum
 // follows the order written in the configuration). Numbers are parsed with
-// strtod, which is correctly rounded, so "%.17g" text round-trips exactly.
+// strtod, which is correctly rounded; the writer prints the shortest digits
+// that read back exactly.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <memory>
 #include <sstream>
@@ -275,6 +278,27 @@ inline std::string quote(const std::string& s) {
     return out + "\"";
 }
 
+// Shortest decimal digits that read back as v (v finite, nonzero, positive):
+// digits "d1..dk" and n with v = 0.d1..dk x 10^n.
+inline void shortestDigits(double v, std::string& digits, int& n) {
+    char buf[40];
+    for (int p = 1; p <= 17; ++p) {
+        std::snprintf(buf, sizeof buf, "%.*e", p - 1, v);
+        if (std::strtod(buf, nullptr) == v || p == 17) break;
+    }
+    // buf = "d.ddde[+-]XX"
+    const char* e = std::strchr(buf, 'e');
+    digits.clear();
+    for (const char* c = buf; c < e; ++c)
+        if (*c != '.') digits += *c;
+    while (digits.size() > 1 && digits.back() == '0') digits.pop_back();
+    n = std::atoi(e + 1) + 1;
+}
+
+// Number text as nlohmann::json writes it: integers as parsed, doubles in
+// their shortest round-trip digits, decimal notation for decimal exponents in
+// (-4, 15] (".0" after an integral value), else d.ddde+XX with at least two
+// exponent digits (nlohmann/detail/conversions/to_chars.hpp format_buffer).
 inline std::string numberText(const Value& v) {
     if (v.integral && std::fabs(v.num) < 9.007199254740992e15) {
         char buf[32];
@@ -282,11 +306,40 @@ inline std::string numberText(const Value& v) {
         return buf;
     }
     if (!std::isfinite(v.num)) return "null";
-    char buf[40];
-    std::snprintf(buf, sizeof buf, "%.17g", v.num);
-    std::string s = buf;
-    if (s.find_first_of(".eE") == std::string::npos) s += ".0";
-    return s;
+    std::string out = std::signbit(v.num) ? "-" : "";
+    const double a = std::fabs(v.num);
+    if (a == 0.0) return out + "0.0";
+    std::string d;
+    int n = 0;
+    shortestDigits(a, d, n);
+    const int k = int(d.size());
+    constexpr int kMinExp = -4, kMaxExp = 15;
+    if (k <= n && n <= kMaxExp) return out + d + std::string(size_t(n - k), '0') + ".0";
+    if (0 < n && n <= kMaxExp) return out + d.substr(0, size_t(n)) + "." + d.substr(size_t(n));
+    if (kMinExp < n && n <= 0) return out + "0." + std::string(size_t(-n), '0') + d;
+    out += d.substr(0, 1);
+    if (k > 1) out += "." + d.substr(1);
+    const int e = n - 1;
+    char buf[16];
+    std::snprintf(buf, sizeof buf, "e%c%02d", e < 0 ? '-' : '+', e < 0 ? -e : e);
+    return out + buf;
+}
+
+// Copy of v with every object's keys in ascending order (nlohmann::json keeps
+// objects in a std::map, so its dumps list keys sorted).
+inline Value sortedKeys(const Value& v) {
+    if (v.kind == Value::Arr) {
+        Value r = v;
+        r.arr = std::make_shared<std::vector<Value>>();
+        for (const Value& x : *v.arr) r.arr->push_back(sortedKeys(x));
+        return r;
+    }
+    if (v.kind != Value::Obj) return v;
+    Value r = Value::makeObject();
+    std::vector<std::pair<std::string, Value>> items(v.obj->begin(), v.obj->end());
+    std::stable_sort(items.begin(), items.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    for (auto& kv : items) r.obj->emplace_back(kv.first, sortedKeys(kv.second));
+    return r;
 }
 
 inline void dump(const Value& v, std::string& out, int indent, int depth) {

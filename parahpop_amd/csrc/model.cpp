@@ -404,12 +404,6 @@ void parseConfig(const json::Value& root, const std::string& configDir,
         if (b.occulting && !(b.radius > 0)) throw std::runtime_error("Occulting body needs a radius: " + b.name);
         if (b.radiation && !(b.radius > 0)) throw std::runtime_error("Sun radius missing for shadow geometry.");
     }
-    if (model.tidesActive()) {
-        if (model.find(NAIF_SUN) < 0 && true) {
-            // Tide generation needs Sun and Moon positions; the analytic ephemeris provides them
-            // whether or not they are force bodies.
-        }
-    }
 }
 
 ShField loadShField(const std::string& path, char delimiter, int headerLines, int degree, int order) {

@@ -436,7 +436,7 @@ void validateTargets(size_t n, const int64_t* id, const double* epoch, const dou
 namespace {
 
 int fail(phx_ctx* c, const std::exception& e) {
-    if (c) c->error = e.what();
+    if (c) c->setError(e.what());
     if (dynamic_cast<const HipError*>(&e)) return PHX_EDEVICE;
     if (dynamic_cast<const std::bad_alloc*>(&e)) return PHX_ENOMEM;
     return PHX_EINVAL;
@@ -921,7 +921,13 @@ int phx_create(const char* config_json_path, phx_ctx** out) {
     }
 }
 
-const char* phx_last_error(const phx_ctx* ctx) { return ctx ? ctx->error.c_str() : g_createError.c_str(); }
+const char* phx_last_error(const phx_ctx* ctx) {
+    // a copy per calling thread: another thread's serving call may replace the
+    // context's message meanwhile
+    thread_local std::string copy;
+    copy = ctx ? ctx->lastError() : g_createError;
+    return copy.c_str();
+}
 
 void phx_destroy(phx_ctx* ctx) {
     if (!ctx) return;
@@ -949,31 +955,31 @@ int phx_set_option(phx_ctx* c, const char* key, double v) {
     const std::string k = key;
     if (k == "waves") {
         const int w = int(v);
-        if (w != 1 && w != 2 && w != 4 && w != 8 && w != 16) { c->error = "waves must be 1, 2, 4, 8 or 16"; return PHX_EINVAL; }
+        if (w != 1 && w != 2 && w != 4 && w != 8 && w != 16) { c->setError("waves must be 1, 2, 4, 8 or 16"); return PHX_EINVAL; }
         c->nWaves = w;
     } else if (k == "steps_per_launch") {
-        if (!(v >= 1)) { c->error = "steps_per_launch must be >= 1"; return PHX_EINVAL; }
+        if (!(v >= 1)) { c->setError("steps_per_launch must be >= 1"); return PHX_EINVAL; }
         c->stepsPerLaunch = int(v);
     } else if (k == "share0") c->share0 = v;
     else if (k == "share_corr") c->shareCorr = v;
     else if (k == "sort_altitude") c->sortAltitude = v != 0.0;
     else if (k == "exact_groups_max") c->exactGroupsMax = int(v);
     else if (k == "grid_step") {
-        if (!(v > 0 && v <= 600)) { c->error = "grid_step must be in (0, 600] s"; return PHX_EINVAL; }
+        if (!(v > 0 && v <= 600)) { c->setError("grid_step must be in (0, 600] s"); return PHX_EINVAL; }
         c->gridStep = v;
     } else if (k == "force_grid") c->forceGrid = v != 0.0;
     else if (k == "split") c->split = v != 0.0;
     else if (k == "split_groups") {
-        if (!(v >= 1.0 && v <= double(kMaxGroups))) { c->error = "split_groups must be 1 .. 4"; return PHX_EINVAL; }
+        if (!(v >= 1.0 && v <= double(kMaxGroups))) { c->setError("split_groups must be 1 .. 4"); return PHX_EINVAL; }
         c->splitGroups = int(v);
     }
     else if (k == "split_species") {
-        if (v != 0 && v != 1 && v != 2 && v != 4 && v != 8) { c->error = "split_species must be 0, 1, 2, 4 or 8"; return PHX_EINVAL; }
+        if (v != 0 && v != 1 && v != 2 && v != 4 && v != 8) { c->setError("split_species must be 0, 1, 2, 4 or 8"); return PHX_EINVAL; }
         c->splitSpecies = int(v);
     }
     else if (k == "split_cost_pg" || k == "split_cost_corr" || k == "split_cost_srp" || k == "split_cost_lane" ||
              k == "split_cost_order" || k == "split_cost_order0") {
-        if (!(v >= 0.0 && v <= 1e4)) { c->error = k + " must be in [0, 10000]"; return PHX_EINVAL; }
+        if (!(v >= 0.0 && v <= 1e4)) { c->setError(k + " must be in [0, 10000]"); return PHX_EINVAL; }
         (k == "split_cost_pg"      ? c->splitCostPg
          : k == "split_cost_corr"  ? c->splitCostCorr
          : k == "split_cost_srp"   ? c->splitCostSrp
@@ -981,18 +987,19 @@ int phx_set_option(phx_ctx* c, const char* key, double v) {
          : k == "split_cost_order" ? c->splitCostOrder
                                    : c->splitCostOrder0) = v;
     } else if (k == "split_waves") {
-        if (v != 4 && v != 8 && v != 16) { c->error = "split_waves must be 4, 8 or 16"; return PHX_EINVAL; }
+        if (v != 4 && v != 8 && v != 16) { c->setError("split_waves must be 4, 8 or 16"); return PHX_EINVAL; }
         c->splitWaves = int(v);
     }
     else if (k == "serve_steps_per_tick") {
-        if (!(v >= 1)) { c->error = "serve_steps_per_tick must be >= 1"; return PHX_EINVAL; }
+        if (!(v >= 1)) { c->setError("serve_steps_per_tick must be >= 1"); return PHX_EINVAL; }
         c->serveStepsPerTick = int(v);
     } else if (k == "serve_slots" || k == "serve_record_mib") {
-        if (c->server) { c->error = k + " must be set before the first phx_submit"; return PHX_EINVAL; }
-        if (!(v >= 1)) { c->error = k + " must be >= 1"; return PHX_EINVAL; }
+        std::lock_guard<std::mutex> lk(c->serverMu);
+        if (c->server) { c->setError(k + " must be set before the first phx_submit"); return PHX_EINVAL; }
+        if (!(v >= 1)) { c->setError(k + " must be >= 1"); return PHX_EINVAL; }
         if (k == "serve_slots") c->serveSlots = (long long)v;
         else c->serveRecordMiB = v;
-    } else { c->error = "unknown option " + k; return PHX_EINVAL; }
+    } else { c->setError("unknown option " + k); return PHX_EINVAL; }
     return PHX_OK;
 }
 
@@ -1023,9 +1030,12 @@ double phx_get_option(const phx_ctx* c, const char* key) {
     if (k == "sh_degree") return c->fm.hasSh ? c->fm.shDegree : 0;
     if (k == "mixed") return c->run.precision.mixed ? 1 : 0;
     if (k == "serve_steps_per_tick") return c->serveStepsPerTick;
-    if (k == "serve_slots" && !c->server) return double(c->serveSlots);
     if (k == "serve_record_mib") return c->serveRecordMiB;
-    if (k.rfind("serve_", 0) == 0) return c->server ? c->server->stat(k) : 0.0;
+    if (k.rfind("serve_", 0) == 0) {
+        std::lock_guard<std::mutex> lk(c->serverMu);
+        if (k == "serve_slots" && !c->server) return double(c->serveSlots);
+        return c->server ? c->server->stat(k) : 0.0;
+    }
     return std::nan("");
 }
 
@@ -1037,7 +1047,7 @@ int phx_batch_create(phx_ctx* c, size_t n, const int64_t* id, const double* epoc
         *out = buildBatch(*c, n, id, epoch, state6, mass, drag, srp, cd, cr, duration, nullptr).release();
         return PHX_OK;
     } catch (const std::length_error& e) {
-        c->error = e.what();
+        c->setError(e.what());
         return PHX_ENOMEM;
     } catch (const std::exception& e) {
         return fail(c, e);
@@ -1052,7 +1062,7 @@ int phx_batch_create_uneven(phx_ctx* c, size_t n, const int64_t* id, const doubl
         *out = buildBatch(*c, n, id, epoch, state6, mass, drag, srp, cd, cr, 0.0, duration_s).release();
         return PHX_OK;
     } catch (const std::length_error& e) {
-        c->error = e.what();
+        c->setError(e.what());
         return PHX_ENOMEM;
     } catch (const std::exception& e) {
         return fail(c, e);
@@ -1108,7 +1118,7 @@ int phx_batch_autotune(phx_batch* b, int trialSteps, phx_stats* best) {
         for (size_t i = 0; i < cands.size(); ++i) {
             apply(cands[i]);
             int rc = phx_batch_reset(b);
-            if (rc) throw std::runtime_error(c.error);
+            if (rc) throw std::runtime_error(c.lastError());
             runBatch(c, *b, trialSteps, &st);
             if (st.object_steps <= 0) continue;
             const double t = st.kernel_seconds / st.object_steps;
@@ -1135,7 +1145,7 @@ int phx_batch_download(phx_batch* b, double* outEpoch, double* outState) {
         downloadBatch(*b->ctx, *b, outEpoch, outState);
         return PHX_OK;
     } catch (const std::range_error& e) {
-        b->ctx->error = e.what();
+        b->ctx->setError(e.what());
         return PHX_ENOTDONE;
     } catch (const std::exception& e) {
         return fail(b->ctx, e);
@@ -1184,15 +1194,14 @@ int phx_submit(phx_ctx* c, const phx_request* req, uint64_t* ticket) {
     try {
         Server* s = nullptr;
         {
-            static std::mutex createMu;
-            std::lock_guard<std::mutex> lk(createMu);
+            std::lock_guard<std::mutex> lk(c->serverMu);
             if (!c->server) c->server = std::make_unique<Server>(*c);
             s = c->server.get();
         }
         *ticket = s->submit(*req);
         return PHX_OK;
     } catch (const std::length_error& e) {
-        c->error = e.what();
+        c->setError(e.what());
         return PHX_ENOMEM;
     } catch (const std::exception& e) {
         return fail(c, e);
@@ -1201,13 +1210,18 @@ int phx_submit(phx_ctx* c, const phx_request* req, uint64_t* ticket) {
 
 int phx_poll(phx_ctx* c, uint64_t ticket, phx_result* result) {
     if (!c) return PHX_EINVAL;
-    if (!c->server) {
-        c->error = "no request was submitted";
+    Server* s = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(c->serverMu);
+        s = c->server.get();
+    }
+    if (!s) {
+        c->setError("no request was submitted");
         return PHX_EINVAL;
     }
     std::string err;
-    const int rc = c->server->poll(ticket, result, err);
-    if (rc != PHX_OK && rc != PHX_EAGAIN) c->error = err;
+    const int rc = s->poll(ticket, result, err);
+    if (rc != PHX_OK && rc != PHX_EAGAIN) c->setError(err);
     return rc;
 }
 

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -65,7 +66,18 @@ struct phx_ctx {
     phx::ForceModel fm{};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // last error message: phx_submit / phx_poll may run on any thread, so it is
+    // written and copied under errMu
     std::string error;
+    mutable std::mutex errMu;
+    void setError(const std::string& m) {
+        std::lock_guard<std::mutex> lk(errMu);
+        error = m;
+    }
+    std::string lastError() const {
+        std::lock_guard<std::mutex> lk(errMu);
+        return error;
+    }
     std::string dataDir;
     // device tables
     phx::DBuf<double> shTerms, shDiag, nrl, freq, waves, terms, eop, weather, nrlSlots;
@@ -98,7 +110,10 @@ struct phx_ctx {
     int serveStepsPerTick = 16;
     long long serveSlots = 262144;      // object slots of the device pool
     double serveRecordMiB = 8192.0;     // epoch-record arena
+    // request server, created by the first phx_submit; serverMu guards its
+    // creation and every read of the pointer from the serving calls
     std::unique_ptr<phx::Server> server;
+    mutable std::mutex serverMu;
 };
 
 struct phx_batch {

@@ -1331,7 +1331,6 @@ template <class EP>
 __device__ __attribute__((always_inline)) inline double globe7Fast(const State& st, NrlCD p, EP E, double dfa, double glat,
                                                                    double ut71, double ut79c, double ut79s, double ut58,
                                                                    NrlCD tr, double& aptOut) {
-    const double hr = 0.2618, dgtr = 1.74533E-2;
     const double(&plg)[4][9] = st.plg;
     const double cd14 = E[GE_CD14], cd18 = E[GE_CD18], f2 = E[GE_F2];
     double tinf = p[30];

@@ -62,7 +62,17 @@ Server::Server(phx_ctx& c)
       slots_(roundUp(std::max<long long>(c.serveSlots, kTile), kTile)),
       recRows_(std::max<long long>(8, (long long)(c.serveRecordMiB * 1048576.0 / (REC_SIZE * sizeof(double))))),
       slotAlloc_(slots_),
-      rowAlloc_(recRows_) {}
+      rowAlloc_(recRows_) {
+    base_ = contextStepArgs(c, waves_);
+    stepsPerTick_ = std::max(1, c.serveStepsPerTick);
+    // split stages (per-lane epochs with the eight-wave drag kernel) when the
+    // context selects them, as for device batches
+    split_ = c.split && c.fm.drag && c.fm.hasSh && c.splitSpecies == 8;
+    if (split_) {
+        waves_ = c.splitWaves;
+        orderBlocks(c, waves_, base_.blockStart, true);
+    }
+}
 
 Server::~Server() {
     {
@@ -304,15 +314,11 @@ void Server::rebuildTiles() {
 }
 
 void Server::tick() {
-    int W = 1;
-    StepArgs a = contextStepArgs(c_, W);
-    const int K = std::max(1, c_.serveStepsPerTick);
-    // split stages (per-lane epochs with the eight-wave drag kernel) when the
-    // context selects them, as for device batches
-    const bool split = c_.split && c_.fm.drag && c_.fm.hasSh && c_.splitSpecies == 8;
+    const int W = waves_;
+    StepArgs a = base_;
+    const int K = stepsPerTick_;
+    const bool split = split_;
     if (split) {
-        W = c_.splitWaves;
-        orderBlocks(c_, W, a.blockStart, true);
         if (!ks_.p) {
             const size_t S = size_t(slots_);
             ks_.alloc(size_t(kStages) * 6 * S);

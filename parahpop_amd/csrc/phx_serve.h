@@ -88,6 +88,13 @@ private:
     void failAll(int rc, const std::string& msg);
 
     phx_ctx& c_;
+    // step shape read once from the context's options when the server is
+    // created, so later option changes (phx_set_option, phx_batch_autotune on
+    // the caller's thread) never race the worker
+    StepArgs base_{};                  // context-wide step arguments and order blocks
+    int waves_ = 1;                    // waves per tile (split SH kernel or fused kernel)
+    bool split_ = false;               // split stages (SH kernel, drag kernel, combine)
+    int stepsPerTick_ = 1;
     const double h_;                   // grid step (s)
     const long long slots_;            // pool slots (multiple of 64)
     const long long recRows_;          // arena rows

@@ -8,7 +8,7 @@ import os
 import numpy as np
 import pytest
 
-import brie_writer
+from parahpop_amd import brie_io as brie_writer
 
 DAY = 86400.0
 EPOCH0_ET = 4035.0 * DAY - 43200.0

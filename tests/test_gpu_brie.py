@@ -11,7 +11,7 @@ import os
 import numpy as np
 import pytest
 
-import brie_writer
+from parahpop_amd import brie_io as brie_writer
 
 pytestmark = pytest.mark.gpu
 

@@ -109,9 +109,16 @@ def test_validate_cases(envisat_like, case, duration, count):
     out = os.path.join(d, case + "_out")
     _launch(c, tp, out)
     _compare_with_oracle(c, tt, out)
-    summary = json.load(open(os.path.join(out, "summary.json")))
+    text = open(os.path.join(out, "summary.json")).read()
+    summary = json.loads(text)
     assert summary["max_end_epoch_error_seconds"] <= 1e-4
     assert summary["targets"] == count and summary["duration_seconds"] == duration
+    # driver.cpp:166-175: nine keys, written by nlohmann::json (sorted, indent 2)
+    keys = ["backend", "duration_seconds", "environment_seconds", "initial_step_seconds", "integrator",
+            "max_end_epoch_error_seconds", "propagation_seconds", "targets", "validation_passed"]
+    assert list(summary) == keys
+    assert text.startswith('{\n  "backend": "device",\n') and text.endswith("}\n")
+    assert '  "targets": %d,\n' % count in text and '  "validation_passed": true\n' in text
 
 
 def test_central_invariants(tmp_path):

@@ -153,3 +153,28 @@ def test_population_distributions():
     assert np.all(r[leo] > synth.R_EARTH + 150) and np.all(r[leo] <= (synth.R_EARTH + 2000.0) * 1.02)
     am = pop["drag_area"] / pop["mass"]
     assert abs(np.median(np.log(am)) - np.log(0.01)) < 0.05
+
+
+def test_json_writer_formats_numbers_as_nlohmann(tmp_path):
+    """summary.json and configuration_resolved.json are written by nlohmann::json
+    in the reference (driver.cpp:166-175): shortest round-trip digits, decimal
+    notation for decimal exponents in (-4, 15], else d.ddde+XX; keys sorted.
+    Expected texts are nlohmann::json::dump() outputs of the same values."""
+    cases = [(600.0, "600.0"), (0.1, "0.1"), (1e-5, "1e-05"), (2.5e-5, "2.5e-05"), (1.5e20, "1.5e+20"),
+             (1e15, "1e+15"), (1e14, "100000000000000.0"), (-0.0, "-0.0"), (0.0001, "0.0001"),
+             (0.001, "0.001"), (3.141592653589793, "3.141592653589793"), (86400.0, "86400.0"),
+             (4035.0416666666665, "4035.0416666666665"), (5e-324, "5e-324"),
+             (1.7976931348623157e308, "1.7976931348623157e+308"), (-123.456, "-123.456"), (1e100, "1e+100")]
+    src = tmp_path / "fmt.cpp"
+    body = "\n".join('    std::printf("%%s\\n", phx::json::numberText(phx::json::Value::of(%r)).c_str());' % v
+                     for v, _ in cases)
+    src.write_text('#include "json_lite.h"\n#include <cstdio>\nint main() {\n%s\n'
+                   '    phx::json::Value o = phx::json::parse("{\\"b\\": 1, \\"a\\": {\\"d\\": [2, 1.0], \\"c\\": -3}}");\n'
+                   '    std::printf("%%s\\n", phx::json::dump(phx::json::sortedKeys(o)).c_str());\n}\n' % body)
+    exe = tmp_path / "fmt"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "parahpop_amd", "csrc"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, stdout=subprocess.PIPE, text=True).stdout.split("\n")
+    assert out[:len(cases)] == [t for _, t in cases]
+    tail = "\n".join(out[len(cases):]).strip()
+    assert tail == '{\n  "a": {\n    "c": -3,\n    "d": [\n      2,\n      1.0\n    ]\n  },\n  "b": 1\n}'

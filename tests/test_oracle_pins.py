@@ -80,12 +80,6 @@ def test_final_step_shortening(central_workload):
         assert st["object_steps"] == t["n"] * (2 if dur == 65.0 else 1)
 
 
-def _potential_gradient_fd(oracle_mod, cfg, r, et):
-    """Finite-difference gradient of the SH potential from the oracle's
-    acceleration being the gradient of U: checks curl-free consistency."""
-    o = oracle_mod.Oracle(cfg)
-    return o
-
 
 def test_sh_zonal_reduces_to_j2(workdir):
     """A field with only C20 gives the textbook J2 acceleration (J2.h:20-36)."""
