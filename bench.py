@@ -80,6 +80,34 @@ def pos_err_mm(a, b):
     return float(np.sqrt(np.mean(d ** 2))), float(np.max(d))
 
 
+def cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may run on,
+    capped at OMP_NUM_THREADS when it is set (one GPU's share of the host)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(omp))) if omp.isdigit() else max(1, n)
+
+
+def host_info():
+    model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    flags = ""
+    try:
+        for ln in open(os.path.join(ROOT, "oracle", "Makefile")):
+            if ln.startswith("CXXFLAGS"):
+                flags = ln.split("=", 1)[1].strip()
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model, "build": "g++ " + flags}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -168,7 +196,16 @@ def leg_main(args, work):
             line["roofline"]["traffic_source"] = os.path.relpath(traffic, ROOT)
     if not args.no_cpu:
         import oracle
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
+        # the timed batch itself: 64 of its objects after the timed steps
+        e_t, s_t = b.state()
+        idx64 = np.linspace(0, t["n"] - 1, 64).astype(int)
+        e_o, s_o, _ = oracle.Oracle(cfg).propagate(subset(t, idx64), steps * 60.0, threads=threads)
+        rms, mx = pos_err_mm(s_t[:, idx64], s_o)
+        line["timed_batch_vs_oracle"] = {
+            "objects": 64, "steps": steps, "pos_mm_rms": rms, "pos_mm_max": mx,
+            "vel_um_s_max": float(np.max(np.linalg.norm(s_t[3:, idx64] - s_o[3:], axis=0)) * 1e9),
+            "epochs_identical": bool(np.array_equal(e_t[idx64], e_o))}
         idx = np.linspace(0, t["n"] - 1, args.cpu_objects).astype(int)
         sub = subset(t, idx)
         e_o, s_o, so = oracle.Oracle(cfg).propagate(sub, args.cpu_seconds, threads=threads)
@@ -177,6 +214,7 @@ def leg_main(args, work):
                                 "kind": "port",
                                 "sample": "%d evenly spaced objects of the same population, %.0f s (%d object-steps)"
                                           % (sub["n"], args.cpu_seconds, int(so["object_steps"]))}
+        line["cpu_baseline"].update(host_info())
         rms, mx = pos_err_mm(s_g, s_o)
         dv = float(np.max(np.linalg.norm(s_g[3:] - s_o[3:], axis=0)) * 1e9)
         line["error_vs_cpu"] = {"pos_mm_rms": rms, "pos_mm_max": mx, "vel_um_s_max": dv,
@@ -214,7 +252,7 @@ def leg_mixed(args, work):
     if not args.no_cpu:
         import oracle
         idx = np.linspace(0, t["n"] - 1, 64).astype(int)
-        e_o, s_o, _ = oracle.Oracle(cfg64).propagate(subset(t, idx), duration, threads=min(16, os.cpu_count() or 1))
+        e_o, s_o, _ = oracle.Oracle(cfg64).propagate(subset(t, idx), duration, threads=cpu_threads())
         rms, mx = pos_err_mm(states["mixed"][:, idx], s_o)
         out["mixed_vs_oracle_pos_mm_64obj"] = {"rms": rms, "max": mx}
         rms, mx = pos_err_mm(states["fp64"][:, idx], s_o)
@@ -280,7 +318,7 @@ def leg_alternate(args, work):
         import oracle
         cfg, t, s_g = states[args.objects]
         idx = np.linspace(0, t["n"] - 1, 64).astype(int)
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         orc = oracle.Oracle(cfg)
         _, s_o, ost = orc.propagate(subset(t, idx), duration, threads=threads)
         rms, mx = pos_err_mm(s_g[:, idx], s_o)

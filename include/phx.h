@@ -39,6 +39,8 @@ typedef struct phx_stats {
     int waves;                 /* waves per 64-object tile used */
     int steps_per_launch;      /* steps per launch of the fused step kernel; 1 on the split-stage path */
     int kernels_per_step;      /* kernels per step on the split-stage path; 0 on the fused path */
+    int sh_groups_min;         /* split-stage path: fewest and most SH workgroups per tile over */
+    int sh_groups_max;         /* the call's steps (1: the one-workgroup kernel); 0 on the fused path */
 } phx_stats;
 
 /* Creates a context from a ParaHPOP CONFIG.json: parses and validates the
@@ -89,6 +91,9 @@ int phx_batch_run(phx_batch* b, int max_steps, phx_stats* stats);
 /* 1 when every object reached its end of simulation, else 0. */
 int phx_batch_done(const phx_batch* b);
 int phx_batch_download(phx_batch* b, double* out_epoch, double* out_state6_soa);
+/* Current epochs (MJD2000 TDB) and states of every object, finished or not
+ * (diagnostic: checks part of a run; no reference counterpart). */
+int phx_batch_state(phx_batch* b, double* out_epoch, double* out_state6_soa);
 /* Copies the batch's initial state back in place so it can be run again. */
 int phx_batch_reset(phx_batch* b);
 void phx_batch_destroy(phx_batch* b);
@@ -100,10 +105,14 @@ void phx_batch_destroy(phx_batch* b);
  *                      device memory; per-lane epochs need split_species 8);
  *                      0: the fused multi-step kernel
  *   "split_waves"      4, 8 or 16 (default) waves per tile of the split SH kernel
- *   "split_groups"     1 .. 4 (default 4): with 8 or 16 waves, a batch with fewer
+ *   "split_groups"     1 .. 4 (default 1): with 8 or 16 waves, a batch with fewer
  *                      active tiles than compute units splits each tile's SH orders
  *                      over up to this many workgroups (min(groups, CUs / tiles));
- *                      the last workgroup to finish sums the parts in a fixed order
+ *                      the last workgroup to finish sums the parts in a fixed
+ *                      order. Faster tails, but the SH sum is then partitioned by
+ *                      the number of active tiles, so an object's last bits
+ *                      (about 1e-15 relative per step) depend on the rest of its
+ *                      batch and on the CU count; with 1 they do not
  *   "split_species"    8 (default): one eight-wave drag workgroup per tile (the
  *                      species and the drag); 1, 2 or 4: species per workgroup and
  *                      a finish kernel; 0: one four-wave drag workgroup per tile

@@ -19,7 +19,7 @@ PHX_OK, PHX_EINVAL, PHX_EDEVICE, PHX_ENOTDONE, PHX_ENOMEM, PHX_EAGAIN = 0, -1, -
 
 EXPORTED = [
     "phx_create", "phx_propagate", "phx_batch_create", "phx_batch_create_uneven", "phx_batch_autotune", "phx_batch_run", "phx_batch_done", "phx_batch_download",
-    "phx_batch_reset", "phx_batch_destroy", "phx_set_option", "phx_get_option", "phx_submit", "phx_poll",
+    "phx_batch_state", "phx_batch_reset", "phx_batch_destroy", "phx_set_option", "phx_get_option", "phx_submit", "phx_poll",
     "phx_last_error", "phx_destroy", "phx_force_probe", "phx_epoch_record", "phx_nrlmsise_probe", "phx_version",
 ]
 
@@ -35,7 +35,7 @@ class Stats(ctypes.Structure):
                 ("epoch_seconds", ctypes.c_double), ("wall_seconds", ctypes.c_double),
                 ("epoch_records", ctypes.c_double), ("launches", ctypes.c_double),
                 ("waves", ctypes.c_int), ("steps_per_launch", ctypes.c_int),
-                ("kernels_per_step", ctypes.c_int)]
+                ("kernels_per_step", ctypes.c_int), ("sh_groups_min", ctypes.c_int), ("sh_groups_max", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -73,6 +73,7 @@ def library():
     lib.phx_batch_run.argtypes = [vp, i, vp]
     lib.phx_batch_done.argtypes = [vp]
     lib.phx_batch_download.argtypes = [vp, vp, vp]
+    lib.phx_batch_state.argtypes = [vp, vp, vp]
     lib.phx_batch_reset.argtypes = [vp]
     lib.phx_batch_destroy.argtypes = [vp]
     lib.phx_batch_destroy.restype = None
@@ -264,6 +265,13 @@ class Batch:
         e = np.zeros(self.n)
         s = np.zeros((6, self.n))
         self.ctx._check(self.ctx.lib.phx_batch_download(self.h, _ptr(e), _ptr(s)))
+        return e, s
+
+    def state(self):
+        """Current (epochs, states[6][n]) of every object, finished or not."""
+        e = np.zeros(self.n)
+        s = np.zeros((6, self.n))
+        self.ctx._check(self.ctx.lib.phx_batch_state(self.h, _ptr(e), _ptr(s)))
         return e, s
 
     def close(self):

@@ -729,6 +729,7 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
     int launches = 0;
     int kernelsPerStep = 0;
     int blocksFor = W;   // wave count the order blocks in `a` were split for
+    int groupsMin = 1 << 30, groupsMax = 0;   // SH workgroups per tile (split path)
     // split stages: shared-epoch batches with drag, and per-lane epochs with the
     // eight-wave drag kernel
     const bool split = c.split && c.fm.drag && c.fm.hasSh && (!b.grid || c.splitSpecies == 8);
@@ -760,6 +761,8 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
             // several workgroups
             int G = 1;
             if (Ws >= 8 && c.splitGroups > 1) G = std::max(1, std::min(c.splitGroups, c.numCU / std::max(1, active)));
+            groupsMin = std::min(groupsMin, G);
+            groupsMax = std::max(groupsMax, G);
             if (G > 1) {
                 if (!b.tileCount.p) {
                     b.shPart.alloc(size_t(kMaxGroups) * 3 * np);
@@ -828,12 +831,14 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
         stats->waves = W;
         stats->steps_per_launch = split ? 1 : c.stepsPerLaunch;
         stats->kernels_per_step = kernelsPerStep;
+        stats->sh_groups_min = groupsMax ? groupsMin : 0;
+        stats->sh_groups_max = groupsMax;
         stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
     }
     return 0;
 }
 
-void downloadBatch(phx_ctx& c, phx_batch& b, double* outEpoch, double* outState) {
+void downloadBatch(phx_ctx& c, phx_batch& b, double* outEpoch, double* outState, bool requireEnd = true) {
     const size_t np = static_cast<size_t>(b.nPad);
     std::vector<double> y(6 * np), cur(np);
     std::vector<uint8_t> st(np);
@@ -843,7 +848,8 @@ void downloadBatch(phx_ctx& c, phx_batch& b, double* outEpoch, double* outState)
     check(hipStreamSynchronize(c.stream), "download");
     for (size_t i = 0; i < b.n; ++i) {
         const int s = b.slotOf[i];
-        if (!(st[size_t(s)] & 2)) throw std::range_error("A target stopped without reaching the requested end of simulation.");
+        if (requireEnd && !(st[size_t(s)] & 2))
+            throw std::range_error("A target stopped without reaching the requested end of simulation.");
         if (outEpoch) outEpoch[i] = cur[size_t(s)] * (1.0 / 86400.0) + 0.5;   // TimeConversion.h:36-40
         if (outState)
             for (int d = 0; d < 6; ++d) outState[size_t(d) * b.n + i] = y[size_t(d) * b.nPad + s];
@@ -1147,6 +1153,16 @@ int phx_batch_download(phx_batch* b, double* outEpoch, double* outState) {
     } catch (const std::range_error& e) {
         b->ctx->setError(e.what());
         return PHX_ENOTDONE;
+    } catch (const std::exception& e) {
+        return fail(b->ctx, e);
+    }
+}
+
+int phx_batch_state(phx_batch* b, double* outEpoch, double* outState) {
+    if (!b) return PHX_EINVAL;
+    try {
+        downloadBatch(*b->ctx, *b, outEpoch, outState, false);
+        return PHX_OK;
     } catch (const std::exception& e) {
         return fail(b->ctx, e);
     }

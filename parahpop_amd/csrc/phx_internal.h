@@ -99,7 +99,8 @@ struct phx_ctx {
     int forceGrid = 0;
     int split = 1;               // shared-epoch batches with drag: split stages
     int splitWaves = 16;         // waves per tile of the drag-free split kernel
-    int splitGroups = 4;         // at most this many SH workgroups per tile while the active tiles leave CUs idle
+    int splitGroups = 1;         // at most this many SH workgroups per tile while the active tiles leave CUs idle
+                                 // (1: one workgroup, whose sums do not depend on the rest of the batch)
     int splitSpecies = 8;        // 8: one eight-wave drag workgroup per tile (species and drag); 1, 2, 4:
                                  // species per workgroup, then a finish kernel; 0: one four-wave workgroup per tile
     // split SH kernel (8 and 16 waves): costs of the waves' extra families and of

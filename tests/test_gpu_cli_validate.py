@@ -171,7 +171,7 @@ def test_process_with_reference_data_formats(tmp_path):
     a piecewise exponential atmosphere with Earth flattening; 32 objects, 2 h,
     final_states.csv against the oracle at compare.py's tolerances."""
     import oracle
-    import brie_writer
+    from parahpop_amd import brie_io as brie_writer
     from parahpop_amd import synth
     d = str(tmp_path)
     cfg, tg = synth.make_alternate_workload(d, n=32, duration=7200.0)
