@@ -78,6 +78,22 @@ int phx_batch_create_uneven(phx_ctx* ctx, size_t n, const int64_t* id, const dou
                             const double* state6_soa, const double* mass_kg, const double* drag_area_m2,
                             const double* srp_area_m2, const double* cd, const double* cr, const double* duration_s,
                             phx_batch** out);
+/* phx_propagate on caller-owned device buffers, allocating no device memory
+ * (no counterpart in the reference; for callers that cannot allocate: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid it).
+ * d_epoch (n, MJD2000 TDB) and d_state6_soa (6n, SoA) are read and overwritten
+ * with the final epochs and states; d_mass, d_drag_area (m^2), d_srp_area (m^2),
+ * d_cd, d_cr (n each) are read. All are device pointers, written by the caller
+ * before the call (synchronize the caller's stream first). Every working array
+ * lives in one fixed device array of the library (20.5 MB per GPU, held
+ * until the process ends): at most 16,384 objects and an epoch span of 2,043
+ * grid steps (34 h at the 60 s grid). Runs the per-lane epoch path: the
+ * results equal phx_propagate's with the option force_grid = 1. Synchronous on
+ * the context's stream. Calls that overlap on one GPU (any context) race on
+ * the array. Errors as phx_propagate (PHX_EINVAL beyond the limits). */
+int phx_propagate_noalloc(phx_ctx* ctx, size_t n, double* d_epoch, double* d_state6_soa, const double* d_mass,
+                          const double* d_drag_area, const double* d_srp_area, const double* d_cd, const double* d_cr,
+                          double duration_s, phx_stats* stats);
 /* Autotuner (no reference counterpart; the reference reads fixed
  * PARAHPOP_PERF_* overrides, AccumulateBodies.cu:20-48): times trial_steps
  * steps of this batch for each candidate shape (fused kernel: waves per

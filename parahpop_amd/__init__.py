@@ -18,7 +18,7 @@ REC_SIZE = 352
 PHX_OK, PHX_EINVAL, PHX_EDEVICE, PHX_ENOTDONE, PHX_ENOMEM, PHX_EAGAIN = 0, -1, -2, -3, -4, -5
 
 EXPORTED = [
-    "phx_create", "phx_propagate", "phx_batch_create", "phx_batch_create_uneven", "phx_batch_autotune", "phx_batch_run", "phx_batch_done", "phx_batch_download",
+    "phx_create", "phx_propagate", "phx_propagate_noalloc", "phx_batch_create", "phx_batch_create_uneven", "phx_batch_autotune", "phx_batch_run", "phx_batch_done", "phx_batch_download",
     "phx_batch_state", "phx_batch_reset", "phx_batch_destroy", "phx_set_option", "phx_get_option", "phx_submit", "phx_poll",
     "phx_last_error", "phx_destroy", "phx_force_probe", "phx_epoch_record", "phx_nrlmsise_probe", "phx_version",
 ]
@@ -67,6 +67,7 @@ def library():
     vp, sz, d, i = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double, ctypes.c_int
     lib.phx_create.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
     lib.phx_propagate.argtypes = [vp, sz] + [vp] * 8 + [d, vp, vp, vp]
+    lib.phx_propagate_noalloc.argtypes = [vp, sz] + [vp] * 7 + [d, vp]
     lib.phx_batch_create.argtypes = [vp, sz] + [vp] * 8 + [d, ctypes.POINTER(vp)]
     lib.phx_batch_create_uneven.argtypes = [vp, sz] + [vp] * 9 + [ctypes.POINTER(vp)]
     lib.phx_batch_autotune.argtypes = [vp, i, vp]
@@ -163,6 +164,14 @@ class Context:
                                     ctypes.byref(st))
         self._check(rc)
         return out_e, out_s, st.as_dict()
+
+    def propagate_noalloc(self, n, d_epoch, d_state, d_mass, d_drag_area, d_srp_area, d_cd, d_cr, duration):
+        """phx_propagate_noalloc on device buffers (integer device addresses; the
+        epoch and state buffers are overwritten); returns the stats."""
+        st = Stats()
+        ptrs = [ctypes.c_void_p(int(p)) for p in (d_epoch, d_state, d_mass, d_drag_area, d_srp_area, d_cd, d_cr)]
+        self._check(self.lib.phx_propagate_noalloc(self.h, int(n), *ptrs, float(duration), ctypes.byref(st)))
+        return st.as_dict()
 
     def batch(self, targets, duration):
         """Device-resident batch; duration is a scalar or one value per object."""

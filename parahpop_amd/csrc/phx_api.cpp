@@ -885,6 +885,7 @@ int phx_create(const char* config_json_path, phx_ctx** out) {
         check(hipEventCreate(&c->ev0), "hipEventCreate");
         check(hipEventCreate(&c->ev1), "hipEventCreate");
         if (uploadTideFactors() != 0) throw HipError("tide factor upload failed");
+        if (noallocInit() != 0) throw HipError("device scratch array of phx_propagate_noalloc not found");
         loadTables(*c);
         if (c->env.hasSh) buildShTables(*c);
         if (!c->env.brie.empty()) {

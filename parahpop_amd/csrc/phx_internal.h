@@ -162,6 +162,9 @@ void validateTargets(size_t n, const int64_t* id, const double* epoch, const dou
 // Context-wide step-kernel arguments (tables, weather, wave split); the caller
 // fills the per-batch buffers.
 StepArgs contextStepArgs(const phx_ctx& c, int& waves);
+// Looks up this device's address of the fixed scratch array that
+// phx_propagate_noalloc works in (once per device, that device current).
+int noallocInit();
 // Steps an object needs to reach its end (propagators/FixedStep.h:7-37).
 int stepsToEnd(double startEt, double endEt, double step);
 
