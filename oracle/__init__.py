@@ -42,6 +42,15 @@ def library():
         lib.orc_geodetic_altitude.restype = d
         lib.orc_exp_density.argtypes = [vp, d]
         lib.orc_exp_density.restype = d
+        lib.orc_time_at.argtypes = [vp, d, vp]
+        lib.orc_nrl_inputs.argtypes = [vp, d, vp, vp]
+        lib.orc_solid_tide.argtypes = [vp, vp, vp, vp, vp, i, i, vp, vp]
+        lib.orc_ocean_waves.argtypes = [vp, vp, d, vp, vp]
+        lib.orc_tide_arguments.argtypes = [vp, vp, vp, vp]
+        lib.orc_third_body.argtypes = [vp, vp, d, vp]
+        lib.orc_shadow.argtypes = [vp, vp, d, d]
+        lib.orc_shadow.restype = d
+        lib.orc_sh_body_fixed.argtypes = [vp, vp, vp]
         lib.orc_last_error.restype = ctypes.c_char_p
         lib.orc_destroy.argtypes = [vp]
         lib.orc_destroy.restype = None
@@ -103,6 +112,59 @@ class Oracle:
 
     def exp_density(self, h):
         return self.lib.orc_exp_density(self.h, float(h))
+
+    def time_at(self, et):
+        """EOP time model at ET: dict(mjd_utc, ut1_utc, tt_utc, xp, yp, valid)."""
+        o = np.zeros(6)
+        self.lib.orc_time_at(self.h, float(et), _p(o))
+        return dict(zip(("mjd_utc", "ut1_utc", "tt_utc", "xp", "yp", "valid"), o))
+
+    def nrl_inputs(self, et, fixed):
+        """NRLMSISE-00 inputs at ET for an Earth-fixed position (km)."""
+        o = np.zeros(19)
+        self.lib.orc_nrl_inputs(self.h, float(et), _p(np.ascontiguousarray(fixed, dtype=np.float64)), _p(o))
+        keys = ("mjd_utc", "doy", "sec", "gast_h", "lst_h", "alt", "lat", "lon", "f107", "f107a", "ap_daily")
+        out = dict(zip(keys, o[:11]))
+        out["ap"] = o[11:18].copy()
+        out["valid"] = bool(o[18] == 1.0)
+        return out
+
+    def solid_tide(self, moon, sun, E, time5, frequency_terms=True, zero_tide=True):
+        C, S = np.zeros(49), np.zeros(49)
+        a = [np.ascontiguousarray(x, dtype=np.float64).ravel() for x in (moon, sun, E, time5)]
+        self.lib.orc_solid_tide(self.h, *[_p(x) for x in a], int(frequency_terms), int(zero_tide), _p(C), _p(S))
+        return C.reshape(7, 7), S.reshape(7, 7)
+
+    def ocean_waves(self, args5, gmst):
+        C, S = np.zeros(49), np.zeros(49)
+        self.lib.orc_ocean_waves(self.h, _p(np.ascontiguousarray(args5, dtype=np.float64)), float(gmst), _p(C), _p(S))
+        return C.reshape(7, 7), S.reshape(7, 7)
+
+    def sh_body_fixed(self, p):
+        out = np.zeros(3)
+        if self.lib.orc_sh_body_fixed(self.h, _p(np.ascontiguousarray(p, dtype=np.float64)), _p(out)) != 0:
+            raise RuntimeError(self.lib.orc_last_error().decode())
+        return out
+
+
+def tide_arguments(time5):
+    """Delaunay arguments (l, l', F, D, Omega), GMST and Doodson arguments for time
+    data (mjdUtc, UT1-UTC, TT-UTC, xp, yp)."""
+    a, g, b = np.zeros(5), np.zeros(1), np.zeros(6)
+    library().orc_tide_arguments(_p(np.ascontiguousarray(time5, dtype=np.float64)), _p(a), _p(g), _p(b))
+    return a, g[0], b
+
+
+def third_body(r, b, gm):
+    out = np.zeros(3)
+    library().orc_third_body(_p(np.ascontiguousarray(r, dtype=np.float64)), _p(np.ascontiguousarray(b, dtype=np.float64)),
+                             float(gm), _p(out))
+    return out
+
+
+def shadow(sun, body, r_sun, r_body):
+    return library().orc_shadow(_p(np.ascontiguousarray(sun, dtype=np.float64)), _p(np.ascontiguousarray(body, dtype=np.float64)),
+                                float(r_sun), float(r_body))
 
 
 def geodetic_altitude(r, pole, req, flattening):

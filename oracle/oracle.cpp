@@ -476,3 +476,103 @@ int orc_ctx_earth_orientation(const orc_ctx* ctx, double et, double* R9, double*
     omega3[0] = o.omega.x; omega3[1] = o.omega.y; omega3[2] = o.omega.z;
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Test hooks: single functions of the restatement, for the pins in
+// tests/test_oracle_pins.py.
+// ---------------------------------------------------------------------------
+
+int orc_time_at(const orc_ctx* ctx, double et, double* out6) {
+    const TimeData t = timeAt(ctx->eop, et);
+    out6[0] = t.mjdUtc; out6[1] = t.ut1Utc; out6[2] = t.ttUtc;
+    out6[3] = t.xpArcsec; out6[4] = t.ypArcsec; out6[5] = t.valid ? 1.0 : 0.0;
+    return t.valid ? 0 : -1;
+}
+
+int orc_nrl_inputs(const orc_ctx* ctx, double et, const double* fixed3, double* out) {
+    using namespace nrl;
+    for (int k = 0; k < 19; ++k) out[k] = std::nan("");
+    const double mjdUtc = etToMjdUtc(et);
+    out[0] = mjdUtc;
+    int year = 0, month = 0, day = 0, hour = 0, minute = 0;
+    double second = 0;
+    calendarDate(mjdUtc, year, month, day, hour, minute, second);
+    out[1] = dayOfYear(year, month, day);
+    out[2] = hour * 3600.0 + minute * 60.0 + second;
+    out[3] = hpopGastHours(mjdUtc);
+    double radius = 6378.137, flat = 0.0033528106647474805;
+    for (const auto& b : ctx->env.model.bodies)
+        if (b.naif == phx::NAIF_EARTH && b.atmosphere) radius = b.radius, flat = b.flattening;
+    double lat = 0, lon = 0, alt = 0;
+    if (geodeticFixed(fixed3[0], fixed3[1], fixed3[2], radius, flat, lat, lon, alt)) {
+        out[4] = wrapHours(out[3] + lon * kRadToDeg / 15.0);
+        out[5] = alt;
+        out[6] = lat * kRadToDeg;
+        out[7] = lon * kRadToDeg;
+    }
+    double f107 = 0, f107a = 0, avgAp = 0, ap[7] = {};
+    if (!fillWeather(ctx->weather, mjdUtc, f107, f107a, avgAp, ap)) return -1;
+    out[8] = f107; out[9] = f107a; out[10] = avgAp;
+    for (int k = 0; k < 7; ++k) out[11 + k] = ap[k];
+    out[18] = 1.0;
+    return 0;
+}
+
+namespace {
+TideInput tideInput(const double* moon3, const double* sun3, const double* E9, const double* time5) {
+    TideInput in{};
+    for (int i = 0; i < 3; ++i) { in.moon[i] = moon3[i]; in.sun[i] = sun3[i]; }
+    for (int i = 0; i < 9; ++i) in.E[i] = E9[i];
+    in.mjdUtc = time5[0]; in.ut1Utc = time5[1]; in.ttUtc = time5[2]; in.xpArcsec = time5[3]; in.ypArcsec = time5[4];
+    return in;
+}
+void unpackCS(const CS7& cs, double* C49, double* S49) {
+    for (int n = 0; n < 7; ++n)
+        for (int m = 0; m < 7; ++m) { C49[7 * n + m] = cs.C[n][m]; S49[7 * n + m] = cs.S[n][m]; }
+}
+}  // namespace
+
+int orc_solid_tide(const orc_ctx* ctx, const double* moon3, const double* sun3, const double* E9, const double* time5,
+                   int frequency_terms, int zero_tide, double* C49, double* S49) {
+    const TideInput in = tideInput(moon3, sun3, E9, time5);
+    TideConstants k = ctx->tideK;
+    k.zeroTide = zero_tide != 0;
+    static const TideTables none;
+    CS7 cs;
+    solidCoefficients(in, k, frequency_terms ? ctx->tides : none, cs);
+    unpackCS(cs, C49, S49);
+    return 0;
+}
+
+int orc_ocean_waves(const orc_ctx* ctx, const double* args5, double gmst, double* C49, double* S49) {
+    CS7 cs;
+    clearCS(cs);
+    oceanWaves(args5, gmst, ctx->tides, cs);
+    unpackCS(cs, C49, S49);
+    return 0;
+}
+
+int orc_tide_arguments(const double* time5, double* args5, double* gmst, double* beta6) {
+    const double zero[9] = {};
+    const TideInput in = tideInput(zero, zero, zero, time5);
+    tideArguments(in, args5, *gmst);
+    doodsonArguments(args5, *gmst, beta6);
+    return 0;
+}
+
+int orc_third_body(const double* r3, const double* b3, double gm, double* out3) {
+    const V3 a = thirdBodyNB(V3{r3[0], r3[1], r3[2]}, V3{b3[0], b3[1], b3[2]}, gm);
+    out3[0] = a.x; out3[1] = a.y; out3[2] = a.z;
+    return 0;
+}
+
+double orc_shadow(const double* sun3, const double* body3, double rSun, double rBody) {
+    return shadowFactor(V3{sun3[0], sun3[1], sun3[2]}, V3{body3[0], body3[1], body3[2]}, rSun, rBody);
+}
+
+int orc_sh_body_fixed(const orc_ctx* ctx, const double* p3, double* out3) {
+    if (!ctx->hasSh) { g_error = "no spherical-harmonic field configured"; return -1; }
+    const V3 a = shBodyFixed(V3{p3[0], p3[1], p3[2]}, ctx->sh);
+    out3[0] = a.x; out3[1] = a.y; out3[2] = a.z;
+    return 0;
+}

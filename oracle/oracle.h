@@ -55,6 +55,29 @@ double orc_geodetic_altitude(const double* r3, const double* pole3, double Req, 
 /* Density (kg/km^3) of the configured Earth exponential atmosphere at geodetic
  * altitude h km; NaN when the configuration has none. */
 double orc_exp_density(const orc_ctx* ctx, double h);
+/* Test hooks: single functions of the restatement (pins).
+ * orc_time_at: the EOP time model at ET (mjdUtc, UT1-UTC s, TT-UTC s, xp, yp
+ * arcsec, valid). orc_nrl_inputs: NRLMSISE-00 inputs at ET for an Earth-fixed
+ * position (mjdUtc, doy, seconds of day, GAST h, local time h, altitude km,
+ * geodetic latitude and longitude deg, F10.7, F10.7A, daily Ap, ap[7], valid).
+ * orc_solid_tide: solid-tide coefficients (C, S [7][7]) for Moon and Sun
+ * positions (m, ICRF), rotation E (row-major ICRF->ITRF) and time data
+ * (mjdUtc, UT1-UTC, TT-UTC, xp, yp); frequency_terms 0 leaves out the
+ * frequency-dependent corrections. orc_ocean_waves: the FES2004 wave sum for
+ * Delaunay arguments (l, l', F, D, Omega) and GMST. orc_tide_arguments: those
+ * arguments and the Doodson arguments for time data. orc_third_body: the
+ * non-barycentric third-body acceleration. orc_shadow: the dual-cone factor
+ * (Sun and occulter relative to the object, km). orc_sh_body_fixed: the SH
+ * acceleration at a body-fixed position. */
+int orc_time_at(const orc_ctx* ctx, double et, double* out6);
+int orc_nrl_inputs(const orc_ctx* ctx, double et, const double* fixed3, double* out19);
+int orc_solid_tide(const orc_ctx* ctx, const double* moon3, const double* sun3, const double* E9, const double* time5,
+                   int frequency_terms, int zero_tide, double* C49, double* S49);
+int orc_ocean_waves(const orc_ctx* ctx, const double* args5, double gmst, double* C49, double* S49);
+int orc_tide_arguments(const double* time5, double* args5, double* gmst, double* beta6);
+int orc_third_body(const double* r3, const double* b3, double gm, double* out3);
+double orc_shadow(const double* sun3, const double* body3, double rSun, double rBody);
+int orc_sh_body_fixed(const orc_ctx* ctx, const double* p3, double* out3);
 double orc_step_seconds(const orc_ctx* ctx);
 double orc_duration_seconds(const orc_ctx* ctx);
 const char* orc_last_error(void);

@@ -778,13 +778,22 @@ inline void solidCoefficients(const TideInput& in, const TideConstants& k, const
     out.S[2][1] += -1.333e-9 * (m2 - 0.0115 * m1);
 }
 
-// HpopCorrections.h:298-322
-inline void oceanCoefficients(const TideInput& in, const TideTables& tt, CS7& out) {
-    clearCS(out);
-    double f[5], gmst;
-    tideArguments(in, f, gmst);
+// HpopCorrections.h:302-305: Doodson arguments (tau, s, h, p, N', p_s) from
+// the Delaunay arguments (l, l', F, D, Omega) and GMST.
+inline void doodsonArguments(const double f[5], double gmst, double beta[6]) {
     const double s = f[2] + f[4], h = s - f[3];
-    const double beta[6] = {gmst + kPi - s, s, h, s - f[0], -f[4], h - f[1]};
+    beta[0] = gmst + kPi - s;
+    beta[1] = s;
+    beta[2] = h;
+    beta[3] = s - f[0];
+    beta[4] = -f[4];
+    beta[5] = h - f[1];
+}
+
+// HpopCorrections.h:306-317: the FES2004 wave sum added into out.
+inline void oceanWaves(const double f[5], double gmst, const TideTables& tt, CS7& out) {
+    double beta[6];
+    doodsonArguments(f, gmst, beta);
     for (const OceanWave& w : tt.waves) {
         double theta = 0.0;
         for (int j = 0; j < 6; ++j) theta += w.doodson[j] * beta[j];
@@ -795,6 +804,14 @@ inline void oceanCoefficients(const TideInput& in, const TideTables& tt, CS7& ou
             if (t.m) out.S[t.n][t.m] += (t.sp - t.sm) * co - (t.cp - t.cm) * si;
         }
     }
+}
+
+// HpopCorrections.h:298-322
+inline void oceanCoefficients(const TideInput& in, const TideTables& tt, CS7& out) {
+    clearCS(out);
+    double f[5], gmst;
+    tideArguments(in, f, gmst);
+    oceanWaves(f, gmst, tt, out);
     double m1, m2;
     poleWobble(in, m1, m2);
     out.C[2][1] += -2.1778e-10 * (m1 - 0.01724 * m2);

@@ -147,26 +147,52 @@ def mjd_to_calendar(mjd):
     return year, month, day
 
 
-def write_space_weather(path, mjd_first, days, f107=150.0, ap=15):
-    """CelesTrak SW-All layout: y m d bsrn nd kp*8 sum ap*8 avg cp c9 isn adjF q adjC81 adjL81 obsF obsC81 obsL81."""
+def varying_weather_day(k):
+    """Space weather of day k of a varying table: adjusted F10.7 and its
+    81-day centred mean, daily Ap and the eight 3-hour ap, every value distinct
+    from the neighbouring days' and slots' (so a wrong day, slot or lag shows)."""
+    f107 = 90.0 + 5.0 * k + 17.0 * math.sin(0.9 * k)
+    f107a = 120.0 + 0.75 * k
+    aps = [3 + (11 * k + 7 * j + (j * j) % 5) % 61 for j in range(8)]
+    ap_daily = 4 + (13 * k) % 47
+    return f107, f107a, ap_daily, aps
+
+
+def write_space_weather(path, mjd_first, days, f107=150.0, ap=15, varying=False):
+    """CelesTrak SW-All layout: y m d bsrn nd kp*8 sum ap*8 avg cp c9 isn adjF q adjC81 adjL81 obsF obsC81 obsL81.
+    Constant (F10.7 = F10.7A = f107, ap = Ap = ap) or varying (varying_weather_day)."""
     lines = []
     for k in range(days):
         y, mo, d = mjd_to_calendar(mjd_first + k)
+        if varying:
+            fk, fa, apd, aps = varying_weather_day(k)
+        else:
+            fk, fa, apd, aps = f107, f107, ap, [ap] * 8
         kp = " ".join(["27"] * 8)
-        aps = " ".join([str(ap)] * 8)
         lines.append("%d %02d %02d 2420 %d %s 216 %s %d 0.8 4 50 %.1f 0 %.1f %.1f %.1f %.1f %.1f" % (
-            y, mo, d, k + 1, kp, aps, ap, f107, f107, f107, f107, f107, f107))
+            y, mo, d, k + 1, kp, " ".join(str(a) for a in aps), apd, fk, fa, fa, fk, fa, fa))
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
 
-def write_eop(path, mjd_first, days):
-    """HPOP EOP-All rows: y m d mjd xp yp ut1-utc lod dpsi deps dx dy tai-utc (constant, zero pole)."""
+def varying_eop_day(k, step_day):
+    """EOP of day k of a varying table: xp, yp (arcsec) and UT1-UTC (s) drifting
+    linearly, TAI-UTC 37 s before day step_day and 38 s from it."""
+    return 0.05 + 0.0013 * k, 0.35 - 0.0009 * k, -0.2 + 0.0011 * k, 37.0 if k < step_day else 38.0
+
+
+def write_eop(path, mjd_first, days, varying=False, step_day=None):
+    """HPOP EOP-All rows: y m d mjd xp yp ut1-utc lod dpsi deps dx dy tai-utc.
+    Constant (zero pole, UT1 = UTC, TAI-UTC 37 s) or varying (varying_eop_day,
+    with the TAI-UTC step on day step_day, by default mid-table)."""
+    if step_day is None:
+        step_day = days // 2
     lines = []
     for k in range(days):
         y, mo, d = mjd_to_calendar(mjd_first + k)
-        lines.append("%d %2d %2d %d 0.000000 0.000000 0.0000000 0.0000000 0.000000 0.000000 0.000000 0.000000 37"
-                     % (y, mo, d, mjd_first + k))
+        xp, yp, dut, dat = varying_eop_day(k, step_day) if varying else (0.0, 0.0, 0.0, 37.0)
+        lines.append("%d %2d %2d %d %.6f %.6f %.7f 0.0000000 0.000000 0.000000 0.000000 0.000000 %d"
+                     % (y, mo, d, mjd_first + k, xp, yp, dut, int(dat)))
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -259,20 +285,24 @@ def write_config(path, model, duration, step, max_steps, precision=None):
         json.dump(cfg, f, indent=2)
 
 
-def environment_files(out, degree, seed, mjd_lo, mjd_hi, kaula=False):
+def environment_files(out, degree, seed, mjd_lo, mjd_hi, kaula=False, varying=False):
     with open(os.path.join(out, "constants.json"), "w") as f:
         json.dump(CONSTANTS, f, indent=1)
     gfile = "gravity_%d.tab" % degree
     write_gravity(os.path.join(out, gfile), degree, seed, kaula)
     first = int(math.floor(mjd_lo)) - 10
     days = int(math.ceil(mjd_hi)) - first + 10
-    write_space_weather(os.path.join(out, "SW-All.txt"), first, days)
-    write_eop(os.path.join(out, "EOP-All.txt"), first, days)
+    write_space_weather(os.path.join(out, "SW-All.txt"), first, days, varying=varying)
+    # the TAI-UTC step falls at the first UTC midnight after the workload starts
+    # (MJD 55579, 69 s after an epoch of ET 4034.5 days)
+    write_eop(os.path.join(out, "EOP-All.txt"), first, days, varying=varying, step_day=10)
     return gfile
 
 
-def make_workload(out, which, n=None, duration=None, step=None, degree=None, seed=None, precision=None):
-    """Write CONFIG.json, TARGETS.json and data files for workload 1..4; returns the paths."""
+def make_workload(out, which, n=None, duration=None, step=None, degree=None, seed=None, precision=None, varying=False):
+    """Write CONFIG.json, TARGETS.json and data files for workload 1..4; returns the paths.
+    varying: space weather and EOP tables whose values change every day and
+    3-hour slot (drifting pole and UT1, one TAI-UTC step) instead of constant ones."""
     os.makedirs(out, exist_ok=True)
     mjd_utc0 = EPOCH0 + 51544.5
     if which == 1:
@@ -291,7 +321,8 @@ def make_workload(out, which, n=None, duration=None, step=None, degree=None, see
         degree = degree or (70 if which == 2 else 120)
         duration = duration or 86400.0
         step = step or (60.0 if which == 2 else 30.0)
-        gfile = environment_files(out, degree, 70 if which == 2 else 120, mjd_utc0, mjd_utc0 + duration / 86400.0)
+        gfile = environment_files(out, degree, 70 if which == 2 else 120, mjd_utc0, mjd_utc0 + duration / 86400.0,
+                                  varying=varying)
         model = full_model(degree, gfile, mjd_utc0 - 1, mjd_utc0 + duration / 86400.0 + 1)
         if which == 3 and precision is None:
             precision = {"mode": "mixed", "fp64Degree": 20}
@@ -303,7 +334,7 @@ def make_workload(out, which, n=None, duration=None, step=None, degree=None, see
         degree = degree or 70
         step = step or 60.0
         duration = duration or 86400.0
-        gfile = environment_files(out, degree, 70, mjd_utc0, mjd_utc0 + 30.0 + 8.0)
+        gfile = environment_files(out, degree, 70, mjd_utc0, mjd_utc0 + 30.0 + 8.0, varying=varying)
         model = full_model(degree, gfile, mjd_utc0 - 1, mjd_utc0 + 39.0)
     else:
         raise ValueError("workload must be 1..4")
