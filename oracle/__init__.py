@@ -51,6 +51,8 @@ def library():
         lib.orc_shadow.argtypes = [vp, vp, d, d]
         lib.orc_shadow.restype = d
         lib.orc_sh_body_fixed.argtypes = [vp, vp, vp]
+        lib.orc_ocean_tide.argtypes = [vp, vp, vp, vp]
+        lib.orc_tide_acceleration.argtypes = [vp, vp, vp, vp, vp, i, vp]
         lib.orc_last_error.restype = ctypes.c_char_p
         lib.orc_destroy.argtypes = [vp]
         lib.orc_destroy.restype = None
@@ -139,6 +141,20 @@ class Oracle:
         C, S = np.zeros(49), np.zeros(49)
         self.lib.orc_ocean_waves(self.h, _p(np.ascontiguousarray(args5, dtype=np.float64)), float(gmst), _p(C), _p(S))
         return C.reshape(7, 7), S.reshape(7, 7)
+
+    def ocean_tide(self, time5):
+        """Whole ocean-tide coefficient set (waves plus the ocean pole tide)."""
+        C, S = np.zeros(49), np.zeros(49)
+        self.lib.orc_ocean_tide(self.h, _p(np.ascontiguousarray(time5, dtype=np.float64)), _p(C), _p(S))
+        return C.reshape(7, 7), S.reshape(7, 7)
+
+    def tide_acceleration(self, r, E, C, S, max_degree):
+        """Acceleration (m/s^2, ICRF) of a degree 2..max_degree potential at r (m, ICRF)."""
+        out = np.zeros(3)
+        a = [np.ascontiguousarray(x, dtype=np.float64).ravel() for x in (r, E, C, S)]
+        if self.lib.orc_tide_acceleration(self.h, *[_p(x) for x in a], int(max_degree), _p(out)) != 0:
+            raise RuntimeError(self.lib.orc_last_error().decode())
+        return out
 
     def sh_body_fixed(self, p):
         out = np.zeros(3)

@@ -570,6 +570,27 @@ double orc_shadow(const double* sun3, const double* body3, double rSun, double r
     return shadowFactor(V3{sun3[0], sun3[1], sun3[2]}, V3{body3[0], body3[1], body3[2]}, rSun, rBody);
 }
 
+int orc_ocean_tide(const orc_ctx* ctx, const double* time5, double* C49, double* S49) {
+    const double zero[9] = {};
+    CS7 cs;
+    oceanCoefficients(tideInput(zero, zero, zero, time5), ctx->tides, cs);
+    unpackCS(cs, C49, S49);
+    return 0;
+}
+
+int orc_tide_acceleration(const orc_ctx* ctx, const double* r3, const double* E9, const double* C49, const double* S49,
+                          int max_degree, double* out3) {
+    if (max_degree < 2 || max_degree > 6) { g_error = "max_degree must be 2..6"; return -1; }
+    const double zero[9] = {};
+    TideInput in = tideInput(zero, zero, E9, zero);
+    for (int i = 0; i < 3; ++i) in.r[i] = r3[i];
+    CS7 cs;
+    for (int n = 0; n < 7; ++n)
+        for (int m = 0; m < 7; ++m) { cs.C[n][m] = C49[7 * n + m]; cs.S[n][m] = S49[7 * n + m]; }
+    tideAcceleration(in, ctx->tideK, cs, out3, max_degree);
+    return 0;
+}
+
 int orc_sh_body_fixed(const orc_ctx* ctx, const double* p3, double* out3) {
     if (!ctx->hasSh) { g_error = "no spherical-harmonic field configured"; return -1; }
     const V3 a = shBodyFixed(V3{p3[0], p3[1], p3[2]}, ctx->sh);

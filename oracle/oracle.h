@@ -68,7 +68,10 @@ double orc_exp_density(const orc_ctx* ctx, double h);
  * arguments and the Doodson arguments for time data. orc_third_body: the
  * non-barycentric third-body acceleration. orc_shadow: the dual-cone factor
  * (Sun and occulter relative to the object, km). orc_sh_body_fixed: the SH
- * acceleration at a body-fixed position. */
+ * acceleration at a body-fixed position. orc_ocean_tide: the whole ocean-tide
+ * coefficient set (waves and the ocean pole tide) for time data.
+ * orc_tide_acceleration: the acceleration (m/s^2, ICRF) of the degree
+ * 2..max_degree potential with coefficients C, S at an ICRF position (m). */
 int orc_time_at(const orc_ctx* ctx, double et, double* out6);
 int orc_nrl_inputs(const orc_ctx* ctx, double et, const double* fixed3, double* out19);
 int orc_solid_tide(const orc_ctx* ctx, const double* moon3, const double* sun3, const double* E9, const double* time5,
@@ -78,6 +81,9 @@ int orc_tide_arguments(const double* time5, double* args5, double* gmst, double*
 int orc_third_body(const double* r3, const double* b3, double gm, double* out3);
 double orc_shadow(const double* sun3, const double* body3, double rSun, double rBody);
 int orc_sh_body_fixed(const orc_ctx* ctx, const double* p3, double* out3);
+int orc_ocean_tide(const orc_ctx* ctx, const double* time5, double* C49, double* S49);
+int orc_tide_acceleration(const orc_ctx* ctx, const double* r3, const double* E9, const double* C49, const double* S49,
+                          int max_degree, double* out3);
 double orc_step_seconds(const orc_ctx* ctx);
 double orc_duration_seconds(const orc_ctx* ctx);
 const char* orc_last_error(void);

@@ -6,9 +6,10 @@
 //       (tselec, glatf, ccor, ccor2, scalh, dnet, splini, splint, spline,
 //        zeta, densm, densu, g0, sumex, sg0, globe7, glob7s, gtd7, gtd7d, gts7)
 // The coefficient blocks are read from nrlmsise00_coefficients.txt.
-// Parity of this restatement with the reference's compiled gtd7d is unpinned
-// by reference-held vectors (none exist); see the tests for the checks that
-// do pin it (monotonicity, published bands, FP32-independent invariants).
+// Pinned by the 15 published test-driver outputs and the ap-array case
+// (tests/golden/nrlmsise00_testdriver.json); the input preparation (ET -> UTC,
+// day of year, seconds, GAST hours, weather record and 3-hour slots) by the
+// published input definitions (tests/test_oracle_pins_env.py).
 #pragma once
 
 #include <cmath>

@@ -12,18 +12,21 @@
 // ephemeris and the .brot2 orientation files that the configurations do not
 // provide) follow the published methods named at each function.
 //
-// What pins each part (tests/test_oracle_pins.py and the NRLMSISE-00 golden
-// vectors): RKF78 tableau identities; closed-form Kepler propagation and the
-// energy / angular-momentum invariants for the central-gravity driver; final
-// step shortening; the SH sum against the analytic J2 formula (zonal field)
-// and against finite differences of the potential; shadow geometry limits;
-// the relativity closed form; the analytic ephemeris against its published
-// accuracy bands; NRLMSISE-00 against the published test-driver outputs.
-// Parity unpinned (restatement only, no reference-held vector or closed form
-// covers it): the EOP time model (timeAt, ET -> UTC iterations), the solid
-// Earth tide coefficients (IERS 2010 frequency-dependent terms), the FES2004
-// ocean tide synthesis, and the NRLMSISE-00 input preparation (ET -> UTC,
-// day of year, GAST hours, weather lookup).
+// What pins each part (tests/test_oracle_pins.py, tests/test_oracle_pins_env.py
+// and the NRLMSISE-00 golden vectors): RKF78 tableau identities; closed-form
+// Kepler propagation and the energy / angular-momentum invariants for the
+// central-gravity driver; final step shortening; the SH sum against the
+// analytic J2 formula (zonal field), finite differences of the potential and
+// its own limit at the pole; the third body against its on-axis form and tidal
+// limit; the shadow against disk-overlap areas, the annular 1 - b^2/a^2 and
+// continuity; the relativity closed form; the analytic ephemeris against its
+// published accuracy bands; the EOP time model against linear tables with a
+// TAI-UTC step; the solid tide against IERS 2010 eqs. 6.6-6.8 and 6.13 and
+// Tables 6.5a-c; the FES2004 sum against eq. 6.15 and its Doodson numbers; the
+// Doodson arguments against the textbook mean longitudes; the pole tides
+// against the IERS 2010 mean pole; the tide acceleration against finite
+// differences of its potential. Parity unpinned: only the ULP-level behaviour
+// of the device math library against glibc.
 #pragma once
 
 #include <cmath>
