@@ -6,9 +6,11 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def build(jobs=8, verbose=False):
+def build(jobs=8, verbose=False, force=True):
+    """Compile libphx.so for gfx950; force recompiles every source even when
+    the objects are newer than their sources."""
     csrc = os.path.join(HERE, "csrc")
-    cmd = ["make", "-C", csrc, "-j%d" % jobs]
+    cmd = ["make", "-C", csrc, "-j%d" % jobs] + (["-B"] if force else [])
     out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if verbose or out.returncode != 0:
         sys.stdout.write(out.stdout)

@@ -56,6 +56,80 @@ const double kRkC[12] = {2. / 27, 1. / 9, 1. / 6, 5. / 12, 1. / 2, 5. / 6, 1. / 
 
 namespace {
 
+// TDB - TT (s) at a TT MJD: the series of EarthCorrections.h:46-56.
+double tdbMinusTtHost(double mjdTt) {
+    const double t = (mjdTt - 51544.5) / 36525.0;
+    return 0.001657 * std::sin(628.3076 * t + 6.2401) + 0.000022 * std::sin(575.3385 * t + 4.2970) +
+           0.000014 * std::sin(1256.6152 * t + 6.1969) + 0.000005 * std::sin(606.9777 * t + 4.0212) +
+           0.000005 * std::sin(52.9691 * t + 0.4444) + 0.000002 * std::sin(21.3299 * t + 5.5431) +
+           0.000010 * std::sin(628.3076 * t + 4.2490);
+}
+
+// TAI-UTC (s) that the EOP time model's ET -> UTC iteration settles on at `et`
+// (EarthCorrections.h:87-105, the same steps as the device eopTimes): UTC =
+// TT - 32.184 s - this value.
+double utcConvention(const EopTable& e, double et) {
+    double ttEt = et;
+    for (int j = 0; j < 2; ++j) ttEt = et - tdbMinusTtHost(51544.5 + ttEt / 86400.0);
+    double utc = 51544.5 + (ttEt - 69.184) / 86400.0;
+    double dat = 0.0;
+    for (int j = 0; j < 3; ++j) {
+        double day = std::floor(utc - e.firstMjdUtc);
+        day = std::fmin(std::fmax(day, 0.0), double(e.days - 1));
+        dat = e.dat[size_t(day)];
+        utc = 51544.5 + (ttEt - 32.184 - dat) / 86400.0;
+    }
+    return dat;
+}
+
+// The UT1 steps of the analytic orientation at the table's leap seconds. The
+// time model's UTC steps back by each leap second while the table's UT1-UTC is
+// interpolated without a step, so UT1 and the Earth rotation angle step with
+// it. Each step of at least half a second (the leap seconds; the fractional
+// TAI-UTC drift of tables before 1972 is left to the interpolation) is located
+// to 1e-9 s where utcConvention changes, scanning the seconds around the
+// table's TAI-UTC change; the offset after it is -(dERA/dUT1) times the sum of
+// the steps so far. Empty without an EOP table or with a .brot2 orientation
+// (whose UT1 - TT has no steps).
+void computeLeaps(phx_ctx& c) {
+    c.leapCount = 0;
+    if (!c.env.hasEop || !c.env.brot.empty()) return;
+    const EopTable& e = c.env.eop;
+    const double rate = 2.0 * 3.141592653589793238462643383279502884 * 1.00273781191135448 / 86400.0;
+    std::vector<double> at, theta;
+    double stepped = 0.0;
+    for (int i = 1; i < e.days; ++i) {
+        if (!(std::fabs(e.dat[size_t(i)] - e.dat[size_t(i - 1)]) >= 0.5)) continue;
+        const double mid = (e.firstMjdUtc + i - 51544.5) * 86400.0 + 32.184 + 0.5 * (e.dat[size_t(i)] + e.dat[size_t(i - 1)]);
+        const double lo0 = mid - 0.5 * std::fabs(e.dat[size_t(i)] - e.dat[size_t(i - 1)]) - 3.0;
+        const int n = 6000;   // 1 ms scan of the window, then bisection
+        const double span = std::fabs(e.dat[size_t(i)] - e.dat[size_t(i - 1)]) + 6.0;
+        double prev = utcConvention(e, lo0), prevT = lo0;
+        for (int k = 1; k <= n; ++k) {
+            const double t = lo0 + span * double(k) / n;
+            const double v = utcConvention(e, t);
+            if (v != prev) {
+                double a = prevT, b = t;
+                for (int it = 0; it < 60 && b - a > 1e-9; ++it) {
+                    const double m = 0.5 * (a + b);
+                    (utcConvention(e, m) == prev ? a : b) = m;
+                }
+                stepped += v - prev;
+                at.push_back(b);
+                theta.push_back(-rate * stepped);
+                prev = v;
+            }
+            prevT = t;
+        }
+    }
+    if (at.empty()) return;
+    if (at.size() > 256) throw std::runtime_error("EOP table: more than 256 leap-second steps");
+    std::vector<double> v(at);
+    v.insert(v.end(), theta.begin(), theta.end());
+    c.leaps.upload(v.data(), v.size(), c.stream);
+    c.leapCount = int(at.size());
+}
+
 void loadTables(phx_ctx& c) {
     const std::string dir = c.dataDir;
     if (c.env.model.tidesActive()) {
@@ -308,6 +382,9 @@ EpochArgs epochArgs(const phx_ctx& c) {
     a.iers = c.iers.p;
     a.iersNut = c.env.brot.empty() ? 0 : c.env.brot.iers[0].nNut;
     a.iersErp = c.env.brot.empty() ? 0 : c.env.brot.iers[0].nErp;
+    a.leaps = c.leaps.p;
+    a.leapCount = c.leapCount;
+    a.gridRows = 0;
     return a;
 }
 
@@ -652,6 +729,7 @@ std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, c
     ea.count = int(epochs.size());
     ea.epochs = dEpochs.p;
     ea.out = b->records.p;
+    ea.gridRows = grid ? 1 : 0;
     check(hipEventRecord(c.ev0, s), "hipEventRecord");
     if (launchEpochRecords(ea, c.fm, s) != 0) throw HipError("epoch-record kernel launch failed");
     check(hipEventRecord(c.ev1, s), "hipEventRecord");
@@ -677,6 +755,8 @@ StepArgs contextStepArgs(const phx_ctx& c, int& W) {
     a.wStart = c.env.hasWeather ? c.env.weather.startMjdUtc : 0.0;
     a.nrlSlots = c.nrlSlots.p;
     a.gridInvH = 1.0 / c.gridStep;
+    a.leaps = c.leaps.p;
+    a.leapCount = c.leapCount;
     // drag models run at most four waves per tile (the NRLMSISE-00 path needs
     // the register budget of one wave per SIMD); without a field the force
     // families spread over at most four waves
@@ -887,6 +967,7 @@ int phx_create(const char* config_json_path, phx_ctx** out) {
         if (uploadTideFactors() != 0) throw HipError("tide factor upload failed");
         if (noallocInit() != 0) throw HipError("device scratch array of phx_propagate_noalloc not found");
         loadTables(*c);
+        computeLeaps(*c);
         if (c->env.hasSh) buildShTables(*c);
         if (!c->env.brie.empty()) {
             std::vector<double> rows(size_t(kEphSlots) * kEphRow, 0.0);

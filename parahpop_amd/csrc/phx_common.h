@@ -132,6 +132,11 @@ struct StepArgs {
     const double* slotT0;
     const int* slotRows;
     const double* nrlSlots;      // [wDays * 8][nrlmsise::SLOT_SIZE] weather-slot rows (uneven epochs)
+    // Grid rows hold the rotation without the UT1 steps of leap seconds; an
+    // object applies R3(theta) of its own stage epoch (leaps: [leapCount] ET of
+    // each step, then [leapCount] offsets theta after it)
+    const double* leaps;
+    int leapCount;
     const double* weather;       // 13 x wDays SoA (CelesTrak components)
     int wDays;
     double wStart;

@@ -57,6 +57,18 @@ __global__ void __launch_bounds__(64) epochRecordKernel(EpochArgs a, ForceModel 
     if (a.iers) iersOrientation(a, et, R, w);
     else orientation(a, et, R, w);
     for (int k = 0; k < 9; ++k) rec[REC_R + k] = R[k];
+    if (a.gridRows && a.leapCount > 0 && !a.iers) {
+        // interpolation rows: R3(-theta) R, smooth across the UT1 steps (the
+        // step kernels apply R3(theta) of each object's own epoch)
+        const double th = leapTheta(a.leaps, a.leapCount, et);
+        if (th != 0.0) {
+            const double c = cos(th), s = sin(th);
+            for (int k = 0; k < 3; ++k) {
+                rec[REC_R + k] = c * R[k] - s * R[3 + k];
+                rec[REC_R + 3 + k] = s * R[k] + c * R[3 + k];
+            }
+        }
+    }
     for (int k = 0; k < 3; ++k) rec[REC_OMEGA + k] = w[k];
     double ind[3] = {0.0, 0.0, 0.0};
     for (int b = 0; b < fm.nBodies; ++b) {

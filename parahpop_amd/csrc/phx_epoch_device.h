@@ -46,6 +46,19 @@ __device__ double tdbMinusTt(double mjdTt) {
 
 struct Times { double mjdUtc, ut1Utc, ttUtc, xp, yp; bool valid; };
 
+// Earth rotation angle offset (rad) that the leap seconds before `et` add to
+// the analytic orientation: the EOP time model's UTC, and with it UT1 = UTC +
+// (UT1-UTC), steps back by each leap second while the table's UT1-UTC stays
+// continuous. leaps: [n] step epochs (ET, ascending), then [n] offsets.
+__device__ inline double leapTheta(const double* leaps, int n, double et) {
+    double th = 0.0;
+    for (int k = 0; k < n; ++k) {
+        if (et < leaps[k]) break;
+        th = leaps[n + k];
+    }
+    return th;
+}
+
 // ET -> UTC/UT1/TT offsets and polar motion from the daily EOP table.
 __device__ Times eopTimes(const EpochArgs& a, double et) {
     Times t;

@@ -83,6 +83,10 @@ struct phx_ctx {
     phx::DBuf<double> shTerms, shDiag, nrl, freq, waves, terms, eop, weather, nrlSlots;
     phx::DBuf<double> eph, ephChain;   // Chebyshev ephemeris data and per-slot chains (empty: analytic)
     phx::DBuf<double> iers;            // IERS2000 orientation payload (empty: analytic model)
+    // UT1 steps of the EOP time model at leap seconds: [leapCount] ET of each
+    // step, then [leapCount] Earth rotation angle offsets after it
+    phx::DBuf<double> leaps;
+    int leapCount = 0;
     std::vector<std::vector<phx::BrieStep>> ephChains;   // host copy, for coverage checks
     phx::DBuf<float> shTerms32;
     phx::DBuf<int> shOrderStart;

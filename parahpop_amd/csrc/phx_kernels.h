@@ -38,6 +38,12 @@ struct EpochArgs {
     // model) and its nutation / ERP node counts
     const double* iers;
     int iersNut, iersErp;
+    // leap-second steps of UT1 ([leapCount] ET, then [leapCount] rotation
+    // offsets); gridRows 1: the records are interpolation rows, whose rotation
+    // is stored without the steps (R3(-theta(et)) R) so that it is smooth
+    const double* leaps;
+    int leapCount;
+    int gridRows;
 };
 
 constexpr int kEphSteps = 6;

@@ -225,6 +225,7 @@ extern "C" int phx_propagate_noalloc(phx_ctx* c, size_t n, double* d_epoch, doub
         hipLaunchKernelGGL(naGridKernel, dim3((std::max(nPad, rows) + 255) / 256), dim3(256), 0, s, base, nPad, rows, gridT0, h);
         check(hipGetLastError(), "grid kernel launch");
         EpochArgs ea = epochArgs(*c);
+        ea.gridRows = 1;
         ea.count = rows;
         ea.epochs = at<double>(base, kOffRowEt);
         ea.out = at<double>(base, kOffRec);

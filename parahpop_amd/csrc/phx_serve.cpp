@@ -288,6 +288,7 @@ bool Server::admitOne(Request& r) {
     }
     h2d(epochStage_.p, epochs.data(), epochs.size() * sizeof(double));
     EpochArgs ea = epochArgs(c_);
+    ea.gridRows = 1;
     ea.count = int(r.rows);
     ea.epochs = epochStage_.p;
     ea.out = records_.p + size_t(rec0) * REC_SIZE;

@@ -77,6 +77,18 @@ struct GridRec {
         return w0 * p[f] + w1 * p[REC_SIZE + f] + w2 * p[2 * REC_SIZE + f] + w3 * p[3 * REC_SIZE + f];
     }
 };
+// Grid rows with the object's leap-second rotation: rows hold R3(-theta) R
+// (smooth); the first two rows of R are turned back by the object's theta
+// (c = cos theta, s = sin theta).
+struct GridRecL {
+    GridRec g;
+    double c, s;
+    __device__ double operator()(int f) const {
+        if (f < 3) return c * g(f) + s * g(f + 3);
+        if (f < 6) return c * g(f) - s * g(f - 3);
+        return g(f);
+    }
+};
 __device__ inline GridRec gridRec(const double* table, int count, double t0, double invH, double ts) {
     const double x = (ts - t0) * invH;
     int k = int(floor(x));
@@ -1248,7 +1260,8 @@ __global__ void __launch_bounds__(64 * W) rkf78Kernel(StepArgs a, ForceModel fm)
             double acc[3];
             if (GRID) {
                 const double ts = xch[6][lane];
-                const GridRec rec = gridRec(gridBase, gridRows, gridT0, a.gridInvH, ts);
+                const double th = a.leapCount > 0 ? epochdev::leapTheta(a.leaps, a.leapCount, ts) : 0.0;
+                const GridRecL rec{gridRec(gridBase, gridRows, gridT0, a.gridInvH, ts), cos(th), sin(th)};
                 double nrl[13];
                 NrlSlot slot{a.nrlSlots, 0.0, 0.0};
                 if (DRAG && fm.drag) {
@@ -1390,6 +1403,13 @@ __device__ inline void splitStageState(const StepArgs& a, const SplitArgs& sp, i
             if (s >= 1) ts += a.nextDt[obj] * kRkC[s - 1];
             const GridRec own = gridRec(a.records + a.slotRecord[obj] * REC_SIZE, a.slotRows[obj], a.slotT0[obj], a.gridInvH, ts);
             const long long ownOff = (long long)(own.p - a.records);
+            // leap seconds: the rows' rotation is smooth; this lane's is R3(theta) of it
+            double lc = 1.0, ls = 0.0;
+            if (a.leapCount > 0) {
+                const double th = epochdev::leapTheta(a.leaps, a.leapCount, ts);
+                lc = cos(th);
+                ls = sin(th);
+            }
             for (int item = (wave - first) * kTile + lane; item < kTile * kRecStaged; item += stagers * kTile) {
                 const int l = item / kRecStaged, f = item - l * kRecStaged;
                 GridRec g;
@@ -1398,7 +1418,15 @@ __device__ inline void splitStageState(const StepArgs& a, const SplitArgs& sp, i
                 g.w1 = __shfl(own.w1, l);
                 g.w2 = __shfl(own.w2, l);
                 g.w3 = __shfl(own.w3, l);
-                recF[GRID ? f : 0][l] = g(f);
+                double v = g(f);
+                if (a.leapCount > 0) {
+                    const double c = __shfl(lc, l), sn = __shfl(ls, l);
+                    if (f < 6) {
+                        const double q = g(f < 3 ? f + 3 : f - 3);
+                        v = f < 3 ? c * v + sn * q : c * v - sn * q;
+                    }
+                }
+                recF[GRID ? f : 0][l] = v;
             }
         }
     }
