@@ -366,6 +366,32 @@ def leg_ephemeris(args, work):
     return out
 
 
+def serve_run(ctx, reqs, gaps):
+    """Submit reqs with the given inter-arrival gaps, polling meanwhile; returns
+    (results by request index, latencies, wall seconds, object-steps)."""
+    steps0 = ctx.option("serve_object_steps")
+    pending, results = {}, {}
+    t0 = time.perf_counter()
+    next_t = t0
+    for i, (r, du) in enumerate(reqs):
+        next_t += gaps[i]
+        while time.perf_counter() < next_t:
+            for j, k in list(pending.items()):
+                res = ctx.poll(k)
+                if res is not None:
+                    results[j] = res
+                    del pending[j]
+            time.sleep(0.001)
+        pending[i] = ctx.submit(r, du)
+        if (i + 1) % 50 == 0:
+            log("serve: submitted %d/%d" % (i + 1, len(reqs)))
+    for j, k in pending.items():
+        results[j] = ctx.wait(k, timeout=900.0)
+    wall = time.perf_counter() - t0
+    lat = np.array([results[j][3] - results[j][2] for j in range(len(reqs))])
+    return results, lat, wall, ctx.option("serve_object_steps") - steps0
+
+
 def leg_serve(args, work):
     import parahpop_amd
     from parahpop_amd import synth
@@ -381,51 +407,48 @@ def leg_serve(args, work):
     nreq = args.serve_requests
     sizes = np.exp(rng.uniform(0.0, math.log(args.serve_max_objects), nreq)).astype(int).clip(1, pool_n)
     durs = np.exp(rng.uniform(math.log(3600.0), math.log(3 * 86400.0), nreq))
-    gaps = rng.exponential(1.0 / args.serve_rate, nreq)
     starts = rng.integers(0, pool_n - sizes + 1)
     reqs = [(subset(t, np.arange(s, s + k)), float(du)) for s, k, du in zip(starts, sizes, durs)]
     ctx = parahpop_amd.Context(cfg_mixed, serve_steps_per_tick=args.serve_steps_per_tick)
     # warm the server (pool allocation, graph capture) with a tiny request
     ctx.wait(ctx.submit(subset(t, [0]), 600.0))
-    steps0 = ctx.option("serve_object_steps")
-    tickets, results = [], {}
-    t0 = time.perf_counter()
-    next_t = t0
-    for i, (r, du) in enumerate(reqs):
-        next_t += gaps[i]
-        while time.perf_counter() < next_t:
-            for k in list(tickets):
-                res = ctx.poll(k)
-                if res is not None:
-                    results[k] = res
-                    tickets.remove(k)
-            time.sleep(0.001)
-        tickets.append(ctx.submit(r, du))
-        if i == 0:
-            first = tickets[-1]
-        log("serve: submitted %d/%d (%d objects, %.0f s)" % (i + 1, nreq, r["n"], du))
-    order = list(tickets)
-    all_tickets = list(range(first, first + nreq))
-    for k in order:
-        results[k] = ctx.wait(k, timeout=900.0)
-    wall = time.perf_counter() - t0
-    lat = np.array([results[k][3] - results[k][2] for k in all_tickets])
-    obj_steps = ctx.option("serve_object_steps") - steps0
-    # error vs the FP64 drop-in path for the smallest requests
-    small = np.argsort(sizes)[: min(4, nreq)]
+    rates = [float(x) for x in args.serve_rates.split(",") if x]
+    out = {"workload": "serving: %d Poisson requests at each of %s /s, 1-%d objects log-U, 1 h-3 d log-U, mixed 70x70"
+                       % (nreq, "/".join("%g" % r for r in rates), args.serve_max_objects),
+           "requests": nreq, "objects_requested": int(sizes.sum()), "rates": []}
+    first = None
+    for rate in rates:
+        gaps = np.random.default_rng(17).exponential(1.0 / rate, nreq)
+        captures0 = ctx.option("serve_graph_captures")
+        results, lat, wall, obj_steps = serve_run(ctx, reqs, gaps)
+        entry = {"rate_per_s": rate, "object_steps_per_s": obj_steps / wall, "object_steps": obj_steps, "wall_seconds": wall,
+                 "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+                 "latency_max_s": float(lat.max()), "graph_captures": ctx.option("serve_graph_captures") - captures0}
+        if first is None:
+            first = results
+        else:
+            # the same requests in another mix: the same bits
+            entry["results_equal_first_rate"] = all(np.array_equal(results[j][1], first[j][1]) for j in range(nreq))
+        out["rates"].append(entry)
+        log("serve: rate %g/s p50 %.2f s p99 %.2f s max %.2f s, %.3g object-steps/s"
+            % (rate, entry["latency_p50_s"], entry["latency_p99_s"], entry["latency_max_s"], entry["object_steps_per_s"]))
+    for k in ("object_steps_per_s", "latency_p50_s", "latency_p99_s", "latency_max_s"):
+        out[k] = out["rates"][-1][k]
+    # error of every served object against the FP64 drop-in path: all requests
+    # as one uneven FP64 batch (per-object durations)
+    idx = np.concatenate([np.arange(s, s + k) for s, k in zip(starts, sizes)])
+    dur = np.concatenate([np.full(k, du) for k, du in zip(sizes, durs)])
     ctx64 = parahpop_amd.Context(cfg)
-    errs = []
-    for j in small:
-        r, du = reqs[j]
-        s64 = ctx64.propagate(r, du)[1]
-        errs.append(pos_err_mm(results[all_tickets[j]][1], s64)[1])
-    return {"workload": "serving: %d Poisson requests at %.2f/s, 1-%d objects log-U, 1 h-3 d log-U, mixed 70x70"
-                        % (nreq, args.serve_rate, args.serve_max_objects),
-            "object_steps_per_s": obj_steps / wall, "object_steps": obj_steps, "wall_seconds": wall,
-            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
-            "latency_max_s": float(lat.max()), "graph_captures": ctx.option("serve_graph_captures"),
-            "ticks": ctx.option("serve_ticks"),
-            "mixed_vs_fp64_pos_mm_max_smallest4": float(max(errs)) if errs else None}
+    every = subset(t, idx)
+    every["id"] = np.arange(len(idx), dtype=np.int64)   # requests overlap in the pool: one id per served object
+    b = ctx64.batch(every, dur)
+    b.run()
+    s64 = b.download()[1]
+    served = np.concatenate([first[j][1] for j in range(nreq)], axis=1)
+    rms, mx = pos_err_mm(served, s64)
+    out["mixed_vs_fp64_pos_mm"] = {"objects": int(len(idx)), "rms": rms, "max": mx}
+    log("serve: mixed vs FP64 over %d served objects: rms %.3g mm, max %.3g mm" % (len(idx), rms, mx))
+    return out
 
 
 def main():
@@ -446,8 +469,8 @@ def main():
     ap.add_argument("--mixed-seconds", type=float, default=86400.0)
     ap.add_argument("--uneven-objects", type=int, default=131072)
     ap.add_argument("--uneven-steps-per-launch", type=int, default=64)
-    ap.add_argument("--serve-requests", type=int, default=24)
-    ap.add_argument("--serve-rate", type=float, default=1.0)
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rates", default="4,16", help="Poisson arrival rates (requests/s), one run of all requests each")
     ap.add_argument("--serve-max-objects", type=int, default=50000)
     ap.add_argument("--serve-pool", type=int, default=65536)
     ap.add_argument("--serve-steps-per-tick", type=int, default=16)
