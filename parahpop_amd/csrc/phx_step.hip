@@ -60,6 +60,8 @@ __constant__ double kRkA[12][12] = {
     {3. / 205, 0, 0, 0, 0, -6. / 41, -3. / 205, -3. / 41, 3. / 41, 6. / 41, 0, 0},
     {-1777. / 4100, 0, 0, -341. / 164, 4496. / 1025, -289. / 82, 2193. / 4100, 51. / 82, 33. / 164, 12. / 41, 0, 1.0}};
 __constant__ double kRkB[13] = {0, 0, 0, 0, 0, 34. / 105, 9. / 35, 9. / 35, 9. / 280, 9. / 280, 0, 41. / 840, 41. / 840};
+// stages with nonzero b_s (5-9, 11, 12): the only k-stages the step combination reads
+constexpr unsigned kRkBNonZero = 0x1BE0u;
 
 enum : uint8_t { ST_TERMINATED = 1, ST_EOS = 2, ST_PAD = 4 };
 
@@ -1345,16 +1347,19 @@ __device__ inline size_t ksAt(int s, int d, int nPad, int obj) { return (size_t(
 
 // Component d of the stage state y_s = y + sum_j (dt a_sj) k_j, in the fused
 // kernel's arithmetic and order (so every kernel that forms it gets the same
-// bits). The k-stage loads do not depend on the coefficients: all are issued
-// at once and the terms of zero coefficients (and of j >= s) are not added.
+// bits). Only the k-stages of nonzero a_sj with j < s are read (55 of the 78
+// lower-triangle slots over a step); the loads are issued before the sums.
 __device__ __attribute__((always_inline)) inline double stageComponent(const StepArgs& a, const SplitArgs& sp, int s, int obj,
                                                                        int d, double dt) {
     const int nPad = a.nPad;
+    const double* arow = kRkA[s >= 1 ? s - 1 : 0];
     double k[kStages - 1];
 #pragma unroll
-    for (int j = 0; j < kStages - 1; ++j) k[j] = sp.ks[ksAt(j, d, nPad, obj)];
+    for (int j = 0; j < kStages - 1; ++j) {
+        k[j] = 0.0;
+        if (j < s && arow[j] != 0.0) k[j] = sp.ks[ksAt(j, d, nPad, obj)];
+    }
     double y = a.y[d * nPad + obj];
-    const double* arow = kRkA[s >= 1 ? s - 1 : 0];
 #pragma unroll
     for (int j = 0; j < kStages - 1; ++j) {
         const double aij = arow[j];
@@ -1861,12 +1866,15 @@ __global__ void __launch_bounds__(64) splitCombineKernel(StepArgs a, SplitArgs s
     if ((st & (ST_TERMINATED | ST_PAD)) == 0) {
         double dt = a.nextDt[obj], cur = a.cur[obj];
         // y += sum_s (dt b_s) k_s over the nonzero b_s, in stage order; the
-        // k-stage loads are issued before the sums
+        // k-stage loads (of the nonzero b_s only) are issued before the sums
         double y[6];
         for (int d = 0; d < 6; ++d) {
             double k[kStages];
 #pragma unroll
-            for (int s = 0; s < kStages; ++s) k[s] = sp.ks[ksAt(s, d, nPad, obj)];
+            for (int s = 0; s < kStages; ++s) {
+                k[s] = 0.0;
+                if ((kRkBNonZero >> s) & 1u) k[s] = sp.ks[ksAt(s, d, nPad, obj)];
+            }
             double v = a.y[d * nPad + obj];
 #pragma unroll
             for (int s = 0; s < kStages; ++s) {
