@@ -41,6 +41,8 @@ typedef struct phx_stats {
     int kernels_per_step;      /* kernels per step on the split-stage path; 0 on the fused path */
     int sh_groups_min;         /* split-stage path: fewest and most SH workgroups per tile over */
     int sh_groups_max;         /* the call's steps (1: the one-workgroup kernel); 0 on the fused path */
+    double graph_replays;      /* split-stage path: graphs of graph_steps steps replayed in the call */
+    double graph_captures;     /* split-stage path: graphs captured in the call (one per active-tile set) */
 } phx_stats;
 
 /* Creates a context from a ParaHPOP CONFIG.json: parses and validates the
@@ -129,6 +131,10 @@ void phx_batch_destroy(phx_batch* b);
  *                      the number of active tiles, so an object's last bits
  *                      (about 1e-15 relative per step) depend on the rest of its
  *                      batch and on the CU count; with 1 they do not
+ *   "graph_steps"      0 .. 256 (default 16): the split stages of this many steps
+ *                      are captured once as a graph and replayed while the active
+ *                      tiles stay the same for all of them (0: every kernel is
+ *                      launched); the results do not change
  *   "split_species"    8 (default): one eight-wave drag workgroup per tile (the
  *                      species and the drag); 1, 2 or 4: species per workgroup and
  *                      a finish kernel; 0: one four-wave drag workgroup per tile

@@ -35,7 +35,8 @@ class Stats(ctypes.Structure):
                 ("epoch_seconds", ctypes.c_double), ("wall_seconds", ctypes.c_double),
                 ("epoch_records", ctypes.c_double), ("launches", ctypes.c_double),
                 ("waves", ctypes.c_int), ("steps_per_launch", ctypes.c_int),
-                ("kernels_per_step", ctypes.c_int), ("sh_groups_min", ctypes.c_int), ("sh_groups_max", ctypes.c_int)]
+                ("kernels_per_step", ctypes.c_int), ("sh_groups_min", ctypes.c_int), ("sh_groups_max", ctypes.c_int),
+                ("graph_replays", ctypes.c_double), ("graph_captures", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -803,6 +803,7 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
     a.slotRows = b.slotRows.p;
     int remaining = b.totalSteps - b.stepsDone;
     if (maxSteps > 0) remaining = std::min(remaining, maxSteps);
+    const long long replays0 = b.graphReplays, captures0 = b.graphCaptures;
     long long before = 0;
     check(hipMemcpyAsync(&before, b.counters.p, sizeof before, hipMemcpyDeviceToHost, c.stream), "counter read");
     check(hipEventRecord(c.ev0, c.stream), "hipEventRecord");
@@ -825,9 +826,21 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
         const int Ws = c.splitWaves;
         orderBlocks(c, Ws, a.blockStart, true);
         SplitArgs sp{b.ks.p, b.ysBuf.p, b.parts.p, b.spec.p, b.lane.p, 0, 0, c.splitSpecies};
+        // the step index of exact records lives on the device (advanced by the
+        // combine kernel), so every step launches the same kernels and arguments
+        if (!b.stepDev.p) b.stepDev.alloc(1);
+        // a graph lives for one call (the options it was captured with may change between calls)
+        if (b.graphExec) check(hipGraphExecDestroy(b.graphExec), "hipGraphExecDestroy");
+        if (b.graph) check(hipGraphDestroy(b.graph), "hipGraphDestroy");
+        b.graphExec = nullptr;
+        b.graph = nullptr;
+        b.stepHost = b.stepsDone;
+        check(hipMemcpyAsync(b.stepDev.p, &b.stepHost, sizeof(int), hipMemcpyHostToDevice, c.stream), "step index");
+        sp.stepDev = b.stepDev.p;
         int groupsFor = 1;   // group count the group blocks in `sp` were split for
         // per stage: SH kernel, then either the drag kernel or species + finish kernels
         kernelsPerStep = kStages * (c.splitSpecies == 0 || c.splitSpecies == 8 ? 2 : 3) + 1;
+        const int K = c.graphSteps;
         while (remaining > 0) {
             int active = b.nTiles;
             while (active > 0 && b.tileSteps[size_t(active - 1)] <= b.stepsDone) --active;
@@ -859,6 +872,44 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
             sp.shPart = b.shPart.p;
             sp.famPart = b.famPart.p;
             sp.tileCount = b.tileCount.p;
+            // K steps as one graph while the active tiles stay the same for all of
+            // them (the next tile ends no sooner than K steps from here)
+            const int untilChange = b.tileSteps[size_t(active - 1)] - b.stepsDone;
+            if (K > 0 && remaining >= K && untilChange >= K) {
+                if (!b.graphExec || b.graphActive != active || b.graphG != G || b.graphK != K) {
+                    if (b.graphExec) check(hipGraphExecDestroy(b.graphExec), "hipGraphExecDestroy");
+                    if (b.graph) check(hipGraphDestroy(b.graph), "hipGraphDestroy");
+                    b.graphExec = nullptr;
+                    b.graph = nullptr;
+                    check(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                    int rc = 0;
+                    for (int k = 0; k < K && rc == 0; ++k) {
+                        for (int s = 0; s < kStages && rc == 0; ++s) {
+                            sp.stage = s;
+                            rc = launchSplitStage(a, c.fm, sp, Ws, c.stream);
+                        }
+                        if (rc == 0) rc = launchSplitCombine(a, sp, c.stream);
+                    }
+                    hipGraph_t g = nullptr;
+                    const hipError_t e = hipStreamEndCapture(c.stream, &g);
+                    if (rc != 0 || e != hipSuccess) {
+                        if (g) (void)hipGraphDestroy(g);
+                        throw HipError(std::string("split step graph capture failed: ") + hipGetErrorString(e));
+                    }
+                    b.graph = g;
+                    check(hipGraphInstantiate(&b.graphExec, b.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+                    b.graphActive = active;
+                    b.graphG = G;
+                    b.graphK = K;
+                    ++b.graphCaptures;
+                }
+                check(hipGraphLaunch(b.graphExec, c.stream), "hipGraphLaunch");
+                ++b.graphReplays;
+                launches += K * kernelsPerStep;
+                b.stepsDone += K;
+                remaining -= K;
+                continue;
+            }
             for (int s = 0; s < kStages; ++s) {
                 sp.stage = s;
                 if (launchSplitStage(a, c.fm, sp, Ws, c.stream) != 0) throw HipError("split stage launch failed");
@@ -913,6 +964,8 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
         stats->kernels_per_step = kernelsPerStep;
         stats->sh_groups_min = groupsMax ? groupsMin : 0;
         stats->sh_groups_max = groupsMax;
+        stats->graph_replays = double(b.graphReplays - replays0);
+        stats->graph_captures = double(b.graphCaptures - captures0);
         stats->wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
     }
     return 0;
@@ -1061,6 +1114,10 @@ int phx_set_option(phx_ctx* c, const char* key, double v) {
         if (!(v >= 1.0 && v <= double(kMaxGroups))) { c->setError("split_groups must be 1 .. 4"); return PHX_EINVAL; }
         c->splitGroups = int(v);
     }
+    else if (k == "graph_steps") {
+        if (!(v >= 0.0 && v <= 256.0)) { c->setError("graph_steps must be 0 .. 256"); return PHX_EINVAL; }
+        c->graphSteps = int(v);
+    }
     else if (k == "split_species") {
         if (v != 0 && v != 1 && v != 2 && v != 4 && v != 8) { c->setError("split_species must be 0, 1, 2, 4 or 8"); return PHX_EINVAL; }
         c->splitSpecies = int(v);
@@ -1103,6 +1160,7 @@ double phx_get_option(const phx_ctx* c, const char* key) {
     if (k == "grid_step") return c->gridStep;
     if (k == "force_grid") return c->forceGrid;
     if (k == "split_groups") return c->splitGroups;
+    if (k == "graph_steps") return c->graphSteps;
     if (k == "split") return c->split;
     if (k == "split_species") return c->splitSpecies;
     if (k == "split_waves") return c->splitWaves;

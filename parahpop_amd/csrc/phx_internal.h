@@ -100,6 +100,7 @@ struct phx_ctx {
     int sortAltitude = 1;
     int exactGroupsMax = 16;     // more distinct (start, end) groups than this -> grid epochs
     double gridStep = 60.0;      // s, spacing of the epoch grid
+    int graphSteps = 16;         // split stages: steps per replayed graph (0: launch every kernel)
     int forceGrid = 0;
     int split = 1;               // shared-epoch batches with drag: split stages
     int splitWaves = 16;         // waves per tile of the drag-free split kernel
@@ -143,6 +144,21 @@ struct phx_batch {
     double epochSeconds = 0.0;
     bool grid = false;
     double gridInvH = 0.0;
+    // split stages: the device step index of exact records, and the graph of
+    // graphK steps captured for graphActive tiles and graphG SH workgroups per tile
+    phx::DBuf<int> stepDev;
+    int stepHost = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    int graphActive = -1, graphG = 0, graphK = 0;
+    long long graphReplays = 0, graphCaptures = 0;
+    phx_batch() = default;
+    phx_batch(const phx_batch&) = delete;
+    phx_batch& operator=(const phx_batch&) = delete;
+    ~phx_batch() {
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
 };
 
 namespace phx {

@@ -89,6 +89,10 @@ struct SplitArgs {
     double* famPart;
     unsigned* tileCount;
     int groupBlockStart[kMaxGroupWaves + 1];
+    // shared-epoch records: the step index read on the device (nullptr: `step`);
+    // the combine kernel advances it, so a graph of several steps replays with
+    // the same arguments
+    int* stepDev;
 };
 int launchSplitStage(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, int shWaves, hipStream_t stream);
 int launchSplitCombine(const StepArgs& a, const SplitArgs& sp, hipStream_t stream);

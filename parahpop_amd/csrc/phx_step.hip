@@ -1345,6 +1345,11 @@ namespace {
 
 __device__ inline size_t ksAt(int s, int d, int nPad, int obj) { return (size_t(s) * 6 + d) * size_t(nPad) + obj; }
 
+// Step index of the exact records of the current stage (uniform).
+__device__ inline long long splitStep(const SplitArgs& sp) {
+    return sp.stepDev ? (long long)__builtin_amdgcn_readfirstlane(*sp.stepDev) : (long long)sp.step;
+}
+
 // Component d of the stage state y_s = y + sum_j (dt a_sj) k_j, in the fused
 // kernel's arithmetic and order (so every kernel that forms it gets the same
 // bits). Only the k-stages of nonzero a_sj with j < s are read (55 of the 78
@@ -1470,7 +1475,7 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
         }
         stageForcesParts<W>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex);
     } else {
-        const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
+        const double* recp = a.records + (a.tileRecord[tile] + splitStep(sp) * kStages + s) * REC_SIZE;
         const ExactRec rec{recp};
         if (fm.drag && wave == W - 4) nrlLaneStore(nrlLane(fm, rec, recp + REC_NRL, px, py, pz), sp.lane, nPad, obj);
         stageForcesParts<W>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex);
@@ -1574,7 +1579,7 @@ __global__ void __launch_bounds__(64 * W) splitShGroupKernel(StepArgs a, ForceMo
         stageForcesParts<W, false>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex, g == 0, rpl);
         if (wave == 0) splitGroupFinish(a, fm, sp, rec, g, tile, obj, lane, px, py, pz, rpl, ax, ex);
     } else {
-        const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
+        const double* recp = a.records + (a.tileRecord[tile] + splitStep(sp) * kStages + s) * REC_SIZE;
         const ExactRec rec{recp};
         if (g == 0 && fm.drag && wave == W - 4) nrlLaneStore(nrlLane(fm, rec, recp + REC_NRL, px, py, pz), sp.lane, nPad, obj);
         stageForcesParts<W, false>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex, g == 0, rpl);
@@ -1594,7 +1599,7 @@ __global__ void __launch_bounds__(256) splitDragKernel(StepArgs a, ForceModel fm
     const int nPad = a.nPad;
     const int s = sp.stage;
     const double px = sp.ys[obj], py = sp.ys[size_t(nPad) + obj], pz = sp.ys[size_t(2) * nPad + obj];
-    const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
+    const double* recp = a.records + (a.tileRecord[tile] + splitStep(sp) * kStages + s) * REC_SIZE;
     const ExactRec rec{recp};
     const double* nrl = recp + REC_NRL;
     const NrlLane nl = nrlLane(fm, rec, nrl, px, py, pz);
@@ -1634,7 +1639,7 @@ __global__ void __launch_bounds__(64 * SPW) splitSpeciesKernel(StepArgs a, Force
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
-    const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
+    const double* recp = a.records + (a.tileRecord[tile] + splitStep(sp) * kStages + s) * REC_SIZE;
     const double* nrl = recp + REC_NRL;
     const NrlLane nl = nrlLaneLoad(sp.lane, nPad, obj, nrl);
     if (__ballot(nl.state == 1) == 0ull) return;
@@ -1702,7 +1707,7 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
     const int s = sp.stage;
     // shared epochs: the tile's record; per-lane epochs: the lane's time inputs
     // and weather-slot row (the globe terms of its slot)
-    const double* recp = GRID ? nullptr : a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
+    const double* recp = GRID ? nullptr : a.records + (a.tileRecord[tile] + splitStep(sp) * kStages + s) * REC_SIZE;
     double nrlLocal[13];
     int row = 0;
     const double* nrl = GRID ? nrlLocal : recp + REC_NRL;
@@ -1829,7 +1834,7 @@ __global__ void __launch_bounds__(64) splitDragFinishKernel(StepArgs a, ForceMod
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
-    const double* recp = a.records + (a.tileRecord[tile] + (long long)sp.step * kStages + s) * REC_SIZE;
+    const double* recp = a.records + (a.tileRecord[tile] + splitStep(sp) * kStages + s) * REC_SIZE;
     const ExactRec rec{recp};
     const double* nrl = recp + REC_NRL;
     const NrlLane nl = nrlLaneLoad(sp.lane, nPad, obj, nrl);
@@ -1898,6 +1903,8 @@ __global__ void __launch_bounds__(64) splitCombineKernel(StepArgs a, SplitArgs s
     }
     for (int off = 32; off > 0; off >>= 1) completed += __shfl_xor(completed, off);
     if (lane == 0 && a.counters && completed) atomicAdd((unsigned long long*)a.counters, completed);
+    // the next step's records (no kernel of this step reads the index any more)
+    if (sp.stepDev && blockIdx.x == 0 && threadIdx.x == 0) *sp.stepDev = *sp.stepDev + 1;
 }
 
 }  // namespace
