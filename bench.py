@@ -46,25 +46,28 @@ sys.path.insert(0, ROOT)
 
 METRIC = "object-steps/sec full-force 70x70 FP64 vs mixed; p99 request latency; mm error"
 FP64_PEAK_TFLOPS = 78.6
-# Algorithmic FLOPs per object-step of the reference algorithm at degree L
-# (FMA = 2, div/sqrt = 1): F = 13 (F_SH(L) + F_other) + F_RK with
-# F_SH(L) = 29 (L+1)(L+2)/2 + 15 (L+1) + 300, F_other = 1465 (central, 10 third
-# bodies, SRP with shadow, relativity, degree-4 and degree-6 tide gradients;
-# NRLMSISE-00 not counted), F_RK = 800.
-F_OTHER = 1465.0
-F_RK = 800.0
-
-
-# Alternate model, FLOPs per force evaluation: central 25, Sun and Moon 2 x 25
-# (direct + indirect), J2 45 (two 3x3 rotations 30 + the Wakker form 15), SRP
-# with the shadow cones 100, relativity 30, drag over the geodetic altitude 85
-# (three fixed-point steps 45, density and v_rel 40).
-F_ALT_RHS = 335.0
+# Algorithmic FLOPs per object-step of the per-object force model and the RKF78
+# step, counted by the FLOP-counting build of the oracle (the oracle's sources
+# with a counting scalar; `make -C oracle flops && python oracle/count_flops.py`):
+# + - * / and sqrt count 1, so an FMA counts 2. Per SH evaluation 29 FLOPs per
+# (n, m) term and 4 per degree, 13 evaluations per step, plus 26,757.4 per
+# object-step for the rest (central and third bodies, SRP with the shadow cones,
+# relativity, the degree-4 and degree-6 tide gradients, the drag force, the
+# step) over the 64-object sample of the headline workload: F(70) = 994,061,
+# F(120) = 2,815,686, of which 2,700,750 are the FP32 degrees 21-120 in mixed
+# precision. Not in the figure: NRLMSISE-00 (6,589 FLOPs and 422
+# transcendental calls per evaluation below the cutoff) and the epoch-only
+# inputs (evaluated once per epoch here).
+F_SH_TERM = 29.0
+F_SH_DEGREE = 4.0
+F_REST = 26757.3875
+# The alternate model (J2, Sun/Moon, SRP with shadow, relativity, exponential
+# drag over the geodetic altitude), counted the same way.
+F_ALT = 5698.1265625
 
 
 def flops_per_object_step(L):
-    f_sh = 29.0 * (L + 1) * (L + 2) / 2 + 15.0 * (L + 1) + 300.0
-    return 13.0 * (f_sh + F_OTHER) + F_RK
+    return 13.0 * (F_SH_TERM * (L + 1) * (L + 2) / 2 + F_SH_DEGREE * (L + 1)) + F_REST
 
 
 def subset(t, idx):
@@ -315,7 +318,7 @@ def leg_alternate(args, work):
     import parahpop_amd
     from parahpop_amd import synth
     duration = 86400.0
-    f = 13.0 * F_ALT_RHS + F_RK
+    f = F_ALT
     out = {"workload": "J2-only gravity, 28-band exponential drag over the geodetic altitude, Sun/Moon, SRP with "
                        "Earth shadow, relativity; seed-2 population, 60 s step, 1 d",
            "flops_per_object_step": f}

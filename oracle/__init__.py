@@ -19,7 +19,9 @@ class OrcStats(ctypes.Structure):
                 ("seconds", ctypes.c_double), ("threads", ctypes.c_int)]
 
 
+FLOPS_LIB_PATH = os.path.join(HERE, "liboracle_flops.so")
 _lib = None
+_flops_lib = None
 
 
 def library():
@@ -27,37 +29,53 @@ def library():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("liboracle.so not built: run `python oracle/build.py`")
-        lib = ctypes.CDLL(LIB_PATH)
-        vp, sz, d, i = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double, ctypes.c_int
-        lib.orc_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
-        lib.orc_propagate.argtypes = [vp, sz] + [vp] * 8 + [d, i, vp, vp, vp]
-        lib.orc_acceleration.argtypes = [vp, vp, vp, d, d, d, d, d, vp]
-        lib.orc_force_term.argtypes = [vp, i, vp, vp, d, d, d, d, d, vp]
-        lib.orc_nrlmsise.argtypes = [ctypes.c_char_p, i] + [d] * 8 + [vp, i, vp, vp]
-        lib.orc_body_position.argtypes = [i, d, vp]
-        lib.orc_earth_orientation.argtypes = [d, vp, vp]
-        lib.orc_ctx_body_position.argtypes = [vp, i, d, vp]
-        lib.orc_ctx_earth_orientation.argtypes = [vp, d, vp, vp]
-        lib.orc_geodetic_altitude.argtypes = [vp, vp, d, d]
-        lib.orc_geodetic_altitude.restype = d
-        lib.orc_exp_density.argtypes = [vp, d]
-        lib.orc_exp_density.restype = d
-        lib.orc_time_at.argtypes = [vp, d, vp]
-        lib.orc_nrl_inputs.argtypes = [vp, d, vp, vp]
-        lib.orc_solid_tide.argtypes = [vp, vp, vp, vp, vp, i, i, vp, vp]
-        lib.orc_ocean_waves.argtypes = [vp, vp, d, vp, vp]
-        lib.orc_tide_arguments.argtypes = [vp, vp, vp, vp]
-        lib.orc_third_body.argtypes = [vp, vp, d, vp]
-        lib.orc_shadow.argtypes = [vp, vp, d, d]
-        lib.orc_shadow.restype = d
-        lib.orc_sh_body_fixed.argtypes = [vp, vp, vp]
-        lib.orc_ocean_tide.argtypes = [vp, vp, vp, vp]
-        lib.orc_tide_acceleration.argtypes = [vp, vp, vp, vp, vp, i, vp]
-        lib.orc_last_error.restype = ctypes.c_char_p
-        lib.orc_destroy.argtypes = [vp]
-        lib.orc_destroy.restype = None
-        _lib = lib
+        _lib = _bind(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+def flops_library():
+    """The FLOP-counting build of the oracle (same sources, counting scalar)."""
+    global _flops_lib
+    if _flops_lib is None:
+        if not os.path.exists(FLOPS_LIB_PATH):
+            raise ImportError("liboracle_flops.so not built: run `make -C oracle flops`")
+        _flops_lib = _bind(ctypes.CDLL(FLOPS_LIB_PATH))
+    return _flops_lib
+
+
+def _bind(lib):
+    vp, sz, d, i = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_double, ctypes.c_int
+    lib.orc_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp)]
+    lib.orc_propagate.argtypes = [vp, sz] + [vp] * 8 + [d, i, vp, vp, vp]
+    lib.orc_acceleration.argtypes = [vp, vp, vp, d, d, d, d, d, vp]
+    lib.orc_force_term.argtypes = [vp, i, vp, vp, d, d, d, d, d, vp]
+    lib.orc_nrlmsise.argtypes = [ctypes.c_char_p, i] + [d] * 8 + [vp, i, vp, vp]
+    lib.orc_body_position.argtypes = [i, d, vp]
+    lib.orc_earth_orientation.argtypes = [d, vp, vp]
+    lib.orc_ctx_body_position.argtypes = [vp, i, d, vp]
+    lib.orc_ctx_earth_orientation.argtypes = [vp, d, vp, vp]
+    lib.orc_geodetic_altitude.argtypes = [vp, vp, d, d]
+    lib.orc_geodetic_altitude.restype = d
+    lib.orc_exp_density.argtypes = [vp, d]
+    lib.orc_exp_density.restype = d
+    lib.orc_time_at.argtypes = [vp, d, vp]
+    lib.orc_nrl_inputs.argtypes = [vp, d, vp, vp]
+    lib.orc_solid_tide.argtypes = [vp, vp, vp, vp, vp, i, i, vp, vp]
+    lib.orc_ocean_waves.argtypes = [vp, vp, d, vp, vp]
+    lib.orc_tide_arguments.argtypes = [vp, vp, vp, vp]
+    lib.orc_third_body.argtypes = [vp, vp, d, vp]
+    lib.orc_shadow.argtypes = [vp, vp, d, d]
+    lib.orc_shadow.restype = d
+    lib.orc_sh_body_fixed.argtypes = [vp, vp, vp]
+    lib.orc_ocean_tide.argtypes = [vp, vp, vp, vp]
+    lib.orc_tide_acceleration.argtypes = [vp, vp, vp, vp, vp, i, vp]
+    lib.orc_last_error.restype = ctypes.c_char_p
+    lib.orc_destroy.argtypes = [vp]
+    lib.orc_destroy.restype = None
+    lib.orc_flop_counts.argtypes = [vp]
+    lib.orc_flop_reset.argtypes = []
+    lib.orc_flop_reset.restype = None
+    return lib
 
 
 def _p(a):
@@ -65,8 +83,8 @@ def _p(a):
 
 
 class Oracle:
-    def __init__(self, config_path):
-        self.lib = library()
+    def __init__(self, config_path, counting=False):
+        self.lib = flops_library() if counting else library()
         h = ctypes.c_void_p()
         if self.lib.orc_create(os.fsencode(config_path), os.fsencode(DATA_DIR), ctypes.byref(h)) != 0:
             raise RuntimeError(self.lib.orc_last_error().decode())
@@ -90,6 +108,21 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(self.lib.orc_last_error().decode())
         return oe, os_, {k: getattr(st, k) for k, _ in OrcStats._fields_}
+
+    def flop_counts(self):
+        """Counts of the calling thread since flop_reset() (counting build only):
+        arithmetic FLOPs and transcendental calls of the per-object force model
+        and step, of the epoch-only inputs and of the NRLMSISE-00 density, and
+        the entries into each category's scopes (NRLMSISE-00: evaluations)."""
+        out = np.zeros(9)
+        if self.lib.orc_flop_counts(_p(out)) != 0:
+            raise RuntimeError("not the FLOP-counting build of the oracle")
+        keys = ("per_object", "epoch", "nrlmsise")
+        return {"flops": dict(zip(keys, out[:3])), "transcendental": dict(zip(keys, out[3:6])),
+                "scope_entries": dict(zip(keys, out[6:]))}
+
+    def flop_reset(self):
+        self.lib.orc_flop_reset()
 
     def force(self, r, v, et, mass, area_km2, cr_eff, cd, which=-1):
         out = np.zeros(3)

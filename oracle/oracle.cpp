@@ -31,6 +31,11 @@
 
 using namespace oracle;
 
+#ifdef ORACLE_FLOP_COUNT
+thread_local oracle_flop::Counts oracle_flop::counts{};
+thread_local int oracle_flop::category = oracle_flop::kPerObject;
+#endif
+
 struct orc_ctx {
     phx::RunConfig run;
     phx::Environment env;
@@ -48,6 +53,7 @@ struct orc_ctx {
 // ICRF -> ITRF rotation and Earth rate: the configured IERS2000 unit, or the
 // analytic model (with the EOP table when loaded).
 Orientation contextOrientation(const orc_ctx& c, double et) {
+    ORACLE_FLOP_SCOPE(kEpoch);
     if (!c.env.brot.empty()) {
         const phx::BrotIers& u = c.env.brot.iers[0];
         return iers2000Orientation(c.env.brot.data.data() + u.offset, u.nNut, u.nErp, et);
@@ -58,6 +64,7 @@ Orientation contextOrientation(const orc_ctx& c, double et) {
 // Geocentric position (km, ICRF) of a body: the configured Chebyshev
 // ephemeris (brie.h), or the analytic series.
 V3 bodyPosition(const orc_ctx& c, int naif, double et) {
+    ORACLE_FLOP_SCOPE(kEpoch);
     if (c.ephChains.empty()) return bodyGeocentric(naif, et);
     auto it = c.ephChains.find(naif);
     if (it == c.ephChains.end()) throw std::runtime_error("no ephemeris chain for body " + std::to_string(naif));
@@ -401,7 +408,8 @@ int orc_propagate(orc_ctx* ctx, size_t n, const int64_t* id, const double* epoch
             }
             objectSteps += double(done);
         }
-        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        // decltype(1.0): the builtin double, also in the FLOP-counting build
+        const double secs = std::chrono::duration<decltype(1.0)>(std::chrono::steady_clock::now() - t0).count();
         for (size_t i = 0; i < n; ++i) {
             if (!eos[i]) throw std::runtime_error("A target stopped without reaching the requested end of simulation.");
             out_epoch[i] = spiceToMjd(cur[i]);
@@ -481,6 +489,27 @@ int orc_ctx_earth_orientation(const orc_ctx* ctx, double et, double* R9, double*
 // Test hooks: single functions of the restatement, for the pins in
 // tests/test_oracle_pins.py.
 // ---------------------------------------------------------------------------
+
+int orc_flop_counts(double* out9) {
+#ifdef ORACLE_FLOP_COUNT
+    for (int c = 0; c < oracle_flop::kCategories; ++c) {
+        out9[c] = f64(oracle_flop::counts.flops[c]);
+        out9[oracle_flop::kCategories + c] = f64(oracle_flop::counts.trans[c]);
+        out9[2 * oracle_flop::kCategories + c] = f64(oracle_flop::counts.calls[c]);
+    }
+    return 0;
+#else
+    (void)out9;
+    return -1;
+#endif
+}
+
+void orc_flop_reset(void) {
+#ifdef ORACLE_FLOP_COUNT
+    oracle_flop::counts = oracle_flop::Counts{};
+    oracle_flop::category = oracle_flop::kPerObject;
+#endif
+}
 
 int orc_time_at(const orc_ctx* ctx, double et, double* out6) {
     const TimeData t = timeAt(ctx->eop, et);

@@ -86,6 +86,13 @@ int orc_tide_acceleration(const orc_ctx* ctx, const double* r3, const double* E9
                           int max_degree, double* out3);
 double orc_step_seconds(const orc_ctx* ctx);
 double orc_duration_seconds(const orc_ctx* ctx);
+/* FLOP counts of the calling thread since orc_flop_reset (liboracle_flops.so,
+ * the build with flop_count.h; returns -1 in the ordinary build): out9 =
+ * arithmetic FLOPs of the per-object force model and step, of the epoch-only
+ * inputs and of the NRLMSISE-00 density, then the transcendental calls of the
+ * same three categories, then the entries into their scopes. */
+int orc_flop_counts(double* out9);
+void orc_flop_reset(void);
 const char* orc_last_error(void);
 void orc_destroy(orc_ctx* ctx);
 

@@ -12,6 +12,8 @@
 // published input definitions (tests/test_oracle_pins_env.py).
 #pragma once
 
+#include "flop_scope.h"
+
 #include <cmath>
 #include <fstream>
 #include <map>
@@ -844,6 +846,7 @@ inline double taiMinusUtc(double mjd) {
 
 // :103-111
 inline double etToMjdUtc(double et) {
+    ORACLE_FLOP_SCOPE(kEpoch);
     const double days = et / kSecondsPerDay;
     const double g = 6.240075674 + 0.01720197034 * days;
     const double tdbMinusTt = 0.001657 * std::sin(g) + 0.000022 * std::sin(2.0 * g);
@@ -854,6 +857,7 @@ inline double etToMjdUtc(double et) {
 
 // :113-131
 inline void calendarDate(double mjd, int& year, int& month, int& day, int& hour, int& minute, double& second) {
+    ORACLE_FLOP_SCOPE(kEpoch);
     const long a = static_cast<long>(std::floor(mjd + 2400001.0));
     long b = 0, c = 0;
     if (a < 2299161) c = a + 1524;
@@ -891,6 +895,7 @@ inline double wrapHours(double hours) {
 
 // :156-187
 inline double hpopGastHours(double mjdUtc) {
+    ORACLE_FLOP_SCOPE(kEpoch);
     const double mjdUt1 = mjdUtc;
     const double mjd0 = std::floor(mjdUt1);
     const double ut1 = kSecondsPerDay * (mjdUt1 - mjd0);
@@ -918,6 +923,7 @@ inline double weatherApAtLag(const NrlWeather& w, int record, int slot, int lag)
 
 // :215-241 (adjusted flux, the reference default)
 inline bool fillWeather(const NrlWeather& w, double mjdUtc, double& f107, double& f107a, double& avgAp, double ap[7]) {
+    ORACLE_FLOP_SCOPE(kEpoch);
     const int record = static_cast<int>(std::floor(mjdUtc - w.startMjdUtc));
     if (record < 3 || record >= w.days) return false;
     int slot = static_cast<int>(std::floor((mjdUtc - std::floor(mjdUtc)) * 8.0));
@@ -970,6 +976,7 @@ inline bool geodeticFixed(double x, double y, double z, double radius, double fl
 // Nrlmsise00.cu:270-327 — density in kg/km^3 (0 above hCutoff, NaN on invalid data).
 inline double nrlmsiseDensity(const NrlmsiseTables& T, const NrlWeather& w, double hCutoff, double densityScale,
                               const V3& fixed, double et, double Req, double flattening) {
+    ORACLE_FLOP_SCOPE(kNrlmsise);
     using namespace nrl;
     const double mjdUtc = etToMjdUtc(et);
     double lat = 0, lon = 0, alt = 0;

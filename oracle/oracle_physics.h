@@ -29,6 +29,8 @@
 // of the device math library against glibc.
 #pragma once
 
+#include "flop_scope.h"
+
 #include <cmath>
 #include <algorithm>
 #include <cstdint>
@@ -139,6 +141,7 @@ inline TimeData timeAtMjdUtc(const Eop& e, double mjdUtc) {
 
 // EarthCorrections.h:87-105
 inline TimeData timeAt(const Eop& e, double epochEt) {
+    ORACLE_FLOP_SCOPE(kEpoch);
     if (e.days < 2 || !std::isfinite(epochEt)) return invalidTime();
     double ttEt = epochEt;
     for (int j = 0; j < 2; ++j) ttEt = epochEt - tdbMinusTt(51544.5 + ttEt / 86400);
@@ -717,6 +720,7 @@ inline void clearCS(CS7& o) { for (int n = 0; n < 7; ++n) for (int m = 0; m < 7;
 
 // HpopCorrections.h:148-169,201-264,326-329
 inline void solidCoefficients(const TideInput& in, const TideConstants& k, const TideTables& tt, CS7& out) {
+    ORACLE_FLOP_SCOPE(kEpoch);
     double moon[3], sun[3];
     rot(in.E, in.moon, moon);
     rot(in.E, in.sun, sun);
@@ -811,6 +815,7 @@ inline void oceanWaves(const double f[5], double gmst, const TideTables& tt, CS7
 
 // HpopCorrections.h:298-322
 inline void oceanCoefficients(const TideInput& in, const TideTables& tt, CS7& out) {
+    ORACLE_FLOP_SCOPE(kEpoch);
     clearCS(out);
     double f[5], gmst;
     tideArguments(in, f, gmst);
