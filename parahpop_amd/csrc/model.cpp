@@ -270,12 +270,14 @@ void parseConfig(const json::Value& root, const std::string& configDir,
     if (!(run.stepSeconds > 0) || !std::isfinite(run.stepSeconds))
         throw std::runtime_error("initialStepSize must be finite and positive.");
     if (const auto* p = ex.find("precision")) {
-        checkKeys(*p, {"mode", "fp64Degree"}, "precision");
+        checkKeys(*p, {"mode", "fp64Degree", "fp64Order"}, "precision");
         const std::string mode = lower(p->at("mode").asString());
         if (mode == "mixed") run.precision.mixed = true;
         else if (mode != "fp64") throw std::runtime_error("precision.mode must be \"fp64\" or \"mixed\".");
         if (p->has("fp64Degree")) run.precision.fp64Degree = int(p->at("fp64Degree").asInt());
         if (run.precision.fp64Degree < 1) throw std::runtime_error("precision.fp64Degree must be >= 1.");
+        if (p->has("fp64Order")) run.precision.fp64Order = int(p->at("fp64Order").asInt());
+        if (run.precision.fp64Order < 0) throw std::runtime_error("precision.fp64Order must be >= 0.");
     }
 
     const json::Value& env = root.at("model").at("environment");

@@ -98,8 +98,9 @@ struct BodySpec {
 };
 
 struct Precision {
-    bool mixed = false;   // FP32 terms for degrees above fp64Degree
+    bool mixed = false;   // FP32 terms for degrees above fp64Degree in orders above fp64Order
     int fp64Degree = 20;
+    int fp64Order = 3;
 };
 
 struct ModelConfig {

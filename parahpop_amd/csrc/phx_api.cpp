@@ -749,6 +749,7 @@ StepArgs contextStepArgs(const phx_ctx& c, int& W) {
     a.shOrderStart = c.shOrderStart.p;
     a.shDiag = c.shDiag.p;
     a.fp64Degree = c.run.precision.mixed ? c.run.precision.fp64Degree : -1;
+    a.fp64Order = c.run.precision.fp64Order;
     a.nrlCoeff = c.nrl.p;
     a.weather = c.weather.p;
     a.wDays = c.env.hasWeather ? c.env.weather.days : 0;

@@ -120,6 +120,7 @@ struct StepArgs {
     const double* shDiag;        // [L+1] sectoral factors c_m, then [L+1] products c_1 ... c_m
     int blockStart[kMaxWaves + 1];  // order blocks per wave: [blockStart[w], blockStart[w+1])
     int fp64Degree;              // mixed precision: degrees above this use FP32 terms (-1: all FP64)
+    int fp64Order;               // mixed precision: orders up to this stay FP64 at every degree
     const double* nrlCoeff;      // NRLMSISE-00 coefficient blocks (device)
     long long* counters;         // [4] diagnostics: object-steps completed
     // Uneven epochs: each slot reads its own segment of grid records, rows

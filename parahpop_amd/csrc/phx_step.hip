@@ -189,13 +189,17 @@ __device__ __attribute__((always_inline)) inline void shRun0(CP t, int count, T 
 // spherical components (radial, latitude, longitude) of sum_nm alpha^n (...).
 // All control flow over (n, m) is uniform across the wave; only the literal
 // pole (x = y = 0 in the body frame) takes a divergent fix-up after the loop.
-// Mixed precision (fp64Degree >= 0): degrees n <= fp64Degree run in FP64; the
-// terms of higher degree run in FP32 (recursion and per-order partial sums, FP32
-// coefficient stream), and each order's FP32 partial sums are accumulated into
-// the FP64 sums; the diagonal seeds and the longitude phasor stay FP64.
+// Mixed precision (fp64Degree >= 0): the orders m <= fp64Order and the degrees
+// n <= fp64Degree run in FP64; the other terms run in FP32 (recursion and
+// per-order partial sums, FP32 coefficient stream), and each order's FP32
+// partial sums are accumulated into the FP64 sums; the diagonal seeds and the
+// longitude phasor stay FP64. The low orders stay FP64 because their latitude
+// and longitude sums are divided by cos(phi) after a cancellation of terms
+// about n times larger than the result near the poles, which an FP32
+// recursion does not survive.
 __device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const double* termsG, const float* terms32G,
-                             int fp64Degree, const int* orderStartG, const double* diagG, bool normalized, double& outR,
-                             double& outPhi, double& outLam) {
+                             int fp64Degree, int fp64Order, const int* orderStartG, const double* diagG, bool normalized,
+                             double& outR, double& outPhi, double& outLam) {
     ConstD terms = (ConstD)termsG;
     ConstF terms32 = (ConstF)terms32G;
     ConstI orderStart = (ConstI)orderStartG;
@@ -205,10 +209,12 @@ __device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const doubl
     L = __builtin_amdgcn_readfirstlane(L);
     // first degree evaluated in FP32 (L + 1: none)
     const int nSplit = __builtin_amdgcn_readfirstlane(fp64Degree < 0 ? L + 1 : min(fp64Degree + 1, L + 1));
+    // orders [m0, mFlat) run entirely in FP64 in the flat loop below
+    const int mFlat = __builtin_amdgcn_readfirstlane(nSplit > L ? m1 : min(m1, fp64Order + 1));
     const double u = g.alpha * g.sinPhi;   // alpha sin(phi)
     const double v = g.alpha * g.cosPhi;   // alpha cos(phi)
     const double w = g.alpha * g.alpha;
-    const float uf = float(u), vf = float(v), wf = float(w);
+    const float uf = float(u), wf = float(w);
     double ar = 0.0, aphiDirect = 0.0, aphiNum = 0.0, alamNum = 0.0;
     // Seed P~_{m0,m0} = (prod_{k <= m0} c_k) (alpha cos phi)^m0 and
     // cos/sin(m0 lambda): the power and the phasor by binary powering.
@@ -231,7 +237,7 @@ __device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const doubl
         }
         Pmm = diag[L + 1 + m0] * vp;
     }
-    if (nSplit > L && m0 < m1) {
+    if (m0 < mFlat) {
         // FP64 stream: order 0 by its own recursion, orders >= 1 as one flat loop
         // over four-term batches (each order's run is padded with zero terms to a
         // multiple of four), so an order change costs no extra round trip for its
@@ -251,7 +257,7 @@ __device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const doubl
             cosML = c;
             m = 1;
         }
-        if (m < m1) {
+        if (m < mFlat) {
             ConstD t = terms + SHT_STRIDE * orderStart[m];
             int left = (L - m + 4) >> 2;
             // the next diagonal factor is prefetched an order ahead
@@ -281,13 +287,14 @@ __device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const doubl
                     ar -= X1;
                     aphiNum += g.alpha * XE - g.sinPhi * (X1 - X5);
                     alamNum -= double(m) * X6;
-                    if (m + 1 >= m1) break;
+                    // next diagonal and phasor (also after the last order: the
+                    // mixed orders below start from them)
                     if (m + 1 <= L) Pmm = dNext * v * Pmm;
                     const double s = g.cosL * sinML + g.sinL * cosML;
                     const double c = g.cosL * cosML - g.sinL * sinML;
                     sinML = s;
                     cosML = c;
-                    ++m;
+                    if (++m >= mFlat) break;
                     dNext = diag[min(m + 1, L)];
                     left = (L - m + 4) >> 2;
                     P0 = Pmm;
@@ -297,24 +304,13 @@ __device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const doubl
             }
         }
     }
-    for (int m = (nSplit > L ? m1 : m0); m < m1; ++m) {
+    // mixed orders (m >= 1: order 0 is always in the FP64 orders)
+    for (int m = max(m0, mFlat); m < m1; ++m) {
         const int first = orderStart[m];
         const int count = L - m + 1;
         const int c64 = max(0, min(count, nSplit - m));   // degrees m .. m + c64 - 1 in FP64
         const int c32 = count - c64;
-        if (m == 0) {
-            double P0 = Pmm, Pm1 = 0.0, dP0 = 0.0, dPm1 = 0.0;
-            double X1 = 0.0, D1 = 0.0;
-            shRun0<double>(terms + SHT_STRIDE * first, c64, u, v, w, P0, Pm1, dP0, dPm1, X1, D1);
-            if (c32 > 0) {
-                float p0 = float(P0), pm1 = float(Pm1), dp0 = float(dP0), dpm1 = float(dPm1), x1 = 0.0f, d1 = 0.0f;
-                shRun0<float>(terms32 + SHT_STRIDE * (first + c64), c32, uf, vf, wf, p0, pm1, dp0, dpm1, x1, d1);
-                X1 += double(x1);
-                D1 += double(d1);
-            }
-            ar -= X1;
-            aphiDirect += D1;
-        } else {
+        {
             double P0 = Pmm, Pm1 = 0.0;
             double A1 = 0.0, A2 = 0.0, A5 = 0.0, A6 = 0.0, B1 = 0.0, B2 = 0.0;
             shRun<double>(terms + SHT_STRIDE * first, c64, u, w, P0, Pm1, A1, A2, A5, A6, B1, B2);
@@ -957,7 +953,7 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
         double sr = 0.0, sp = 0.0, sl = 0.0;
         g = shGeometry(rec, px, py, pz, fm.shRadius);
         if (m0 < m1)
-            shOrderBlock(g, m0, m1, fm.shDegree, a.shTerms, a.shTerms32, a.fp64Degree, a.shOrderStart, a.shDiag, fm.shNormalized, sr,
+            shOrderBlock(g, m0, m1, fm.shDegree, a.shTerms, a.shTerms32, a.fp64Degree, a.fp64Order, a.shOrderStart, a.shDiag, fm.shNormalized, sr,
                          sp, sl);
         part[wave][0][lane] = sr;
         part[wave][1][lane] = sp;
@@ -1116,7 +1112,7 @@ __device__ inline void stageForcesParts(const StepArgs& a, const ForceModel& fm,
         double sr = 0.0, sp = 0.0, sl = 0.0;
         g = shGeometry(rec, px, py, pz, fm.shRadius);
         if (m0 < m1)
-            shOrderBlock(g, m0, m1, fm.shDegree, a.shTerms, a.shTerms32, a.fp64Degree, a.shOrderStart, a.shDiag, fm.shNormalized, sr,
+            shOrderBlock(g, m0, m1, fm.shDegree, a.shTerms, a.shTerms32, a.fp64Degree, a.fp64Order, a.shOrderStart, a.shDiag, fm.shNormalized, sr,
                          sp, sl);
         part[wave][0][lane] = sr;
         part[wave][1][lane] = sp;
@@ -1934,7 +1930,7 @@ __global__ void __launch_bounds__(64) forceProbeKernel(int n, const double* pos,
     if ((which < 0 || which == 1) && fm.hasSh) {
         const ShGeom g = shGeometry(rec, px, py, pz, fm.shRadius);
         double r, p, l;
-        shOrderBlock(g, 0, fm.shOrder + 1, fm.shDegree, shTerms, nullptr, -1, shOrderStart, shDiag, fm.shNormalized, r, p, l);
+        shOrderBlock(g, 0, fm.shOrder + 1, fm.shDegree, shTerms, nullptr, -1, 0, shOrderStart, shDiag, fm.shNormalized, r, p, l);
         shToInertial(rec, g, r, p, l, fm.shGm, bx, by, bz);
         ax += bx, ay += by, az += bz;
     } else if ((which < 0 || which == 1) && fm.hasJ2) {

@@ -115,6 +115,14 @@ def test_execution_rejections(tmp_path):
     json.dump(c, open(p, "w"))
     rc, msg = _create_error(p)
     assert rc == -1 and "maxNumSteps and maxThreads" in msg
+    for prec, expect in [({"mode": "half"}, "precision.mode"), ({"mode": "mixed", "fp64Degree": 0}, "fp64Degree must be"),
+                         ({"mode": "mixed", "fp64Order": -1}, "fp64Order must be"),
+                         ({"mode": "mixed", "fp32Degree": 30}, "fp32Degree")]:
+        c = json.loads(json.dumps(cfg))
+        c["execution"]["precision"] = prec
+        json.dump(c, open(p, "w"))
+        rc, msg = _create_error(p)
+        assert rc == -1 and expect in msg, msg
 
 
 def test_cli_contract(tmp_path):
