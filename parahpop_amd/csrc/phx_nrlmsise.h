@@ -922,10 +922,11 @@ __device__ inline double densuUpperFrom(const DensuShared& h, double zg2, double
     return dlb * exp(((1.0 + alpha + gamma)) * lnT) * expl;
 }
 
-__device__ inline void densuShare(DensuShared& h, State& st, double alt, double tinf, double tlb, double s2, double zlb,
-                                  double xmm, const double* zn1, double* tn1, double* tgn1) {
-    constexpr int MN = 5;
-    const double rgas = 831.4;
+// densuShare in three parts that depend on each other only through the common
+// fields, so two waves can form the lane part and the lower-thermosphere part
+// at once; together they perform densu's statements once each.
+__device__ inline void densuShareCommon(DensuShared& h, const State& st, double alt, double tinf, double tlb, double s2,
+                                        double zlb, double xmm, const double* zn1) {
     h.re = st.re;
     h.za = zn1[0];
     h.zlb = zlb;
@@ -936,15 +937,32 @@ __device__ inline void densuShare(DensuShared& h, State& st, double alt, double 
     h.glb = st.gsurf / pow((1.0 + zlb / st.re), 2.0);
     h.zLane = alt;
     h.laneUpper = !(alt < h.za);
-    const double zL = alt > h.za ? alt : h.za;
-    h.zg2L = zeta(st, zL, zlb);
-    h.ttL = tinf - (tinf - tlb) * exp(-s2 * h.zg2L);
-    h.lnL = log(tlb / h.ttL);
-    // below za: z = za
-    h.zg2A = zeta(st, h.za, zlb);
-    h.ttA = tinf - (tinf - tlb) * exp(-s2 * h.zg2A);
-    h.lnA = log(tlb / h.ttA);
-    const double dta = (tinf - h.ttA) * s2 * pow(((st.re + zlb) / (st.re + h.za)), 2.0);
+}
+
+// The lane altitude: zeta, the Bates temperature and its logarithm, and the
+// mixing-density factor (xm = xmm, alpha = 0).
+__device__ inline void densuShareLane(DensuShared& h, const State& st) {
+    const double rgas = 831.4;
+    const double zL = h.zLane > h.za ? h.zLane : h.za;
+    h.zg2L = zeta(st, zL, h.zlb);
+    h.ttL = h.tinf - (h.tinf - h.tlb) * exp(-h.s2 * h.zg2L);
+    h.lnL = log(h.tlb / h.ttL);
+    const double gamma = h.xmm * h.glb / (h.s2 * rgas * h.tinf);
+    double expl = exp(-h.s2 * gamma * h.zg2L);
+    if (expl > 50.0) expl = 50.0;
+    if (h.ttL <= 0) expl = 50.0;
+    h.pmm = exp(((1.0 + 0. + gamma)) * h.lnL);
+    h.emm = expl;
+}
+
+// Below za: the temperature at za, the lower-thermosphere node update (tn1[0],
+// tgn1[0]) and the five-node spline of 1/T.
+__device__ inline void densuShareLower(DensuShared& h, const State& st, const double* zn1, double* tn1, double* tgn1) {
+    constexpr int MN = 5;
+    h.zg2A = zeta(st, h.za, h.zlb);
+    h.ttA = h.tinf - (h.tinf - h.tlb) * exp(-h.s2 * h.zg2A);
+    h.lnA = log(h.tlb / h.ttA);
+    const double dta = (h.tinf - h.ttA) * h.s2 * pow(((st.re + h.zlb) / (st.re + h.za)), 2.0);
     tgn1[0] = dta;
     tn1[0] = h.ttA;
     h.z1 = zn1[0];
@@ -961,13 +979,13 @@ __device__ inline void densuShare(DensuShared& h, State& st, double alt, double 
     const double yd2 = -tgn1[1] / (t2 * t2) * h.zgdif * pow(((st.re + h.z2) / (st.re + h.z1)), 2.0);
     splineN(h.xs, h.ys, yd1, yd2, h.y2out);
     h.glb1 = st.gsurf / pow((1.0 + h.z1 / st.re), 2.0);
-    // mixing-density factor at the lane altitude
-    const double gamma = xmm * h.glb / (s2 * rgas * tinf);
-    double expl = exp(-s2 * gamma * h.zg2L);
-    if (expl > 50.0) expl = 50.0;
-    if (h.ttL <= 0) expl = 50.0;
-    h.pmm = exp(((1.0 + 0. + gamma)) * h.lnL);
-    h.emm = expl;
+}
+
+__device__ inline void densuShare(DensuShared& h, State& st, double alt, double tinf, double tlb, double s2, double zlb,
+                                  double xmm, const double* zn1, double* tn1, double* tgn1) {
+    densuShareCommon(h, st, alt, tinf, tlb, s2, zlb, xmm, zn1);
+    densuShareLane(h, st);
+    densuShareLower(h, st, zn1, tn1, tgn1);
     h.db28 = h.b28 = 0.0;
 }
 
@@ -1014,14 +1032,28 @@ __device__ inline double densuAt(const DensuShared& h, State& st, double alt, do
 // lane's DensuShared (gts7SpeciesShare). Densu's side effects
 // (lower-thermosphere nodes, tz) depend only on (tinf, tlb, s), so the shared
 // quantities may be evaluated before any species.
-template <class Globe>
+// The lane part of the shared quantities alone (densuShareLane's fields and the
+// common ones), for a wave beside the one that runs gts7SpeciesShare<false>.
+__device__ inline void gts7ShareLane(DensuShared& h, const State& st, const NrlTables& T, const Input& in, const Pre& pre) {
+    densuShareCommon(h, st, in.alt, pre.tinf, pre.tlb, pre.s, ((NrlCD)(T.base + OFF_PTM))[5], pre.xmm, pre.zn1);
+    densuShareLane(h, st);
+}
+
+// LANE = false leaves densuShareLane's fields to another wave (gts7ShareLane).
+template <class Globe, bool LANE = true>
 __device__ __attribute__((always_inline)) inline void gts7SpeciesShare(DensuShared& h, bool needDb28, State& st,
                                                                        const NrlTables& T, Input& in, const Flags& fl,
                                                                        const Globe& G, const Pre& pre) {
     double zn1[5] = {pre.zn1[0], pre.zn1[1], pre.zn1[2], pre.zn1[3], pre.zn1[4]};
     const NrlCD pdm = (NrlCD)(T.base + OFF_PDM);
     const double z = in.alt;
-    densuShare(h, st, z, pre.tinf, pre.tlb, pre.s, ((NrlCD)(T.base + OFF_PTM))[5], pre.xmm, zn1, st.meso_tn1, st.meso_tgn1);
+    if constexpr (LANE) {
+        densuShare(h, st, z, pre.tinf, pre.tlb, pre.s, ((NrlCD)(T.base + OFF_PTM))[5], pre.xmm, zn1, st.meso_tn1, st.meso_tgn1);
+    } else {
+        densuShareCommon(h, st, z, pre.tinf, pre.tlb, pre.s, ((NrlCD)(T.base + OFF_PTM))[5], pre.xmm, zn1);
+        densuShareLower(h, st, zn1, st.meso_tn1, st.meso_tgn1);
+        h.db28 = h.b28 = 0.0;
+    }
     const bool wantB28 = !(z > 450.0 && z >= zn1[0]);
     if (wantB28 || needDb28) {
         const double g28 = fl.sw[21] * G(st, 4, (T.base + OFF_PD + 150 * (2)), in, fl);

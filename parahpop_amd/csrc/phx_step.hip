@@ -1659,6 +1659,8 @@ __device__ __attribute__((always_inline)) inline void densuSharedFields(H& h, F&
 }
 constexpr int kDensuSharedFields = 40;
 static_assert(sizeof(nrlmsise::DensuShared) == kDensuSharedFields * sizeof(double), "every densu field is exchanged");
+// Fields of densuShareLane (zg2L, ttL, lnL, pmm, emm) in densuSharedFields order.
+__device__ constexpr bool densuLaneField(int i) { return i == 9 || i == 10 || i == 11 || i == 36 || i == 37; }
 
 // Globe policy serving precomputed values: globe7 of table j from row j
 // (with the Ap function of pt, ps and pd[3] in rows 11-13 for the apt[0] that
@@ -1683,11 +1685,18 @@ struct SharedGlobe {
     }
 };
 
+// Species of each wave of the drag tile kernel: N2 on wave 0, which forms the
+// lower-thermosphere part of the shared densu quantities while wave 4 forms
+// their lane part; the four species whose mixing corrections are longest
+// (O, H, N, O2) on four different SIMDs (waves w and w + 4 share a SIMD);
+// anomalous O (formed before the hand-over) beside O2.
+__constant__ int kTileSpecies[8] = {0, 1, 4, 3, 2, 5, 6, 7};
+
 // Drag of one tile and stage in one eight-wave workgroup. The waves first
 // evaluate the eleven globe7 tables the tile's lanes need (two per wave at
 // most) and, when a lane is below 300 km, the five glob7s node tables; from
 // those values each wave forms gts7's prefix and the shared densu quantities
-// and one species (kSpeciesOrder[wave]); wave 0 then assembles the density
+// and one species (kTileSpecies[wave]); wave 0 then assembles the density
 // and the drag, k_s = ax + (extra + drag), as the fused kernel. Every value is
 // the one the serial evaluation forms.
 template <bool GRID>
@@ -1722,10 +1731,14 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
         } else {
             F = nrlmsise::FastGlobe{nrl + (REC_NRL_GLOBE - REC_NRL), kGlobeTrig};
         }
-        const int k = kSpeciesOrder[wave];
+        const int k = kTileSpecies[wave];
+        // globe7 table of this wave's species
+        const int jk = k == 7 ? 10 : nrlmsise::kSpeciesPar[k].globe;
         {
-            // globe7 of the prefix tables pt, ps, pd[3] and of pd[2] (N2), one per
-            // wave; with a lane below 300 km, then the five glob7s node tables
+            // globe7 of the prefix tables pt, ps, pd[3] and of pd[2] (N2) on waves
+            // 0-3 and of the species tables of waves 4-7 (O, H, N, anomalous O),
+            // one per wave; with a lane below 300 km, then the five glob7s node
+            // tables on waves 3-7
             nrlmsise::State st;
             F.basis(st, in0, fl);
             if (wave < 4) {
@@ -1733,6 +1746,9 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
                 double apt = 0.0;
                 gv[j][lane] = F.value(st, j, nrlmsise::globeTable(T, j), in0, apt);
                 if (wave < 3) gv[11 + wave][lane] = apt;
+            } else {
+                double apt = 0.0;
+                gv[jk][lane] = F.value(st, jk, nrlmsise::globeTable(T, jk), in0, apt);
             }
             if (__syncthreads_or(in0.alt < 300.0)) {
                 if (wave >= 3) {
@@ -1756,27 +1772,35 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
         nrlmsise::Pre pre;
         nrlmsise::gts7Prefix(st, T, in, fl, G, pre);
         nrlmsise::DensuShared h;
-        if (wave == 0) {
-            // the densu quantities every species shares, formed once (with the N2
-            // turbopause density) and handed to the other waves; then N2
-            nrlmsise::gts7SpeciesShare(h, true, st, T, in, fl, G, pre);
+        if (wave == 0 || wave == 5) {
+            // the densu quantities every species shares, formed once and handed
+            // to every wave: wave 0 the lower-thermosphere part (with the N2
+            // turbopause density), wave 4 the lane part, at once
+            const bool lanePart = wave == 5;
+            if (lanePart) {
+                nrlmsise::gts7ShareLane(h, st, T, in, pre);
+            } else {
+                nrlmsise::gts7SpeciesShare<SharedGlobe, false>(h, true, st, T, in, fl, G, pre);
+                hx[kDensuSharedFields][lane] = st.meso_tn1[0];
+                hx[kDensuSharedFields + 1][lane] = st.meso_tgn1[0];
+            }
             int i = 0;
-            densuSharedFields(h, [&](auto& x) { hx[i++][lane] = double(x); });
-            hx[kDensuSharedFields][lane] = st.meso_tn1[0];
-            hx[kDensuSharedFields + 1][lane] = st.meso_tgn1[0];
-            spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
-        } else {
-            // meanwhile each other wave evaluates its species' globe7 table; the
-            // anomalous oxygen (its own densu profile) needs nothing else
+            densuSharedFields(h, [&](auto& x) {
+                if (densuLaneField(i) == lanePart) hx[i][lane] = double(x);
+                ++i;
+            });
+        } else if (wave < 4) {
+            // meanwhile waves 1-3 evaluate their species' globe7 tables
             nrlmsise::State sb;
             F.basis(sb, in, fl);
-            const int j = k == 7 ? 10 : nrlmsise::kSpeciesPar[k].globe;
             double apt = 0.0;
-            gv[j][lane] = F.value(sb, j, nrlmsise::globeTable(T, j), in, apt);
-            if (k == 7) spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
+            gv[jk][lane] = F.value(sb, jk, nrlmsise::globeTable(T, jk), in, apt);
+        } else if (k == 7) {
+            // the anomalous oxygen (its own densu profile) needs nothing else
+            spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
         }
         __syncthreads();
-        if (wave != 0 && k != 7) {
+        if (k != 7) {
             int i = 0;
             densuSharedFields(h, [&](auto& x) {
                 using T = typename std::remove_reference<decltype(x)>::type;
