@@ -87,7 +87,7 @@ int phx_batch_create_uneven(phx_ctx* ctx, size_t n, const int64_t* id, const dou
  * with the final epochs and states; d_mass, d_drag_area (m^2), d_srp_area (m^2),
  * d_cd, d_cr (n each) are read. All are device pointers, written by the caller
  * before the call (synchronize the caller's stream first). Every working array
- * lives in one fixed device array of the library (20.5 MB per GPU, held
+ * lives in one fixed device array of the library (21.1 MB per GPU, held
  * until the process ends): at most 16,384 objects and an epoch span of 2,043
  * grid steps (34 h at the 60 s grid). Runs the per-lane epoch path: the
  * results equal phx_propagate's with the option force_grid = 1. Synchronous on

@@ -54,6 +54,7 @@ def _bind(lib):
     lib.orc_earth_orientation.argtypes = [d, vp, vp]
     lib.orc_ctx_body_position.argtypes = [vp, i, d, vp]
     lib.orc_ctx_earth_orientation.argtypes = [vp, d, vp, vp]
+    lib.orc_body_axes.argtypes = [vp, i, d, vp]
     lib.orc_geodetic_altitude.argtypes = [vp, vp, d, d]
     lib.orc_geodetic_altitude.restype = d
     lib.orc_exp_density.argtypes = [vp, d]
@@ -144,6 +145,14 @@ class Oracle:
         R, w = np.zeros(9), np.zeros(3)
         self.lib.orc_ctx_earth_orientation(self.h, float(et), _p(R), _p(w))
         return R.reshape(3, 3), w
+
+    def body_axes(self, naif, et):
+        """ICRF -> axes of the configured field of body `naif`; (R, rotation target)."""
+        R = np.zeros(9)
+        target = self.lib.orc_body_axes(self.h, int(naif), float(et), _p(R))
+        if target < 0:
+            raise RuntimeError(self.lib.orc_last_error().decode())
+        return R.reshape(3, 3), target
 
     def exp_density(self, h):
         return self.lib.orc_exp_density(self.h, float(h))

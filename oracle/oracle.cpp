@@ -48,7 +48,26 @@ struct orc_ctx {
     NrlWeather weather;
     bool anyOcculter = false;
     std::map<int, std::vector<phx::BrieStep>> ephChains;   // Chebyshev ephemeris chains to the Earth
+    std::vector<ShTables> rotSh;   // fields of the bodies other than the Earth (env.rot order)
 };
+
+// ICRF -> axes of the k-th body field other than the Earth's.
+M3 rotBodyAxes(const orc_ctx& c, size_t k, double et) {
+    ORACLE_FLOP_SCOPE(kEpoch);
+    const phx::RotBody& r = c.env.rot[k];
+    const phx::BrotBody& u = c.env.brot.bodies[size_t(r.unit)];
+    const double* d = c.env.brot.data.data() + u.offset;
+    if (u.type == 3) return binaryFrameAxes(d, u.p[2], u.p[3], u.p[0], et);
+    const int deg[3] = {u.p[0], u.p[1], u.p[2]}, np[3] = {u.p[5], u.p[6], u.p[7]};
+    const double* ang = nullptr;
+    int angDeg = 0;
+    if (r.bary >= 0) {
+        const phx::BrotBody& b = c.env.brot.bodies[size_t(r.bary)];
+        ang = c.env.brot.data.data() + b.offset;
+        angDeg = b.p[0];
+    }
+    return iauBodyAxes(d, deg, np, ang, angDeg, et);
+}
 
 // ICRF -> ITRF rotation and Earth rate: the configured IERS2000 unit, or the
 // analytic model (with the EOP table when loaded).
@@ -132,6 +151,11 @@ int orc_create(const char* config_json_path, const char* data_dir, orc_ctx** out
             const phx::ShField& f = ctx->env.sh;
             ctx->sh = ShTables{f.degree, f.order, f.normalized, f.radius, f.gm, f.C.data(), f.S.data(), f.factors.data()};
             ctx->hasSh = true;
+        }
+        for (const auto& r : ctx->env.rot) {
+            const phx::ShField& f = r.field;
+            ctx->rotSh.push_back(ShTables{f.degree, f.order, f.normalized, f.radius, f.gm, f.C.data(), f.S.data(),
+                                          f.factors.data()});
         }
         if (ctx->env.hasEop) {
             const phx::EopTable& e = ctx->env.eop;
@@ -230,7 +254,20 @@ V3 acceleration(const orc_ctx& c, V3 pos, V3 vel, double et, const Params& p, in
                 acc = acc + thirdBodyNB(pos, bodyPos, b.gm);
             }
         }
-        if (b.shape && c.hasSh && (only < 0 || only == 1))
+        if (b.naif != phx::NAIF_EARTH && (b.shape || b.oblateness)) {
+            // SphericalHarmonics.cu:90-144, J2.h:75-90: the body-relative
+            // position in the body's axes, nothing beyond its sphere of influence
+            if (only < 0 || only == 2) {
+                size_t k = 0;
+                while (c.env.model.bodies[size_t(c.env.rot[k].body)].naif != b.naif) ++k;
+                const V3 d = pos - bodyPos;
+                if (!(norm(d) > b.soi)) {
+                    const M3 R = rotBodyAxes(c, k, et);
+                    if (b.shape) acc = acc + sphericalHarmonics(d, R, c.rotSh[k]);
+                    else acc = acc + j2Gravity(d, R, b.gm, b.j2, b.radius, b.soi);
+                }
+            }
+        } else if (b.shape && c.hasSh && (only < 0 || only == 1))
             acc = acc + sphericalHarmonics(pos, orientation().R, c.sh);
         else if (b.oblateness && !b.shape && (only < 0 || only == 1))
             acc = acc + j2Gravity(pos, orientation().R, b.gm, b.j2, b.radius, b.soi);
@@ -483,6 +520,17 @@ int orc_ctx_earth_orientation(const orc_ctx* ctx, double et, double* R9, double*
     for (int i = 0; i < 9; ++i) R9[i] = o.R.m[i];
     omega3[0] = o.omega.x; omega3[1] = o.omega.y; omega3[2] = o.omega.z;
     return 0;
+}
+
+int orc_body_axes(const orc_ctx* ctx, int naif, double et, double* R9) {
+    for (size_t k = 0; k < ctx->env.rot.size(); ++k) {
+        if (ctx->env.model.bodies[size_t(ctx->env.rot[k].body)].naif != naif) continue;
+        const M3 R = rotBodyAxes(*ctx, k, et);
+        for (int i = 0; i < 9; ++i) R9[i] = R.m[i];
+        return ctx->env.rot[k].target;
+    }
+    g_error = "no field of body " + std::to_string(naif) + " in its own axes is configured";
+    return -1;
 }
 
 // ---------------------------------------------------------------------------

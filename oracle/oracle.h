@@ -49,6 +49,9 @@ int orc_ctx_body_position(const orc_ctx* ctx, int naif, double et, double* pos3)
  * orientation: its IERS2000 unit when the configuration names a .brot2 file,
  * else the analytic model. */
 int orc_ctx_earth_orientation(const orc_ctx* ctx, double et, double* R9, double* omega3);
+// ICRF -> axes of the configured field of body `naif` (not the Earth); returns
+// the rotation target the body resolved to, -1 when the body has no such field.
+int orc_body_axes(const orc_ctx* ctx, int naif, double et, double* R9);
 /* Geodetic altitude (km) of r above the ellipsoid (equatorial radius Req km,
  * flattening f) whose polar axis is the unit vector pole, r and pole in one frame. */
 double orc_geodetic_altitude(const double* r3, const double* pole3, double Req, double f);

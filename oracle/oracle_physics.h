@@ -515,6 +515,100 @@ inline V3 j2BodyFixed(V3 p, double gm, double j2, double radius) {
     return {c * p.x * v, c * p.y * v, c * p.z * (v + 2.0)};
 }
 
+// ---------------------------------------------------------------------------
+// Axes of a body other than the Earth from its .brot2 orientation unit:
+// orientations/RefEphUnit.h:190-420 (IAU body model with the nutation-precession
+// sums of core/Polynomial.h:30-115; binary frame Euler angles), frames/Frames.h:
+// 40-130,250-270 (radians, ECLIPJ2000 bias) and frames/Rotation.h:40-210 (the
+// passive 3-1-3 rotation as the product of half-angle quaternions q3 q2 q1).
+// ---------------------------------------------------------------------------
+struct Quat { double w, x, y, z; };
+inline Quat qmul(const Quat& a, const Quat& b) {
+    return {a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+            a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x, a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w};
+}
+// passive (frame) rotation by theta about principal axis `axis` (1 x, 3 z)
+inline Quat passiveAxis(int axis, double theta) {
+    const double s = std::sin(0.5 * theta), c = std::cos(0.5 * theta);
+    return axis == 1 ? Quat{c, -s, 0.0, 0.0} : Quat{c, 0.0, 0.0, -s};
+}
+// the matrix of v -> q v q^-1
+inline M3 quatMatrix(const Quat& q) {
+    const double w = q.w, x = q.x, y = q.y, z = q.z;
+    return {{1.0 - 2.0 * (y * y + z * z), 2.0 * (x * y - w * z), 2.0 * (x * z + w * y),
+             2.0 * (x * y + w * z), 1.0 - 2.0 * (x * x + z * z), 2.0 * (y * z - w * x),
+             2.0 * (x * z - w * y), 2.0 * (y * z + w * x), 1.0 - 2.0 * (x * x + y * y)}};
+}
+
+// ICRF -> body axes from RA, Dec, W (radians)
+inline M3 raDecPmAxes(double ra, double dec, double pm, bool eclipticBase) {
+    const double halfPi = M_PI / 2;
+    const Quat q1 = passiveAxis(3, halfPi + ra), q2 = passiveAxis(1, halfPi - dec), q3 = passiveAxis(3, pm);
+    Quat q = qmul(qmul(q3, q2), q1);
+    if (eclipticBase) q = qmul(q, passiveAxis(1, 84381.448 / 3600.0 * (M_PI / 180.0)));
+    return quatMatrix(q);
+}
+
+// IAU body unit: d = RA[degRa+1] Dec[degDec+1] PM[degPm+1] ampRA[nRa] ampDec[nDec]
+// ampPM[nPm] (ascending powers; RA, Dec in Julian centuries, PM in days past
+// J2000 TDB; degrees); ang = the barycenter's angle polynomials (nullptr: none)
+inline M3 iauBodyAxes(const double* d, const int deg[3], const int np[3], const double* ang, int angDeg, double et) {
+    const double tc = et / 3155760000.0, td = et / 86400.0;
+    auto poly = [](const double* c, int n, double t) {
+        double r = 0.0;
+        for (int i = n - 1; i >= 0; --i) r = std::fma(r, t, c[i]);
+        return r;
+    };
+    auto trig = [&](const double* amp, int n, bool useSin) {
+        double r = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double v = 0.0, tp = 1.0;
+            for (int j = 0; j <= angDeg; ++j) {
+                v = std::fma(ang[i * (angDeg + 1) + j], tp, v);
+                tp *= tc;
+            }
+            r = std::fma(amp[i], useSin ? std::sin(v * (M_PI / 180.0)) : std::cos(v * (M_PI / 180.0)), r);
+        }
+        return r;
+    };
+    double ra = poly(d, deg[0] + 1, tc);
+    double dec = poly(d + deg[0] + 1, deg[1] + 1, tc);
+    double pm = poly(d + deg[0] + 1 + deg[1] + 1, deg[2] + 1, td);
+    if (ang) {
+        const double* amp = d + (deg[0] + 1) + (deg[1] + 1) + (deg[2] + 1);
+        ra += trig(amp, np[0], true);
+        dec += trig(amp + np[0], np[1], false);
+        pm += trig(amp + np[0] + np[1], np[2], true);
+    }
+    const double k = M_PI / 180.0;
+    return raDecPmAxes(std::fmod(ra, 360.0) * k, std::fmod(dec, 360.0) * k, std::fmod(pm, 360.0) * k, false);
+}
+
+// Binary frame unit: Chebyshev Euler angles (phi, theta, psi) in the ephemeris
+// body-unit layout (radius, interval starts, coefficients [angle][degree][interval]),
+// evaluated here by the forward recurrence T_{k+1} = 2 s T_k - T_{k-1}.
+inline M3 binaryFrameAxes(const double* d, int nIntervals, int pdeg, int inertFrame, double et) {
+    const double r = d[0];
+    long long idx = (long long)std::floor((et - d[1]) / (2.0 * r));
+    if (idx < 0) idx = 0;
+    if (idx > nIntervals - 1) idx = nIntervals - 1;
+    const double s = (et - d[1 + idx]) / r - 1.0;
+    double e[3];
+    for (int k = 0; k < 3; ++k) {
+        double t0 = 1.0, t1 = s, v = 0.0;
+        for (int i = 0; i <= pdeg; ++i) {
+            const double c = d[1 + nIntervals + (long long)k * nIntervals * (pdeg + 1) + (long long)i * nIntervals + idx];
+            const double T = i == 0 ? 1.0 : (i == 1 ? s : 2.0 * s * t1 - t0);
+            if (i >= 2) { t0 = t1; t1 = T; }
+            v += c * T;
+        }
+        e[k] = v;
+    }
+    const double halfPi = M_PI / 2;
+    const double ra = e[0] - halfPi, dec = halfPi - e[1], pm = std::fmod(e[2], 2.0 * M_PI);
+    return raDecPmAxes(ra, dec, pm, inertFrame == 17);
+}
+
 inline V3 j2Gravity(V3 pos, const M3& R, double gm, double j2, double radius, double soi) {
     if (norm(pos) > soi) return V3{};
     return mulT(R, j2BodyFixed(mul(R, pos), gm, j2, radius));

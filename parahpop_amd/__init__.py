@@ -12,8 +12,9 @@ import weakref
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libphx.so")
-REC_SIZE = 352
+# PHX_LIB_PATH names another build of the library (the tables are read from its data/ folder)
+LIB_PATH = os.environ.get("PHX_LIB_PATH") or os.path.join(HERE, "libphx.so")
+REC_SIZE = 384   # doubles per epoch record (phx_common.h REC_SIZE)
 
 PHX_OK, PHX_EINVAL, PHX_EDEVICE, PHX_ENOTDONE, PHX_ENOMEM, PHX_EAGAIN = 0, -1, -2, -3, -4, -5
 

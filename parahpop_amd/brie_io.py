@@ -188,3 +188,74 @@ def iau2006_channels(xp_arcsec=0.0, yp_arcsec=0.0, wobble=False):
         return (X, Y, s, xp_arcsec * asec * np.cos(a), yp_arcsec * asec * np.sin(a) if wobble else yp_arcsec * asec,
                 -69.184 / 86400.0, 0.0, 0.0)
     return ch
+
+
+# ---------------------------------------------------------------------------
+# Body-orientation units of .brot2 files (third_party/brie/brie/orientations/
+# metadata/Layout.h:19-60): twelve int fields per unit, field-major (unit type,
+# target, eight unit fields, data offset, data size) and the envelope (start,
+# final ET seconds) as the real fields.
+
+def iau_body_unit(target, ra, dec, pm, np_ra=(), np_dec=(), np_pm=(), barycenter=0, radii=()):
+    """Type-1 unit: RA and Dec polynomials in Julian centuries, the prime
+    meridian polynomial in days past J2000 TDB (degrees, ascending powers), then
+    the nutation-precession amplitudes of RA (sine), Dec (cosine) and PM (sine)
+    over the angles of the `barycenter` type-2 unit."""
+    data = np.concatenate([np.asarray(x, dtype=float) for x in (ra, dec, pm, np_ra, np_dec, np_pm, radii)])
+    ints = [len(ra) - 1, len(dec) - 1, len(pm) - 1, len(radii), barycenter, len(np_ra), len(np_dec), len(np_pm)]
+    return {"type": 1, "target": target, "ints": ints, "data": data, "start": -1e12, "final": 1e12}
+
+
+def np_angles_unit(target, angles):
+    """Type-2 unit: nutation-precession angle polynomials (degrees, Julian
+    centuries, ascending powers), all of one degree."""
+    a = np.asarray(angles, dtype=float)
+    return {"type": 2, "target": target, "ints": [a.shape[1] - 1, a.shape[0], 0, 0, 0, 0, 0, 0],
+            "data": a.reshape(-1), "start": -1e12, "final": 1e12}
+
+
+def binary_frame_unit(target, euler, t0, t1, interval, pdeg, inert_frame=1):
+    """Type-3 unit: Chebyshev fit of euler(et) -> (phi, theta, psi) (radians,
+    3-1-3 angles from the base frame: 1 ICRF, 17 ECLIPJ2000) in the ephemeris
+    body-unit layout; the envelope is the fitted span."""
+    data, n = fit_unit(euler, t0, t1, interval, pdeg)
+    return {"type": 3, "target": target, "ints": [inert_frame, 2, n, pdeg, 0, 0, 0, 0], "data": data,
+            "start": float(data[1]), "final": float(data[1] + 2 * data[0] * n)}
+
+
+def write_brot2_units(path, units):
+    """One .brot2 file (layout 2) holding the given units (dicts from the unit
+    builders above)."""
+    N = len(units)
+    fields = [[] for _ in range(12)]
+    offset = 0
+    for u in units:
+        vals = [u["type"], u["target"]] + list(u["ints"]) + [offset, len(u["data"])]
+        for f, v in enumerate(vals):
+            fields[f].append(int(v))
+        offset += len(u["data"])
+    doc = {"layout": 2, "core": {"metadata": {"nBodyUnits": N, "intMetadata": [v for f in fields for v in f],
+                                              "doubleMetadata": [float(u["start"]) for u in units] +
+                                                                [float(u["final"]) for u in units]},
+                                 "data": np.concatenate([np.asarray(u["data"], dtype=float) for u in units])}}
+    with open(path, "wb") as f:
+        f.write(cbor(doc))
+
+
+def moon_iau_units(terms=4):
+    """The Moon's IAU rotation model (WGCCRE 2009 / NAIF pck00010 Moon block,
+    truncated to the first `terms` nutation-precession angles E1..E13 of the
+    Earth barycenter, in degrees and centuries): a type-1 unit for body 301 and
+    the type-2 unit of barycenter 3."""
+    E = np.array([[125.045, -1935.5364525], [250.089, -3871.072905], [260.008, 475263.3328725],
+                  [176.625, 487269.629985], [357.529, 35999.0509575], [311.589, 964468.49931],
+                  [134.963, 477198.869325], [276.617, 12006.300765], [34.226, 63863.5132425],
+                  [15.134, -5806.6093575], [119.743, 131.84064], [239.961, 6003.1503825],
+                  [25.053, 473327.79642]])[:terms]
+    amp_ra = [-3.8787, -0.1204, 0.0700, -0.0172, 0.0, 0.0072, 0.0, 0.0, 0.0, -0.0052, 0.0, 0.0, 0.0043][:terms]
+    amp_dec = [1.5419, 0.0239, -0.0278, 0.0068, 0.0, -0.0029, 0.0009, 0.0, 0.0, 0.0008, 0.0, 0.0, -0.0009][:terms]
+    amp_pm = [3.5610, 0.1208, -0.0642, 0.0158, 0.0252, -0.0066, -0.0047, -0.0046, 0.0028, 0.0052, 0.0040, 0.0019,
+              -0.0044][:terms]
+    body = iau_body_unit(301, [269.9949, 0.0031], [66.5392, 0.0130], [38.3213, 13.17635815, -1.4e-12],
+                         amp_ra, amp_dec, amp_pm, barycenter=3, radii=(1737.4, 1737.4, 1737.4))
+    return [body, np_angles_unit(3, E)]

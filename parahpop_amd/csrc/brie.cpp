@@ -277,8 +277,38 @@ void loadBrot2(const std::string& path, BrotTable& t) {
     if (nUnits <= 0 || ints.size() != size_t(kInt * nUnits) || reals.size() != size_t(2 * nUnits))
         c.fail("orientation metadata sizes do not match nBodyUnits");
     const long long base = (long long)t.data.size();
+    size_t found = 0;
     for (long long u = 0; u < nUnits; ++u) {
         auto f = [&](int field) { return (long long)ints[size_t(field * nUnits + u)]; };
+        if (f(0) >= 1 && f(0) <= 3) {
+            BrotBody b;
+            b.type = int(f(0));
+            b.target = int(f(1));
+            for (int k = 0; k < 8; ++k) b.p[k] = int(f(2 + k));
+            b.offset = base + f(10);
+            b.size = f(11);
+            b.start = reals[size_t(u)];
+            b.final = reals[size_t(nUnits + u)];
+            long long need = 0;
+            if (b.type == 1) {
+                for (int k = 0; k < 3; ++k)
+                    if (b.p[k] < 0) c.fail("body orientation unit with a negative polynomial degree");
+                for (int k = 5; k < 8; ++k)
+                    if (b.p[k] < 0) c.fail("body orientation unit with a negative term count");
+                need = (long long)b.p[0] + b.p[1] + b.p[2] + 3 + b.p[5] + b.p[6] + b.p[7];
+            } else if (b.type == 2) {
+                if (b.p[0] < 0 || b.p[1] < 0) c.fail("nutation-precession unit with negative sizes");
+                need = (long long)(b.p[0] + 1) * b.p[1];
+            } else {
+                if (b.p[2] < 1 || b.p[3] < 0) c.fail("binary frame unit without intervals");
+                need = 1 + (long long)b.p[2] + 3LL * (b.p[3] + 1) * b.p[2];
+            }
+            if (f(10) < 0 || b.size < need || f(10) + b.size > (long long)data.size())
+                c.fail("orientation unit with inconsistent sizes");
+            t.bodies.push_back(b);
+            ++found;
+            continue;
+        }
         if (f(0) != 4 || f(2) != 0) continue;   // unit_type 4, model 0 (IERS2000)
         BrotIers b;
         b.target = int(f(1));
@@ -293,9 +323,10 @@ void loadBrot2(const std::string& path, BrotTable& t) {
         if (b.nNut < 6 || b.nErp < 4 || size < need || f(10) + size > (long long)data.size())
             c.fail("IERS2000 unit with inconsistent sizes");
         t.iers.push_back(b);
+        ++found;
     }
     t.data.insert(t.data.end(), data.begin(), data.end());
-    if (t.iers.empty()) throw std::runtime_error(path + ": no IERS2000 orientation unit");
+    if (found == 0) throw std::runtime_error(path + ": no IERS2000 or body orientation unit");
 }
 
 namespace {

@@ -53,14 +53,42 @@ struct BrotIers {
     double start = 0.0, final = 0.0;   // envelope, ET seconds
 };
 
-struct BrotTable {
-    std::vector<BrotIers> iers;
-    std::vector<double> data;
-    bool empty() const { return iers.empty(); }
+// Body-orientation unit of a .brot2 file (orientations/metadata/Layout.h:19-60):
+//   type 1  IAU body model (orientations/RefEphUnit.h:190-330): RA and Dec
+//           polynomials in Julian centuries and the prime meridian polynomial
+//           in days (degrees, coefficients in ascending powers), then the
+//           nutation-precession amplitudes of RA, Dec and PM; p = {PDEG_RA,
+//           PDEG_DEC, PDEG_PM, NUM_DIMS_RADII, BARYCENTER, NUM_NP_TERMS_RA,
+//           NUM_NP_TERMS_DEC, NUM_NP_TERMS_PM}
+//   type 2  nutation-precession angles of a barycenter: NUM_NP_ANGLES
+//           polynomials of degree PDEG_NP in centuries (degrees); p = {PDEG_NP,
+//           NUM_NP_ANGLES}
+//   type 3  binary frame (core/Type2BodyUnit.h): Chebyshev 3-1-3 Euler angles
+//           (radians) in the ephemeris body-unit layout; p = {INERT_FRAME,
+//           INTERPOL_TYPE, NINTERVALS, PDEG}
+struct BrotBody {
+    int type = 0, target = 0;
+    int p[8] = {};
+    long long offset = 0, size = 0;   // into BrotTable::data
+    double start = 0.0, final = 0.0;  // envelope, ET seconds
 };
 
-// Read one .brot2 file (layout 2) and append its IERS2000 units to `t`; other
-// unit types are skipped.
+struct BrotTable {
+    std::vector<BrotIers> iers;
+    std::vector<BrotBody> bodies;
+    std::vector<double> data;
+    bool empty() const { return iers.empty(); }
+    // the first unit of `target` (brie resolves queries through the first
+    // matching row), -1 when none
+    int findBody(int target) const {
+        for (size_t i = 0; i < bodies.size(); ++i)
+            if (bodies[i].target == target) return int(i);
+        return -1;
+    }
+};
+
+// Read one .brot2 file (layout 2) and append its IERS2000 (type 4, model 0) and
+// body-orientation (types 1-3) units to `t`; throws when it holds neither.
 void loadBrot2(const std::string& path, BrotTable& t);
 
 // Read one .brie file (layout 1 or 2) and append its units to `t`.

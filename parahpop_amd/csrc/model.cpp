@@ -370,7 +370,7 @@ void parseConfig(const json::Value& root, const std::string& configDir,
                 else if (q == "radius") b.radius = p.second.asDouble();
                 else if (q == "soi") b.soi = p.second.asDouble();
                 else if (q == "j2") b.j2 = p.second.asDouble();
-                else if (q == "orientation") (void)p.second.asString();  // frame name; Earth uses the IERS2000 unit
+                else if (q == "orientation") b.orientation = p.second.asString();  // Earth uses the IERS2000 unit
                 else if (q == "amean") (void)p.second.asDouble();  // accepted; not used by these forces
                 else throw std::runtime_error("Unsupported body option: " + p.first);
             }
@@ -398,14 +398,21 @@ void parseConfig(const json::Value& root, const std::string& configDir,
             throw std::runtime_error("No gravitational parameter for body " + b.name);
         if (b.radiation && b.naif != NAIF_SUN) throw std::runtime_error("Radiation is supported for the Sun only.");
         if (b.atmosphere && b.naif != NAIF_EARTH) throw std::runtime_error("Atmospheres are supported for Earth only.");
-        if (b.shape && b.naif != NAIF_EARTH) throw std::runtime_error("Spherical harmonics are supported for Earth only.");
-        if (b.oblateness && !b.shape && b.naif != NAIF_EARTH)
-            throw std::runtime_error("J2 (oblateness) is supported for Earth only.");
+        if ((b.shape || b.oblateness) && b.naif != NAIF_EARTH && !(b.radius > 0))
+            throw std::runtime_error("Body field needs the body radius: " + b.name);
         if (b.atmosphere && !b.nrlmsise && !(b.radius > 0))
             throw std::runtime_error("Exponential atmosphere needs the body radius: " + b.name);
         if (b.occulting && !(b.radius > 0)) throw std::runtime_error("Occulting body needs a radius: " + b.name);
         if (b.radiation && !(b.radius > 0)) throw std::runtime_error("Sun radius missing for shadow geometry.");
     }
+    int rotating = 0;
+    for (const auto& b : model.bodies) rotating += (b.shape || b.oblateness) && b.naif != NAIF_EARTH ? 1 : 0;
+    if (rotating > kMaxRotBodies)
+        throw std::runtime_error("At most " + std::to_string(kMaxRotBodies) +
+                                 " bodies other than the Earth may carry a gravity field.");
+    if (rotating > 0 && model.orientationFiles.empty())
+        throw std::runtime_error("A gravity field of a body other than the Earth needs its rotation data: list the "
+                                 "defining .brot2 files under model.environment.orientations.");
 }
 
 ShField loadShField(const std::string& path, char delimiter, int headerLines, int degree, int order) {
@@ -578,6 +585,77 @@ EopTable loadEop(const std::string& path) {
     return t;
 }
 
+namespace {
+
+// Rotation-data target of a body frame name (naming/orientations/Registry.h:
+// 580-640 builtins; <BODY>_BODY_FIXED and IAU_<BODY> name a body's IAU model).
+// Sets `body` to the body the frame describes.
+int frameTarget(const std::string& name, int& body) {
+    std::string k;
+    for (char c : name) if (c != ' ') k += char(std::toupper((unsigned char)c));
+    for (const char* inertial : {"ICRF", "J2000", "EME2000", "ECLIPJ2000", "ECLIPTIC", "ECLIPTICJ2000"})
+        if (k == inertial)
+            throw std::runtime_error("Body orientation '" + name + "' is an inertial frame; a body's force-model "
+                                     "rotation needs a body-fixed or binary frame");
+    if (k == "MOON_PA") { body = NAIF_MOON; return 31008; }
+    if (k == "ITRF93" || k == "ITRF" || k == "EARTH_FIXED") { body = NAIF_EARTH; return 3000; }
+    if (k == "ITRF_IERS2000" || k == "ITRF_IPF" || k == "ITRF_NATIVE") { body = NAIF_EARTH; return 3001; }
+    std::string token;
+    const std::string suffix = "_BODY_FIXED";
+    if (k.size() > suffix.size() && k.compare(k.size() - suffix.size(), suffix.size(), suffix) == 0)
+        token = k.substr(0, k.size() - suffix.size());
+    else if (k.size() > 4 && k.compare(0, 4, "IAU_") == 0)
+        token = k.substr(4);
+    if (!token.empty()) {
+        try {
+            body = naifFromName(token);
+            return body;
+        } catch (const std::runtime_error&) {
+        }
+    }
+    throw std::runtime_error("Body orientation '" + name + "' is not a known orientation frame; name a builtin "
+                             "(MOON_PA, <BODY>_BODY_FIXED, IAU_<BODY>)");
+}
+
+// The rotation target of a body's field (naming/orientations/Resolver.h:45-92):
+// an explicit frame name, else the unique loaded binary frame of the body,
+// else the body's IAU model (its own NAIF id).
+void resolveRotBody(const Environment& env, const BodySpec& b, RotBody& r) {
+    const BrotTable& t = env.brot;
+    if (!b.orientation.empty()) {
+        int body = 0;
+        r.target = frameTarget(b.orientation, body);
+        if (body != b.naif)
+            throw std::runtime_error("Body orientation '" + b.orientation + "' describes body " + std::to_string(body) +
+                                     ", not body " + std::to_string(b.naif) + "; set it on the body it belongs to");
+    } else {
+        r.target = b.naif;
+        if (b.naif == NAIF_MOON && t.findBody(31008) >= 0) r.target = 31008;
+    }
+    r.unit = t.findBody(r.target);
+    if (r.unit < 0 || (t.bodies[size_t(r.unit)].type != 1 && t.bodies[size_t(r.unit)].type != 3))
+        throw std::runtime_error("Body orientation of " + b.name + " (rotation target " + std::to_string(r.target) +
+                                 ") needs its rotation data loaded; add the defining .brot2 to the environment's "
+                                 "orientations");
+    const BrotBody& u = t.bodies[size_t(r.unit)];
+    r.bary = -1;
+    if (u.type == 1) {
+        const int bary = t.findBody(u.p[4]);
+        if (bary >= 0) {
+            if (t.bodies[size_t(bary)].type != 2)
+                throw std::runtime_error("Barycenter " + std::to_string(u.p[4]) + " of body orientation " +
+                                         std::to_string(r.target) + " is not a nutation-precession unit");
+            const BrotBody& bu = t.bodies[size_t(bary)];
+            if (bu.p[1] < std::max(u.p[5], std::max(u.p[6], u.p[7])))
+                throw std::runtime_error("Barycenter unit " + std::to_string(u.p[4]) + " holds fewer angles than body "
+                                         "orientation " + std::to_string(r.target) + " has terms");
+            r.bary = bary;
+        }
+    }
+}
+
+}  // namespace
+
 void loadEnvironment(Environment& env) {
     const int e = env.model.find(NAIF_EARTH);
     const BodySpec& earth = env.model.bodies[size_t(e)];
@@ -595,9 +673,22 @@ void loadEnvironment(Environment& env) {
     }
     if (env.model.orientation == "brot2") {
         for (const auto& f : env.model.orientationFiles) loadBrot2(f, env.brot);
-        if (env.brot.iers.size() != 1)
-            throw std::runtime_error("orientations: expected one IERS2000 Earth orientation unit, found " +
+        // the Earth's IERS2000 unit, when one is loaded (the analytic model otherwise)
+        if (env.brot.iers.size() > 1)
+            throw std::runtime_error("orientations: expected at most one IERS2000 Earth orientation unit, found " +
                                      std::to_string(env.brot.iers.size()));
+    }
+    env.rot.clear();
+    for (size_t i = 0; i < env.model.bodies.size(); ++i) {
+        const BodySpec& b = env.model.bodies[i];
+        if (b.naif == NAIF_EARTH || !(b.shape || b.oblateness)) continue;
+        RotBody r;
+        r.body = int(i);
+        resolveRotBody(env, b, r);
+        r.sh = b.shape;
+        if (b.shape) r.field = loadShField(b.shFile, b.shDelimiter, b.shHeaderLines, b.shDegree, b.shOrder);
+        else if (!(b.j2 != 0.0)) throw std::runtime_error("No J2 for body " + b.name);
+        env.rot.push_back(std::move(r));
     }
     if (env.model.ephemeris == "brie") {
         for (const auto& f : env.model.ephemerisFiles) loadBrie(f, env.brie);

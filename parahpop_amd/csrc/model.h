@@ -11,6 +11,7 @@
 
 #include "brie.h"
 #include "json_lite.h"
+#include "phx_common.h"
 
 namespace phx {
 
@@ -89,6 +90,9 @@ struct BodySpec {
     int shOrder = -1;
     // gravity.oblateness (J2 only; a shape field takes precedence)
     bool oblateness = false;
+    // bodies.<name>.orientation: frame whose rotation the body's field uses
+    // (empty: the unique loaded binary frame of the body, else its IAU model)
+    std::string orientation;
     double j2 = 0.0;          // unnormalized J2 (constants.json "J2" or inline "j2")
     // atmosphere.nrlmsise00
     bool nrlmsise = false;
@@ -138,10 +142,25 @@ struct RunConfig {
     Precision precision;
 };
 
+// A body other than the Earth whose field (spherical harmonics or J2) acts in
+// its own rotating axes (physics/reduced/PrivateDimensional.h:527-559): the
+// orientation unit of its resolved frame (naming/orientations/Resolver.h:45-92)
+// and, for an IAU body unit, the nutation-precession unit of its barycenter.
+constexpr int kMaxRotBodies = kMaxRotFields;
+struct RotBody {
+    int body = -1;           // index into ModelConfig::bodies
+    int target = 0;          // rotation-data target (NAIF body or frame id)
+    int unit = -1;           // index into BrotTable::bodies
+    int bary = -1;           // type-1 units: barycenter unit (-1: none)
+    bool sh = false;         // field from its coefficient file, else J2
+    ShField field;
+};
+
 struct Environment {
     ModelConfig model;
     BrieTable brie;        // Chebyshev ephemeris (empty: analytic series)
-    BrotTable brot;        // IERS2000 Earth orientation (empty: analytic model)
+    BrotTable brot;        // IERS2000 Earth orientation (empty: analytic model) and body units
+    std::vector<RotBody> rot;
     ShField sh;            // Earth field (empty degree 0 when not configured)
     bool hasSh = false;
     WeatherTable weather;

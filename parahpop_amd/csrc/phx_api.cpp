@@ -206,14 +206,15 @@ void loadTables(phx_ctx& c) {
     }
 }
 
-// Packed SH term stream (see ShTerm in phx_common.h).
-void buildShTables(phx_ctx& c) {
-    const ShField& f = c.env.sh;
+// Packed SH term stream (see ShTerm in phx_common.h), its order starts and the
+// diagonal factors followed by their cumulative products.
+void packShField(const ShField& f, std::vector<double>& terms, std::vector<float>& terms32, std::vector<int>& start,
+                 std::vector<double>& diag) {
     const int L = f.degree, M = f.order;
-    std::vector<double> terms;
-    std::vector<float> terms32;
-    std::vector<int> start(static_cast<size_t>(L) + 2, 0);
-    std::vector<double> diag(static_cast<size_t>(L) + 1, 0.0);
+    terms.clear();
+    terms32.clear();
+    start.assign(static_cast<size_t>(L) + 2, 0);
+    diag.assign(static_cast<size_t>(L) + 1, 0.0);
     int k = 0;
     for (int m = 0; m <= L; ++m) {
         diag[size_t(m)] = f.factors[2 * f.index(m, m)];
@@ -249,9 +250,6 @@ void buildShTables(phx_ctx& c) {
         terms.push_back(0.0);
         terms32.push_back(0.0f);
     }
-    c.shTerms.upload(terms.data(), terms.size(), c.stream);
-    c.shTerms32.upload(terms32.data(), terms32.size(), c.stream);
-    c.shOrderStart.upload(start.data(), start.size(), c.stream);
     // then the cumulative products prod_{k=1..m} c_k (the seed of an order block)
     double prod = 1.0;
     diag.push_back(1.0);
@@ -259,7 +257,38 @@ void buildShTables(phx_ctx& c) {
         prod *= diag[size_t(m)];
         diag.push_back(prod);
     }
+}
+
+void buildShTables(phx_ctx& c) {
+    std::vector<double> terms, diag;
+    std::vector<float> terms32;
+    std::vector<int> start;
+    packShField(c.env.sh, terms, terms32, start, diag);
+    c.shTerms.upload(terms.data(), terms.size(), c.stream);
+    c.shTerms32.upload(terms32.data(), terms32.size(), c.stream);
+    c.shOrderStart.upload(start.data(), start.size(), c.stream);
     c.shDiag.upload(diag.data(), diag.size(), c.stream);
+}
+
+// Term streams of the fields of the bodies other than the Earth and the
+// orientation table of their axes; fills the device pointers of ForceModel.
+void buildRotTables(phx_ctx& c) {
+    if (c.env.rot.empty()) return;
+    c.brotBody.upload(c.env.brot.data.data(), c.env.brot.data.size(), c.stream);
+    for (size_t k = 0; k < c.env.rot.size(); ++k) {
+        const RotBody& r = c.env.rot[k];
+        if (!r.sh) continue;
+        std::vector<double> terms, diag;
+        std::vector<float> terms32;
+        std::vector<int> start;
+        packShField(r.field, terms, terms32, start, diag);
+        c.rotTerms[k].upload(terms.data(), terms.size(), c.stream);
+        c.rotStart[k].upload(start.data(), start.size(), c.stream);
+        c.rotDiag[k].upload(diag.data(), diag.size(), c.stream);
+        c.fm.rotTerms[k] = c.rotTerms[k].p;
+        c.fm.rotOrderStart[k] = c.rotStart[k].p;
+        c.fm.rotDiag[k] = c.rotDiag[k].p;
+    }
 }
 
 // Body of ephemeris slot `slot`: the configured bodies in order, then the Sun
@@ -328,6 +357,26 @@ void buildForceModel(phx_ctx& c) {
         }
     }
     fm.nOccluders = occ;
+    // fields of the bodies other than the Earth (device pointers: buildRotTables)
+    fm.nRot = int(c.env.rot.size());
+    for (size_t k = 0; k < c.env.rot.size(); ++k) {
+        const RotBody& r = c.env.rot[k];
+        const BodySpec& b = m.bodies[size_t(r.body)];
+        fm.rotBody[k] = r.body;
+        fm.rotSh[k] = r.sh ? 1 : 0;
+        fm.rotSoi[k] = b.soi;
+        if (r.sh) {
+            fm.rotDegree[k] = r.field.degree;
+            fm.rotOrder[k] = r.field.order;
+            fm.rotNormalized[k] = r.field.normalized ? 1 : 0;
+            fm.rotGm[k] = r.field.gm;
+            fm.rotRadius[k] = r.field.radius;
+        } else {
+            fm.rotGm[k] = b.gm;
+            fm.rotRadius[k] = b.radius;
+            fm.rotJ2[k] = b.j2;
+        }
+    }
     if (c.env.hasSh) {
         fm.hasSh = 1;
         fm.shDegree = c.env.sh.degree;
@@ -385,10 +434,52 @@ EpochArgs epochArgs(const phx_ctx& c) {
     a.leaps = c.leaps.p;
     a.leapCount = c.leapCount;
     a.gridRows = 0;
+    a.brot = c.brotBody.p;
+    a.nRot = int(c.env.rot.size());
+    for (size_t k = 0; k < c.env.rot.size(); ++k) {
+        const RotBody& r = c.env.rot[k];
+        const BrotBody& u = c.env.brot.bodies[size_t(r.unit)];
+        RotUnit& d = a.rot[k];
+        d.type = u.type;
+        d.off = u.offset;
+        d.bary = -1;
+        if (u.type == 1) {
+            d.degRa = u.p[0];
+            d.degDec = u.p[1];
+            d.degPm = u.p[2];
+            d.npRa = u.p[5];
+            d.npDec = u.p[6];
+            d.npPm = u.p[7];
+            if (r.bary >= 0) {
+                const BrotBody& bu = c.env.brot.bodies[size_t(r.bary)];
+                d.bary = bu.offset;
+                d.baryDeg = bu.p[0];
+                d.baryAngles = bu.p[1];
+            }
+        } else {
+            d.inertFrame = u.p[0];
+            d.nIntervals = u.p[2];
+            d.pdeg = u.p[3];
+        }
+    }
     return a;
 }
 
+bool needsCoverageCheck(const phx_ctx& c) { return !c.ephChains.empty() || !c.env.brot.empty() || !c.env.rot.empty(); }
+
 void checkEphemerisCoverage(const phx_ctx& c, double lo, double hi) {
+    // binary frames of the body fields carry an envelope (model-level coverage,
+    // interface/config/model/environment/Orientations.h:150-200)
+    for (const RotBody& r : c.env.rot) {
+        const BrotBody& u = c.env.brot.bodies[size_t(r.unit)];
+        if (u.type == 3 && (lo < u.start || hi > u.final)) {
+            std::ostringstream m;
+            m.precision(17);
+            m << "Epoch range [" << lo << ", " << hi << "] not in the available orientation range [" << u.start << ", "
+              << u.final << "] of rotation target " << u.target;
+            throw std::runtime_error(m.str());
+        }
+    }
     if (!c.env.brot.empty()) {
         const BrotIers& u = c.env.brot.iers[0];
         if (lo < u.start || hi > u.final) {
@@ -550,7 +641,7 @@ std::unique_ptr<phx_batch> buildBatch(phx_ctx& c, size_t n, const int64_t* id, c
             throw std::invalid_argument("Duration/step cannot be represented by the integration time/index types.");
         count = std::max(count, (long long)std::ceil(span / step) + 1);
     }
-    if (!c.ephChains.empty() || !c.env.brot.empty()) {
+    if (needsCoverageCheck(c)) {
         double lo = start[0], hi = endEt[0];
         for (size_t i = 0; i < n; ++i) {
             lo = std::min(lo, start[i]);
@@ -1023,6 +1114,7 @@ int phx_create(const char* config_json_path, phx_ctx** out) {
         loadTables(*c);
         computeLeaps(*c);
         if (c->env.hasSh) buildShTables(*c);
+        buildRotTables(*c);
         if (!c->env.brie.empty()) {
             std::vector<double> rows(size_t(kEphSlots) * kEphRow, 0.0);
             const ModelConfig& m = c->env.model;

@@ -13,6 +13,7 @@ constexpr int kMaxWaves = 16;      // waves per tile that share one object's wor
 constexpr int kMaxGroups = 4;      // SH workgroups per tile when few tiles are active
 constexpr int kMaxGroupWaves = kMaxGroups * kMaxWaves;
 constexpr int kMaxExpSegments = 32; // bands of a piecewise exponential atmosphere
+constexpr int kMaxRotFields = 3;    // bodies other than the Earth with a gravity field
 
 // Per-epoch record: everything the force model needs that depends only on the
 // stage epoch, computed once per distinct epoch and read by every object.
@@ -35,9 +36,10 @@ enum EpochRecord : int {
     REC_NRL_GLOBE = REC_NRL + 13, // epoch-only globe7 terms of the 11 tables (nrlmsise::GE_COUNT = 199),
                                   // then the glob7s terms of the 5 lower-thermosphere tables (nrlmsise::GL_COUNT = 30)
     REC_USED = REC_NRL_GLOBE + 199 + 30,
-    REC_SIZE = 352
+    REC_BROT = 352,      // 9 per rotating body (ForceModel::rot*): ICRF -> body axes, row-major
+    REC_SIZE = REC_BROT + 9 * kMaxRotFields + 5
 };
-static_assert(REC_USED <= REC_SIZE, "epoch record overflow");
+static_assert(REC_USED <= REC_BROT, "epoch record overflow");
 
 // Packed spherical-harmonic term stream. For order m the terms n = m..L are
 // consecutive; each carries what the order-block evaluation consumes for that
@@ -92,6 +94,16 @@ struct ForceModel {
     int tides;                    // solid or ocean active
     int relativity;
     double muEarthSI, radiusSI, clightSI;
+    // fields of bodies other than the Earth (spherical harmonics or J2), each in
+    // its own axes (record rows REC_BROT + 9 k), zero beyond its sphere of influence
+    int nRot;
+    int rotBody[kMaxRotFields];   // index of the body in the body list
+    int rotSh[kMaxRotFields];     // 1: coefficient field, 0: J2
+    int rotDegree[kMaxRotFields], rotOrder[kMaxRotFields], rotNormalized[kMaxRotFields];
+    double rotGm[kMaxRotFields], rotRadius[kMaxRotFields], rotSoi[kMaxRotFields], rotJ2[kMaxRotFields];
+    const double* rotTerms[kMaxRotFields];     // packed SH term stream of the field (FP64)
+    const int* rotOrderStart[kMaxRotFields];
+    const double* rotDiag[kMaxRotFields];
 };
 
 // kernel arguments travel by value (StepArgs + ForceModel + SplitArgs per launch)

@@ -90,6 +90,10 @@ struct phx_ctx {
     std::vector<std::vector<phx::BrieStep>> ephChains;   // host copy, for coverage checks
     phx::DBuf<float> shTerms32;
     phx::DBuf<int> shOrderStart;
+    // fields of the bodies other than the Earth: term streams and the
+    // orientation table their axes come from
+    phx::DBuf<double> rotTerms[phx::kMaxRotFields], rotDiag[phx::kMaxRotFields], brotBody;
+    phx::DBuf<int> rotStart[phx::kMaxRotFields];
     int freqCount[3] = {0, 0, 0};
     int waveCount = 0;
     // options
@@ -176,6 +180,7 @@ void splitOrderBlocks(const phx_ctx& c, int W, int G, int* out);
 // Throws unless the Chebyshev ephemeris and the IERS2000 orientation (when
 // configured) cover ET [lo, hi].
 void checkEphemerisCoverage(const phx_ctx& c, double lo, double hi);
+bool needsCoverageCheck(const phx_ctx& c);
 
 void validateTargets(size_t n, const int64_t* id, const double* epoch, const double* state6, const double* mass,
                      const double* drag, const double* srp, const double* cd, const double* cr);

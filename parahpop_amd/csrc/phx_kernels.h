@@ -8,6 +8,20 @@
 
 namespace phx {
 
+// Orientation unit of a body with a field of its own (ForceModel::rot*), in a
+// .brot2 body table on the device (offsets in doubles): type 1 IAU body model
+// (RA, Dec, PM polynomials, then nutation-precession amplitudes; `bary` the
+// angle polynomials of its barycenter, -1 none) or type 3 binary frame
+// (Chebyshev 3-1-3 Euler angles in the ephemeris body-unit layout).
+struct RotUnit {
+    int type;
+    long long off;
+    int degRa, degDec, degPm, npRa, npDec, npPm;     // type 1
+    long long bary;
+    int baryDeg, baryAngles;
+    int inertFrame, nIntervals, pdeg;                // type 3
+};
+
 struct EpochArgs {
     int count;
     const double* epochs;   // [count] ET seconds
@@ -44,6 +58,10 @@ struct EpochArgs {
     const double* leaps;
     int leapCount;
     int gridRows;
+    // orientation units of the bodies with fields of their own
+    const double* brot;
+    int nRot;
+    RotUnit rot[kMaxRotFields];
 };
 
 constexpr int kEphSteps = 6;

@@ -213,7 +213,7 @@ extern "C" int phx_propagate_noalloc(phx_ctx* c, size_t n, double* d_epoch, doub
         if (sum.flags & 1) throw std::invalid_argument("State contains a nonfinite value.");
         if (sum.flags & 2) throw std::invalid_argument("Invalid target epoch or physical parameter.");
         if (sum.flags & 4) throw std::invalid_argument("Duration/step cannot be represented by the integration time/index types.");
-        if (!c->ephChains.empty() || !c->env.brot.empty()) checkEphemerisCoverage(*c, sum.lo, sum.hi);
+        if (needsCoverageCheck(*c)) checkEphemerisCoverage(*c, sum.lo, sum.hi);
         // the grid of the batch path: two rows of margin each side
         const double gridT0 = sum.lo - 2.0 * h;
         const double rowsD = std::ceil((sum.hi - gridT0) / h) + 3;

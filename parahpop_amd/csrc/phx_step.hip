@@ -829,9 +829,96 @@ __device__ inline void bodySlots(const ForceModel& fm, const Rec& rec, double px
     }
 }
 
-// Central and third-body point gravity (PointGravity.h:16-81); the indirect
-// third-body term is an epoch-only sum taken from the record.
+// Record view whose rotation rows (REC_R ..) are the axes of body field k.
 template <class Rec>
+struct BodyAxesRec {
+    const Rec& r;
+    int off;
+    __device__ double operator()(int f) const { return r(off + (f - REC_R)); }
+};
+
+// One body field of ForceModel::rot*, selected by a uniform index with
+// constant-index reads (a dynamic index into the by-value kernel argument
+// would copy it to scratch memory).
+struct RotField {
+    int body, sh, degree, order, normalized;
+    double gm, radius, soi, j2;
+    const double* terms;
+    const int* orderStart;
+    const double* diag;
+};
+
+template <class T>
+__device__ inline T pick3(int k, const T (&v)[kMaxRotFields]) {
+    static_assert(kMaxRotFields == 3, "one select per field");
+    return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]);
+}
+
+__device__ inline RotField rotField(const ForceModel& fm, int k) {
+    RotField f;
+    f.body = pick3(k, fm.rotBody);
+    f.sh = pick3(k, fm.rotSh);
+    f.degree = pick3(k, fm.rotDegree);
+    f.order = pick3(k, fm.rotOrder);
+    f.normalized = pick3(k, fm.rotNormalized);
+    f.gm = pick3(k, fm.rotGm);
+    f.radius = pick3(k, fm.rotRadius);
+    f.soi = pick3(k, fm.rotSoi);
+    f.j2 = pick3(k, fm.rotJ2);
+    f.terms = pick3(k, fm.rotTerms);
+    f.orderStart = pick3(k, fm.rotOrderStart);
+    f.diag = pick3(k, fm.rotDiag);
+    return f;
+}
+
+// Field k of the bodies other than the Earth (SphericalHarmonics.cu:90-144,
+// J2.h:75-90): evaluated at the position relative to the body in its own axes
+// (record rows REC_BROT + 9 k) and rotated back; zero beyond the body's sphere
+// of influence. Spherical harmonics run all orders on the calling wave.
+template <class Rec>
+__device__ inline void rotFieldAccel(const RotField f, int k, const Rec& rec, double px, double py, double pz, double& ax,
+                                     double& ay, double& az) {
+    {
+        const int q = REC_BODY + 3 * f.body;
+        const double dx = px - rec(q), dy = py - rec(q + 1), dz = pz - rec(q + 2);
+        if (!(sqrt(dx * dx + dy * dy + dz * dz) <= f.soi)) return;
+        const BodyAxesRec<Rec> br{rec, REC_BROT + 9 * k};
+        double bx, by, bz;
+        if (f.sh) {
+            const ShGeom g = shGeometry(br, dx, dy, dz, f.radius);
+            double r, p, l;
+            shOrderBlock(g, 0, f.order + 1, f.degree, f.terms, nullptr, -1, 0, f.orderStart, f.diag, f.normalized != 0, r,
+                         p, l);
+            shToInertial(br, g, r, p, l, f.gm, bx, by, bz);
+        } else {
+            // J2 (Wakker eq. 20.6) in the body's axes
+            double E[9];
+#pragma unroll
+            for (int j = 0; j < 9; ++j) E[j] = br(REC_R + j);
+            const double fx = E[0] * dx + E[1] * dy + E[2] * dz;
+            const double fy = E[3] * dx + E[4] * dy + E[5] * dz;
+            const double fz = E[6] * dx + E[7] * dy + E[8] * dz;
+            const double invr = 1.0 / sqrt(fx * fx + fy * fy + fz * fz);
+            const double invr2 = invr * invr;
+            const double invr5 = invr2 * invr2 * invr;
+            const double v = 1.0 - 5.0 * (fz * fz) * invr2;
+            const double c = -1.5 * f.gm * f.j2 * invr5 * f.radius * f.radius;
+            const double cx = c * fx * v, cy = c * fy * v, cz = c * fz * (v + 2.0);
+            bx = E[0] * cx + E[3] * cy + E[6] * cz;
+            by = E[1] * cx + E[4] * cy + E[7] * cz;
+            bz = E[2] * cx + E[5] * cy + E[8] * cz;
+        }
+        ax += bx;
+        ay += by;
+        az += bz;
+    }
+}
+
+// Central and third-body point gravity (PointGravity.h:16-81), then (ROT) the
+// fields of the bodies other than the Earth; the indirect third-body term is an
+// epoch-only sum taken from the record. ROT = false leaves the field code out
+// of the kernel (the launchers choose it when no such field is configured).
+template <bool ROT = true, class Rec>
 __device__ void pointGravity(const ForceModel& fm, const Rec& rec, double px, double py, double pz, double& ax, double& ay,
                              double& az) {
     ax = ay = az = 0.0;
@@ -845,6 +932,8 @@ __device__ void pointGravity(const ForceModel& fm, const Rec& rec, double px, do
     // a few bodies (Earth, Sun, Moon, one more) or every slot the record holds
     if (fm.nBodies <= 4) bodySlots<4>(fm, rec, px, py, pz, ax, ay, az);
     else bodySlots<kMaxBodies>(fm, rec, px, py, pz, ax, ay, az);
+    if constexpr (ROT)
+        for (int k = 0; k < fm.nRot; ++k) rotFieldAccel(rotField(fm, k), k, rec, px, py, pz, ax, ay, az);
     ax += rec(REC_INDIRECT + 0);
     ay += rec(REC_INDIRECT + 1);
     az += rec(REC_INDIRECT + 2);
@@ -1078,7 +1167,7 @@ __device__ inline void stageForces(const StepArgs& a, const ForceModel& fm, cons
 // (`extras`) evaluates the other families, and wave 0 returns the raw sums:
 // the workgroup's SH components (radial, latitude, longitude) in rpl, point
 // gravity in ax and the corrections + SRP sum in ex.
-template <int W, bool FINAL = true, class Rec>
+template <int W, bool FINAL = true, bool ROT = true, class Rec>
 __device__ inline void stageForcesParts(const StepArgs& a, const ForceModel& fm, const Rec& rec, int wave, int lane, int m0,
                                         int m1, double ams, double (*xch)[kTile], double (*part)[3][kTile],
                                         double (*extra)[3][kTile], double (*pg)[kTile], double acc[3], double ax[3],
@@ -1090,7 +1179,7 @@ __device__ inline void stageForcesParts(const StepArgs& a, const ForceModel& fm,
     if (!extras) {
     } else if (wave == W - 1) {
         double gx, gy, gz;
-        pointGravity(fm, rec, px, py, pz, gx, gy, gz);
+        pointGravity<ROT>(fm, rec, px, py, pz, gx, gy, gz);
         pg[0][lane] = gx;
         pg[1][lane] = gy;
         pg[2][lane] = gz;
@@ -1438,7 +1527,7 @@ __device__ inline void splitStageState(const StepArgs& a, const SplitArgs& sp, i
     }
 }
 
-template <int W, bool GRID>
+template <int W, bool GRID, bool ROT = true>
 __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
     __shared__ double xch[7][kTile];
     __shared__ double recF[GRID ? kRecStaged : 1][kTile];   // per-lane epochs: staged record fields
@@ -1469,12 +1558,12 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
             nrlLaneStore(L, sp.lane, nPad, obj);
             nrlLaneStoreGrid(sec, L.in.doy, row, sp.lane, nPad, obj);
         }
-        stageForcesParts<W>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex);
+        stageForcesParts<W, true, ROT>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex);
     } else {
         const double* recp = a.records + (a.tileRecord[tile] + splitStep(sp) * kStages + s) * REC_SIZE;
         const ExactRec rec{recp};
         if (fm.drag && wave == W - 4) nrlLaneStore(nrlLane(fm, rec, recp + REC_NRL, px, py, pz), sp.lane, nPad, obj);
-        stageForcesParts<W>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex);
+        stageForcesParts<W, true, ROT>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex);
     }
     if (wave == 0) {
         for (int d = 0; d < 3; ++d) {
@@ -1941,7 +2030,7 @@ __global__ void __launch_bounds__(64) forceProbeKernel(int n, const double* pos,
     double ax = 0.0, ay = 0.0, az = 0.0, bx, by, bz;
     if (which < 0 || which == 0 || which == 2) {
         ForceModel f = fm;
-        if (which == 0) f.nBodies = 0;        // central term only
+        if (which == 0) f.nBodies = f.nRot = 0;   // central term only
         if (which == 2) f.earthPointGravity = 0;  // third bodies only
         pointGravity(f, rec, px, py, pz, bx, by, bz);
         if (which == 0) {
@@ -2116,14 +2205,20 @@ static int launchSplitSh(const StepArgs& a, const ForceModel& fm, const SplitArg
         switch (shWaves) {
             case 4: hipLaunchKernelGGL((splitShKernel<4, true>), grid, dim3(256), 0, stream, a, fm, sp); break;
             case 8: hipLaunchKernelGGL((splitShKernel<8, true>), grid, dim3(512), 0, stream, a, fm, sp); break;
-            case 16: hipLaunchKernelGGL((splitShKernel<16, true>), grid, dim3(1024), 0, stream, a, fm, sp); break;
+            case 16:
+                if (fm.nRot == 0) hipLaunchKernelGGL((splitShKernel<16, true, false>), grid, dim3(1024), 0, stream, a, fm, sp);
+                else hipLaunchKernelGGL((splitShKernel<16, true>), grid, dim3(1024), 0, stream, a, fm, sp);
+                break;
             default: return -2;
         }
     } else {
         switch (shWaves) {
             case 4: hipLaunchKernelGGL((splitShKernel<4, false>), grid, dim3(256), 0, stream, a, fm, sp); break;
             case 8: hipLaunchKernelGGL((splitShKernel<8, false>), grid, dim3(512), 0, stream, a, fm, sp); break;
-            case 16: hipLaunchKernelGGL((splitShKernel<16, false>), grid, dim3(1024), 0, stream, a, fm, sp); break;
+            case 16:
+                if (fm.nRot == 0) hipLaunchKernelGGL((splitShKernel<16, false, false>), grid, dim3(1024), 0, stream, a, fm, sp);
+                else hipLaunchKernelGGL((splitShKernel<16, false>), grid, dim3(1024), 0, stream, a, fm, sp);
+                break;
             default: return -2;
         }
     }

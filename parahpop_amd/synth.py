@@ -113,11 +113,10 @@ def write_targets(path, pop, epochs, ids=None):
         json.dump({"description": "synthetic population", "targets": targets}, f, separators=(",", ":"))
 
 
-def write_gravity(path, degree, seed, kaula=False):
-    """Fully normalized PDS-SHA field: C20 = -4.8417e-4, other C/S ~ N(0, (s/n^2)^2), S_n0 = 0."""
+def write_gravity(path, degree, seed, kaula=False, radius=R_FIELD, gm=GM_EARTH, c20=-4.8417e-4, scale=1e-5):
+    """Fully normalized PDS-SHA field: C20 = c20, other C/S ~ N(0, (scale/n^2)^2), S_n0 = 0."""
     rng = np.random.default_rng(seed)
-    scale = 1e-5
-    lines = ["%.10f, %.10f, 0.0, %d, %d, 1, 0.0, 0.0" % (R_FIELD, GM_EARTH, degree, degree)]
+    lines = ["%.10f, %.10f, 0.0, %d, %d, 1, 0.0, 0.0" % (radius, gm, degree, degree)]
     for n in range(1, degree + 1):
         for m in range(n + 1):
             if n == 1:
@@ -127,7 +126,7 @@ def write_gravity(path, degree, seed, kaula=False):
                 c = rng.normal(0.0, sig)
                 s = rng.normal(0.0, sig) if m > 0 else 0.0
                 if n == 2 and m == 0:
-                    c = -4.8417e-4
+                    c = c20
             lines.append("%d, %d, %.17e, %.17e" % (n, m, c, s))
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
@@ -268,6 +267,69 @@ def make_alternate_workload(out, n=16384, duration=86400.0, step=60.0, seed=2, m
     write_config(os.path.join(out, "CONFIG.json"), model or alternate_model(), duration, step, max_steps)
     write_targets(os.path.join(out, "TARGETS.json"), pop, epochs)
     return os.path.join(out, "CONFIG.json"), os.path.join(out, "TARGETS.json")
+
+
+GM_MOON = 4902.800192171393
+R_MOON_FIELD = 1738.0
+
+
+def moon_pa_euler(et, libration=0.01):
+    """3-1-3 Euler angles (phi, theta, psi; radians) of a stand-in lunar
+    principal-axis frame: the IAU model's (pi/2 + RA, pi/2 - Dec, W) without
+    nutation-precession terms, plus a 27.3-day libration of `libration` rad in
+    each angle."""
+    T = et / 3155760000.0
+    d = et / 86400.0
+    a = np.radians(269.9949 + 0.0031 * T)
+    dl = np.radians(66.5392 + 0.0130 * T)
+    w = np.radians(38.3213 + 13.17635815 * d)
+    lib = libration * np.sin(2.0 * np.pi * d / 27.321661)
+    return np.array([np.pi / 2 + a + lib, np.pi / 2 - dl + 0.5 * lib, w - lib])
+
+
+def moon_ecliptic_euler(et):
+    """3-1-3 Euler angles from ECLIPJ2000 of a stand-in lunar frame: the node of
+    the lunar equator on the ecliptic (regressing in 18.6 years), its 1.5424 deg
+    inclination and the rotation angle."""
+    d = et / 86400.0
+    node = np.radians(125.045 - 0.0529921 * d)
+    return np.array([node, np.radians(1.5424), np.radians(180.0 + 38.3213 + 13.17635815 * d) - node])
+
+
+def make_lunar_workload(out, field="sh", frame="iau", degree=8, duration=7200.0, step=30.0, seed=3):
+    """A model with a field of the Moon in its own axes: Earth (point mass with
+    a degree-4 field), Sun and Moon point masses, the Moon's degree-`degree`
+    coefficient field ("sh") or J2 ("j2"), its axes from a .brot2 file holding
+    the IAU Moon model (types 1 and 2) and, for frame "moon_pa" / "moon_pa_ecl",
+    a MOON_PA binary frame (type 3, ICRF or ECLIPJ2000 base) that the body
+    resolves to (frame "iau_named" names IAU_MOON explicitly). Returns the
+    CONFIG.json path; the targets are the caller's."""
+    from . import brie_io
+    os.makedirs(out, exist_ok=True)
+    with open(os.path.join(out, "constants.json"), "w") as f:
+        json.dump(CONSTANTS, f, indent=1)
+    write_gravity(os.path.join(out, "earth4.txt"), 4, seed)
+    write_gravity(os.path.join(out, "moon.txt"), degree, seed + 1, radius=R_MOON_FIELD, gm=GM_MOON, c20=-9.09e-5,
+                  scale=2e-5)
+    units = brie_io.moon_iau_units()
+    et0 = (EPOCH0 - 2.0) * 86400.0 - 43200.0
+    if frame in ("moon_pa", "moon_pa_ecl"):
+        euler = moon_pa_euler if frame == "moon_pa" else moon_ecliptic_euler
+        units.append(brie_io.binary_frame_unit(31008, euler, et0, et0 + 6 * 86400.0, 86400.0 / 4, 12,
+                                               inert_frame=1 if frame == "moon_pa" else 17))
+    brie_io.write_brot2_units(os.path.join(out, "moon.brot2"), units)
+    moon = {"gravity": {"shape": {"body": "Moon", "file": "moon.txt", "delimiter": ",", "headerLines": 0,
+                                  "degree": degree, "order": degree}} if field == "sh" else {"oblateness": True}}
+    if frame == "iau_named":
+        moon["orientation"] = "IAU_MOON"
+    bodies = {"Earth": {"gravity": {"shape": {"body": "Earth", "file": "earth4.txt", "delimiter": ",", "headerLines": 0,
+                                              "degree": 4, "order": 4}}},
+              "Sun": {"gravity": True}, "Moon": moon}
+    model = {"environment": {"bodies": bodies, "ephemeris": "analytic", "orientations": ["moon.brot2"],
+                             "constants": "constants.json", "preferSSB": True}}
+    cfg = os.path.join(out, "CONFIG.json")
+    write_config(cfg, model, duration, step, int(math.ceil(duration / step)) + 2)
+    return cfg
 
 
 def central_model():

@@ -99,7 +99,13 @@ def test_orientation_envelope_and_unit_errors(tmp_path):
     with pytest.raises(RuntimeError, match="orientation range"):
         o.propagate(t, 5 * DAY, threads=1)
     with open(tmp_path / "empty.brot2", "wb") as f:
+        f.write(brie_writer.cbor({"layout": 2, "core": {"metadata": {"nBodyUnits": 1, "intMetadata": [5] + [0] * 11,
+                                                                     "doubleMetadata": [0.0, 1.0]}, "data": [0.0]}}))
+    with pytest.raises(RuntimeError, match="no IERS2000 or body orientation unit"):
+        oracle.Oracle(_config(str(tmp_path / "e"), ["../empty.brot2"]))
+    # a body unit whose sizes exceed its data
+    with open(tmp_path / "short.brot2", "wb") as f:
         f.write(brie_writer.cbor({"layout": 2, "core": {"metadata": {"nBodyUnits": 1, "intMetadata": [1] + [0] * 11,
                                                                      "doubleMetadata": [0.0, 1.0]}, "data": [0.0]}}))
-    with pytest.raises(RuntimeError, match="no IERS2000 orientation unit"):
-        oracle.Oracle(_config(str(tmp_path / "e"), ["../empty.brot2"]))
+    with pytest.raises(RuntimeError, match="inconsistent sizes"):
+        oracle.Oracle(_config(str(tmp_path / "s"), ["../short.brot2"]))

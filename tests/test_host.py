@@ -85,7 +85,9 @@ def _create_error(path):
     ({"environment": {"bodies": {"Earth": {"atmosphere": {"jacchia": {}}}}, "constants": "constants.json"}},
      "Unknown atmosphere model"),
     ({"environment": {"bodies": {"Earth": {"gravity": True}, "Moon": {"gravity": {"oblateness": True}}},
-                      "constants": "constants.json"}}, "J2 (oblateness) is supported for Earth only"),
+                      "constants": "constants.json"}}, "needs its rotation data"),
+    ({"environment": {"bodies": {"Earth": {"gravity": True}, "Moon": {"gravity": {"oblateness": True}, "radius": 0.0}},
+                      "constants": "constants.json"}}, "Body field needs the body radius"),
     ({"environment": {"bodies": {"Earth": {"gravity": ["oblateness", "rings"]}}, "constants": "constants.json"}},
      "Unsupported gravity feature"),
     ({"environment": {"bodies": {"Earth": {"gravity": True, "flattening": 1.0}}, "constants": "constants.json"}},
