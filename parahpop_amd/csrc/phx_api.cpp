@@ -130,6 +130,22 @@ void computeLeaps(phx_ctx& c) {
     c.leapCount = int(at.size());
 }
 
+// The coefficients the NRLMSISE-00 chains read one at a time, gathered behind
+// the table in the kernels' HOT_* layout: pdl, ptm and pdm whole (contiguous in
+// the table), pt[0..14], ps[0], pd[k][0], ptl[k][0], pma[8][0].
+void appendNrlHot(std::vector<double>& v) {
+    // table offsets of the blocks (the reference's coefficient file order)
+    constexpr int pt = 0, pd = 150, ps = 1500, pdl = 1650, ptl = 1790, pma = 2190;
+    std::vector<double> h(v.begin() + pdl, v.begin() + pdl + 140);
+    for (int i = 0; i < 15; ++i) h.push_back(v[size_t(pt + i)]);
+    h.push_back(v[size_t(ps)]);
+    for (int k = 0; k < 9; ++k) h.push_back(v[size_t(pd + 150 * k)]);
+    for (int k = 0; k < 4; ++k) h.push_back(v[size_t(ptl + 100 * k)]);
+    h.push_back(v[size_t(pma + 100 * 8)]);
+    if (h.size() != 170) throw std::logic_error("NRLMSISE-00 gathered coefficient block");
+    v.insert(v.end(), h.begin(), h.end());
+}
+
 void loadTables(phx_ctx& c) {
     const std::string dir = c.dataDir;
     if (c.env.model.tidesActive()) {
@@ -190,8 +206,9 @@ void loadTables(phx_ctx& c) {
             while (s >> x) v.push_back(x);
         }
         if (v.size() != 3300) throw std::runtime_error("NRLMSISE-00 coefficient table must hold 3300 values");
-        c.nrl.upload(v.data(), v.size(), c.stream);
         if (uploadGlobeTrig(v.data()) != 0) throw HipError("globe phase upload failed");
+        appendNrlHot(v);
+        c.nrl.upload(v.data(), v.size(), c.stream);
         const WeatherTable& wt = c.env.weather;
         c.weather.upload(wt.data.data(), wt.data.size(), c.stream);
     }
@@ -1538,6 +1555,7 @@ int phx_nrlmsise_probe(phx_ctx* c, size_t n, const double* in, int useAp, double
                 while (s >> x) v.push_back(x);
             }
             if (v.size() != 3300) throw std::runtime_error("NRLMSISE-00 coefficient table must hold 3300 values");
+            appendNrlHot(v);
             c->nrl.upload(v.data(), v.size(), c->stream);
         }
         DBuf<double> di, dout;

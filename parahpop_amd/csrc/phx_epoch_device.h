@@ -625,7 +625,7 @@ __device__ void nrlEpoch(const EpochArgs& a, double et, double* dst, double* glo
     weatherInputs(a, record, slot, rec + REC_NRL_F107);
     rec[REC_NRL_VALID] = 1.0;
     if (globe && a.nrlCoeff) {
-        nrlmsise::NrlTables T{a.nrlCoeff};
+        nrlmsise::NrlTables T = nrlmsise::nrlTables(a.nrlCoeff);
         nrlmsise::globeEpoch(T, doy, rec[REC_NRL_SEC], rec[REC_NRL_F107], rec[REC_NRL_F107A], rec + REC_NRL_AP, globe);
         nrlmsise::lowerEpoch(T, doy, globe + nrlmsise::GE_COUNT);
     }
@@ -643,7 +643,7 @@ __device__ void nrlSlotRow(const EpochArgs& a, int record, int slot, double* row
     double sec;
     doySecOf(mjd, doy, sec);
     weatherInputs(a, record, slot, row + SLOT_F107);
-    NrlTables T{a.nrlCoeff};
+    NrlTables T = nrlTables(a.nrlCoeff);
     globeSlot(T, doy, row[SLOT_F107], row[SLOT_F107A], row + SLOT_AP, row);
     row[SLOT_DOY] = double(doy);
     row[SLOT_VALID] = 1.0;

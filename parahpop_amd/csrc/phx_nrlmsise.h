@@ -27,10 +27,31 @@ enum : int {
     OFF_PTL = 1790, OFF_PMA = 2190, OFF_SAM = 3190, OFF_PAVGM = 3290, COEFF_COUNT = 3300
 };
 
+// Coefficients that gts7's prefix, the shared densu quantities and the species
+// read one at a time on their dependency chains, gathered after the table
+// (COEFF_COUNT values, then these): pdl, ptm and pdm whole, pt[0..14], ps[0] and
+// the first entries of the pd, ptl and pma[8] blocks.
+enum : int {
+    HOT_PDL = 0,      // pdl[2][25]
+    HOT_PTM = 50,     // ptm[10]
+    HOT_PDM = 60,     // pdm[8][10]
+    HOT_PT = 140,     // pt[0..14]
+    HOT_PS0 = 155,    // ps[0]
+    HOT_PD0 = 156,    // pd[k][0], k = 0..8
+    HOT_PTL0 = 165,   // ptl[k][0], k = 0..3
+    HOT_PMA80 = 169,  // pma[8][0]
+    HOT_COUNT = 170
+};
+
+// `hot` is the gathered block: behind the table in device memory, or a copy in
+// LDS (the drag tile kernel's), read on the chains instead of scalar loads.
 struct NrlTables {
     const double* base;
+    const double* hot;
     __device__ const double* pt() const { return base + OFF_PT; }
 };
+
+__device__ inline NrlTables nrlTables(const double* coeff) { return NrlTables{coeff, coeff + COEFF_COUNT}; }
 
 
 
@@ -830,39 +851,40 @@ struct Pre {
 
 template <class Globe>
 __device__ inline void gts7Prefix(State& st, const NrlTables& T, Input& in, const Flags& fl, const Globe& G, Pre& pre) {
+    const double* H = T.hot;
     double zn1[5] = {120.0, 110.0, 100.0, 90.0, 72.5};
     const double dgtr = 1.74533E-2, dr = 1.72142E-2;
-    const double za = ((NrlCD)(T.base+OFF_PDL+25*(1)))[15];
+    const double za = H[HOT_PDL + 25 + 15];
     zn1[0] = za;
     st.g0Valid = false;
     G.basis(st, in, fl);
     double tinf;
-    if (in.alt > zn1[0]) tinf = ((NrlCD)(T.base+OFF_PTM))[0] * ((NrlCD)(T.base+OFF_PT))[0] * (1.0 + fl.sw[16] * G(st, 0, (T.base+OFF_PT), in, fl));
-    else tinf = ((NrlCD)(T.base+OFF_PTM))[0] * ((NrlCD)(T.base+OFF_PT))[0];
+    if (in.alt > zn1[0]) tinf = H[HOT_PTM + 0] * H[HOT_PT + 0] * (1.0 + fl.sw[16] * G(st, 0, (T.base+OFF_PT), in, fl));
+    else tinf = H[HOT_PTM + 0] * H[HOT_PT + 0];
     double g0v;
-    if (in.alt > zn1[4]) g0v = ((NrlCD)(T.base+OFF_PTM))[3] * ((NrlCD)(T.base+OFF_PS))[0] * (1.0 + fl.sw[19] * G(st, 1, (T.base+OFF_PS), in, fl));
-    else g0v = ((NrlCD)(T.base+OFF_PTM))[3] * ((NrlCD)(T.base+OFF_PS))[0];
-    const double tlb = ((NrlCD)(T.base+OFF_PTM))[1] * (1.0 + fl.sw[17] * G(st, 5, (T.base+OFF_PD+150*(3)), in, fl)) * ((NrlCD)(T.base+OFF_PD+150*(3)))[0];
+    if (in.alt > zn1[4]) g0v = H[HOT_PTM + 3] * H[HOT_PS0] * (1.0 + fl.sw[19] * G(st, 1, (T.base+OFF_PS), in, fl));
+    else g0v = H[HOT_PTM + 3] * H[HOT_PS0];
+    const double tlb = H[HOT_PTM + 1] * (1.0 + fl.sw[17] * G(st, 5, (T.base+OFF_PD+150*(3)), in, fl)) * H[HOT_PD0 + 3];
     const double s = g0v / (tinf - tlb);
     if (in.alt < 300.0) {
-        st.meso_tn1[1] = ((NrlCD)(T.base+OFF_PTM))[6] * ((NrlCD)(T.base+OFF_PTL+100*(0)))[0] / (1.0 - fl.sw[18] * G.lower(st, 0, (T.base+OFF_PTL+100*(0)), in, fl));
-        st.meso_tn1[2] = ((NrlCD)(T.base+OFF_PTM))[2] * ((NrlCD)(T.base+OFF_PTL+100*(1)))[0] / (1.0 - fl.sw[18] * G.lower(st, 1, (T.base+OFF_PTL+100*(1)), in, fl));
-        st.meso_tn1[3] = ((NrlCD)(T.base+OFF_PTM))[7] * ((NrlCD)(T.base+OFF_PTL+100*(2)))[0] / (1.0 - fl.sw[18] * G.lower(st, 2, (T.base+OFF_PTL+100*(2)), in, fl));
-        st.meso_tn1[4] = ((NrlCD)(T.base+OFF_PTM))[4] * ((NrlCD)(T.base+OFF_PTL+100*(3)))[0] / (1.0 - fl.sw[18] * fl.sw[20] * G.lower(st, 3, (T.base+OFF_PTL+100*(3)), in, fl));
-        st.meso_tgn1[1] = ((NrlCD)(T.base+OFF_PTM))[8] * ((NrlCD)(T.base+OFF_PMA+100*(8)))[0] * (1.0 + fl.sw[18] * fl.sw[20] * G.lower(st, 4, (T.base+OFF_PMA+100*(8)), in, fl)) * st.meso_tn1[4] *
-                          st.meso_tn1[4] / (pow((((NrlCD)(T.base+OFF_PTM))[4] * ((NrlCD)(T.base+OFF_PTL+100*(3)))[0]), 2.0));
+        st.meso_tn1[1] = H[HOT_PTM + 6] * H[HOT_PTL0 + 0] / (1.0 - fl.sw[18] * G.lower(st, 0, (T.base+OFF_PTL+100*(0)), in, fl));
+        st.meso_tn1[2] = H[HOT_PTM + 2] * H[HOT_PTL0 + 1] / (1.0 - fl.sw[18] * G.lower(st, 1, (T.base+OFF_PTL+100*(1)), in, fl));
+        st.meso_tn1[3] = H[HOT_PTM + 7] * H[HOT_PTL0 + 2] / (1.0 - fl.sw[18] * G.lower(st, 2, (T.base+OFF_PTL+100*(2)), in, fl));
+        st.meso_tn1[4] = H[HOT_PTM + 4] * H[HOT_PTL0 + 3] / (1.0 - fl.sw[18] * fl.sw[20] * G.lower(st, 3, (T.base+OFF_PTL+100*(3)), in, fl));
+        st.meso_tgn1[1] = H[HOT_PTM + 8] * H[HOT_PMA80] * (1.0 + fl.sw[18] * fl.sw[20] * G.lower(st, 4, (T.base+OFF_PMA+100*(8)), in, fl)) * st.meso_tn1[4] *
+                          st.meso_tn1[4] / (pow((H[HOT_PTM + 4] * H[HOT_PTL0 + 3]), 2.0));
     } else {
-        st.meso_tn1[1] = ((NrlCD)(T.base+OFF_PTM))[6] * ((NrlCD)(T.base+OFF_PTL+100*(0)))[0];
-        st.meso_tn1[2] = ((NrlCD)(T.base+OFF_PTM))[2] * ((NrlCD)(T.base+OFF_PTL+100*(1)))[0];
-        st.meso_tn1[3] = ((NrlCD)(T.base+OFF_PTM))[7] * ((NrlCD)(T.base+OFF_PTL+100*(2)))[0];
-        st.meso_tn1[4] = ((NrlCD)(T.base+OFF_PTM))[4] * ((NrlCD)(T.base+OFF_PTL+100*(3)))[0];
-        st.meso_tgn1[1] = ((NrlCD)(T.base+OFF_PTM))[8] * ((NrlCD)(T.base+OFF_PMA+100*(8)))[0] * st.meso_tn1[4] * st.meso_tn1[4] / (pow((((NrlCD)(T.base+OFF_PTM))[4] * ((NrlCD)(T.base+OFF_PTL+100*(3)))[0]), 2.0));
+        st.meso_tn1[1] = H[HOT_PTM + 6] * H[HOT_PTL0 + 0];
+        st.meso_tn1[2] = H[HOT_PTM + 2] * H[HOT_PTL0 + 1];
+        st.meso_tn1[3] = H[HOT_PTM + 7] * H[HOT_PTL0 + 2];
+        st.meso_tn1[4] = H[HOT_PTM + 4] * H[HOT_PTL0 + 3];
+        st.meso_tgn1[1] = H[HOT_PTM + 8] * H[HOT_PMA80] * st.meso_tn1[4] * st.meso_tn1[4] / (pow((H[HOT_PTM + 4] * H[HOT_PTL0 + 3]), 2.0));
     }
     pre.tinf = tinf;
     pre.tlb = tlb;
     pre.s = s;
-    pre.zhf = ((NrlCD)(T.base+OFF_PDL+25*(1)))[24] * (1.0 + fl.sw[5] * ((NrlCD)(T.base+OFF_PDL+25*(0)))[24] * sin(dgtr * in.g_lat) * cos(dr * (in.doy - ((NrlCD)(T.base+OFF_PT))[13])));
-    pre.xmm = ((NrlCD)(T.base+OFF_PDM+10*(2)))[4];
+    pre.zhf = H[HOT_PDL + 25 + 24] * (1.0 + fl.sw[5] * H[HOT_PDL + 24] * sin(dgtr * in.g_lat) * cos(dr * (in.doy - H[HOT_PT + 13])));
+    pre.xmm = H[HOT_PDM + 24];
     for (int k = 0; k < 5; ++k) pre.zn1[k] = zn1[k];
 }
 
@@ -1035,7 +1057,7 @@ __device__ inline double densuAt(const DensuShared& h, State& st, double alt, do
 // The lane part of the shared quantities alone (densuShareLane's fields and the
 // common ones), for a wave beside the one that runs gts7SpeciesShare<false>.
 __device__ inline void gts7ShareLane(DensuShared& h, const State& st, const NrlTables& T, const Input& in, const Pre& pre) {
-    densuShareCommon(h, st, in.alt, pre.tinf, pre.tlb, pre.s, ((NrlCD)(T.base + OFF_PTM))[5], pre.xmm, pre.zn1);
+    densuShareCommon(h, st, in.alt, pre.tinf, pre.tlb, pre.s, T.hot[HOT_PTM + 5], pre.xmm, pre.zn1);
     densuShareLane(h, st);
 }
 
@@ -1045,19 +1067,19 @@ __device__ __attribute__((always_inline)) inline void gts7SpeciesShare(DensuShar
                                                                        const NrlTables& T, Input& in, const Flags& fl,
                                                                        const Globe& G, const Pre& pre) {
     double zn1[5] = {pre.zn1[0], pre.zn1[1], pre.zn1[2], pre.zn1[3], pre.zn1[4]};
-    const NrlCD pdm = (NrlCD)(T.base + OFF_PDM);
+    const double* pdm = T.hot + HOT_PDM;
     const double z = in.alt;
     if constexpr (LANE) {
-        densuShare(h, st, z, pre.tinf, pre.tlb, pre.s, ((NrlCD)(T.base + OFF_PTM))[5], pre.xmm, zn1, st.meso_tn1, st.meso_tgn1);
+        densuShare(h, st, z, pre.tinf, pre.tlb, pre.s, T.hot[HOT_PTM + 5], pre.xmm, zn1, st.meso_tn1, st.meso_tgn1);
     } else {
-        densuShareCommon(h, st, z, pre.tinf, pre.tlb, pre.s, ((NrlCD)(T.base + OFF_PTM))[5], pre.xmm, zn1);
+        densuShareCommon(h, st, z, pre.tinf, pre.tlb, pre.s, T.hot[HOT_PTM + 5], pre.xmm, zn1);
         densuShareLower(h, st, zn1, st.meso_tn1, st.meso_tgn1);
         h.db28 = h.b28 = 0.0;
     }
     const bool wantB28 = !(z > 450.0 && z >= zn1[0]);
     if (wantB28 || needDb28) {
         const double g28 = fl.sw[21] * G(st, 4, (T.base + OFF_PD + 150 * (2)), in, fl);
-        h.db28 = pdm[20] * exp(g28) * ((NrlCD)(T.base + OFF_PD + 150 * (2)))[0];
+        h.db28 = pdm[20] * exp(g28) * T.hot[HOT_PD0 + 2];
     }
     if (wantB28) {
         const double zh28 = pdm[22] * pre.zhf;
@@ -1075,12 +1097,12 @@ __device__ __attribute__((always_inline)) inline double gts7Species(int k, const
     double tz;
     double* tn1 = st.meso_tn1;
     double* tgn1 = st.meso_tgn1;
-    const NrlCD pdm = (NrlCD)(T.base + OFF_PDM);   // pdm[row][i] = pdm[10 row + i]
-    const NrlCD pdl = (NrlCD)(T.base + OFF_PDL);   // pdl[blk][i] = pdl[25 blk + i]
-    const double zlbT = ((NrlCD)(T.base + OFF_PTM))[5];
+    const double* pdm = T.hot + HOT_PDM;   // pdm[row][i] = pdm[10 row + i]
+    const double* pdl = T.hot + HOT_PDL;   // pdl[blk][i] = pdl[25 blk + i]
+    const double zlbT = T.hot[HOT_PTM + 5];
     if (k == 7) {  // anomalous O (d[8])
         const double g16h = fl.sw[21] * G(st, 10, (T.base + OFF_PD + 150 * (8)), in, fl);
-        const double db16h = pdm[70] * exp(g16h) * ((NrlCD)(T.base + OFF_PD + 150 * (8)))[0];
+        const double db16h = pdm[70] * exp(g16h) * T.hot[HOT_PD0 + 8];
         const double tho = pdm[79] * pdl[6];
         const double ddo = densu(st, z, db16h, tho, tho, 16., 0.0, &tz, zlbT, s, mn1, zn1, tn1, tgn1);
         const double zsht = pdm[75];
@@ -1090,13 +1112,12 @@ __device__ __attribute__((always_inline)) inline double gts7Species(int k, const
     }
     const SpeciesPar P = kSpeciesPar[k];
     const double* pdkG = T.base + OFF_PD + 150 * P.pd;
-    const NrlCD pdk = (NrlCD)pdkG;
-    const NrlCD pdmk = pdm + 10 * P.row;
+    const double* pdmk = pdm + 10 * P.row;
     const double b28 = h.b28;
     double dbk = h.db28;
     if (k != 0) {
         const double gk = fl.sw[21] * G(st, P.globe, pdkG, in, fl);
-        dbk = pdmk[0] * exp(gk) * pdk[0];
+        dbk = pdmk[0] * exp(gk) * T.hot[HOT_PD0 + P.pd];
     }
     double d = densuLane(h, st, dbk, P.xm, P.alpha, zn1, tn1, tgn1);
     const double zhm28 = pdm[23] * pdl[25 + 5];

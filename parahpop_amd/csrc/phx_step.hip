@@ -606,7 +606,7 @@ __device__ __attribute__((always_inline)) inline void nrlSpecies(const NrlLane& 
     nrlmsise::Input in = L.in;
     const nrlmsise::Flags fl = nrlFlags();
     nrlmsise::State st;
-    const nrlmsise::NrlTables T{coeff};
+    const nrlmsise::NrlTables T = nrlmsise::nrlTables(coeff);
     nrlmsise::Pre pre;
     // gtd7's gravity/radius at this latitude (switch 2 on), Nrlmsise00Model.inl:976-980
     nrlmsise::glatf(in.g_lat, st.gsurf, st.re);
@@ -651,7 +651,7 @@ __device__ __noinline__ double nrlDensity(double densityScale, const double* nrl
     fl.switches[9] = -1;
     nrlmsise::Output out;
     nrlmsise::State st;
-    nrlmsise::NrlTables T{coeff};
+    nrlmsise::NrlTables T = nrlmsise::nrlTables(coeff);
     if (MODE == 1) nrlmsise::gtd7d(st, T, in, fl, out, nrlmsise::FastGlobe{nrl + (REC_NRL_GLOBE - REC_NRL), kGlobeTrig});
     else nrlmsise::gtd7d(st, T, in, fl, out, nrlmsise::GenericGlobe{});
     if (!isfinite(out.d[5]) || out.d[5] < 0.0) return nan("");
@@ -666,7 +666,7 @@ __device__ __noinline__ double nrlDensityGlobe(double densityScale, const double
     nrlmsise::Flags fl = nrlFlags();
     nrlmsise::Output out;
     nrlmsise::State st;
-    nrlmsise::NrlTables T{coeff};
+    nrlmsise::NrlTables T = nrlmsise::nrlTables(coeff);
     nrlmsise::gtd7d(st, T, in, fl, out, F);
     if (!isfinite(out.d[5]) || out.d[5] < 0.0) return nan("");
     return (out.d[5] * 1.0e9) * densityScale;
@@ -1793,12 +1793,15 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
     __shared__ double gv[19][kTile];
     __shared__ double spec[8][kTile];
     __shared__ double hx[kDensuSharedFields + 2][kTile];   // wave 0's densu quantities and lower-boundary node
+    __shared__ double hot[nrlmsise::HOT_COUNT];             // the chains' coefficients, read from LDS
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int tile = __builtin_amdgcn_readfirstlane(a.tileList ? a.tileList[blockIdx.x] : int(blockIdx.x));
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
+    // the copy completes before the first barrier below, long before the prefix
+    if (threadIdx.x < nrlmsise::HOT_COUNT) hot[threadIdx.x] = a.nrlCoeff[nrlmsise::COEFF_COUNT + threadIdx.x];
     // shared epochs: the tile's record; per-lane epochs: the lane's time inputs
     // and weather-slot row (the globe terms of its slot)
     const double* recp = GRID ? nullptr : a.records + (a.tileRecord[tile] + splitStep(sp) * kStages + s) * REC_SIZE;
@@ -1811,7 +1814,7 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
     if (__ballot(nl.state == 1) != 0ull) {
         const nrlmsise::Input in0 = nl.in;
         const nrlmsise::Flags fl = nrlFlags();
-        const nrlmsise::NrlTables T{a.nrlCoeff};
+        const nrlmsise::NrlTables T{a.nrlCoeff, hot};
         Globe F;
         if constexpr (GRID) {
             const double sr = 7.2722E-5;
@@ -2092,7 +2095,7 @@ __global__ void __launch_bounds__(64) nrlProbeKernel(int n, const double* in, in
     if (useApArray) fl.switches[9] = -1;
     nrlmsise::Output o;
     nrlmsise::State st;
-    nrlmsise::NrlTables T{coeff};
+    nrlmsise::NrlTables T = nrlmsise::nrlTables(coeff);
     nrlmsise::gtd7d(st, T, x, fl, o, nrlmsise::GenericGlobe{});
     for (int k = 0; k < 9; ++k) out[11 * i + k] = o.d[k];
     out[11 * i + 9] = o.t[0];
