@@ -126,7 +126,8 @@ def run_steps(b, total, chunk):
             acc[key] += st[key]
         for key in ("waves", "steps_per_launch", "kernels_per_step"):
             acc[key] = st[key]
-        acc["graph_replays"] = acc.get("graph_replays", 0.0) + st.get("graph_replays", 0.0)
+        for key in ("graph_replays", "graph_captures"):
+            acc[key] = acc.get(key, 0.0) + st.get(key, 0.0)
         done += k
     return acc
 
@@ -175,7 +176,7 @@ def leg_main(args, work):
         "config": {"workload": "full-force 70x70, 16384 LEO/MEO objects, 60 s step, 1 day",
                    "objects": args.objects, "degree": args.degree, "waves": st["waves"],
                    "steps_per_launch": st["steps_per_launch"], "graph_steps": int(ctx.option("graph_steps")),
-                   "graph_replays": st.get("graph_replays", 0.0),
+                   "graph_replays": st.get("graph_replays", 0.0), "graph_captures": st.get("graph_captures", 0.0),
                    "path": "split-stage" if st["kernels_per_step"] else "fused multi-step"},
         "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "flops_per_object_step": fps},

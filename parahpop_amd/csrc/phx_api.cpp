@@ -938,11 +938,9 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
         // the step index of exact records lives on the device (advanced by the
         // combine kernel), so every step launches the same kernels and arguments
         if (!b.stepDev.p) b.stepDev.alloc(1);
-        // a graph lives for one call (the options it was captured with may change between calls)
-        if (b.graphExec) check(hipGraphExecDestroy(b.graphExec), "hipGraphExecDestroy");
-        if (b.graph) check(hipGraphDestroy(b.graph), "hipGraphDestroy");
-        b.graphExec = nullptr;
-        b.graph = nullptr;
+        // graphs captured under other options have expired
+        for (size_t i = b.graphs.size(); i-- > 0;)
+            if (b.graphs[i].gen != c.optGen) b.dropGraph(i);
         b.stepHost = b.stepsDone;
         check(hipMemcpyAsync(b.stepDev.p, &b.stepHost, sizeof(int), hipMemcpyHostToDevice, c.stream), "step index");
         sp.stepDev = b.stepDev.p;
@@ -982,17 +980,27 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
             sp.famPart = b.famPart.p;
             sp.tileCount = b.tileCount.p;
             // K steps as one graph while the active tiles stay the same for all of
-            // them (the next tile ends no sooner than K steps from here)
+            // them (the next tile ends no sooner than K steps from here); while the
+            // whole batch is active, shorter runs replay graphs of a power of two
+            // steps, captured together with the K-step graph, so that a call of any
+            // length after the first reuses graphs instead of capturing
             const int untilChange = b.tileSteps[size_t(active - 1)] - b.stepsDone;
-            if (K > 0 && remaining >= K && untilChange >= K) {
-                if (!b.graphExec || b.graphActive != active || b.graphG != G || b.graphK != K) {
-                    if (b.graphExec) check(hipGraphExecDestroy(b.graphExec), "hipGraphExecDestroy");
-                    if (b.graph) check(hipGraphDestroy(b.graph), "hipGraphDestroy");
-                    b.graphExec = nullptr;
-                    b.graph = nullptr;
+            const int room = std::min(remaining, untilChange);
+            int k = 0;
+            if (K > 0 && room >= K) k = K;
+            else if (K > 0 && active == b.nTiles)
+                for (int p = 2; p < K && p <= room; p *= 2) k = p;
+            if (k > 0) {
+                auto find = [&](int steps) -> int {
+                    for (size_t i = 0; i < b.graphs.size(); ++i)
+                        if (b.graphs[i].active == active && b.graphs[i].groups == G && b.graphs[i].steps == steps) return int(i);
+                    return -1;
+                };
+                auto capture = [&](int steps) {
+                    if (b.graphs.size() >= phx_batch::kMaxStepGraphs) b.dropGraph(0);
                     check(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
                     int rc = 0;
-                    for (int k = 0; k < K && rc == 0; ++k) {
+                    for (int j = 0; j < steps && rc == 0; ++j) {
                         for (int s = 0; s < kStages && rc == 0; ++s) {
                             sp.stage = s;
                             rc = launchSplitStage(a, c.fm, sp, Ws, c.stream);
@@ -1005,18 +1013,30 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
                         if (g) (void)hipGraphDestroy(g);
                         throw HipError(std::string("split step graph capture failed: ") + hipGetErrorString(e));
                     }
-                    b.graph = g;
-                    check(hipGraphInstantiate(&b.graphExec, b.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-                    b.graphActive = active;
-                    b.graphG = G;
-                    b.graphK = K;
+                    hipGraphExec_t x = nullptr;
+                    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+                    if (ei != hipSuccess) {
+                        (void)hipGraphDestroy(g);
+                        throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+                    }
+                    b.graphs.push_back({active, G, steps, c.optGen, g, x});
                     ++b.graphCaptures;
+                };
+                if (find(k) < 0) {
+                    capture(k);
+                    if (active == b.nTiles) {
+                        if (k != K && find(K) < 0) capture(K);
+                        for (int p = 2; p < K; p *= 2)
+                            if (find(p) < 0) capture(p);
+                    }
                 }
-                check(hipGraphLaunch(b.graphExec, c.stream), "hipGraphLaunch");
+                const int gi = find(k);
+                if (gi < 0) throw HipError("split step graph missing after capture");
+                check(hipGraphLaunch(b.graphs[size_t(gi)].exec, c.stream), "hipGraphLaunch");
                 ++b.graphReplays;
-                launches += K * kernelsPerStep;
-                b.stepsDone += K;
-                remaining -= K;
+                launches += k * kernelsPerStep;
+                b.stepsDone += k;
+                remaining -= k;
                 continue;
             }
             for (int s = 0; s < kStages; ++s) {
@@ -1255,6 +1275,7 @@ int phx_set_option(phx_ctx* c, const char* key, double v) {
         if (k == "serve_slots") c->serveSlots = (long long)v;
         else c->serveRecordMiB = v;
     } else { c->setError("unknown option " + k); return PHX_EINVAL; }
+    ++c->optGen;
     return PHX_OK;
 }
 
@@ -1362,6 +1383,7 @@ int phx_batch_autotune(phx_batch* b, int trialSteps, phx_stats* best) {
         }
         const Cand keep{c.nWaves, c.share0, c.split ? 1 : 0, c.splitWaves, c.splitSpecies};
         auto apply = [&c](const Cand& k) {
+            ++c.optGen;
             c.nWaves = k.waves;
             c.share0 = k.share0;
             c.split = k.split != 0;

@@ -105,6 +105,7 @@ struct phx_ctx {
     int exactGroupsMax = 16;     // more distinct (start, end) groups than this -> grid epochs
     double gridStep = 60.0;      // s, spacing of the epoch grid
     int graphSteps = 16;         // split stages: steps per replayed graph (0: launch every kernel)
+    unsigned long long optGen = 0;   // advanced by every option change (captured step graphs expire)
     int forceGrid = 0;
     int split = 1;               // shared-epoch batches with drag: split stages
     int splitWaves = 16;         // waves per tile of the drag-free split kernel
@@ -148,20 +149,31 @@ struct phx_batch {
     double epochSeconds = 0.0;
     bool grid = false;
     double gridInvH = 0.0;
-    // split stages: the device step index of exact records, and the graph of
-    // graphK steps captured for graphActive tiles and graphG SH workgroups per tile
+    // split stages: the device step index of exact records, and the captured
+    // graphs of `steps` split steps for `active` tiles and `groups` SH
+    // workgroups per tile, valid while the context's option generation is `gen`
+    // (kept across calls; the oldest is dropped beyond kMaxStepGraphs)
     phx::DBuf<int> stepDev;
     int stepHost = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graphExec = nullptr;
-    int graphActive = -1, graphG = 0, graphK = 0;
+    struct StepGraph {
+        int active, groups, steps;
+        unsigned long long gen;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    static constexpr size_t kMaxStepGraphs = 16;
+    std::vector<StepGraph> graphs;
     long long graphReplays = 0, graphCaptures = 0;
     phx_batch() = default;
     phx_batch(const phx_batch&) = delete;
     phx_batch& operator=(const phx_batch&) = delete;
+    void dropGraph(size_t i) {
+        (void)hipGraphExecDestroy(graphs[i].exec);
+        (void)hipGraphDestroy(graphs[i].graph);
+        graphs.erase(graphs.begin() + long(i));
+    }
     ~phx_batch() {
-        if (graphExec) (void)hipGraphExecDestroy(graphExec);
-        if (graph) (void)hipGraphDestroy(graph);
+        while (!graphs.empty()) dropGraph(graphs.size() - 1);
     }
 };
 
