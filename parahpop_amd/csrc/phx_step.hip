@@ -514,6 +514,23 @@ __device__ inline void nrlLaneStore(const NrlLane& L, double* lane, int nPad, in
     lane[size_t(4) * nPad + obj] = double(L.state);
 }
 
+// nrlLaneStore for the one-workgroup SH kernel: the state row always (the drag
+// kernels leave a tile whose lanes are all in state 0 after reading it), the
+// other rows only when a lane of the tile is below the cutoff. Returns whether
+// one is (uniform across the wave).
+__device__ inline bool nrlLaneStoreTile(const NrlLane& L, double* lane, int nPad, int obj) {
+    const bool any = __ballot(L.state != 0) != 0ull;
+    if (any) nrlLaneStore(L, lane, nPad, obj);
+    else lane[size_t(4) * nPad + obj] = 0.0;
+    return any;
+}
+
+// Whether a lane of the tile needs the drag kernels at this stage, from the
+// state row the SH kernel leaves for every tile (uniform across the workgroup).
+__device__ inline bool nrlTileActive(const double* lane, int nPad, int obj) {
+    return __ballot(lane[size_t(4) * nPad + obj] != 0.0) != 0ull;
+}
+
 // nrlLane's result from the kept lane inputs and the stage's epoch record.
 __device__ inline NrlLane nrlLaneLoad(const double* lane, int nPad, int obj, const double* nrl) {
     const double* r = nrl - REC_NRL;
@@ -1465,12 +1482,13 @@ __device__ __attribute__((always_inline)) inline double stageComponent(const Ste
 // finish kernel adds the drag.
 // Stage state of a split SH workgroup: y_s = y + sum_j (dt a_sj) k_j and
 // t_s = t + dt c_s (as the fused kernel), one component per wave, into xch;
-// with `write` also into ys and the velocity k-stage slots. Per-lane epochs:
+// with `write` also into the velocity k-stage slots and with `writeYs` into ys
+// (the drag kernels' copy of the stage state). Per-lane epochs:
 // the waves from 6 on (wave 6 % W when there are fewer) interpolate every
 // record field below the NRLMSISE-00 block at the lane's stage epoch into recF.
 template <int W, bool GRID>
 __device__ inline void splitStageState(const StepArgs& a, const SplitArgs& sp, int wave, int lane, int obj, bool write,
-                                       double (*xch)[kTile], double (*recF)[kTile]) {
+                                       bool writeYs, double (*xch)[kTile], double (*recF)[kTile]) {
     const int nPad = a.nPad;
     const int s = sp.stage;
     for (int d = wave; d < 7; d += W) {
@@ -1484,7 +1502,7 @@ __device__ inline void splitStageState(const StepArgs& a, const SplitArgs& sp, i
             if (s >= 1) v += dt * kRkC[s - 1];
         }
         xch[d][lane] = v;
-        if (write) sp.ys[size_t(d) * nPad + obj] = v;
+        if (writeYs) sp.ys[size_t(d) * nPad + obj] = v;
     }
     if (GRID) {
         // the idle waves' threads walk (lane, field) items field-fastest, so one
@@ -1534,13 +1552,15 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
     __shared__ double part[W][3][kTile];
     __shared__ double extra[2][3][kTile];
     __shared__ double pg[3][kTile];
+    __shared__ int dragTile;   // a lane of the tile needs the drag kernel at this stage
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int tile = __builtin_amdgcn_readfirstlane(a.tileList ? a.tileList[blockIdx.x] : int(blockIdx.x));
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
-    splitStageState<W, GRID>(a, sp, wave, lane, obj, true, xch, recF);
+    if (threadIdx.x == 0) dragTile = 0;
+    splitStageState<W, GRID>(a, sp, wave, lane, obj, true, false, xch, recF);
     __syncthreads();
     const int m0 = a.blockStart[wave], m1 = a.blockStart[wave + 1];
     double acc[3], ax[3], ex[3];
@@ -1555,21 +1575,31 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
             double nrl[13], sec;
             const int row = laneNrlInputs(a, ts, nrl, sec);
             const NrlLane L = nrlLane(fm, rec, nrl, px, py, pz);
-            nrlLaneStore(L, sp.lane, nPad, obj);
-            nrlLaneStoreGrid(sec, L.in.doy, row, sp.lane, nPad, obj);
+            if (nrlLaneStoreTile(L, sp.lane, nPad, obj)) {
+                nrlLaneStoreGrid(sec, L.in.doy, row, sp.lane, nPad, obj);
+                if (lane == 0) dragTile = 1;
+            }
         }
         stageForcesParts<W, true, ROT>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex);
     } else {
         const double* recp = a.records + (a.tileRecord[tile] + splitStep(sp) * kStages + s) * REC_SIZE;
         const ExactRec rec{recp};
-        if (fm.drag && wave == W - 4) nrlLaneStore(nrlLane(fm, rec, recp + REC_NRL, px, py, pz), sp.lane, nPad, obj);
+        if (fm.drag && wave == W - 4 && nrlLaneStoreTile(nrlLane(fm, rec, recp + REC_NRL, px, py, pz), sp.lane, nPad, obj) &&
+            lane == 0)
+            dragTile = 1;
         stageForcesParts<W, true, ROT>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex);
     }
     if (wave == 0) {
-        for (int d = 0; d < 3; ++d) {
-            sp.ks[ksAt(s, 3 + d, nPad, obj)] = acc[d];
-            sp.parts[size_t(d) * nPad + obj] = ax[d];
-            sp.parts[size_t(3 + d) * nPad + obj] = ex[d];
+        // k_s without drag; a tile with a lane below the cutoff also leaves the
+        // stage state and the parts for the drag kernel (which adds the drag to
+        // k_s), a tile without one leaves nothing else
+        for (int d = 0; d < 3; ++d) sp.ks[ksAt(s, 3 + d, nPad, obj)] = acc[d];
+        if (dragTile) {
+            for (int d = 0; d < 7; ++d) sp.ys[size_t(d) * nPad + obj] = xch[d][lane];
+            for (int d = 0; d < 3; ++d) {
+                sp.parts[size_t(d) * nPad + obj] = ax[d];
+                sp.parts[size_t(3 + d) * nPad + obj] = ex[d];
+            }
         }
     }
 }
@@ -1645,7 +1675,7 @@ __global__ void __launch_bounds__(64 * W) splitShGroupKernel(StepArgs a, ForceMo
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
-    splitStageState<W, GRID>(a, sp, wave, lane, obj, g == 0, xch, recF);
+    splitStageState<W, GRID>(a, sp, wave, lane, obj, g == 0, g == 0, xch, recF);
     __syncthreads();
     const int m0 = sp.groupBlockStart[g * W + wave], m1 = sp.groupBlockStart[g * W + wave + 1];
     double acc[3], ax[3] = {0.0, 0.0, 0.0}, ex[3] = {0.0, 0.0, 0.0}, rpl[3] = {0.0, 0.0, 0.0};
@@ -1683,6 +1713,8 @@ __global__ void __launch_bounds__(256) splitDragKernel(StepArgs a, ForceModel fm
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
+    // the stage state is kept for tiles with a lane below the cutoff only
+    if (!nrlTileActive(sp.lane, nPad, obj)) return;
     const double px = sp.ys[obj], py = sp.ys[size_t(nPad) + obj], pz = sp.ys[size_t(2) * nPad + obj];
     const double* recp = a.records + (a.tileRecord[tile] + splitStep(sp) * kStages + s) * REC_SIZE;
     const ExactRec rec{recp};
@@ -1800,6 +1832,8 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
+    // a tile without a lane below the cutoff: k_s is complete (every wave leaves)
+    if (!nrlTileActive(sp.lane, nPad, obj)) return;
     // the copy completes before the first barrier below, long before the prefix
     if (threadIdx.x < nrlmsise::HOT_COUNT) hot[threadIdx.x] = a.nrlCoeff[nrlmsise::COEFF_COUNT + threadIdx.x];
     // shared epochs: the tile's record; per-lane epochs: the lane's time inputs
