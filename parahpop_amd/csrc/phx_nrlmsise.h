@@ -1061,8 +1061,23 @@ __device__ inline void gts7ShareLane(DensuShared& h, const State& st, const NrlT
     densuShareLane(h, st);
 }
 
-// LANE = false leaves densuShareLane's fields to another wave (gts7ShareLane).
-template <class Globe, bool LANE = true>
+// The N2 mixed density at its turbopause height (b28) from the shared
+// lower-thermosphere quantities and db28; the last statement of
+// gts7SpeciesShare, which B28 = false leaves out.
+__device__ inline void gts7ShareB28(DensuShared& h, State& st, const NrlTables& T, const Input& in, const Pre& pre) {
+    double zn1[5] = {pre.zn1[0], pre.zn1[1], pre.zn1[2], pre.zn1[3], pre.zn1[4]};
+    const double* pdm = T.hot + HOT_PDM;
+    const double z = in.alt;
+    const bool wantB28 = !(z > 450.0 && z >= zn1[0]);
+    if (wantB28) {
+        const double zh28 = pdm[22] * pre.zhf;
+        h.b28 = densuAt(h, st, zh28, h.db28, 28.0 - pre.xmm, (0.0 - 1.0), zn1, st.meso_tn1, st.meso_tgn1);
+    }
+}
+
+// LANE = false leaves densuShareLane's fields to another wave (gts7ShareLane);
+// B28 = false leaves b28 to gts7ShareB28.
+template <class Globe, bool LANE = true, bool B28 = true>
 __device__ __attribute__((always_inline)) inline void gts7SpeciesShare(DensuShared& h, bool needDb28, State& st,
                                                                        const NrlTables& T, Input& in, const Flags& fl,
                                                                        const Globe& G, const Pre& pre) {
@@ -1081,39 +1096,37 @@ __device__ __attribute__((always_inline)) inline void gts7SpeciesShare(DensuShar
         const double g28 = fl.sw[21] * G(st, 4, (T.base + OFF_PD + 150 * (2)), in, fl);
         h.db28 = pdm[20] * exp(g28) * T.hot[HOT_PD0 + 2];
     }
-    if (wantB28) {
-        const double zh28 = pdm[22] * pre.zhf;
-        h.b28 = densuAt(h, st, zh28, h.db28, 28.0 - pre.xmm, (0.0 - 1.0), zn1, st.meso_tn1, st.meso_tgn1);
-    }
+    if constexpr (B28) gts7ShareB28(h, st, T, in, pre);
 }
 
+// Species k of 0-6 up to the statements that need b28 (the N2 mixed density at
+// its turbopause height): the diffusive and mixed densities, and every
+// correction factor that does not depend on b28, either applied (when gts7
+// applies it before the b28 factor, or when there is none) or kept (c1, c2) for
+// the tail, which applies the rest in gts7's order. A factor of 1.0 leaves the
+// density's bits unchanged.
+struct SpeciesHead {
+    double d;    // density so far
+    double bk;   // the species' density at its turbopause height (tail 2)
+    double c1, c2;   // factors after the b28 one (1.0: none)
+    int tail;    // 0: d is final; 1: N2 mixing (dm28 and dnet); 2: ccor of log(b28 ... / bk), then c1, c2
+};
+
 template <class Globe>
-__device__ __attribute__((always_inline)) inline double gts7Species(int k, const DensuShared& h, State& st, const NrlTables& T,
-                                                                    Input& in, const Flags& fl, const Globe& G, const Pre& pre) {
-    const int mn1 = 5;
-    const double s = pre.s, xmm = pre.xmm;
+__device__ __attribute__((always_inline)) inline SpeciesHead gts7SpeciesHead(int k, const DensuShared& h, State& st,
+                                                                             const NrlTables& T, Input& in, const Flags& fl,
+                                                                             const Globe& G, const Pre& pre) {
+    const double xmm = pre.xmm;
     double zn1[5] = {pre.zn1[0], pre.zn1[1], pre.zn1[2], pre.zn1[3], pre.zn1[4]};
     const double z = in.alt;
-    double tz;
     double* tn1 = st.meso_tn1;
     double* tgn1 = st.meso_tgn1;
     const double* pdm = T.hot + HOT_PDM;   // pdm[row][i] = pdm[10 row + i]
     const double* pdl = T.hot + HOT_PDL;   // pdl[blk][i] = pdl[25 blk + i]
-    const double zlbT = T.hot[HOT_PTM + 5];
-    if (k == 7) {  // anomalous O (d[8])
-        const double g16h = fl.sw[21] * G(st, 10, (T.base + OFF_PD + 150 * (8)), in, fl);
-        const double db16h = pdm[70] * exp(g16h) * T.hot[HOT_PD0 + 8];
-        const double tho = pdm[79] * pdl[6];
-        const double ddo = densu(st, z, db16h, tho, tho, 16., 0.0, &tz, zlbT, s, mn1, zn1, tn1, tgn1);
-        const double zsht = pdm[75];
-        const double zmho = pdm[74];
-        const double zsho = scalh(st, zmho, 16.0, tho);
-        return ddo * exp(-zsht / zsho * (exp(-(z - zmho) / zsht) - 1.));
-    }
+    SpeciesHead r{0.0, 0.0, 1.0, 1.0, 0};
     const SpeciesPar P = kSpeciesPar[k];
     const double* pdkG = T.base + OFF_PD + 150 * P.pd;
     const double* pdmk = pdm + 10 * P.row;
-    const double b28 = h.b28;
     double dbk = h.db28;
     if (k != 0) {
         const double gk = fl.sw[21] * G(st, P.globe, pdkG, in, fl);
@@ -1125,8 +1138,7 @@ __device__ __attribute__((always_inline)) inline double gts7Species(int k, const
     if (mix) {
         // mixed density at the lane altitude: densu(z, b, ..., xmm, 0)
         if (k == 0) {
-            const double dm28 = h.laneUpper ? b28 * h.pmm * h.emm : densuLane(h, st, b28, xmm, 0.0, zn1, tn1, tgn1);
-            d = dnetFast(d, dm28, zhm28, xmm, 28.0);
+            r.tail = 1;
         } else {
             const double zh = pdmk[2];
             const double bk = densuAt(h, st, zh, dbk, P.xm - xmm, P.alpha - 1.0, zn1, tn1, tgn1);
@@ -1136,11 +1148,14 @@ __device__ __attribute__((always_inline)) inline double gts7Species(int k, const
                 const double rl = pdmk[1] * pdl[25 + 16] * (1.0 + fl.sw[1] * pdl[23] * (in.f107A - 150.0));
                 d = d * ccor2(z, rl, pdmk[5] * pdl[P.hIdx], pdmk[4] * pdl[P.zIdx], pdmk[5] * pdl[25 + 4]);
             } else {
-                const double f = P.fIdx >= 0 ? sqrt(pdl[P.fIdx] * pdl[P.fIdx]) : 1.0;
-                const double rl = P.fIdx >= 0 ? log(b28 * pdmk[1] * f / bk) : log(b28 * pdmk[1] / bk);
-                d = d * ccor(z, rl, pdmk[5] * pdl[P.hIdx], pdmk[4] * pdl[P.zIdx]);
+                r.bk = bk;
+                r.tail = 2;
             }
-            if (P.rcIdx >= 0) d = d * ccor(z, pdmk[3] * pdl[P.rcIdx], pdmk[7] * pdl[P.hccIdx], pdmk[6] * pdl[P.zccIdx]);
+            if (P.rcIdx >= 0) {
+                const double c = ccor(z, pdmk[3] * pdl[P.rcIdx], pdmk[7] * pdl[P.hccIdx], pdmk[6] * pdl[P.zccIdx]);
+                if (r.tail == 2) r.c1 = c;
+                else d = d * c;
+            }
         }
     }
     if (k == 3 && fl.sw[15]) {
@@ -1148,9 +1163,63 @@ __device__ __attribute__((always_inline)) inline double gts7Species(int k, const
         const double hcc232 = pdmk[7] * pdl[22];
         const double zcc32 = pdmk[6] * pdl[25 + 21];
         const double rc32 = pdmk[3] * pdl[25 + 23] * (1. + fl.sw[1] * pdl[23] * (in.f107A - 150.));
-        d = d * ccor2(z, rc32, hcc32, zcc32, hcc232);
+        const double c = ccor2(z, rc32, hcc32, zcc32, hcc232);
+        if (r.tail == 2) r.c2 = c;
+        else d = d * c;
     }
+    r.d = d;
+    return r;
+}
+
+// The rest of species k from its head and b28 (h.b28).
+__device__ __attribute__((always_inline)) inline double gts7SpeciesTail(int k, const SpeciesHead& r, const DensuShared& h,
+                                                                       State& st, const NrlTables& T, const Input& in,
+                                                                       const Pre& pre) {
+    if (r.tail == 0) return r.d;
+    const double xmm = pre.xmm;
+    const double* pdm = T.hot + HOT_PDM;
+    const double* pdl = T.hot + HOT_PDL;
+    const double zhm28 = pdm[23] * pdl[25 + 5];
+    const double b28 = h.b28;
+    if (r.tail == 1) {
+        double zn1[5] = {pre.zn1[0], pre.zn1[1], pre.zn1[2], pre.zn1[3], pre.zn1[4]};
+        const double dm28 = h.laneUpper ? b28 * h.pmm * h.emm : densuLane(h, st, b28, xmm, 0.0, zn1, st.meso_tn1, st.meso_tgn1);
+        return dnetFast(r.d, dm28, zhm28, xmm, 28.0);
+    }
+    const SpeciesPar P = kSpeciesPar[k];
+    const double* pdmk = pdm + 10 * P.row;
+    const double z = in.alt;
+    const double f = P.fIdx >= 0 ? sqrt(pdl[P.fIdx] * pdl[P.fIdx]) : 1.0;
+    const double rl = P.fIdx >= 0 ? log(b28 * pdmk[1] * f / r.bk) : log(b28 * pdmk[1] / r.bk);
+    double d = r.d * ccor(z, rl, pdmk[5] * pdl[P.hIdx], pdmk[4] * pdl[P.zIdx]);
+    d = d * r.c1;
+    d = d * r.c2;
     return d;
+}
+
+template <class Globe>
+__device__ __attribute__((always_inline)) inline double gts7Species(int k, const DensuShared& h, State& st, const NrlTables& T,
+                                                                    Input& in, const Flags& fl, const Globe& G, const Pre& pre) {
+    if (k == 7) {  // anomalous O (d[8])
+        const int mn1 = 5;
+        const double s = pre.s;
+        double zn1[5] = {pre.zn1[0], pre.zn1[1], pre.zn1[2], pre.zn1[3], pre.zn1[4]};
+        const double z = in.alt;
+        double tz;
+        const double* pdm = T.hot + HOT_PDM;
+        const double* pdl = T.hot + HOT_PDL;
+        const double zlbT = T.hot[HOT_PTM + 5];
+        const double g16h = fl.sw[21] * G(st, 10, (T.base + OFF_PD + 150 * (8)), in, fl);
+        const double db16h = pdm[70] * exp(g16h) * T.hot[HOT_PD0 + 8];
+        const double tho = pdm[79] * pdl[6];
+        const double ddo = densu(st, z, db16h, tho, tho, 16., 0.0, &tz, zlbT, s, mn1, zn1, st.meso_tn1, st.meso_tgn1);
+        const double zsht = pdm[75];
+        const double zmho = pdm[74];
+        const double zsho = scalh(st, zmho, 16.0, tho);
+        return ddo * exp(-zsht / zsho * (exp(-(z - zmho) / zsht) - 1.));
+    }
+    const SpeciesHead r = gts7SpeciesHead(k, h, st, T, in, fl, G, pre);
+    return gts7SpeciesTail(k, r, h, st, T, in, pre);
 }
 
 // Mass-density weights of the species in gtd7d (amu): d[5] = 1.66e-24 sum w_k d_k.

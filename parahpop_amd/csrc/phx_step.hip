@@ -1825,6 +1825,7 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
     __shared__ double gv[19][kTile];
     __shared__ double spec[8][kTile];
     __shared__ double hx[kDensuSharedFields + 2][kTile];   // wave 0's densu quantities and lower-boundary node
+    __shared__ double hb[kTile];                            // b28 of wave 0
     __shared__ double hot[nrlmsise::HOT_COUNT];             // the chains' coefficients, read from LDS
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1900,13 +1901,13 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
         nrlmsise::DensuShared h;
         if (wave == 0 || wave == 5) {
             // the densu quantities every species shares, formed once and handed
-            // to every wave: wave 0 the lower-thermosphere part (with the N2
-            // turbopause density), wave 4 the lane part, at once
+            // to every wave: wave 0 the lower-thermosphere part, wave 5 the lane
+            // part, at once (the N2 turbopause density b28 follows the hand-over)
             const bool lanePart = wave == 5;
             if (lanePart) {
                 nrlmsise::gts7ShareLane(h, st, T, in, pre);
             } else {
-                nrlmsise::gts7SpeciesShare<SharedGlobe, false>(h, true, st, T, in, fl, G, pre);
+                nrlmsise::gts7SpeciesShare<SharedGlobe, false, false>(h, true, st, T, in, fl, G, pre);
                 hx[kDensuSharedFields][lane] = st.meso_tn1[0];
                 hx[kDensuSharedFields + 1][lane] = st.meso_tgn1[0];
             }
@@ -1926,6 +1927,9 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
             spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
         }
         __syncthreads();
+        // every species up to its b28 factor while wave 0 forms b28 (before its
+        // own N2 head), then the tails from b28
+        nrlmsise::SpeciesHead head{};
         if (k != 7) {
             int i = 0;
             densuSharedFields(h, [&](auto& x) {
@@ -1934,7 +1938,16 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
             });
             st.meso_tn1[0] = hx[kDensuSharedFields][lane];
             st.meso_tgn1[0] = hx[kDensuSharedFields + 1][lane];
-            spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
+            if (wave == 0) {
+                nrlmsise::gts7ShareB28(h, st, T, in, pre);
+                hb[lane] = h.b28;
+            }
+            head = nrlmsise::gts7SpeciesHead(k, h, st, T, in, fl, G, pre);
+        }
+        __syncthreads();
+        if (k != 7) {
+            h.b28 = hb[lane];
+            spec[k][lane] = nrlmsise::gts7SpeciesTail(k, head, h, st, T, in, pre);
         }
     }
     __syncthreads();
