@@ -116,6 +116,25 @@ __device__ inline double ccor(double alt, double r, double h1, double zh) {
     return exp(e);
 }
 
+// ccor split at its argument r: the part that depends on the altitude alone
+// (the branch taken and 1 + exp((alt - zh) / h1)), then the factor for r;
+// together the same operations as ccor.
+struct CcorPre {
+    int mode;     // 0: exp(0), 1: exp(r), 2: exp(r / den)
+    double den;
+};
+__device__ inline CcorPre ccorPre(double alt, double h1, double zh) {
+    const double e = (alt - zh) / h1;
+    if (e > 70) return {0, 0.0};
+    if (e < -70) return {1, 0.0};
+    return {2, 1.0 + exp(e)};
+}
+__device__ inline double ccorFrom(const CcorPre& c, double r) {
+    if (c.mode == 0) return exp(0.0);
+    if (c.mode == 1) return exp(r);
+    return exp(r / c.den);
+}
+
 // :145-164
 __device__ inline double ccor2(double alt, double r, double h1, double zh, double h2) {
     const double e1 = (alt - zh) / h1, e2 = (alt - zh) / h2;
@@ -1109,13 +1128,52 @@ struct SpeciesHead {
     double d;    // density so far
     double bk;   // the species' density at its turbopause height (tail 2)
     double c1, c2;   // factors after the b28 one (1.0: none)
+    CcorPre cA;  // the altitude part of the b28 factor
     int tail;    // 0: d is final; 1: N2 mixing (dm28 and dnet); 2: ccor of log(b28 ... / bk), then c1, c2
 };
+
+// The correction factors of species k that depend on the lane altitude (and
+// F10.7A) alone, so a wave can form them before the shared densu quantities
+// exist: O's chemistry ccor2, the ccor of O, H and N with rcIdx, O2's ccor2 and
+// the altitude part of the ccor that the tail applies (1.0 or unused when the
+// species has none at this altitude).
+struct SpeciesPre {
+    double c2, rc, c32;
+    CcorPre cA;
+};
+
+__device__ inline SpeciesPre gts7SpeciesPre(int k, const NrlTables& T, const Input& in, const Flags& fl) {
+    SpeciesPre q{1.0, 1.0, 1.0, {0, 0.0}};
+    const double z = in.alt;
+    const double* pdm = T.hot + HOT_PDM;
+    const double* pdl = T.hot + HOT_PDL;
+    const SpeciesPar P = kSpeciesPar[k];
+    const double* pdmk = pdm + 10 * P.row;
+    const bool mix = fl.sw[15] && (k == 1 ? z < P.altl : z <= P.altl);
+    if (mix && k != 0) {
+        if (k == 2) {
+            const double rl = pdmk[1] * pdl[25 + 16] * (1.0 + fl.sw[1] * pdl[23] * (in.f107A - 150.0));
+            q.c2 = ccor2(z, rl, pdmk[5] * pdl[P.hIdx], pdmk[4] * pdl[P.zIdx], pdmk[5] * pdl[25 + 4]);
+        } else {
+            q.cA = ccorPre(z, pdmk[5] * pdl[P.hIdx], pdmk[4] * pdl[P.zIdx]);
+        }
+        if (P.rcIdx >= 0) q.rc = ccor(z, pdmk[3] * pdl[P.rcIdx], pdmk[7] * pdl[P.hccIdx], pdmk[6] * pdl[P.zccIdx]);
+    }
+    if (k == 3 && fl.sw[15]) {
+        const double hcc32 = pdmk[7] * pdl[25 + 22];
+        const double hcc232 = pdmk[7] * pdl[22];
+        const double zcc32 = pdmk[6] * pdl[25 + 21];
+        const double rc32 = pdmk[3] * pdl[25 + 23] * (1. + fl.sw[1] * pdl[23] * (in.f107A - 150.));
+        q.c32 = ccor2(z, rc32, hcc32, zcc32, hcc232);
+    }
+    return q;
+}
 
 template <class Globe>
 __device__ __attribute__((always_inline)) inline SpeciesHead gts7SpeciesHead(int k, const DensuShared& h, State& st,
                                                                              const NrlTables& T, Input& in, const Flags& fl,
-                                                                             const Globe& G, const Pre& pre) {
+                                                                             const Globe& G, const Pre& pre,
+                                                                             const SpeciesPre& q) {
     const double xmm = pre.xmm;
     double zn1[5] = {pre.zn1[0], pre.zn1[1], pre.zn1[2], pre.zn1[3], pre.zn1[4]};
     const double z = in.alt;
@@ -1123,7 +1181,7 @@ __device__ __attribute__((always_inline)) inline SpeciesHead gts7SpeciesHead(int
     double* tgn1 = st.meso_tgn1;
     const double* pdm = T.hot + HOT_PDM;   // pdm[row][i] = pdm[10 row + i]
     const double* pdl = T.hot + HOT_PDL;   // pdl[blk][i] = pdl[25 blk + i]
-    SpeciesHead r{0.0, 0.0, 1.0, 1.0, 0};
+    SpeciesHead r{0.0, 0.0, 1.0, 1.0, {0, 0.0}, 0};
     const SpeciesPar P = kSpeciesPar[k];
     const double* pdkG = T.base + OFF_PD + 150 * P.pd;
     const double* pdmk = pdm + 10 * P.row;
@@ -1145,27 +1203,21 @@ __device__ __attribute__((always_inline)) inline SpeciesHead gts7SpeciesHead(int
             const double dmk = h.laneUpper ? bk * h.pmm * h.emm : densuLane(h, st, bk, xmm, 0., zn1, tn1, tgn1);
             d = dnetFast(d, dmk, zhm28, xmm, P.xm);
             if (k == 2) {
-                const double rl = pdmk[1] * pdl[25 + 16] * (1.0 + fl.sw[1] * pdl[23] * (in.f107A - 150.0));
-                d = d * ccor2(z, rl, pdmk[5] * pdl[P.hIdx], pdmk[4] * pdl[P.zIdx], pdmk[5] * pdl[25 + 4]);
+                d = d * q.c2;
             } else {
                 r.bk = bk;
+                r.cA = q.cA;
                 r.tail = 2;
             }
             if (P.rcIdx >= 0) {
-                const double c = ccor(z, pdmk[3] * pdl[P.rcIdx], pdmk[7] * pdl[P.hccIdx], pdmk[6] * pdl[P.zccIdx]);
-                if (r.tail == 2) r.c1 = c;
-                else d = d * c;
+                if (r.tail == 2) r.c1 = q.rc;
+                else d = d * q.rc;
             }
         }
     }
     if (k == 3 && fl.sw[15]) {
-        const double hcc32 = pdmk[7] * pdl[25 + 22];
-        const double hcc232 = pdmk[7] * pdl[22];
-        const double zcc32 = pdmk[6] * pdl[25 + 21];
-        const double rc32 = pdmk[3] * pdl[25 + 23] * (1. + fl.sw[1] * pdl[23] * (in.f107A - 150.));
-        const double c = ccor2(z, rc32, hcc32, zcc32, hcc232);
-        if (r.tail == 2) r.c2 = c;
-        else d = d * c;
+        if (r.tail == 2) r.c2 = q.c32;
+        else d = d * q.c32;
     }
     r.d = d;
     return r;
@@ -1173,8 +1225,7 @@ __device__ __attribute__((always_inline)) inline SpeciesHead gts7SpeciesHead(int
 
 // The rest of species k from its head and b28 (h.b28).
 __device__ __attribute__((always_inline)) inline double gts7SpeciesTail(int k, const SpeciesHead& r, const DensuShared& h,
-                                                                       State& st, const NrlTables& T, const Input& in,
-                                                                       const Pre& pre) {
+                                                                       State& st, const NrlTables& T, const Pre& pre) {
     if (r.tail == 0) return r.d;
     const double xmm = pre.xmm;
     const double* pdm = T.hot + HOT_PDM;
@@ -1188,10 +1239,9 @@ __device__ __attribute__((always_inline)) inline double gts7SpeciesTail(int k, c
     }
     const SpeciesPar P = kSpeciesPar[k];
     const double* pdmk = pdm + 10 * P.row;
-    const double z = in.alt;
     const double f = P.fIdx >= 0 ? sqrt(pdl[P.fIdx] * pdl[P.fIdx]) : 1.0;
     const double rl = P.fIdx >= 0 ? log(b28 * pdmk[1] * f / r.bk) : log(b28 * pdmk[1] / r.bk);
-    double d = r.d * ccor(z, rl, pdmk[5] * pdl[P.hIdx], pdmk[4] * pdl[P.zIdx]);
+    double d = r.d * ccorFrom(r.cA, rl);
     d = d * r.c1;
     d = d * r.c2;
     return d;
@@ -1218,8 +1268,9 @@ __device__ __attribute__((always_inline)) inline double gts7Species(int k, const
         const double zsho = scalh(st, zmho, 16.0, tho);
         return ddo * exp(-zsht / zsho * (exp(-(z - zmho) / zsht) - 1.));
     }
-    const SpeciesHead r = gts7SpeciesHead(k, h, st, T, in, fl, G, pre);
-    return gts7SpeciesTail(k, r, h, st, T, in, pre);
+    const SpeciesPre q = gts7SpeciesPre(k, T, in, fl);
+    const SpeciesHead r = gts7SpeciesHead(k, h, st, T, in, fl, G, pre, q);
+    return gts7SpeciesTail(k, r, h, st, T, pre);
 }
 
 // Mass-density weights of the species in gtd7d (amu): d[5] = 1.66e-24 sum w_k d_k.

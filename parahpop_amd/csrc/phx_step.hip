@@ -1926,6 +1926,10 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
             // the anomalous oxygen (its own densu profile) needs nothing else
             spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
         }
+        // each species' correction factors of the lane altitude alone, in the
+        // time its wave waits for the shared quantities
+        nrlmsise::SpeciesPre q{};
+        if (k != 7) q = nrlmsise::gts7SpeciesPre(k, T, in, fl);
         __syncthreads();
         // every species up to its b28 factor while wave 0 forms b28 (before its
         // own N2 head), then the tails from b28
@@ -1942,12 +1946,12 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
                 nrlmsise::gts7ShareB28(h, st, T, in, pre);
                 hb[lane] = h.b28;
             }
-            head = nrlmsise::gts7SpeciesHead(k, h, st, T, in, fl, G, pre);
+            head = nrlmsise::gts7SpeciesHead(k, h, st, T, in, fl, G, pre, q);
         }
         __syncthreads();
         if (k != 7) {
             h.b28 = hb[lane];
-            spec[k][lane] = nrlmsise::gts7SpeciesTail(k, head, h, st, T, in, pre);
+            spec[k][lane] = nrlmsise::gts7SpeciesTail(k, head, h, st, T, pre);
         }
     }
     __syncthreads();
