@@ -1080,6 +1080,19 @@ __device__ inline void gts7ShareLane(DensuShared& h, const State& st, const NrlT
     densuShareLane(h, st);
 }
 
+// The N2 density at Zlb (db28) from its globe7 table, where a species needs it.
+template <class Globe>
+__device__ __attribute__((always_inline)) inline void gts7ShareDb28(DensuShared& h, bool needDb28, State& st, const NrlTables& T,
+                                                                    Input& in, const Flags& fl, const Globe& G, const Pre& pre) {
+    const double* pdm = T.hot + HOT_PDM;
+    const double z = in.alt;
+    const bool wantB28 = !(z > 450.0 && z >= pre.zn1[0]);
+    if (wantB28 || needDb28) {
+        const double g28 = fl.sw[21] * G(st, 4, (T.base + OFF_PD + 150 * (2)), in, fl);
+        h.db28 = pdm[20] * exp(g28) * T.hot[HOT_PD0 + 2];
+    }
+}
+
 // The N2 mixed density at its turbopause height (b28) from the shared
 // lower-thermosphere quantities and db28; the last statement of
 // gts7SpeciesShare, which B28 = false leaves out.
@@ -1095,13 +1108,13 @@ __device__ inline void gts7ShareB28(DensuShared& h, State& st, const NrlTables& 
 }
 
 // LANE = false leaves densuShareLane's fields to another wave (gts7ShareLane);
-// B28 = false leaves b28 to gts7ShareB28.
-template <class Globe, bool LANE = true, bool B28 = true>
+// B28 = false leaves b28 to gts7ShareB28 and DB28 = false also db28 (to
+// gts7ShareDb28).
+template <class Globe, bool LANE = true, bool B28 = true, bool DB28 = true>
 __device__ __attribute__((always_inline)) inline void gts7SpeciesShare(DensuShared& h, bool needDb28, State& st,
                                                                        const NrlTables& T, Input& in, const Flags& fl,
                                                                        const Globe& G, const Pre& pre) {
     double zn1[5] = {pre.zn1[0], pre.zn1[1], pre.zn1[2], pre.zn1[3], pre.zn1[4]};
-    const double* pdm = T.hot + HOT_PDM;
     const double z = in.alt;
     if constexpr (LANE) {
         densuShare(h, st, z, pre.tinf, pre.tlb, pre.s, T.hot[HOT_PTM + 5], pre.xmm, zn1, st.meso_tn1, st.meso_tgn1);
@@ -1110,11 +1123,7 @@ __device__ __attribute__((always_inline)) inline void gts7SpeciesShare(DensuShar
         densuShareLower(h, st, zn1, st.meso_tn1, st.meso_tgn1);
         h.db28 = h.b28 = 0.0;
     }
-    const bool wantB28 = !(z > 450.0 && z >= zn1[0]);
-    if (wantB28 || needDb28) {
-        const double g28 = fl.sw[21] * G(st, 4, (T.base + OFF_PD + 150 * (2)), in, fl);
-        h.db28 = pdm[20] * exp(g28) * T.hot[HOT_PD0 + 2];
-    }
+    if constexpr (DB28) gts7ShareDb28(h, needDb28, st, T, in, fl, G, pre);
     if constexpr (B28) gts7ShareB28(h, st, T, in, pre);
 }
 
@@ -1500,6 +1509,19 @@ enum : int { GT_C131 = 0, GT_S131, GT_C97, GT_S97, GT_C136, GT_S136, GT_COUNT };
 // expansion is term by term the published one (Nrlmsise00Model.inl:688-850)
 // with cos(x - p) = cos x cos p + sin x sin p for the longitude / local-time
 // phases and powers of the Ap decay factor taken by products.
+// The Ap function of a globe7 table with p[51] != 0 (the apt[0] its call
+// leaves), from the table's epoch terms E.
+template <class EP>
+__device__ __attribute__((always_inline)) inline double globeApt(NrlCD p, EP E, double glat) {
+    if (p[138] == 0.0) return E[GE_APT0];
+    double ex = exp(-10800.0 * sqrt(p[51] * p[51]) / (1.0 + p[138] * (45.0 - sqrt(glat * glat))));
+    if (ex > 0.99999) ex = 0.99999;
+    const auto g = E + GE_G0;
+    const double e2 = ex * ex, e3 = e2 * ex, e4 = e2 * e2, e8 = e4 * e4, e12 = e8 * e4, e16 = e8 * e8, e19 = e16 * e3;
+    const double sx = 1.0 + (1.0 - e19) / (1.0 - ex) * sqrt(ex);
+    return (g[0] + (g[1] * ex + g[2] * e2 + g[3] * e3 + (g[4] * e4 + g[5] * e12) * (1.0 - e8) / (1.0 - ex))) / sx;
+}
+
 template <class EP>
 __device__ __attribute__((always_inline)) inline double globe7Fast(const State& st, NrlCD p, EP E, double dfa, double glat,
                                                                    double ut71, double ut79c, double ut79s, double ut58,
@@ -1524,16 +1546,7 @@ __device__ __attribute__((always_inline)) inline double globe7Fast(const State& 
     }
     double apt0 = 0.0;
     if (p[51] != 0) {
-        if (p[138] == 0.0) {
-            apt0 = E[GE_APT0];
-        } else {
-            double ex = exp(-10800.0 * sqrt(p[51] * p[51]) / (1.0 + p[138] * (45.0 - sqrt(glat * glat))));
-            if (ex > 0.99999) ex = 0.99999;
-            const auto g = E + GE_G0;
-            const double e2 = ex * ex, e3 = e2 * ex, e4 = e2 * e2, e8 = e4 * e4, e12 = e8 * e4, e16 = e8 * e8, e19 = e16 * e3;
-            const double sx = 1.0 + (1.0 - e19) / (1.0 - ex) * sqrt(ex);
-            apt0 = (g[0] + (g[1] * ex + g[2] * e2 + g[3] * e3 + (g[4] * e4 + g[5] * e12) * (1.0 - e8) / (1.0 - ex))) / sx;
-        }
+        apt0 = globeApt(p, E, glat);
         const double c131 = st.ctloc * tr[GT_C131] + st.stloc * tr[GT_S131];
         tinf += apt0 * (p[50] + p[96] * plg[0][2] + p[54] * plg[0][4] +
                         (p[125] * plg[0][1] + p[126] * plg[0][3] + p[127] * plg[0][5]) * cd14 +
@@ -1620,6 +1633,10 @@ struct FastGlobe {
         return globe7Fast(st, (NrlCD)p, ec, ((NrlCD)E)[GE_DFA], in.g_lat, ec[GE_UT71], ec[GE_UT79C], ec[GE_UT79S], ec[GE_UT58],
                           (NrlCD)(T + GT_COUNT * j), apt0);
     }
+    // the Ap function globe7 of table j (p[51] != 0) leaves in apt[0]
+    __device__ __attribute__((always_inline)) double apt(int j, const double* p, const Input& in) const {
+        return globeApt((NrlCD)p, (NrlCD)(E + GE_STRIDE * j), in.g_lat);
+    }
     __device__ __attribute__((always_inline)) double operator()(State& st, int j, const double* p, const Input& in, const Flags&) const {
         // globe7 also leaves apt[0] for glob7s (lower thermosphere)
         double apt0 = 0.0;
@@ -1687,6 +1704,9 @@ struct SlotGlobe {
         const double ut79s = ss * e[GS_C79] - cs * e[GS_S79];
         const double ut58 = cs * e[GS_C58] + ss * e[GS_S58];
         return globe7Fast(st, (NrlCD)p, e, S[SLOT_DFA], in.g_lat, ut71, ut79c, ut79s, ut58, (NrlCD)(T + GT_COUNT * j), apt0);
+    }
+    __device__ __attribute__((always_inline)) double apt(int j, const double* p, const Input& in) const {
+        return globeApt((NrlCD)p, S + GS_STRIDE * j, in.g_lat);
     }
     __device__ __attribute__((always_inline)) double operator()(State& st, int j, const double* p, const Input& in,
                                                                const Flags&) const {

@@ -1807,10 +1807,10 @@ struct SharedGlobe {
 };
 
 // Species of each wave of the drag tile kernel: N2 on wave 0, which forms the
-// lower-thermosphere part of the shared densu quantities while wave 4 forms
+// lower-thermosphere part of the shared densu quantities while wave 5 forms
 // their lane part; the four species whose mixing corrections are longest
 // (O, H, N, O2) on four different SIMDs (waves w and w + 4 share a SIMD);
-// anomalous O (formed before the hand-over) beside O2.
+// anomalous O beside O2.
 __constant__ int kTileSpecies[8] = {0, 1, 4, 3, 2, 5, 6, 7};
 
 // Drag of one tile and stage in one eight-wave workgroup. The waves first
@@ -1861,36 +1861,40 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
         const int k = kTileSpecies[wave];
         // globe7 table of this wave's species
         const int jk = k == 7 ? 10 : nrlmsise::kSpeciesPar[k].globe;
+        // a lane below 300 km: the glob7s node tables (the same in every wave)
+        const bool low = __ballot(in0.alt < 300.0) != 0ull;
         {
-            // globe7 of the prefix tables pt, ps, pd[3] and of pd[2] (N2) on waves
-            // 0-3 and of the species tables of waves 4-7 (O, H, N, anomalous O),
-            // one per wave; with a lane below 300 km, then the five glob7s node
-            // tables on waves 3-7
+            // globe7 of the prefix tables pt, ps and pd[3] on waves 0-2; on waves
+            // 3-7, with a lane below 300 km, first one of the five glob7s node
+            // tables (each wave forms the apt[0] that the globe7 calls of pt, ps
+            // and pd[3] leave, their Ap functions, itself), then the globe7 table
+            // of pd[2] (N2) or of the wave's species (O, H, N, anomalous O)
             nrlmsise::State st;
             F.basis(st, in0, fl);
-            if (wave < 4) {
-                const int j = wave == 2 ? 5 : (wave == 3 ? 4 : wave);
+            if (wave < 3) {
+                const int j = wave == 2 ? 5 : wave;
                 double apt = 0.0;
                 gv[j][lane] = F.value(st, j, nrlmsise::globeTable(T, j), in0, apt);
-                if (wave < 3) gv[11 + wave][lane] = apt;
+                gv[11 + wave][lane] = apt;
             } else {
-                double apt = 0.0;
-                gv[jk][lane] = F.value(st, jk, nrlmsise::globeTable(T, jk), in0, apt);
-            }
-            if (__syncthreads_or(in0.alt < 300.0)) {
-                if (wave >= 3) {
-                    // apt[0] as gts7's globe7 calls of pt, ps and pd[3] leave it
+                if (low) {
                     const double za = ((nrlmsise::NrlCD)(T.base + nrlmsise::OFF_PDL + 25 * 1))[15];
+                    const double* t0 = nrlmsise::globeTable(T, 0);
+                    const double* t1 = nrlmsise::globeTable(T, 1);
+                    const double* t5 = nrlmsise::globeTable(T, 5);
                     double apt = 0.0;
-                    if (in0.alt > za && ((nrlmsise::NrlCD)nrlmsise::globeTable(T, 0))[51] != 0) apt = gv[11][lane];
-                    if (in0.alt > 72.5 && ((nrlmsise::NrlCD)nrlmsise::globeTable(T, 1))[51] != 0) apt = gv[12][lane];
-                    if (((nrlmsise::NrlCD)nrlmsise::globeTable(T, 5))[51] != 0) apt = gv[13][lane];
+                    if (in0.alt > za && ((nrlmsise::NrlCD)t0)[51] != 0) apt = F.apt(0, t0, in0);
+                    if (in0.alt > 72.5 && ((nrlmsise::NrlCD)t1)[51] != 0) apt = F.apt(1, t1, in0);
+                    if (((nrlmsise::NrlCD)t5)[51] != 0) apt = F.apt(5, t5, in0);
                     st.apt[0] = apt;
                     const int j = wave - 3;
                     gv[14 + j][lane] = F.lower(st, j, nrlmsise::lowerTable(T, j), in0, fl);
                 }
-                __syncthreads();
+                const int j = wave == 3 ? 4 : jk;
+                double apt = 0.0;
+                gv[j][lane] = F.value(st, j, nrlmsise::globeTable(T, j), in0, apt);
             }
+            __syncthreads();
         }
         nrlmsise::Input in = in0;
         nrlmsise::State st;
@@ -1902,12 +1906,12 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
         if (wave == 0 || wave == 5) {
             // the densu quantities every species shares, formed once and handed
             // to every wave: wave 0 the lower-thermosphere part, wave 5 the lane
-            // part, at once (the N2 turbopause density b28 follows the hand-over)
+            // part, at once (db28 and b28 follow the hand-over)
             const bool lanePart = wave == 5;
             if (lanePart) {
                 nrlmsise::gts7ShareLane(h, st, T, in, pre);
             } else {
-                nrlmsise::gts7SpeciesShare<SharedGlobe, false, false>(h, true, st, T, in, fl, G, pre);
+                nrlmsise::gts7SpeciesShare<SharedGlobe, false, false, false>(h, true, st, T, in, fl, G, pre);
                 hx[kDensuSharedFields][lane] = st.meso_tn1[0];
                 hx[kDensuSharedFields + 1][lane] = st.meso_tgn1[0];
             }
@@ -1922,9 +1926,6 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
             F.basis(sb, in, fl);
             double apt = 0.0;
             gv[jk][lane] = F.value(sb, jk, nrlmsise::globeTable(T, jk), in, apt);
-        } else if (k == 7) {
-            // the anomalous oxygen (its own densu profile) needs nothing else
-            spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
         }
         // each species' correction factors of the lane altitude alone, in the
         // time its wave waits for the shared quantities
@@ -1943,10 +1944,14 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
             st.meso_tn1[0] = hx[kDensuSharedFields][lane];
             st.meso_tgn1[0] = hx[kDensuSharedFields + 1][lane];
             if (wave == 0) {
+                nrlmsise::gts7ShareDb28(h, true, st, T, in, fl, G, pre);
                 nrlmsise::gts7ShareB28(h, st, T, in, pre);
                 hb[lane] = h.b28;
             }
             head = nrlmsise::gts7SpeciesHead(k, h, st, T, in, fl, G, pre, q);
+        } else {
+            // the anomalous oxygen (its own densu profile) needs nothing else
+            spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
         }
         __syncthreads();
         if (k != 7) {
