@@ -277,6 +277,8 @@ def leg_uneven(args, work):
     rng = np.random.default_rng(44)
     dur = np.exp(rng.uniform(math.log(600.0), math.log(604800.0), n))
     ctx = parahpop_amd.Context(cfg, steps_per_launch=args.uneven_steps_per_launch)
+    if args.graph_steps >= 0:
+        ctx.set_option("graph_steps", args.graph_steps)
     b = ctx.batch(t, dur)
     tune = b.autotune(4)
     # run to completion in ten chunks of steps, to see the tail's step time
@@ -477,15 +479,23 @@ def leg_serve(args, work):
     b.run()
     s64 = b.download()[1]
     served = np.concatenate([first[j][1] for j in range(nreq)], axis=1)
-    # objects that reenter end non-finite on both paths (NRLMSISE-00 is undefined
-    # below the ground); the error is stated over the others
+    # objects that reenter end non-finite (NRLMSISE-00 is undefined below the
+    # ground) or, having passed through the Earth's field, at a meaningless state
+    # that differs between any two paths; the error is stated over the objects
+    # that end in orbit (80 km above the equatorial radius to 100,000 km) on both
     fin = np.all(np.isfinite(served), axis=0) & np.all(np.isfinite(s64), axis=0)
-    rms, mx = pos_err_mm(served[:, fin], s64[:, fin])
-    out["mixed_vs_fp64_pos_mm"] = {"objects": int(fin.sum()), "objects_nonfinite": int((~fin).sum()), "rms": rms, "max": mx,
+    r_s = np.linalg.norm(np.where(fin, served[:3], 0.0), axis=0)
+    r_64 = np.linalg.norm(np.where(fin, s64[:3], 0.0), axis=0)
+    orbit = fin & (np.minimum(r_s, r_64) > 6378.1363 + 80.0) & (np.maximum(r_s, r_64) < 1e5)
+    rms, mx = pos_err_mm(served[:, orbit], s64[:, orbit])
+    err = np.linalg.norm(served[:3, orbit] - s64[:3, orbit], axis=0) * 1e6
+    out["mixed_vs_fp64_pos_mm"] = {"objects": int(orbit.sum()), "objects_reentered": int((~orbit).sum()),
+                                   "objects_nonfinite": int((~fin).sum()), "rms": rms, "max": mx,
+                                   "p99": float(np.percentile(err, 99)) if err.size else None,
                                    "nonfinite_on_both_paths": bool(np.array_equal(np.isfinite(served).all(axis=0),
                                                                                   np.isfinite(s64).all(axis=0)))}
-    log("serve: mixed vs FP64 over %d finite served objects (%d non-finite): rms %.3g mm, max %.3g mm"
-        % (fin.sum(), (~fin).sum(), rms, mx))
+    log("serve: mixed vs FP64 over %d served objects in orbit (%d reentered, %d of them non-finite): rms %.3g mm, max %.3g mm"
+        % (orbit.sum(), (~orbit).sum(), (~fin).sum(), rms, mx))
     return out
 
 
