@@ -1826,6 +1826,7 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
     __shared__ double spec[8][kTile];
     __shared__ double hx[kDensuSharedFields + 2][kTile];   // wave 0's densu quantities and lower-boundary node
     __shared__ double hb[kTile];                            // b28 of wave 0
+    __shared__ int b28Ready;                                // set by wave 0 once hb holds b28
     __shared__ double hot[nrlmsise::HOT_COUNT];             // the chains' coefficients, read from LDS
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1835,6 +1836,7 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
     const int s = sp.stage;
     // a tile without a lane below the cutoff: k_s is complete (every wave leaves)
     if (!nrlTileActive(sp.lane, nPad, obj)) return;
+    if (threadIdx.x == 0) b28Ready = 0;   // ordered before wave 0 sets it by the barriers below
     // the copy completes before the first barrier below, long before the prefix
     if (threadIdx.x < nrlmsise::HOT_COUNT) hot[threadIdx.x] = a.nrlCoeff[nrlmsise::COEFF_COUNT + threadIdx.x];
     // shared epochs: the tile's record; per-lane epochs: the lane's time inputs
@@ -1933,7 +1935,8 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
         if (k != 7) q = nrlmsise::gts7SpeciesPre(k, T, in, fl);
         __syncthreads();
         // every species up to its b28 factor while wave 0 forms b28 (before its
-        // own N2 head), then the tails from b28
+        // own N2 head), then the tails from b28: a wave with a tail waits for
+        // wave 0's flag, not for the other species' heads
         nrlmsise::SpeciesHead head{};
         if (k != 7) {
             int i = 0;
@@ -1947,16 +1950,23 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
                 nrlmsise::gts7ShareDb28(h, true, st, T, in, fl, G, pre);
                 nrlmsise::gts7ShareB28(h, st, T, in, pre);
                 hb[lane] = h.b28;
+                // the release orders the stores of hb before the flag
+                __hip_atomic_store(&b28Ready, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
             head = nrlmsise::gts7SpeciesHead(k, h, st, T, in, fl, G, pre, q);
+            if (wave != 0 && __ballot(head.tail != 0) != 0ull) {
+                // wave 0 of this workgroup sets the flag unconditionally above;
+                // the wait is bounded all the same (about 8M cycles)
+                for (int spin = 0; spin < (1 << 16) &&
+                                   __hip_atomic_load(&b28Ready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) == 0;
+                     ++spin)
+                    __builtin_amdgcn_s_sleep(2);
+            }
+            h.b28 = hb[lane];
+            spec[k][lane] = nrlmsise::gts7SpeciesTail(k, head, h, st, T, pre);
         } else {
             // the anomalous oxygen (its own densu profile) needs nothing else
             spec[k][lane] = nrlmsise::gts7Species(k, h, st, T, in, fl, G, pre);
-        }
-        __syncthreads();
-        if (k != 7) {
-            h.b28 = hb[lane];
-            spec[k][lane] = nrlmsise::gts7SpeciesTail(k, head, h, st, T, pre);
         }
     }
     __syncthreads();
