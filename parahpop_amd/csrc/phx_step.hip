@@ -1460,16 +1460,21 @@ __device__ __attribute__((always_inline)) inline double stageComponent(const Ste
                                                                        int d, double dt) {
     const int nPad = a.nPad;
     const double* arow = kRkA[s >= 1 ? s - 1 : 0];
+    // the row's coefficients first (one batch of scalar loads), so that the
+    // k-stage loads below issue back to back instead of one per scalar round trip
+    double ar[kStages - 1];
+#pragma unroll
+    for (int j = 0; j < kStages - 1; ++j) ar[j] = arow[j];
     double k[kStages - 1];
 #pragma unroll
     for (int j = 0; j < kStages - 1; ++j) {
         k[j] = 0.0;
-        if (j < s && arow[j] != 0.0) k[j] = sp.ks[ksAt(j, d, nPad, obj)];
+        if (j < s && ar[j] != 0.0) k[j] = sp.ks[ksAt(j, d, nPad, obj)];
     }
     double y = a.y[d * nPad + obj];
 #pragma unroll
     for (int j = 0; j < kStages - 1; ++j) {
-        const double aij = arow[j];
+        const double aij = ar[j];
         const double coef = dt * aij;
         const double v = y + coef * k[j];
         y = (j < s && aij != 0.0) ? v : y;
