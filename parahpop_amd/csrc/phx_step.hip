@@ -531,22 +531,38 @@ __device__ inline bool nrlTileActive(const double* lane, int nPad, int obj) {
     return __ballot(lane[size_t(4) * nPad + obj] != 0.0) != 0ull;
 }
 
+// The five lane input rows of one lane (altitude, latitude, longitude, local
+// time, state), loaded together.
+struct LaneRows {
+    double v[5];
+};
+__device__ inline LaneRows laneRows(const double* lane, int nPad, int obj) {
+    LaneRows r;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) r.v[i] = lane[size_t(i) * nPad + obj];
+    return r;
+}
+
 // nrlLane's result from the kept lane inputs and the stage's epoch record.
-__device__ inline NrlLane nrlLaneLoad(const double* lane, int nPad, int obj, const double* nrl) {
+__device__ inline NrlLane nrlLaneFrom(const LaneRows& rows, const double* nrl) {
     const double* r = nrl - REC_NRL;
     NrlLane L;
-    L.state = int(lane[size_t(4) * nPad + obj]);
+    L.state = int(rows.v[4]);
     L.in.doy = int(r[REC_NRL_DOY]);
     L.in.sec = r[REC_NRL_SEC];
-    L.in.alt = lane[obj];
-    L.in.g_lat = lane[size_t(nPad) + obj];
-    L.in.g_long = lane[size_t(2) * nPad + obj];
-    L.in.lst = lane[size_t(3) * nPad + obj];
+    L.in.alt = rows.v[0];
+    L.in.g_lat = rows.v[1];
+    L.in.g_long = rows.v[2];
+    L.in.lst = rows.v[3];
     L.in.f107A = r[REC_NRL_F107A];
     L.in.f107 = r[REC_NRL_F107];
     L.in.ap = r[REC_NRL_AP];
     for (int k = 0; k < 7; ++k) L.in.apa[k] = r[REC_NRL_AP + k];
     return L;
+}
+
+__device__ inline NrlLane nrlLaneLoad(const double* lane, int nPad, int obj, const double* nrl) {
+    return nrlLaneFrom(laneRows(lane, nPad, obj), nrl);
 }
 
 // This lane's NRLMSISE-00 time inputs (in the epoch record's REC_NRL block
@@ -579,14 +595,21 @@ __device__ inline void nrlLaneStoreGrid(double sec, int doy, int row, double* la
     lane[size_t(7) * nPad + obj] = double(row);
 }
 
-__device__ inline NrlLane nrlLaneLoadGrid(const double* lane, int nPad, int obj, const double* slots, int& row,
-                                          double (&nrl)[13]) {
+// Per-lane epochs: nrlLane's result from the five lane input rows (loaded
+// together), the lane's UT second, day of year and weather-slot row.
+__device__ inline NrlLane nrlLaneFromGrid(const LaneRows& rows, const double* lane, int nPad, int obj, const double* slots,
+                                          int& row, double (&nrl)[13]) {
     row = int(lane[size_t(7) * nPad + obj]);
     nrl[REC_NRL_DOY - REC_NRL] = lane[size_t(6) * nPad + obj];
     nrl[REC_NRL_SEC - REC_NRL] = lane[size_t(5) * nPad + obj];
     for (int k = 0; k < 9; ++k)
         nrl[REC_NRL_F107 - REC_NRL + k] = slots ? slots[size_t(row) * nrlmsise::SLOT_SIZE + nrlmsise::SLOT_F107 + k] : nan("");
-    return nrlLaneLoad(lane, nPad, obj, nrl);
+    return nrlLaneFrom(rows, nrl);
+}
+
+__device__ inline NrlLane nrlLaneLoadGrid(const double* lane, int nPad, int obj, const double* slots, int& row,
+                                          double (&nrl)[13]) {
+    return nrlLaneFromGrid(laneRows(lane, nPad, obj), lane, nPad, obj, slots, row, nrl);
 }
 
 __device__ inline nrlmsise::Flags nrlFlags() {
@@ -1839,8 +1862,10 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
     const int obj = tile * kTile + lane;
     const int nPad = a.nPad;
     const int s = sp.stage;
-    // a tile without a lane below the cutoff: k_s is complete (every wave leaves)
-    if (!nrlTileActive(sp.lane, nPad, obj)) return;
+    // the lane input rows in one batch of loads; a tile without a lane below the
+    // cutoff has k_s complete (every wave leaves on the state row)
+    const LaneRows rows = laneRows(sp.lane, nPad, obj);
+    if (__ballot(rows.v[4] != 0.0) == 0ull) return;
     if (threadIdx.x == 0) b28Ready = 0;   // ordered before wave 0 sets it by the barriers below
     // the copy completes before the first barrier below, long before the prefix
     if (threadIdx.x < nrlmsise::HOT_COUNT) hot[threadIdx.x] = a.nrlCoeff[nrlmsise::COEFF_COUNT + threadIdx.x];
@@ -1850,7 +1875,7 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
     double nrlLocal[13];
     int row = 0;
     const double* nrl = GRID ? nrlLocal : recp + REC_NRL;
-    const NrlLane nl = GRID ? nrlLaneLoadGrid(sp.lane, nPad, obj, a.nrlSlots, row, nrlLocal) : nrlLaneLoad(sp.lane, nPad, obj, nrl);
+    const NrlLane nl = GRID ? nrlLaneFromGrid(rows, sp.lane, nPad, obj, a.nrlSlots, row, nrlLocal) : nrlLaneFrom(rows, nrl);
     using Globe = typename std::conditional<GRID, nrlmsise::SlotGlobe, nrlmsise::FastGlobe>::type;
     // the same lanes in every wave: the branch is uniform across the workgroup
     if (__ballot(nl.state == 1) != 0ull) {
