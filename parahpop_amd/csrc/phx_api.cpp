@@ -1020,6 +1020,8 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
                         throw HipError(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
                     }
                     b.graphs.push_back({active, G, steps, c.optGen, g, x});
+                    // the graph's device-side setup now, not on its first replay
+                    check(hipGraphUpload(x, c.stream), "hipGraphUpload");
                     ++b.graphCaptures;
                 };
                 if (find(k) < 0) {
