@@ -131,7 +131,7 @@ void phx_batch_destroy(phx_batch* b);
  *                      the number of active tiles, so an object's last bits
  *                      (about 1e-15 relative per step) depend on the rest of its
  *                      batch and on the CU count; with 1 they do not
- *   "graph_steps"      0 .. 256 (default 16): the split stages of this many steps
+ *   "graph_steps"      0 .. 256 (default 8): the split stages of this many steps
  *                      are captured once as a graph and replayed while the active
  *                      tiles stay the same for all of them (0: every kernel is
  *                      launched); the results do not change

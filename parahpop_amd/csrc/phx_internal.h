@@ -104,7 +104,7 @@ struct phx_ctx {
     int sortAltitude = 1;
     int exactGroupsMax = 16;     // more distinct (start, end) groups than this -> grid epochs
     double gridStep = 60.0;      // s, spacing of the epoch grid
-    int graphSteps = 16;         // split stages: steps per replayed graph (0: launch every kernel)
+    int graphSteps = 8;          // split stages: steps per replayed graph (0: launch every kernel)
     unsigned long long optGen = 0;   // advanced by every option change (captured step graphs expire)
     int forceGrid = 0;
     int split = 1;               // shared-epoch batches with drag: split stages
