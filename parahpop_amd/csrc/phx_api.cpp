@@ -931,10 +931,12 @@ int runBatch(phx_ctx& c, phx_batch& b, int maxSteps, phx_stats* stats) {
             b.parts.alloc(6 * np);
             b.spec.alloc(8 * np);
             b.lane.alloc(8 * np);
+            b.trig.alloc(6 * np);
         }
         const int Ws = c.splitWaves;
         orderBlocks(c, Ws, a.blockStart, true);
         SplitArgs sp{b.ks.p, b.ysBuf.p, b.parts.p, b.spec.p, b.lane.p, 0, 0, c.splitSpecies};
+        sp.trig = b.trig.p;
         // the step index of exact records lives on the device (advanced by the
         // combine kernel), so every step launches the same kernels and arguments
         if (!b.stepDev.p) b.stepDev.alloc(1);

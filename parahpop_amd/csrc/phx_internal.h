@@ -137,7 +137,7 @@ struct phx_batch {
     std::vector<int> tileSteps;  // per tile: steps until its last object ends (upper bound)
     int stepsDone = 0;
     phx::DBuf<double> y, cur, end, nextDt, ams, bcd, records, slotT0;
-    phx::DBuf<double> ks, ysBuf, parts, spec, lane;   // split stages (allocated on first use)
+    phx::DBuf<double> ks, ysBuf, parts, spec, lane, trig;   // split stages (allocated on first use)
     phx::DBuf<double> shPart, famPart;               // grouped SH workgroups: partial sums
     phx::DBuf<unsigned> tileCount;                   // grouped SH workgroups: arrivals per tile (zeroed)
     phx::DBuf<uint8_t> status;

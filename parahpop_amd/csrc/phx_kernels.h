@@ -111,6 +111,10 @@ struct SplitArgs {
     // the combine kernel advances it, so a graph of several steps replays with
     // the same arguments
     int* stepDev;
+    // [6][nPad] the globe basis trigonometry of the stage's lane inputs
+    // (nrlmsise::BasisTrig), left by the SH kernel for the drag tile kernel
+    // (nullptr: the drag tile kernel forms it)
+    double* trig;
 };
 int launchSplitStage(const StepArgs& a, const ForceModel& fm, const SplitArgs& sp, int shWaves, hipStream_t stream);
 int launchSplitCombine(const StepArgs& a, const SplitArgs& sp, hipStream_t stream);

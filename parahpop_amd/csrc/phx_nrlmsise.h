@@ -1579,11 +1579,28 @@ __device__ __attribute__((always_inline)) inline double globe7Fast(const State& 
     return tinf;
 }
 
-// Globe basis of the fast policies: Legendre functions of the latitude,
-// local-time and longitude trigonometry (double and triple angles by products).
-__device__ __attribute__((always_inline)) inline void fastBasis(State& st, const Input& in, double dfa) {
+// The trigonometry of the globe basis: sin and cos of the latitude, of the
+// local time's hour angle and of the longitude (fastBasis's arguments).
+struct BasisTrig {
+    double c, s, sl, cl, cg, sg;
+};
+__device__ __attribute__((always_inline)) inline BasisTrig basisTrig(const Input& in) {
     const double dgtr = 1.74533E-2, hr = 0.2618;
-    const double c = sin(in.g_lat * dgtr), s = cos(in.g_lat * dgtr);
+    BasisTrig t;
+    t.c = sin(in.g_lat * dgtr);
+    t.s = cos(in.g_lat * dgtr);
+    t.sl = sin(hr * in.lst);
+    t.cl = cos(hr * in.lst);
+    t.cg = cos(dgtr * in.g_long);
+    t.sg = sin(dgtr * in.g_long);
+    return t;
+}
+
+// Globe basis of the fast policies from its trigonometry: Legendre functions
+// of the latitude, local-time and longitude harmonics (double and triple
+// angles by products).
+__device__ __attribute__((always_inline)) inline void fastBasisFrom(State& st, const BasisTrig& t, double dfa) {
+    const double c = t.c, s = t.s;
     const double c2 = c * c, c4 = c2 * c2, s2 = s * s;
     double(&plg)[4][9] = st.plg;
     plg[0][1] = c;
@@ -1609,16 +1626,20 @@ __device__ __attribute__((always_inline)) inline void fastBasis(State& st, const
     plg[3][5] = (9.0 * c * plg[3][4] - 7. * plg[3][3]) / 2.0;
     plg[3][6] = (11.0 * c * plg[3][5] - 8. * plg[3][4]) / 3.0;
     // local time harmonics from one sin/cos pair (double and triple angles)
-    const double sl = sin(hr * in.lst), cl = cos(hr * in.lst);
+    const double sl = t.sl, cl = t.cl;
     st.stloc = sl;
     st.ctloc = cl;
     st.s2tloc = 2.0 * sl * cl;
     st.c2tloc = cl * cl - sl * sl;
     st.s3tloc = sl * (3.0 - 4.0 * sl * sl);
     st.c3tloc = cl * (4.0 * cl * cl - 3.0);
-    st.cglong = cos(dgtr * in.g_long);
-    st.sglong = sin(dgtr * in.g_long);
+    st.cglong = t.cg;
+    st.sglong = t.sg;
     st.dfa = dfa;
+}
+
+__device__ __attribute__((always_inline)) inline void fastBasis(State& st, const Input& in, double dfa) {
+    fastBasisFrom(st, basisTrig(in), dfa);
 }
 
 // Fast policy: epoch terms from the per-epoch record.
@@ -1626,6 +1647,7 @@ struct FastGlobe {
     const double* E;  // GE_COUNT values
     const double* T;  // GE_TABLES x GT_COUNT table phases
     __device__ __attribute__((always_inline)) void basis(State& st, const Input& in, const Flags&) const { fastBasis(st, in, E[GE_DFA]); }
+    __device__ __attribute__((always_inline)) void basisFrom(State& st, const BasisTrig& t) const { fastBasisFrom(st, t, E[GE_DFA]); }
     // globe7 of table j without side effects; apt0 is the table's Ap function
     __device__ __attribute__((always_inline)) double value(const State& st, int j, const double* p, const Input& in,
                                                           double& apt0) const {
@@ -1695,6 +1717,7 @@ struct SlotGlobe {
     __device__ __attribute__((always_inline)) void basis(State& st, const Input& in, const Flags&) const {
         fastBasis(st, in, S[SLOT_DFA]);
     }
+    __device__ __attribute__((always_inline)) void basisFrom(State& st, const BasisTrig& t) const { fastBasisFrom(st, t, S[SLOT_DFA]); }
     // globe7 of table j without side effects; apt0 is the table's Ap function
     __device__ __attribute__((always_inline)) double value(const State& st, int j, const double* p, const Input& in,
                                                           double& apt0) const {

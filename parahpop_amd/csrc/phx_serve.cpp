@@ -326,6 +326,7 @@ void Server::tick() {
             ys_.alloc(7 * S);
             parts_.alloc(6 * S);
             lane_.alloc(8 * S);
+            trig_.alloc(6 * S);
         }
     }
     a.nTiles = activeTiles_;
@@ -358,6 +359,7 @@ void Server::tick() {
         if (split) {
             // K steps of 13 stages (SH kernel, drag kernel) and the end-of-step kernel
             SplitArgs sp{ks_.p, ys_.p, parts_.p, nullptr, lane_.p, 0, 0, 8};
+            sp.trig = trig_.p;
             for (int k = 0; k < K && rc == 0; ++k) {
                 sp.step = k;
                 for (int s = 0; s < kStages && rc == 0; ++s) {

@@ -115,7 +115,7 @@ private:
     hipGraphExec_t exec_ = nullptr;
     int graphTiles_ = -1, graphWaves_ = -1, graphSteps_ = -1, graphSplit_ = -1;
     // split-stage buffers over the pool (allocated on first use)
-    DBuf<double> ks_, ys_, parts_, lane_;
+    DBuf<double> ks_, ys_, parts_, lane_, trig_;
 
     // shared with callers
     mutable std::mutex mu_;

@@ -525,6 +525,28 @@ __device__ inline bool nrlLaneStoreTile(const NrlLane& L, double* lane, int nPad
     return any;
 }
 
+// The globe basis trigonometry of a lane's inputs, kept beside them for the
+// drag tile kernel (rows of `trig`, BasisTrig order).
+__device__ inline void nrlTrigStore(const NrlLane& L, double* trig, int nPad, int obj) {
+    const nrlmsise::BasisTrig t = nrlmsise::basisTrig(L.in);
+    trig[obj] = t.c;
+    trig[size_t(nPad) + obj] = t.s;
+    trig[size_t(2) * nPad + obj] = t.sl;
+    trig[size_t(3) * nPad + obj] = t.cl;
+    trig[size_t(4) * nPad + obj] = t.cg;
+    trig[size_t(5) * nPad + obj] = t.sg;
+}
+__device__ inline nrlmsise::BasisTrig nrlTrigLoad(const double* trig, int nPad, int obj) {
+    nrlmsise::BasisTrig t;
+    t.c = trig[obj];
+    t.s = trig[size_t(nPad) + obj];
+    t.sl = trig[size_t(2) * nPad + obj];
+    t.cl = trig[size_t(3) * nPad + obj];
+    t.cg = trig[size_t(4) * nPad + obj];
+    t.sg = trig[size_t(5) * nPad + obj];
+    return t;
+}
+
 // Whether a lane of the tile needs the drag kernels at this stage, from the
 // state row the SH kernel leaves for every tile (uniform across the workgroup).
 __device__ inline bool nrlTileActive(const double* lane, int nPad, int obj) {
@@ -1605,6 +1627,7 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
             const NrlLane L = nrlLane(fm, rec, nrl, px, py, pz);
             if (nrlLaneStoreTile(L, sp.lane, nPad, obj)) {
                 nrlLaneStoreGrid(sec, L.in.doy, row, sp.lane, nPad, obj);
+                if (sp.trig) nrlTrigStore(L, sp.trig, nPad, obj);
                 if (lane == 0) dragTile = 1;
             }
         }
@@ -1612,9 +1635,13 @@ __global__ void __launch_bounds__(64 * W) splitShKernel(StepArgs a, ForceModel f
     } else {
         const double* recp = a.records + (a.tileRecord[tile] + splitStep(sp) * kStages + s) * REC_SIZE;
         const ExactRec rec{recp};
-        if (fm.drag && wave == W - 4 && nrlLaneStoreTile(nrlLane(fm, rec, recp + REC_NRL, px, py, pz), sp.lane, nPad, obj) &&
-            lane == 0)
-            dragTile = 1;
+        if (fm.drag && wave == W - 4) {
+            const NrlLane L = nrlLane(fm, rec, recp + REC_NRL, px, py, pz);
+            if (nrlLaneStoreTile(L, sp.lane, nPad, obj)) {
+                if (sp.trig) nrlTrigStore(L, sp.trig, nPad, obj);
+                if (lane == 0) dragTile = 1;
+            }
+        }
         stageForcesParts<W, true, ROT>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex);
     }
     if (wave == 0) {
@@ -1718,13 +1745,18 @@ __global__ void __launch_bounds__(64 * W) splitShGroupKernel(StepArgs a, ForceMo
             const NrlLane L = nrlLane(fm, rec, nrl, px, py, pz);
             nrlLaneStore(L, sp.lane, nPad, obj);
             nrlLaneStoreGrid(sec, L.in.doy, row, sp.lane, nPad, obj);
+            if (sp.trig) nrlTrigStore(L, sp.trig, nPad, obj);
         }
         stageForcesParts<W, false>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex, g == 0, rpl);
         if (wave == 0) splitGroupFinish(a, fm, sp, rec, g, tile, obj, lane, px, py, pz, rpl, ax, ex);
     } else {
         const double* recp = a.records + (a.tileRecord[tile] + splitStep(sp) * kStages + s) * REC_SIZE;
         const ExactRec rec{recp};
-        if (g == 0 && fm.drag && wave == W - 4) nrlLaneStore(nrlLane(fm, rec, recp + REC_NRL, px, py, pz), sp.lane, nPad, obj);
+        if (g == 0 && fm.drag && wave == W - 4) {
+            const NrlLane L = nrlLane(fm, rec, recp + REC_NRL, px, py, pz);
+            nrlLaneStore(L, sp.lane, nPad, obj);
+            if (sp.trig) nrlTrigStore(L, sp.trig, nPad, obj);
+        }
         stageForcesParts<W, false>(a, fm, rec, wave, lane, m0, m1, a.ams[obj], xch, part, extra, pg, acc, ax, ex, g == 0, rpl);
         if (wave == 0) splitGroupFinish(a, fm, sp, rec, g, tile, obj, lane, px, py, pz, rpl, ax, ex);
     }
@@ -1865,6 +1897,9 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
     // the lane input rows in one batch of loads; a tile without a lane below the
     // cutoff has k_s complete (every wave leaves on the state row)
     const LaneRows rows = laneRows(sp.lane, nPad, obj);
+    // the basis trigonometry the SH kernel formed beside the lane inputs
+    nrlmsise::BasisTrig trig{};
+    if (sp.trig) trig = nrlTrigLoad(sp.trig, nPad, obj);
     if (__ballot(rows.v[4] != 0.0) == 0ull) return;
     if (threadIdx.x == 0) b28Ready = 0;   // ordered before wave 0 sets it by the barriers below
     // the copy completes before the first barrier below, long before the prefix
@@ -1902,7 +1937,8 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
             // and pd[3] leave, their Ap functions, itself), then the globe7 table
             // of pd[2] (N2) or of the wave's species (O, H, N, anomalous O)
             nrlmsise::State st;
-            F.basis(st, in0, fl);
+            if (sp.trig) F.basisFrom(st, trig);
+            else F.basis(st, in0, fl);
             if (wave < 3) {
                 const int j = wave == 2 ? 5 : wave;
                 double apt = 0.0;
@@ -1955,7 +1991,8 @@ __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceMode
         } else if (wave < 4) {
             // meanwhile waves 1-3 evaluate their species' globe7 tables
             nrlmsise::State sb;
-            F.basis(sb, in, fl);
+            if (sp.trig) F.basisFrom(sb, trig);
+            else F.basis(sb, in, fl);
             double apt = 0.0;
             gv[jk][lane] = F.value(sb, jk, nrlmsise::globeTable(T, jk), in, apt);
         }
