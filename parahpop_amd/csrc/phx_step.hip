@@ -1874,17 +1874,20 @@ struct SharedGlobe {
 __constant__ int kTileSpecies[8] = {0, 1, 4, 3, 2, 5, 6, 7};
 
 // Drag of one tile and stage in one eight-wave workgroup. The waves first
-// evaluate the eleven globe7 tables the tile's lanes need (two per wave at
-// most) and, when a lane is below 300 km, the five glob7s node tables; from
-// those values each wave forms gts7's prefix and the shared densu quantities
-// and one species (kTileSpecies[wave]); wave 0 then assembles the density
-// and the drag, k_s = ax + (extra + drag), as the fused kernel. Every value is
-// the one the serial evaluation forms.
+// evaluate the globe7 tables of gts7's prefix and, when a lane is below
+// 300 km, the five glob7s node tables (two tables per wave at most, from the
+// basis trigonometry the SH kernel left); every wave forms the prefix; wave 0
+// forms the lower-thermosphere densu quantities every species shares and wave
+// 5 the lane ones while the others evaluate the remaining species tables;
+// after the hand-over each wave runs its species (kTileSpecies[wave]) up to the
+// factor that needs b28, which wave 0 forms and flags, then its tail; wave 0
+// assembles the density and the drag, k_s = ax + (extra + drag), as the fused
+// kernel. Every value is the one the serial evaluation forms.
 template <bool GRID>
 __global__ void __launch_bounds__(512) splitDragTileKernel(StepArgs a, ForceModel fm, SplitArgs sp) {
     __shared__ double gv[19][kTile];
     __shared__ double spec[8][kTile];
-    __shared__ double hx[kDensuSharedFields + 2][kTile];   // wave 0's densu quantities and lower-boundary node
+    __shared__ double hx[kDensuSharedFields + 2][kTile];   // the shared densu quantities and lower-boundary node
     __shared__ double hb[kTile];                            // b28 of wave 0
     __shared__ int b28Ready;                                // set by wave 0 once hb holds b28
     __shared__ double hot[nrlmsise::HOT_COUNT];             // the chains' coefficients, read from LDS
