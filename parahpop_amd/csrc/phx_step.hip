@@ -304,8 +304,10 @@ __device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const doubl
             }
         }
     }
-    // mixed orders (m >= 1: order 0 is always in the FP64 orders)
-    for (int m = max(m0, mFlat); m < m1; ++m) {
+    // mixed orders (m >= 1: order 0 is always in the FP64 orders) with degrees
+    // in FP64 and in FP32, one order at a time
+    const int mFp32 = __builtin_amdgcn_readfirstlane(min(m1, max(max(m0, mFlat), nSplit)));
+    for (int m = max(m0, mFlat); m < mFp32; ++m) {
         const int first = orderStart[m];
         const int count = L - m + 1;
         const int c64 = max(0, min(count, nSplit - m));   // degrees m .. m + c64 - 1 in FP64
@@ -339,6 +341,59 @@ __device__ void shOrderBlock(const ShGeom& g, int m0, int m1, int L, const doubl
         const double c = g.cosL * cosML - g.sinL * sinML;
         sinML = s;
         cosML = c;
+    }
+    // orders whose degrees are all in FP32 (m >= nSplit): one flat loop over the
+    // four-term batches of the padded FP32 stream, as the FP64 loop above; each
+    // order's FP32 sums are those of the per-order form (the padding terms add
+    // exact zeros)
+    if (mFp32 < m1) {
+        int m = mFp32;
+        ConstF t = terms32 + SHT_STRIDE * orderStart[m];
+        int left = (L - m + 4) >> 2;
+        float p0 = float(Pmm), pm1 = 0.0f;
+        float a1 = 0.0f, a2 = 0.0f, a5 = 0.0f, a6 = 0.0f, b1 = 0.0f, b2 = 0.0f;
+        while (true) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                ConstF q = t + j * SHT_STRIDE;
+                a1 = fma(p0, q[SHT_C1], a1);
+                a2 = fma(p0, q[SHT_S1], a2);
+                a5 = fma(p0, q[SHT_C], a5);
+                a6 = fma(p0, q[SHT_S], a6);
+                b1 = fma(pm1, q[SHT_EC], b1);
+                b2 = fma(pm1, q[SHT_ES], b2);
+                const float pn = fma(q[SHT_A] * uf, p0, -(q[SHT_B] * wf) * pm1);
+                pm1 = p0;
+                p0 = pn;
+            }
+            t += 4 * SHT_STRIDE;
+            if (--left == 0) {
+                double A1 = 0.0, A2 = 0.0, A5 = 0.0, A6 = 0.0, B1 = 0.0, B2 = 0.0;
+                A1 += double(a1);
+                A2 += double(a2);
+                A5 += double(a5);
+                A6 += double(a6);
+                B1 += double(b1);
+                B2 += double(b2);
+                const double X1 = cosML * A1 + sinML * A2;
+                const double X5 = cosML * A5 + sinML * A6;
+                const double XE = cosML * B1 + sinML * B2;
+                const double X6 = sinML * A5 - cosML * A6;
+                ar -= X1;
+                aphiNum += g.alpha * XE - g.sinPhi * (X1 - X5);
+                alamNum -= double(m) * X6;
+                if (m + 1 <= L) Pmm = diag[m + 1] * v * Pmm;
+                const double s = g.cosL * sinML + g.sinL * cosML;
+                const double c = g.cosL * cosML - g.sinL * sinML;
+                sinML = s;
+                cosML = c;
+                if (++m >= m1) break;
+                left = (L - m + 4) >> 2;
+                p0 = float(Pmm);
+                pm1 = 0.0f;
+                a1 = a2 = a5 = a6 = b1 = b2 = 0.0f;
+            }
+        }
     }
     outR = ar;
     if (g.cosPhi != 0.0) {
